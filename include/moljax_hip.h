@@ -1,0 +1,134 @@
+/*
+ * moljax_hip.h -- C ABI of the MI355X (gfx950) kernels behind the moljax_amd
+ * Python API.  Plain pointers, sizes and a hipStream_t passed as void*; no
+ * torch types.  Every function returns 0 on success or a negative MJX_E_* code
+ * and leaves a message for mjx_last_error() (thread-local).  No C++ exception
+ * crosses this boundary.  All device pointers must be valid on the current HIP
+ * device; all work is enqueued on `stream` (NULL = the legacy default stream)
+ * and the calls are asynchronous unless stated otherwise.
+ *
+ * Array conventions (reference moljax/core/grid.py:115-208): a 2-D field is
+ * row-major with axis 0 = y (rows) and axis 1 = x (columns).  Pointers passed
+ * as `*_in` / `*_out` point at the first INTERIOR element; `pitch` is the row
+ * stride and `istride` the stride between batch instances, both in elements.
+ * A ghost-padded (ny+2, nx+2) array is therefore passed as base + pitch + 1
+ * with pitch = nx + 2.  dtype: MJX_F32 or MJX_F64 for every real array of a
+ * call; complex work buffers hold interleaved (re, im) pairs of that dtype.
+ *
+ * Per-instance parameter table `inst`: a device array of B rows of 8 float64:
+ *   [beta_a, beta_b, alpha_a, alpha_b, F, k, dt, 0]
+ * The spectral multiplier applied to field f at wavenumber (kx, ky) is
+ *   ((1 + alpha_f*lam) / (1 - beta_f*lam))^nrep / (nx*ny),
+ * with lam = lamx[kx] + lamy[ky] (or lam2d[ky*nx + kx]) and |1 - beta*lam|
+ * clamped below at 1e-14 as in moljax/core/fft_solvers.py:280-283.
+ */
+#ifndef MOLJAX_HIP_H
+#define MOLJAX_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MJX_F32 0
+#define MJX_F64 1
+
+#define MJX_OK 0
+#define MJX_E_ARG (-1)     /* invalid argument (shape, dtype, pointer) */
+#define MJX_E_SIZE (-2)    /* grid size not supported by the kernels */
+#define MJX_E_HIP (-3)     /* HIP runtime error */
+#define MJX_E_LIMIT (-4)   /* request exceeds a fixed capacity */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mjx_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* mjx_last_error(void);
+
+/* Fill the FFT twiddle tables on the current device (FP64 sincospi, rounded
+ * once to FP32) and wait for completion.  Called implicitly by every entry
+ * point on its first use on a device; idempotent. */
+int mjx_init(void);
+
+/* Number of bytes of complex work buffer needed by the periodic spectral
+ * entry points for B instances of ny x nx in the given dtype. */
+long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx);
+
+/* Periodic 2-D spectral multiply of up to two real fields per instance:
+ *   out_f = F^-1[ m_f(kx, ky) * F[in_f] ]   for f in {a, b},
+ * m_f as described above.  One row pass (x-FFT of a + i*b, split into the two
+ * Hermitian half-spectra), one column pass (y-FFT, multiply, inverse y-FFT)
+ * and one row pass back.  b_in may be NULL (treated as zero; b_out ignored).
+ * fill_ghosts != 0 also writes the periodic ghost ring of the outputs
+ * (moljax/core/bc.py:202-215 order).  lam2d != NULL selects a full (ny, nx)
+ * symbol array instead of the separable lamx/lamy tables.
+ * Replaces: moljax/core/fft_solvers.py:261-294 (solve_helmholtz_2d),
+ *           fft_solvers.py:368-445 (solve_diffusion_fft_field,
+ *           apply_diffusion_inverse_fft) on periodic 2-D grids, and the
+ *           FFT-CN step of benchmarks/benchmark_vs_scipy.py:74-91
+ *           (alpha = beta = dt*D/2, lam = -(kx^2+ky^2)).
+ * Sizes: nx, ny powers of two in [16, 4096]. */
+int mjx_spectral_pair(int dtype, int B, int ny, int nx,
+                      const void* a_in, const void* b_in, long long in_pitch, long long in_istride,
+                      void* a_out, void* b_out, long long out_pitch, long long out_istride,
+                      int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d,
+                      const double* inst, int nrep, void* work, void* stream);
+
+/* Gray-Scott IMEX-Strang, n_steps fixed steps, in place on (u, v):
+ *   y* = (I - dt/2 D lap)^-1 y ; Heun on R(u,v) = (-uv^2 + F(1-u), uv^2 - (F+k)v)
+ *   y** = y* + dt/2 (R(y*) + R(y* + dt R(y*))) ; y_new = (I - dt/2 D lap)^-1 y**
+ * with the FD symbol of the periodic 5-point Laplacian in lamx/lamy (or the
+ * full (ny, nx) array lam2d when non-NULL).  The
+ * trailing half-solve of step s and the leading half-solve of step s+1 are
+ * applied as one spectral multiply by the squared denominator (identical
+ * operator; one FFT round trip per step instead of two).  inst rows carry
+ * beta_a = dt/2*Du, beta_b = dt/2*Dv, F, k, dt per instance.
+ * Replaces: moljax/core/stepping.py:370-435 (imex_strang_step) driven by
+ *           stepping.py:1374-1436 (integrate_imex_fixed_dt) for the model of
+ *           moljax/core/model.py:529-584 (create_gray_scott_periodic_fft).
+ * work: mjx_spectral_workspace_bytes(dtype, B, ny, nx) bytes. */
+int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch,
+                       long long istride, int fill_ghosts, const void* lamx, const void* lamy,
+                       const void* lam2d, const double* inst, int n_steps, void* work, void* stream);
+
+/* Opt-in timing of the fused IMEX chain's hot kernels (kind 0: the fused
+ * Gray-Scott row pass, kind 1: the column pass).  While enabled every such
+ * launch is bracketed by HIP events on its stream.  Enabling resets the
+ * totals; mjx_kernel_timing waits for the recorded events and returns the
+ * summed milliseconds and the launch count.  Measurement support with no
+ * counterpart in the reference (its timing is wall clock around
+ * block_until_ready, benchmarks/benchmark_utils.py:112-149). */
+int mjx_kernel_timing_enable(int on);
+int mjx_kernel_timing(int kind, double* total_ms, long long* count);
+
+/* Finite-difference operator on ghost-padded arrays; writes the whole padded
+ * output (interior = stencil, ghost ring = 0).  in/out point at the first
+ * interior element; pitches must be >= nx + 2.
+ *   op 0 Laplacian (d2/dy2 + d2/dx2)       moljax/core/operators.py:311-322
+ *   op 1 second derivative along axis      operators.py:271-308
+ *   op 2 central first derivative          operators.py:159-197
+ *   op 3 upwind first derivative, velocity vel (array, same layout rules) or
+ *        vel_scalar when vel == NULL       operators.py:200-268
+ * axis: 0 = y (rows), 1 = x (columns). */
+int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long long in_pitch,
+                long long in_istride, void* out, long long out_pitch, long long out_istride, int axis,
+                double dx, double dy, const void* vel, long long vel_pitch, long long vel_istride,
+                double vel_scalar, void* stream);
+
+/* Ghost-ring fill of a padded field in place.  kind 0 periodic, 1 Dirichlet
+ * (ghost = 2 g - interior), 2 Neumann (ghost = interior -/+ 2 h flux);
+ * side_values = {left, right, bottom, top} (NULL = zeros).  Replaces
+ * moljax/core/bc.py:202-307 (_apply_bc_2d_{periodic,dirichlet,neumann}),
+ * corners as the reference's x-then-y order produces them. */
+int mjx_bc_fill2d(int dtype, int B, int ny, int nx, void* f, long long pitch, long long istride, int kind,
+                  const double* side_values, double dx, double dy, void* stream);
+
+/* Gray-Scott reaction on n contiguous points: ru = -u v^2 + F(1-u),
+ * rv = u v^2 - (F+k) v.  Replaces moljax/core/operators.py:644-667. */
+int mjx_gs_reaction(int dtype, long long n, const void* u, const void* v, void* ru, void* rv, double F,
+                    double k, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOLJAX_HIP_H */

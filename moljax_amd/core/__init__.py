@@ -1,0 +1,16 @@
+"""moljax_amd.core: drop-in host API of moljax/core over the MI355X kernels."""
+from .grid import Grid1D, Grid2D, GridType
+from .bc import BCType, FieldBCSpec, BCSpec, create_default_bc_spec
+from .bc_fill import apply_bc
+from .state import (StateDict, tree_add, tree_sub, tree_scale, tree_axpy, tree_zeros_like, tree_ones_like,
+                    tree_norm2, tree_norm_inf, tree_vdot, scaled_error_norm, flatten_to_vec)
+from .utils import StatusCode, get_interior
+from .operators import (LinearOp, NonlinearOp, laplacian_2d, d2_central_2d, d1_central_2d, d1_upwind_2d,
+                        gray_scott_reaction_op, fd_laplacian_symbol_1d, fd_laplacian_symbol_2d)
+from .fft_solvers import (FFTCache1D, FFTCache2D, build_wavenumbers_1d, build_wavenumbers_2d,
+                          laplacian_symbol_1d, laplacian_symbol_2d, create_fft_cache, create_fft_cache_1d,
+                          create_fft_cache_2d, solve_helmholtz_2d, solve_helmholtz, extract_interior,
+                          embed_interior, solve_diffusion_fft_field, apply_diffusion_inverse_fft, fft_cn_step)
+from .model import (MOLModel, create_gray_scott_model, create_gray_scott_periodic_fft, get_fft_cache,
+                    get_diffusivities)
+from .stepping import IntegratorType, imex_strang_step, integrate_imex_fixed_dt

@@ -1,0 +1,144 @@
+"""Finite-difference operators and reaction terms (moljax/core/operators.py).
+
+2-D operators take a ghost-padded (ny+2, nx+2) array (or a batch
+(..., ny+2, nx+2)) whose ghosts are already filled and return a new padded
+array with the stencil on the interior and zeros in the ghost ring, computed
+by the mjx_fd_op2d kernel.
+"""
+from dataclasses import dataclass
+from typing import Any, Callable, Dict, List, Optional, Union
+
+import numpy as np
+import torch
+
+from .. import _hip
+from ._arrays import as_device, field_dtype, view2d
+from .grid import Grid2D, GridType
+from .state import StateDict, tree_add, tree_zeros_like
+
+FD_LAP, FD_D2, FD_D1C, FD_D1UP = 0, 1, 2, 3
+
+
+@dataclass(frozen=True)
+class LinearOp:
+    """operators.py:329-346."""
+    name: str
+    apply: Callable[[StateDict, GridType, float, Dict[str, Any]], StateDict]
+    dt_bound: Optional[Callable[[GridType, Dict[str, Any]], float]] = None
+
+
+@dataclass(frozen=True)
+class NonlinearOp:
+    """operators.py:349-361."""
+    name: str
+    apply: Callable[[StateDict, GridType, float, Dict[str, Any]], StateDict]
+    dt_bound: Optional[Callable[[GridType, StateDict, Dict[str, Any]], float]] = None
+
+
+def op_sum(ops: List[Union[LinearOp, NonlinearOp]]) -> Callable:
+    """operators.py:364-380."""
+    def combined_apply(state, grid, t, params):
+        result = ops[0].apply(state, grid, t, params)
+        for op in ops[1:]:
+            result = tree_add(result, op.apply(state, grid, t, params))
+        return result
+    return combined_apply
+
+
+def state_zero_like(state: StateDict) -> StateDict:
+    return tree_zeros_like(state)
+
+
+def _check_padded(f: torch.Tensor, grid: Grid2D):
+    if f.shape[-2:] != (grid.ny_total, grid.nx_total):
+        raise ValueError(f"expected padded shape (..., {grid.ny_total}, {grid.nx_total}), got {tuple(f.shape)}")
+    if grid.n_ghost != 1:
+        raise ValueError("the FD kernels serve n_ghost = 1 grids")
+
+
+def _fd(f, grid: Grid2D, op: int, axis: int = 1, vel=None) -> torch.Tensor:
+    t = as_device(f)
+    t = t.to(field_dtype(t))
+    _check_padded(t, grid)
+    src, pitch, istr, B = view2d(t)
+    out = torch.empty((B, grid.ny_total, grid.nx_total), dtype=t.dtype, device=t.device)
+    vel_t, vp, vs, vsc = None, 0, 0, 0.0
+    if vel is not None:
+        if isinstance(vel, (int, float)) or (isinstance(vel, torch.Tensor) and vel.dim() == 0) \
+                or (isinstance(vel, np.ndarray) and vel.ndim == 0):
+            vsc = float(vel)
+        else:
+            vt = as_device(vel, t.dtype)
+            if vt.shape[-2:] != t.shape[-2:]:
+                raise ValueError("velocity array must have the padded field shape")
+            if vt.dim() < t.dim():
+                vt = vt.expand(t.shape).contiguous()
+            vel_t, vp, vs, _ = view2d(vt)
+            vel_t = vel_t[:, 1:, 1:]
+    lib = _hip.lib()
+    rc = lib.mjx_fd_op2d(_hip.dtype_code(t.dtype), op, B, grid.ny, grid.nx,
+                         _hip.ptr(src[:, 1:, 1:]), pitch, istr,
+                         _hip.ptr(out[:, 1:, 1:]), out.stride(1), out.stride(0), int(axis),
+                         float(grid.dx), float(grid.dy), _hip.ptr(vel_t), vp, vs, vsc, _hip.stream_ptr())
+    _hip.check(rc, "mjx_fd_op2d")
+    return out.view(t.shape)
+
+
+def laplacian_2d(f, grid: Grid2D) -> torch.Tensor:
+    """operators.py:311-322: d2/dy2 + d2/dx2 on the interior, zero ghosts."""
+    return _fd(f, grid, FD_LAP)
+
+
+def d2_central_2d(f, grid: Grid2D, axis: int) -> torch.Tensor:
+    """operators.py:271-308."""
+    return _fd(f, grid, FD_D2, axis)
+
+
+def d1_central_2d(f, grid: Grid2D, axis: int) -> torch.Tensor:
+    """operators.py:159-197."""
+    return _fd(f, grid, FD_D1C, axis)
+
+
+def d1_upwind_2d(f, v, grid: Grid2D, axis: int) -> torch.Tensor:
+    """operators.py:200-268: backward difference where v >= 0, forward
+    otherwise; v is a scalar or a padded array."""
+    return _fd(f, grid, FD_D1UP, axis, vel=v)
+
+
+def gray_scott_reaction(u, v, F: float, k: float):
+    """(-u v^2 + F(1-u), u v^2 - (F+k) v) point-wise (mjx_gs_reaction)."""
+    tu = as_device(u)
+    tu = tu.to(field_dtype(tu)).contiguous()
+    tv = as_device(v, tu.dtype).contiguous()
+    if tu.shape != tv.shape:
+        raise ValueError("u and v must have the same shape")
+    ru = torch.empty_like(tu)
+    rv = torch.empty_like(tv)
+    rc = _hip.lib().mjx_gs_reaction(_hip.dtype_code(tu.dtype), tu.numel(), _hip.ptr(tu), _hip.ptr(tv),
+                                    _hip.ptr(ru), _hip.ptr(rv), float(F), float(k), _hip.stream_ptr())
+    _hip.check(rc, "mjx_gs_reaction")
+    return ru, rv
+
+
+def gray_scott_reaction_op() -> NonlinearOp:
+    """operators.py:644-677."""
+    def apply_reaction(state, grid, t, params):
+        ru, rv = gray_scott_reaction(state["u"], state["v"], params.get("F", 0.04), params.get("k", 0.06))
+        return {"u": ru, "v": rv}
+
+    def dt_bound(grid, state, params):
+        return 0.5 / (params.get("F", 0.04) + params.get("k", 0.06) + 1e-14)
+
+    return NonlinearOp(name="gray_scott_reaction", apply=apply_reaction, dt_bound=dt_bound)
+
+
+def fd_laplacian_symbol_2d(ny: int, nx: int, dy: float, dx: float, dtype=torch.float64) -> torch.Tensor:
+    """operators.py:734-766 (same as fft_solvers.laplacian_symbol_2d)."""
+    from .fft_solvers import laplacian_symbol_2d
+    return laplacian_symbol_2d(ny, nx, dy, dx, dtype)
+
+
+def fd_laplacian_symbol_1d(nx: int, dx: float, dtype=torch.float64) -> torch.Tensor:
+    """operators.py:708-731."""
+    from .fft_solvers import laplacian_symbol_1d
+    return laplacian_symbol_1d(nx, dx, dtype)

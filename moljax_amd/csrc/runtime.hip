@@ -1,0 +1,92 @@
+// Library runtime: error reporting and the per-device FFT twiddle tables.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "mjx_common.h"
+#include "mjx_fft.h"
+
+namespace mjx {
+
+// exp(-2*pi*i*t/MJX_TWN), evaluated in FP64 (sincospi) and rounded once to FP32.
+__global__ void k_init_twiddles(cpx<double>* tw64, cpx<float>* tw32) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < MJX_TWN) {
+    double s, c;
+    sincospi(2.0 * (double)t / (double)MJX_TWN, &s, &c);
+    tw64[t] = mk(c, -s);
+    tw32[t] = mk((float)c, (float)-s);
+  }
+}
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MJX_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return MJX_OK;
+}
+
+static constexpr int kMaxDev = 64;
+static std::mutex g_init_mu;
+static cpx<double>* g_tw64[kMaxDev] = {nullptr};
+static cpx<float>* g_tw32[kMaxDev] = {nullptr};
+
+static int current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return -1;
+  return dev;
+}
+
+// Allocates and fills the current device's tables on first use (a one-time,
+// synchronous set-up; later calls only read the cached pointers).
+int ensure_tables(hipStream_t) {
+  const int dev = current_device();
+  if (dev < 0) return fail(MJX_E_HIP, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_tw64[dev]) return MJX_OK;
+  cpx<double>* t64 = nullptr;
+  cpx<float>* t32 = nullptr;
+  if (hipMalloc((void**)&t64, sizeof(cpx<double>) * MJX_TWN) != hipSuccess) return fail(MJX_E_HIP, "hipMalloc twiddles");
+  if (hipMalloc((void**)&t32, sizeof(cpx<float>) * MJX_TWN) != hipSuccess) {
+    (void)hipFree(t64);
+    return fail(MJX_E_HIP, "hipMalloc twiddles");
+  }
+  hipLaunchKernelGGL(k_init_twiddles, dim3((MJX_TWN + 255) / 256), dim3(256), 0, 0, t64, t32);
+  int rc = check_launch("init_twiddles");
+  if (rc) return rc;
+  const hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(MJX_E_HIP, std::string("init_twiddles sync: ") + hipGetErrorString(e));
+  g_tw32[dev] = t32;
+  g_tw64[dev] = t64;
+  return MJX_OK;
+}
+
+const cpx<float>* twiddles_f32() {
+  const int dev = current_device();
+  return dev < 0 ? nullptr : g_tw32[dev];
+}
+const cpx<double>* twiddles_f64() {
+  const int dev = current_device();
+  return dev < 0 ? nullptr : g_tw64[dev];
+}
+
+}  // namespace mjx
+
+extern "C" {
+
+int mjx_version(void) { return 100; }
+
+const char* mjx_last_error(void) { return mjx::g_err.c_str(); }
+
+int mjx_init(void) { return mjx::ensure_tables(nullptr); }
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// C-ABI entry points of the periodic spectral path (kernels in
+// spectral_kernels.h, instantiated in spectral_f32.hip / spectral_f64.hip).
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mjx_common.h"
+#include "spectral_api.h"
+
+namespace mjx {
+
+// Opt-in per-kernel timing: when enabled, every launch of the two hot kernels
+// of the fused IMEX chain (row_gs, col_apply) is bracketed by HIP events on
+// its own stream; totals are read back by mjx_kernel_timing().
+enum { KT_ROW_GS = 0, KT_COL_APPLY = 1, KT_NUM = 2 };
+struct Timed {
+  int kind;
+  hipEvent_t e0, e1;
+};
+static std::mutex g_kt_mu;
+static bool g_kt_on = false;
+static std::vector<Timed> g_kt_pending;
+static double g_kt_ms[KT_NUM] = {0, 0};
+static long long g_kt_n[KT_NUM] = {0, 0};
+
+template <class Fn>
+static int timed_launch(int kind, hipStream_t st, Fn&& fn) {
+  bool on;
+  {
+    std::lock_guard<std::mutex> lk(g_kt_mu);
+    on = g_kt_on;
+  }
+  if (!on) return fn();
+  Timed t;
+  t.kind = kind;
+  if (hipEventCreate(&t.e0) != hipSuccess || hipEventCreate(&t.e1) != hipSuccess)
+    return fail(MJX_E_HIP, "hipEventCreate failed");
+  (void)hipEventRecord(t.e0, st);
+  const int rc = fn();
+  (void)hipEventRecord(t.e1, st);
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt_pending.push_back(t);
+  return rc;
+}
+
+static int check_grid(int dtype, int B, int ny, int nx) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (B < 1) return fail(MJX_E_ARG, "batch must be >= 1");
+  if (!is_pow2_len(nx) || !is_pow2_len(ny))
+    return fail(MJX_E_SIZE, "grid " + std::to_string(ny) + "x" + std::to_string(nx) +
+                                " not supported: nx and ny must be powers of two in [16, 4096]");
+  if ((long long)B * ny * (nx / 16) > 0x7fffffffLL) return fail(MJX_E_LIMIT, "batch too large for one launch");
+  return MJX_OK;
+}
+
+template <typename T>
+static int spectral_pair_t(int B, int ny, int nx, const T* a_in, const T* b_in, long long ip_, long long is_,
+                           T* a_out, T* b_out, long long op_, long long os_, int ghosts, const T* lamx,
+                           const T* lamy, const T* lam2d, const InstParams* inst, int nrep, cpx<T>* W,
+                           hipStream_t st) {
+  int rc = spec_row_fwd_pack(nx, B, ny, a_in, b_in, ip_, is_, W, st);
+  if (rc) return rc;
+  rc = spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, nrep, st);
+  if (rc) return rc;
+  return spec_row_inv_unpack(nx, B, ny, W, a_out, b_in ? b_out : nullptr, op_, os_, ghosts, st);
+}
+
+template <typename T>
+static int gs_imex_t(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                     const T* lamx, const T* lamy, const T* lam2d, const InstParams* inst, int n_steps, cpx<T>* W,
+                     hipStream_t st) {
+  int rc = spec_row_fwd_pack(nx, B, ny, u, v, pitch, istride, W, st);
+  if (rc) return rc;
+  rc = spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, 1, st);
+  if (rc) return rc;
+  for (int s = 0; s < n_steps; ++s) {
+    rc = timed_launch(KT_ROW_GS, st, [&] { return spec_row_gs(nx, B, ny, W, inst, st); });
+    if (rc) return rc;
+    const int nrep = (s == n_steps - 1) ? 1 : 2;
+    rc = timed_launch(KT_COL_APPLY, st,
+                      [&] { return spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, nrep, st); });
+    if (rc) return rc;
+  }
+  return spec_row_inv_unpack(nx, B, ny, W, u, v, pitch, istride, ghosts, st);
+}
+
+}  // namespace mjx
+
+using namespace mjx;
+
+extern "C" {
+
+long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx) {
+  const long long es = dtype == MJX_F64 ? 16 : 8;
+  return (long long)B * ny * nx * es;
+}
+
+int mjx_spectral_pair(int dtype, int B, int ny, int nx, const void* a_in, const void* b_in, long long in_pitch,
+                      long long in_istride, void* a_out, void* b_out, long long out_pitch, long long out_istride,
+                      int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d, const double* inst,
+                      int nrep, void* work, void* stream) {
+  int rc = check_grid(dtype, B, ny, nx);
+  if (rc) return rc;
+  if (!a_in || !a_out || !work || !inst) return fail(MJX_E_ARG, "null pointer argument");
+  if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
+  if (b_in && !b_out) return fail(MJX_E_ARG, "b_out required when b_in is given");
+  if (nrep < 0) return fail(MJX_E_ARG, "nrep must be >= 0");
+  if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (fill_ghosts && out_pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs out_pitch >= nx + 2");
+  rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (dtype == MJX_F32)
+    return spectral_pair_t<float>(B, ny, nx, (const float*)a_in, (const float*)b_in, in_pitch, in_istride,
+                                  (float*)a_out, (float*)b_out, out_pitch, out_istride, fill_ghosts,
+                                  (const float*)lamx, (const float*)lamy, (const float*)lam2d, ip, nrep,
+                                  (cpx<float>*)work, st);
+  return spectral_pair_t<double>(B, ny, nx, (const double*)a_in, (const double*)b_in, in_pitch, in_istride,
+                                 (double*)a_out, (double*)b_out, out_pitch, out_istride, fill_ghosts,
+                                 (const double*)lamx, (const double*)lamy, (const double*)lam2d, ip, nrep,
+                                 (cpx<double>*)work, st);
+}
+
+int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                       int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d, const double* inst,
+                       int n_steps, void* work, void* stream) {
+  int rc = check_grid(dtype, B, ny, nx);
+  if (rc) return rc;
+  if (!u || !v || !inst || !work) return fail(MJX_E_ARG, "null pointer argument");
+  if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
+  if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
+  if (pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (fill_ghosts && pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= nx + 2");
+  if (n_steps == 0) return MJX_OK;
+  rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (dtype == MJX_F32)
+    return gs_imex_t<float>(B, ny, nx, (float*)u, (float*)v, pitch, istride, fill_ghosts, (const float*)lamx,
+                            (const float*)lamy, (const float*)lam2d, ip, n_steps, (cpx<float>*)work, st);
+  return gs_imex_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)lamx,
+                           (const double*)lamy, (const double*)lam2d, ip, n_steps, (cpx<double>*)work, st);
+}
+
+int mjx_kernel_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt_on = on != 0;
+  if (on) {
+    for (auto& t : g_kt_pending) {
+      (void)hipEventDestroy(t.e0);
+      (void)hipEventDestroy(t.e1);
+    }
+    g_kt_pending.clear();
+    for (int i = 0; i < KT_NUM; ++i) {
+      g_kt_ms[i] = 0;
+      g_kt_n[i] = 0;
+    }
+  }
+  return MJX_OK;
+}
+
+int mjx_kernel_timing(int kind, double* total_ms, long long* count) {
+  if (kind < 0 || kind >= KT_NUM || !total_ms || !count) return fail(MJX_E_ARG, "bad kernel timing query");
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  for (auto& t : g_kt_pending) {
+    if (hipEventSynchronize(t.e1) != hipSuccess) return fail(MJX_E_HIP, "hipEventSynchronize failed");
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.e0, t.e1) != hipSuccess) return fail(MJX_E_HIP, "hipEventElapsedTime failed");
+    g_kt_ms[t.kind] += ms;
+    g_kt_n[t.kind] += 1;
+    (void)hipEventDestroy(t.e0);
+    (void)hipEventDestroy(t.e1);
+  }
+  g_kt_pending.clear();
+  *total_ms = g_kt_ms[kind];
+  *count = g_kt_n[kind];
+  return MJX_OK;
+}
+
+}  // extern "C"

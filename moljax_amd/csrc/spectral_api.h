@@ -1,0 +1,24 @@
+// Host-side launch API of the periodic spectral kernels, one overload per
+// precision (defined in spectral_f32.hip / spectral_f64.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mjx_common.h"
+#include "mjx_fft.h"
+
+namespace mjx {
+
+#define MJX_SPEC_DECL(T)                                                                                     \
+  int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,   \
+                        cpx<T>* W, hipStream_t st);                                                          \
+  int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,              \
+                          long long istride, int ghosts, hipStream_t st);                                    \
+  int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st);                  \
+  int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,        \
+                     const InstParams* ip, int nrep, hipStream_t st);
+
+MJX_SPEC_DECL(float)
+MJX_SPEC_DECL(double)
+#undef MJX_SPEC_DECL
+
+}  // namespace mjx

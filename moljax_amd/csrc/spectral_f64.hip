@@ -1,0 +1,25 @@
+// FP64 instantiation of the periodic spectral launchers.
+#include "spectral_api.h"
+#include "spectral_kernels.h"
+
+namespace mjx {
+
+using T = double;
+
+int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
+                      hipStream_t st) {
+  return Launch<T>::row_fwd_pack(nx, B, ny, a, b, pitch, istride, W, st);
+}
+int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
+                        int ghosts, hipStream_t st) {
+  return Launch<T>::row_inv_unpack(nx, B, ny, W, a, b, pitch, istride, ghosts, st);
+}
+int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
+  return Launch<T>::row_gs(nx, B, ny, W, ip, st);
+}
+int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
+                   const InstParams* ip, int nrep, hipStream_t st) {
+  return Launch<T>::col_apply(ny, B, nx, W, lamx, lamy, lam2d, ip, nrep, st);
+}
+
+}  // namespace mjx

@@ -1,0 +1,496 @@
+// Periodic 2-D spectral kernels and their launchers, included once per
+// precision by spectral_f32.hip / spectral_f64.hip (explicit instantiation of
+// Launch<T>) so the two precisions compile in parallel.
+// Kernels: row passes (x-FFT with two-real-field
+// packing), column pass (y-FFT, spectral multiply, inverse y-FFT) and the
+// fused Gray-Scott row pass (inverse x-FFT, Heun reaction, forward x-FFT).
+//
+// Packed x-spectral layout (one complex row of nx values per grid row), for
+// the real fields a and b with row spectra A(kx), B(kx) (Hermitian in kx):
+//   col 0        A(0) + i*A(nx/2)        (both real)
+//   col nx/2     B(0) + i*B(nx/2)        (both real)
+//   col p, 0<p<nx/2          A(p)
+//   col nx/2+p, 0<p<nx/2     B(p)
+// so that every column except 0 and nx/2 carries one field at one kx and the
+// column pass is a plain complex y-FFT with a real multiplier.  Columns 0 and
+// nx/2 carry two real sequences with different multipliers ma, mb; after the
+// y-FFT, out(ky) = (ma+mb)/2 Z(ky) + (ma-mb)/2 conj(Z(-ky)).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "mjx_common.h"
+#include "mjx_fft.h"
+
+namespace mjx {
+
+// Current device's twiddle tables (runtime.hip).
+const cpx<float>* twiddles_f32();
+const cpx<double>* twiddles_f64();
+template <typename T>
+inline const cpx<T>* twiddles() {
+  if constexpr (std::is_same<T, float>::value) return twiddles_f32();
+  else return twiddles_f64();
+}
+
+// ------------------------------------------------------------ pack/unpack
+// Natural-order complex row spectrum Z of a + i*b in lds[0..nx); returns the
+// packed value for column p.
+template <typename T, int NX>
+__device__ __forceinline__ cpx<T> pack_at(const cpx<T>* lds, int p) {
+  constexpr int H = NX / 2;
+  if (p == 0) return mk(lds[0].re, lds[H].re);
+  if (p == H) return mk(lds[0].im, lds[H].im);
+  if (p < H) {
+    const cpx<T> z = lds[p], zm = lds[NX - p];
+    return mk((z.re + zm.re) * (T)0.5, (z.im - zm.im) * (T)0.5);
+  }
+  const int e = p - H;
+  const cpx<T> z = lds[e], zm = lds[NX - e];
+  // (z - conj(zm)) / (2i)
+  const T dre = z.re - zm.re, dim = z.im + zm.im;
+  return mk(dim * (T)0.5, -dre * (T)0.5);
+}
+
+// Packed row in lds[0..nx); returns Z(e) = A(e) + i*B(e).
+template <typename T, int NX>
+__device__ __forceinline__ cpx<T> unpack_at(const cpx<T>* lds, int e) {
+  constexpr int H = NX / 2;
+  if (e == 0) return mk(lds[0].re, lds[H].re);
+  if (e == H) return mk(lds[0].im, lds[H].im);
+  cpx<T> A, B;
+  if (e < H) {
+    A = lds[e];
+    B = lds[H + e];
+  } else {
+    A = conjg(lds[NX - e]);
+    B = conjg(lds[H + NX - e]);
+  }
+  return mk(A.re - B.im, A.im + B.re);
+}
+
+template <typename T>
+__device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
+  T den = (T)1 - beta * lam;
+  if (fabs(den) < (T)1e-14) den = (T)1e-14;
+  const T g = ((T)1 + alpha * lam) / den;
+  T m = (T)1, b = g;
+  int n = nrep;
+  while (n > 0) {
+    if (n & 1) m *= b;
+    b *= b;
+    n >>= 1;
+  }
+  return m * s;
+}
+
+// Gray-Scott reaction terms, moljax/core/operators.py:655-667.
+template <typename T>
+__device__ __forceinline__ void gs_react(T u, T v, T F, T k, T& ru, T& rv) {
+  const T uvv = u * v * v;
+  ru = -uvv + F * ((T)1 - u);
+  rv = uvv - (F + k) * v;
+}
+
+// ------------------------------------------------------------- row passes
+template <int NX, int TR_>
+struct RowCfg {
+  static constexpr int P = plan_p(NX);
+  static constexpr int TN = NX / P;
+  static constexpr int TR = TR_;      // rows per workgroup
+  static constexpr int BT = TR * TN;  // threads per workgroup
+  static constexpr int LP = NX;       // lds row pitch (complex elements)
+};
+
+// Real fields (a, b) -> packed x-spectra in W.
+template <typename T, int NX, int TR>
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(const T* __restrict__ a, const T* __restrict__ b,
+                                                      long long pitch, long long istride,
+                                                      cpx<T>* __restrict__ W, int ny, int rows_per_inst_tiles,
+                                                      const cpx<T>* __restrict__ tw) {
+  using C = RowCfg<NX, TR>;
+  __shared__ cpx<T> lds[C::TR * C::LP];
+  const int G = gridDim.x;
+  const int w = xcd_remap(blockIdx.x, G);
+  const int inst = w / rows_per_inst_tiles;
+  const int tile = w - inst * rows_per_inst_tiles;
+  const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
+  const int y = tile * C::TR + rl;
+  cpx<T>* lrow = lds + rl * C::LP;
+  const T* ra = a + inst * istride + (long long)y * pitch;
+  const T* rb = b ? b + inst * istride + (long long)y * pitch : nullptr;
+  cpx<T> v[C::P];
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int x = j + C::TN * m;
+    v[m] = mk(ra[x], rb ? rb[x] : (T)0);
+  }
+  line_fft<T, NX, -1, 1>(v, j, lrow, tw);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) lrow[j + C::TN * m] = v[m];
+  __syncthreads();
+  cpx<T>* wrow = W + ((long long)inst * ny + y) * NX;
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int p = j + C::TN * m;
+    wrow[p] = pack_at<T, NX>(lrow, p);
+  }
+}
+
+// Load one packed row into LDS and return the unpacked Z(e) at e = j + TN*m.
+template <typename T, int NX, int TR>
+__device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, cpx<T>* lrow, int j,
+                                            cpx<T> (&v)[RowCfg<NX, TR>::P]) {
+  using C = RowCfg<NX, TR>;
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) lrow[j + C::TN * m] = wrow[j + C::TN * m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) v[m] = unpack_at<T, NX>(lrow, j + C::TN * m);
+  __syncthreads();
+}
+
+// Packed x-spectra in W -> real fields (a, b), optional periodic ghost ring.
+template <typename T, int NX, int TR>
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(const cpx<T>* __restrict__ W, T* __restrict__ a,
+                                                        T* __restrict__ b, long long pitch, long long istride,
+                                                        int ny, int rows_per_inst_tiles, int ghosts,
+                                                        const cpx<T>* __restrict__ tw) {
+  using C = RowCfg<NX, TR>;
+  __shared__ cpx<T> lds[C::TR * C::LP];
+  const int G = gridDim.x;
+  const int w = xcd_remap(blockIdx.x, G);
+  const int inst = w / rows_per_inst_tiles;
+  const int tile = w - inst * rows_per_inst_tiles;
+  const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
+  const int y = tile * C::TR + rl;
+  cpx<T>* lrow = lds + rl * C::LP;
+  cpx<T> v[C::P];
+  load_unpack<T, NX, TR>(W + ((long long)inst * ny + y) * NX, lrow, j, v);
+  line_fft<T, NX, +1, 1>(v, j, lrow, tw);
+  T* ra = a + inst * istride + (long long)y * pitch;
+  T* rb = b ? b + inst * istride + (long long)y * pitch : nullptr;
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int x = j + C::TN * m;
+    ra[x] = v[m].re;
+    if (rb) rb[x] = v[m].im;
+  }
+  if (ghosts) {
+    // Periodic ghost ring, x first then y (moljax/core/bc.py:202-215): row y
+    // is also written to the ghost row on the opposite side when it is the
+    // first/last interior row; the x ghosts follow the row.
+    const long long offs[3] = {0, (y == 0) ? (long long)ny * pitch : 0,
+                               (y == ny - 1) ? -(long long)ny * pitch : 0};
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      if (o > 0 && offs[o] == 0) continue;
+      T* qa = ra + offs[o];
+      T* qb = rb ? rb + offs[o] : nullptr;
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) {
+        const int x = j + C::TN * m;
+        if (o > 0) {
+          qa[x] = v[m].re;
+          if (qb) qb[x] = v[m].im;
+        }
+        if (x == 0) {
+          qa[NX] = v[m].re;
+          if (qb) qb[NX] = v[m].im;
+        }
+        if (x == NX - 1) {
+          qa[-1] = v[m].re;
+          if (qb) qb[-1] = v[m].im;
+        }
+      }
+    }
+  }
+}
+
+// Fused Gray-Scott row pass: packed spectra -> physical (u, v) -> Heun
+// reaction step -> packed spectra, in place.
+// Heun exactly as moljax/core/stepping.py:414-425:
+//   R1 = R(y*), y~ = y* + dt R1, R2 = R(y~), y** = y* + (dt/2)(R1 + R2).
+template <typename T, int NX, int TR>
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W, const InstParams* __restrict__ ip, int ny,
+                                                int rows_per_inst_tiles, const cpx<T>* __restrict__ tw) {
+  using C = RowCfg<NX, TR>;
+  __shared__ cpx<T> lds[C::TR * C::LP];
+  const int G = gridDim.x;
+  const int w = xcd_remap(blockIdx.x, G);
+  const int inst = w / rows_per_inst_tiles;
+  const int tile = w - inst * rows_per_inst_tiles;
+  const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
+  const int y = tile * C::TR + rl;
+  cpx<T>* lrow = lds + rl * C::LP;
+  cpx<T>* wrow = W + ((long long)inst * ny + y) * NX;
+  const T F = (T)ip[inst].F, kk = (T)ip[inst].k, dt = (T)ip[inst].dt;
+  const T hdt = (T)(0.5 * ip[inst].dt);
+  cpx<T> v[C::P];
+  load_unpack<T, NX, TR>(wrow, lrow, j, v);
+  line_fft<T, NX, +1, 1>(v, j, lrow, tw);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const T u0 = v[m].re, v0 = v[m].im;
+    T r1u, r1v, r2u, r2v;
+    gs_react(u0, v0, F, kk, r1u, r1v);
+    const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
+    gs_react(ut, vt, F, kk, r2u, r2v);
+    v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
+  }
+  line_fft<T, NX, -1, 1>(v, j, lrow, tw);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) lrow[j + C::TN * m] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int p = j + C::TN * m;
+    wrow[p] = pack_at<T, NX>(lrow, p);
+  }
+}
+
+// ----------------------------------------------------------- column pass
+template <int NY, int TC_>
+struct ColCfg {
+  static constexpr int P = plan_p(NY);
+  static constexpr int TN = NY / P;
+  static constexpr int TC = TC_;      // columns per workgroup
+  static constexpr int BT = TC * TN;  // threads per workgroup
+};
+
+template <typename T, int NY, int TC_, bool LAM2D>
+__global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(cpx<T>* __restrict__ W, int nx, const T* __restrict__ lamx,
+                                                   const T* __restrict__ lamy, const T* __restrict__ lam2d,
+                                                   const InstParams* __restrict__ ip, int nrep, T s,
+                                                   int col_tiles, const cpx<T>* __restrict__ tw) {
+  using C = ColCfg<NY, TC_>;
+  constexpr int TC = C::TC;
+  __shared__ cpx<T> lds[NY * TC];
+  const int G = gridDim.x;
+  const int w = xcd_remap(blockIdx.x, G);
+  const int inst = w / col_tiles;
+  const int tile = w - inst * col_tiles;
+  const int c = threadIdx.x % TC, j = threadIdx.x / TC;
+  const int p0 = tile * TC;
+  const int p = p0 + c;
+  const int half = nx >> 1;
+  cpx<T>* col = W + (long long)inst * NY * nx + p;
+  cpx<T> v[C::P];
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) v[m] = col[(long long)(j + C::TN * m) * nx];
+  line_fft<T, NY, -1, TC>(v, j, lds + c, tw);
+
+  const T alpha_a = (T)ip[inst].alpha_a, alpha_b = (T)ip[inst].alpha_b;
+  const T beta_a = (T)ip[inst].beta_a, beta_b = (T)ip[inst].beta_b;
+  const bool fb = p >= half;
+  const int kx = fb ? p - half : p;
+  const T alpha = fb ? alpha_b : alpha_a, beta = fb ? beta_b : beta_a;
+  const bool special_tile = (p0 == 0) || (p0 == half);
+  if (special_tile) {
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v[m];
+    __syncthreads();
+  }
+  if (special_tile && (p == 0 || p == half)) {
+    // Two real sequences with multipliers at kx = 0 (ma) and kx = nx/2 (mb).
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int ky = j + C::TN * m;
+      const int kym = (NY - ky) & (NY - 1);
+      const cpx<T> zm = conjg(lds[kym * TC + c]);
+      T la, lb;
+      if constexpr (LAM2D) {
+        la = lam2d[(long long)ky * nx + 0];
+        lb = lam2d[(long long)ky * nx + half];
+      } else {
+        la = lamx[0] + lamy[ky];
+        lb = lamx[half] + lamy[ky];
+      }
+      const T ma = spec_mult<T>(la, alpha, beta, nrep, s);
+      const T mb = spec_mult<T>(lb, alpha, beta, nrep, s);
+      const T cp = (T)0.5 * (ma + mb), cm = (T)0.5 * (ma - mb);
+      v[m] = mk(cp * v[m].re + cm * zm.re, cp * v[m].im + cm * zm.im);
+    }
+  } else {
+    const T lx = LAM2D ? (T)0 : lamx[kx];
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int ky = j + C::TN * m;
+      T lam;
+      if constexpr (LAM2D) {
+        lam = lam2d[(long long)ky * nx + kx];
+      } else {
+        lam = lx + lamy[ky];
+      }
+      v[m] = scale(v[m], spec_mult<T>(lam, alpha, beta, nrep, s));
+    }
+  }
+  if (special_tile) __syncthreads();
+  line_fft<T, NY, +1, TC>(v, j, lds + c, tw);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) col[(long long)(j + C::TN * m) * nx] = v[m];
+}
+
+// ------------------------------------------------------------- dispatch
+// Lines per workgroup: about 256 threads, at least one full line, and never
+// more lines than the grid has (rows: ny, column tile: nx/2 so that the two
+// packed special columns 0 and nx/2 fall at tile starts).
+template <int N>
+constexpr int lines_default() {
+  return (N / plan_p(N)) >= 256 ? 1 : 256 / (N / plan_p(N));
+}
+
+inline int pick_lines(int def, int limit) { return def < limit ? def : limit; }
+
+template <typename T, int N, int L>
+struct RowLaunch {
+  static int fwd_pack(int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
+                      hipStream_t st) {
+    const int tiles = ny / L;
+    hipLaunchKernelGGL((k_row_fwd_pack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, a, b, pitch,
+                       istride, W, ny, tiles, twiddles<T>());
+    return check_launch("row_fwd_pack");
+  }
+  static int inv_unpack(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
+                        int ghosts, hipStream_t st) {
+    const int tiles = ny / L;
+    hipLaunchKernelGGL((k_row_inv_unpack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, a, b,
+                       pitch, istride, ny, tiles, ghosts, twiddles<T>());
+    return check_launch("row_inv_unpack");
+  }
+  static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
+    const int tiles = ny / L;
+    hipLaunchKernelGGL((k_row_gs<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip, ny, tiles,
+                       twiddles<T>());
+    return check_launch("row_gs");
+  }
+};
+
+template <typename T, int N, int L>
+struct ColLaunch {
+  static int apply(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
+                   int nrep, T s, hipStream_t st) {
+    const int tiles = nx / L;
+    if (lam2d)
+      hipLaunchKernelGGL((k_col_apply<T, N, L, true>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W, nx,
+                         lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>());
+    else
+      hipLaunchKernelGGL((k_col_apply<T, N, L, false>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W, nx,
+                         lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>());
+    return check_launch("col_apply");
+  }
+};
+
+// Calls F<L>::run(args...) for the runtime line count l (a power of two no
+// larger than lines_default<N>()).
+template <int N, template <int> class F, typename... A>
+int with_lines(int l, A... args) {
+  constexpr int D = lines_default<N>();
+  if constexpr (D >= 64) if (l == 64) return F<64>::run(args...);
+  if constexpr (D >= 32) if (l == 32) return F<32>::run(args...);
+  if constexpr (D >= 16) if (l == 16) return F<16>::run(args...);
+  if constexpr (D >= 8) if (l == 8) return F<8>::run(args...);
+  if constexpr (D >= 4) if (l == 4) return F<4>::run(args...);
+  if constexpr (D >= 2) if (l == 2) return F<2>::run(args...);
+  if (l == 1) return F<1>::run(args...);
+  return fail(MJX_E_SIZE, "no kernel for tile of " + std::to_string(l) + " lines");
+}
+
+#define MJX_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+template <typename T>
+struct Launch {
+  template <int N>
+  struct FwdPack {
+    template <int L>
+    struct F {
+      static int run(int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
+                     hipStream_t st) {
+        return RowLaunch<T, N, L>::fwd_pack(B, ny, a, b, pitch, istride, W, st);
+      }
+    };
+  };
+  template <int N>
+  struct InvUnpack {
+    template <int L>
+    struct F {
+      static int run(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride, int ghosts,
+                     hipStream_t st) {
+        return RowLaunch<T, N, L>::inv_unpack(B, ny, W, a, b, pitch, istride, ghosts, st);
+      }
+    };
+  };
+  template <int N>
+  struct Gs {
+    template <int L>
+    struct F {
+      static int run(int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
+        return RowLaunch<T, N, L>::gs(B, ny, W, ip, st);
+      }
+    };
+  };
+  template <int N>
+  struct Col {
+    template <int L>
+    struct F {
+      static int run(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
+                     const InstParams* ip, int nrep, T s, hipStream_t st) {
+        return ColLaunch<T, N, L>::apply(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      }
+    };
+  };
+
+  static int row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,
+                          cpx<T>* W, hipStream_t st) {
+    switch (nx) {
+#define X(N) \
+  case N:    \
+    return with_lines<N, FwdPack<N>::template F>(pick_lines(lines_default<N>(), ny), B, ny, a, b, pitch, istride, W, st);
+      MJX_FOR_SIZES(X)
+#undef X
+      default:
+        return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
+    }
+  }
+  static int row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,
+                            long long istride, int ghosts, hipStream_t st) {
+    switch (nx) {
+#define X(N)                                                                                              \
+  case N:                                                                                                 \
+    return with_lines<N, InvUnpack<N>::template F>(pick_lines(lines_default<N>(), ny), B, ny, W, a, b, pitch, \
+                                                   istride, ghosts, st);
+      MJX_FOR_SIZES(X)
+#undef X
+      default:
+        return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
+    }
+  }
+  static int row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
+    switch (nx) {
+#define X(N) \
+  case N:    \
+    return with_lines<N, Gs<N>::template F>(pick_lines(lines_default<N>(), ny), B, ny, W, ip, st);
+      MJX_FOR_SIZES(X)
+#undef X
+      default:
+        return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
+    }
+  }
+  static int col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
+                       const InstParams* ip, int nrep, hipStream_t st) {
+    const T s = (T)(1.0 / ((double)nx * (double)ny));
+    switch (ny) {
+#define X(N)                                                                                                 \
+  case N:                                                                                                    \
+    return with_lines<N, Col<N>::template F>(pick_lines(lines_default<N>(), nx / 2), B, nx, W, lamx, lamy, lam2d, \
+                                             ip, nrep, s, st);
+      MJX_FOR_SIZES(X)
+#undef X
+      default:
+        return fail(MJX_E_SIZE, "unsupported ny " + std::to_string(ny));
+    }
+  }
+};
+
+}  // namespace mjx

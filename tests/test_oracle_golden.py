@@ -1,0 +1,176 @@
+"""Pins the CPU oracle against the reference's own golden numbers and against
+closed forms (no GPU).  A plausible mistake in the oracle (wrong symbol sign
+or axis, dropped Heun term, swapped half-steps) fails one of these."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from oracle import fd, gray_scott, spectral
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+# ------------------------------------------------------------- FFT-CN
+def test_fft_cn_scipy_comparison_golden():
+    """benchmarks/results/scipy_comparison.json:18-22 (NumPy FFT-CN, 128^2,
+    D=0.01, dt=1e-3, 1000 steps, max error vs exp(-8 pi^2 D t) u0)."""
+    g = _golden("fft_cn_golden.json")["scipy_comparison"]
+    u0 = spectral.sine_mode_ij(g["N"])
+    u = spectral.fft_cn_integrate(u0, g["D"], g["dt"], g["n_steps"])
+    exact = np.exp(-8 * np.pi ** 2 * g["D"] * g["dt"] * g["n_steps"]) * u0
+    err = np.max(np.abs(u - exact))
+    assert err == pytest.approx(g["numpy_error"], rel=1e-9)
+
+
+@pytest.mark.parametrize("N", [64, 128, 256])
+def test_fft_cn_bc_matrix_golden(N):
+    """benchmarks/results/bc_matrix.json:8,28,48 (periodic FFT-CN, D=0.1,
+    T=0.1, 100 steps; JAX-GPU rounding, hence rel 1e-7)."""
+    g = _golden("fft_cn_golden.json")["bc_matrix"]
+    u0 = spectral.sine_mode_ij(N)
+    u = spectral.fft_cn_integrate(u0, g["D"], g["T"] / g["n_steps"], g["n_steps"])
+    exact = np.exp(-8 * np.pi ** 2 * g["D"] * g["T"]) * u0
+    assert np.max(np.abs(u - exact)) == pytest.approx(g["errors"][str(N)], rel=1e-7)
+
+
+def test_fft_cn_config1_closed_form():
+    """128^2, D=0.1, dt=0.01, 100 steps: equals G^n u0 to 1e-12 (100 FFT
+    round trips of rounding) and the
+    analytic decay to the CN time error (4.1e-3 relative L2)."""
+    N, D, dt, n = 128, 0.1, 0.01, 100
+    u0 = spectral.sine_mode_ij(N)
+    u = spectral.fft_cn_integrate(u0, D, dt, n)
+    lam = -8 * np.pi ** 2  # continuous eigenvalue of the (1,1) mode
+    G = (1 + 0.5 * dt * D * lam) / (1 - 0.5 * dt * D * lam)
+    assert gray_scott.rel_l2(u, G ** n * u0) < 1e-12
+    r = gray_scott.rel_l2(u, np.exp(lam * D * dt * n) * u0)
+    assert 3e-3 < r < 5e-3
+
+
+# ------------------------------------------------------------- symbols
+def test_fd_symbol_is_stencil_eigenvalue():
+    """lam(kx, ky) are exactly the eigenvalues of the periodic 5-point
+    Laplacian (axis 0 = y): FFT(L_h u) = lam * FFT(u)."""
+    rng = np.random.default_rng(0)
+    ny, nx, dy, dx = 32, 48, 0.3, 0.7
+    u = rng.standard_normal((ny, nx))
+    lam = spectral.laplacian_symbol_2d(ny, nx, dy, dx)
+    lhs = np.fft.fft2(spectral.periodic_fd_laplacian(u, dy, dx))
+    assert np.max(np.abs(lhs - lam * np.fft.fft2(u))) < 1e-10 * np.max(np.abs(lhs))
+
+
+def test_helmholtz_fd_residual_and_identity():
+    rng = np.random.default_rng(1)
+    ny, nx, dy, dx, dt, D = 64, 32, 2.5 / 64, 2.5 / 32, 0.7, 0.013
+    lam = spectral.laplacian_symbol_2d(ny, nx, dy, dx)
+    rhs = rng.standard_normal((ny, nx))
+    u = spectral.solve_helmholtz_2d(rhs, lam, dt, D)
+    res = u - dt * D * spectral.periodic_fd_laplacian(u, dy, dx) - rhs
+    assert np.max(np.abs(res)) < 1e-12
+    assert np.max(np.abs(spectral.solve_helmholtz_2d(rhs, lam, dt, 0.0) - rhs)) < 1e-14
+
+
+# ------------------------------------------------------------- FD / BC
+def test_fd_operators_on_sine_modes():
+    """Discrete eigenvalues/derivatives of sin modes are exact."""
+    ny, nx = 24, 40
+    dy, dx = 1.0 / ny, 1.0 / nx
+    y = (np.arange(-1, ny + 1) + 0.5) * dy
+    x = (np.arange(-1, nx + 1) + 0.5) * dx
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    f = np.sin(2 * np.pi * X) * np.sin(4 * np.pi * Y)
+    lap = fd.laplacian_2d(f, dy, dx)
+    ev = (2 * np.cos(2 * np.pi * dx) - 2) / dx ** 2 + (2 * np.cos(4 * np.pi * dy) - 2) / dy ** 2
+    assert np.max(np.abs(lap[1:-1, 1:-1] - ev * f[1:-1, 1:-1])) < 1e-9 * abs(ev)
+    assert np.all(lap[0] == 0) and np.all(lap[:, 0] == 0)
+    d1 = fd.d1_central_2d(f, dy, dx, 1)
+    exact = np.sin(2 * np.pi * dx) / dx * np.cos(2 * np.pi * X) * np.sin(4 * np.pi * Y)
+    assert np.max(np.abs(d1[1:-1, 1:-1] - exact[1:-1, 1:-1])) < 1e-12
+
+
+def test_bc_fill_periodic_matches_wrap():
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((10, 14))
+    g = fd.apply_bc(f, 0)
+    inner = g[1:-1, 1:-1]
+    assert np.array_equal(g[0, 1:-1], inner[-1]) and np.array_equal(g[-1, 1:-1], inner[0])
+    assert np.array_equal(g[1:-1, 0], inner[:, -1]) and g[0, 0] == inner[-1, -1]
+    # Dirichlet: ghost + interior = 2 g at every boundary face
+    d = fd.apply_bc(f, 1, (1.0, 2.0, 3.0, 4.0))
+    assert np.allclose(d[1:-1, 0] + d[1:-1, 1], 2.0) and np.allclose(d[0, 1:-1] + d[1, 1:-1], 6.0)
+
+
+# ------------------------------------------------------------- Gray-Scott
+def _gs_grid(n):
+    return 2.5 / n
+
+
+def test_gs_fixed_point():
+    n = 32
+    lam = spectral.laplacian_symbol_2d(n, n, _gs_grid(n), _gs_grid(n))
+    y = {"u": np.ones((n + 2, n + 2)), "v": np.zeros((n + 2, n + 2))}
+    out, _ = gray_scott.integrate(y, 1.0, 5, 0.035, 0.065, 2e-5, 1e-5, lam)
+    assert np.max(np.abs(out["u"] - 1)) == 0 and np.max(np.abs(out["v"])) == 0
+
+
+def test_gs_pure_diffusion_closed_form():
+    """F = k = 0 and v = 0: each Strang step is (1 - dt/2 Du lam)^-2 in
+    Fourier space, so n steps equal one closed-form multiply."""
+    n, dt, Du, steps = 32, 0.8, 3e-3, 7
+    h = _gs_grid(n)
+    lam = spectral.laplacian_symbol_2d(n, n, h, h)
+    rng = np.random.default_rng(3)
+    u0 = rng.standard_normal((n, n))
+    U = np.zeros((n + 2, n + 2))
+    U[1:-1, 1:-1] = u0
+    y = {"u": gray_scott.apply_bc_periodic(U), "v": np.zeros((n + 2, n + 2))}
+    out, _ = gray_scott.integrate(y, dt, steps, 0.0, 0.0, Du, 1e-5, lam)
+    expect = np.fft.ifft2(np.fft.fft2(u0) * (1 - 0.5 * dt * Du * lam) ** (-2 * steps)).real
+    assert np.max(np.abs(out["u"][1:-1, 1:-1] - expect)) < 1e-13
+
+
+def test_gs_reaction_heun_second_order():
+    """Spatially uniform state: the Strang step reduces to Heun on the GS
+    reaction ODE; the error against a tight ODE solution drops 4x per halving
+    of dt (a dropped Heun term would give first order)."""
+    F, k, T = 0.035, 0.065, 4.0
+    u0, v0 = 0.6, 0.3
+
+    def rhs(t, y):
+        u, v = y
+        return [-u * v * v + F * (1 - u), u * v * v - (F + k) * v]
+
+    ref = solve_ivp(rhs, (0, T), [u0, v0], rtol=1e-13, atol=1e-14, method="DOP853").y[:, -1]
+    n = 16
+    lam = spectral.laplacian_symbol_2d(n, n, 0.1, 0.1)
+    errs = []
+    for steps in (8, 16, 32):
+        y = {"u": np.full((n + 2, n + 2), u0), "v": np.full((n + 2, n + 2), v0)}
+        out, _ = gray_scott.integrate(y, T / steps, steps, F, k, 1e-3, 1e-3, lam)
+        errs.append(abs(out["u"][5, 5] - ref[0]) + abs(out["v"][5, 5] - ref[1]))
+    rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(1.8 < r < 2.2 for r in rates), rates
+
+
+def test_gs_initial_condition_layout():
+    y = gray_scott.initial_condition(64, 64, seed=0)
+    u = y["u"]
+    assert u.shape == (66, 66)
+    inner = u[1:-1, 1:-1]
+    assert abs(inner[32, 32] - 0.5) <= 0.01 and abs(inner[0, 0] - 1.0) <= 0.01
+    assert np.array_equal(u[0, 1:-1], inner[-1])
+
+
+def test_fixed_dt_step_count_quirk():
+    """stepping.py:1402: int((t_end - t0)/dt) + 1 steps."""
+    assert gray_scott.fixed_dt_n_steps(0.0, 10.0, 1.0) == 11
+    assert gray_scott.fixed_dt_n_steps(0.0, 1.0, 0.3) == 4
