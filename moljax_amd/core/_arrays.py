@@ -35,16 +35,17 @@ def field_dtype(t: torch.Tensor) -> torch.dtype:
 
 
 def view2d(t: torch.Tensor) -> Tuple[torch.Tensor, int, int, int]:
-    """Return (tensor, pitch, instance stride, batch) for a (..., ny, nx)
-    tensor whose last axis is unit-stride; batch axes are flattened (a copy is
-    made only when they cannot be expressed with one stride)."""
+    """Return a (B, ny, nx) view, its pitch, instance stride and B for a
+    (..., ny, nx) tensor whose last axis is unit-stride; batch axes are
+    flattened (a copy is made only when they cannot be expressed with one
+    stride)."""
     if t.dim() < 2:
         raise ValueError("expected an array with at least 2 dimensions")
     if t.stride(-1) != 1:
         t = t.contiguous()
     ny, nx = t.shape[-2], t.shape[-1]
     if t.dim() == 2:
-        return t, t.stride(0), ny * t.stride(0), 1
+        return t.unsqueeze(0), t.stride(0), ny * t.stride(0), 1
     lead = t.shape[:-2]
     B = int(np.prod(lead))
     try:

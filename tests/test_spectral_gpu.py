@@ -107,18 +107,26 @@ def test_apply_diffusion_inverse_fft_padded(gpu, dtype):
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_fft_cn_step_config1(gpu, dtype):
     """FFT-CN 128^2, D=0.1, dt=0.01, 100 steps (pseudo-spectral symbol):
-    against the oracle's 100 round trips and the closed form."""
+    against the oracle's 100 round trips, fused (G^n in one round trip) and
+    step by step.  The (1,1) mode decays by 2.7e3 while the undamped k = 0
+    mode keeps the input's rounding, so float32 is measured as max error
+    relative to max|u0| (<= 1e-6) on the float32-rounded input."""
     from moljax_amd.core.fft_solvers import fft_cn_step
     N, D, dt, n = 128, 0.1, 0.01, 100
     u0 = osp.sine_mode_ij(N)
+    if dtype == torch.float32:
+        u0 = u0.astype(np.float32).astype(np.float64)
     ref = osp.fft_cn_integrate(u0, D, dt, n)
+
+    def err(x):
+        return np.max(np.abs(x - ref)) / np.max(np.abs(u0))
+    tol = 1e-14 if dtype == torch.float64 else 1e-6
     fused = _to_np(fft_cn_step(torch.from_numpy(u0).to("cuda", dtype), D, 1.0 / N, dt, n_steps=n))
-    tol = 1e-12 if dtype == torch.float64 else 1e-5
-    assert ogs.rel_l2(fused, ref) <= tol
+    assert err(fused) <= tol
     step = torch.from_numpy(u0).to("cuda", dtype)
     for _ in range(n):
         step = fft_cn_step(step, D, 1.0 / N, dt)
-    assert ogs.rel_l2(_to_np(step), ref) <= tol
+    assert err(_to_np(step)) <= tol
 
 
 def test_fft_cn_scipy_comparison_golden_gpu(gpu):
