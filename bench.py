@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-curve", action="store_true")
     ap.add_argument("--sweep", type=str, default="", help="comma-separated batch sizes to time as well")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         if int(os.environ.get("RANK", "0")) == 0:
@@ -147,6 +149,13 @@ def main():
     import moljax_amd
     moljax_amd.load_library(True)
     B, K, W = args.batch, args.steps, args.warmup
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     log(f"fp32 batched B={B} steps={K}")
     t32, kt, fin32 = time_form(torch.float32, B, K, W, kernel_timing=True)

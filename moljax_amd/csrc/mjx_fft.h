@@ -185,21 +185,62 @@ struct SFor<E, E> {
 // ----------------------------------------------------------------- twiddles
 // tw: device table of MJX_TWN entries exp(-2*pi*i*t/MJX_TWN) in precision T,
 // owned by the library (one per device), passed to every kernel.
-// Returns exp(SIGN * 2*pi*i * t / MJX_TWN).
-template <int SIGN, typename T>
-__device__ __forceinline__ cpx<T> twiddle(const cpx<T>* __restrict__ tw, int t) {
-  cpx<T> w = tw[t];
-  return SIGN < 0 ? w : conjg(w);
+
+// ---------------------------------------------------------- twiddle cache
+// The twiddles a thread needs depend only on its lane index j, so a forward
+// and an inverse FFT in the same thread share one set of table loads:
+// stage s >= 1, butterfly q, input r >= 1 uses w^(r * k), k = b mod Ns,
+// b = j + TN*q (forward convention, conjugated for the inverse).
+__host__ __device__ constexpr int tw_count_stage(int n, int s) {
+  return (plan_p(n) / plan_radix(n, s)) * (plan_radix(n, s) - 1);
 }
+__host__ __device__ constexpr int tw_offset(int n, int s) {
+  return s <= 1 ? 0 : tw_offset(n, s - 1) + tw_count_stage(n, s - 1);
+}
+__host__ __device__ constexpr int tw_count(int n) { return tw_offset(n, plan_nstages(n)); }
+
+template <typename T, int N>
+struct TwCache {
+  cpx<T> w[tw_count(N)];
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void load_twiddles(TwCache<T, N>& c, int j, const cpx<T>* __restrict__ tw) {
+  constexpr int P = plan_p(N);
+  constexpr int TN = N / P;
+  constexpr int NS = plan_nstages(N);
+  SFor<1, NS>::run([&](auto S_) {
+    constexpr int s = decltype(S_)::value;
+    constexpr int R = plan_radix(N, s);
+    constexpr int Ns = plan_ns_before(N, s);
+    constexpr int NB = P / R;
+    constexpr int TSTEP = MJX_TWN / (Ns * R);
+    constexpr int off = tw_offset(N, s);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int k = (j + TN * q) & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) c.w[off + q * (R - 1) + (r - 1)] = tw[r * k * TSTEP];
+    }
+  });
+}
+
+// LDS address of natural index idx: idx * ES, or, with PAD, one spare slot
+// after every 16 elements (idx + idx/16) so that the stride-R writes of a
+// radix-R stage spread over the LDS banks.
+template <int ES, int PAD>
+__device__ __forceinline__ int lds_addr(int idx) {
+  return (PAD ? idx + (idx >> 4) : idx) * ES;
+}
+__host__ __device__ constexpr int lds_line_len(int n, int pad) { return pad ? n + n / 16 : n; }
 
 // ------------------------------------------------------------- line FFT
 // v     : P registers, element m <-> natural index j + TN*m (in and out)
-// lds   : this line's LDS base; natural index idx lives at lds[idx * ES]
-// tw    : twiddle table (see twiddle())
+// lds   : this line's LDS base; natural index idx lives at lds[lds_addr(idx)]
+// twc   : this thread's twiddles (load_twiddles)
 // The caller guarantees every thread of the block calls this (barriers).
-template <typename T, int N, int SIGN, int ES>
-__device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* lds,
-                                         const cpx<T>* __restrict__ tw) {
+template <typename T, int N, int SIGN, int ES, int PAD>
+__device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* lds, const TwCache<T, N>& twc) {
   constexpr int P = plan_p(N);
   constexpr int TN = N / P;
   constexpr int NS = plan_nstages(N);
@@ -214,21 +255,22 @@ __device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* 
       for (int q = 0; q < NB; ++q) {
         const int b = j + TN * q;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q + r * NB] = lds[(b + r * (N / R)) * ES];
+        for (int r = 0; r < R; ++r) v[q + r * NB] = lds[lds_addr<ES, PAD>(b + r * (N / R))];
       }
       __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-      const int b = j + TN * q;
       cpx<T> t[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) t[r] = v[q + r * NB];
       if constexpr (s > 0) {
-        const int k = b & (Ns - 1);
-        constexpr int TSTEP = MJX_TWN / (Ns * R);
+        constexpr int off = tw_offset(N, s);
 #pragma unroll
-        for (int r = 1; r < R; ++r) t[r] = cmul(t[r], twiddle<SIGN, T>(tw, r * k * TSTEP));
+        for (int r = 1; r < R; ++r) {
+          const cpx<T> w = twc.w[off + q * (R - 1) + (r - 1)];
+          t[r] = cmul(t[r], SIGN < 0 ? w : conjg(w));
+        }
       }
       dft<R, SIGN>(t);
 #pragma unroll
@@ -241,7 +283,7 @@ __device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* 
         const int k = b & (Ns - 1);
         const int idxD = (b / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[(idxD + r * Ns) * ES] = v[q + r * NB];
+        for (int r = 0; r < R; ++r) lds[lds_addr<ES, PAD>(idxD + r * Ns)] = v[q + r * NB];
       }
       __syncthreads();
     }

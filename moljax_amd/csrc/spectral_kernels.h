@@ -35,54 +35,84 @@ inline const cpx<T>* twiddles() {
 }
 
 // ------------------------------------------------------------ pack/unpack
-// Natural-order complex row spectrum Z of a + i*b in lds[0..nx); returns the
-// packed value for column p.
+// Row LDS layout: one padded line per row (lds_addr<1, 1>).
+template <typename T>
+__device__ __forceinline__ cpx<T> lrd(const cpx<T>* l, int idx) { return l[lds_addr<1, 1>(idx)]; }
+template <typename T>
+__device__ __forceinline__ void lwr(cpx<T>* l, int idx, cpx<T> v) { l[lds_addr<1, 1>(idx)] = v; }
+
+// Natural-order complex row spectrum Z of a + i*b in LDS; returns the packed
+// value for column p.
 template <typename T, int NX>
-__device__ __forceinline__ cpx<T> pack_at(const cpx<T>* lds, int p) {
+__device__ __forceinline__ cpx<T> pack_at(const cpx<T>* l, int p) {
   constexpr int H = NX / 2;
-  if (p == 0) return mk(lds[0].re, lds[H].re);
-  if (p == H) return mk(lds[0].im, lds[H].im);
+  if (p == 0) return mk(lrd(l, 0).re, lrd(l, H).re);
+  if (p == H) return mk(lrd(l, 0).im, lrd(l, H).im);
   if (p < H) {
-    const cpx<T> z = lds[p], zm = lds[NX - p];
+    const cpx<T> z = lrd(l, p), zm = lrd(l, NX - p);
     return mk((z.re + zm.re) * (T)0.5, (z.im - zm.im) * (T)0.5);
   }
   const int e = p - H;
-  const cpx<T> z = lds[e], zm = lds[NX - e];
+  const cpx<T> z = lrd(l, e), zm = lrd(l, NX - e);
   // (z - conj(zm)) / (2i)
   const T dre = z.re - zm.re, dim = z.im + zm.im;
   return mk(dim * (T)0.5, -dre * (T)0.5);
 }
 
-// Packed row in lds[0..nx); returns Z(e) = A(e) + i*B(e).
+// Packed row in LDS; returns Z(e) = A(e) + i*B(e).
 template <typename T, int NX>
-__device__ __forceinline__ cpx<T> unpack_at(const cpx<T>* lds, int e) {
+__device__ __forceinline__ cpx<T> unpack_at(const cpx<T>* l, int e) {
   constexpr int H = NX / 2;
-  if (e == 0) return mk(lds[0].re, lds[H].re);
-  if (e == H) return mk(lds[0].im, lds[H].im);
+  if (e == 0) return mk(lrd(l, 0).re, lrd(l, H).re);
+  if (e == H) return mk(lrd(l, 0).im, lrd(l, H).im);
   cpx<T> A, B;
   if (e < H) {
-    A = lds[e];
-    B = lds[H + e];
+    A = lrd(l, e);
+    B = lrd(l, H + e);
   } else {
-    A = conjg(lds[NX - e]);
-    B = conjg(lds[H + NX - e]);
+    A = conjg(lrd(l, NX - e));
+    B = conjg(lrd(l, H + NX - e));
   }
   return mk(A.re - B.im, A.im + B.re);
 }
 
+// 1/d: FP64 exact division; FP32 hardware reciprocal refined by one Newton
+// step (error below 1 ulp).
 template <typename T>
+__device__ __forceinline__ T recip(T d) {
+  if constexpr (std::is_same<T, double>::value) {
+    return 1.0 / d;
+  } else {
+    float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, r, 1.0f), r, r);
+  }
+}
+
+// s * ((1 + alpha*lam) / (1 - beta*lam))^nrep with |1 - beta*lam| clamped
+// below at 1e-14 (moljax/core/fft_solvers.py:280-283).  NREP > 0 fixes the
+// power at compile time; NREP == 0 takes the runtime nrep.
+template <typename T, int NREP>
 __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
   T den = (T)1 - beta * lam;
   if (fabs(den) < (T)1e-14) den = (T)1e-14;
-  const T g = ((T)1 + alpha * lam) / den;
-  T m = (T)1, b = g;
-  int n = nrep;
-  while (n > 0) {
-    if (n & 1) m *= b;
-    b *= b;
-    n >>= 1;
+  const T num = (T)1 + alpha * lam;
+  T g;
+  if constexpr (std::is_same<T, double>::value) g = num / den;
+  else g = num * recip(den);
+  if constexpr (NREP == 1) {
+    return g * s;
+  } else if constexpr (NREP == 2) {
+    return (g * g) * s;
+  } else {
+    T m = (T)1, b = g;
+    int n = nrep;
+    while (n > 0) {
+      if (n & 1) m *= b;
+      b *= b;
+      n >>= 1;
+    }
+    return m * s;
   }
-  return m * s;
 }
 
 // Gray-Scott reaction terms, moljax/core/operators.py:655-667.
@@ -98,21 +128,19 @@ template <int NX, int TR_>
 struct RowCfg {
   static constexpr int P = plan_p(NX);
   static constexpr int TN = NX / P;
-  static constexpr int TR = TR_;      // rows per workgroup
-  static constexpr int BT = TR * TN;  // threads per workgroup
-  static constexpr int LP = NX;       // lds row pitch (complex elements)
+  static constexpr int TR = TR_;                       // rows per workgroup
+  static constexpr int BT = TR * TN;                   // threads per workgroup
+  static constexpr int LP = lds_line_len(NX, 1);       // padded LDS row length
 };
 
 // Real fields (a, b) -> packed x-spectra in W.
 template <typename T, int NX, int TR>
-__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(const T* __restrict__ a, const T* __restrict__ b,
-                                                      long long pitch, long long istride,
-                                                      cpx<T>* __restrict__ W, int ny, int rows_per_inst_tiles,
-                                                      const cpx<T>* __restrict__ tw) {
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(
+    const T* __restrict__ a, const T* __restrict__ b, long long pitch, long long istride, cpx<T>* __restrict__ W,
+    int ny, int rows_per_inst_tiles, const cpx<T>* __restrict__ tw) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
-  const int G = gridDim.x;
-  const int w = xcd_remap(blockIdx.x, G);
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / rows_per_inst_tiles;
   const int tile = w - inst * rows_per_inst_tiles;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
@@ -126,9 +154,11 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(const T* 
     const int x = j + C::TN * m;
     v[m] = mk(ra[x], rb ? rb[x] : (T)0);
   }
-  line_fft<T, NX, -1, 1>(v, j, lrow, tw);
+  TwCache<T, NX> twc;
+  load_twiddles(twc, j, tw);
+  line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) lrow[j + C::TN * m] = v[m];
+  for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
   __syncthreads();
   cpx<T>* wrow = W + ((long long)inst * ny + y) * NX;
 #pragma unroll
@@ -143,8 +173,11 @@ template <typename T, int NX, int TR>
 __device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, cpx<T>* lrow, int j,
                                             cpx<T> (&v)[RowCfg<NX, TR>::P]) {
   using C = RowCfg<NX, TR>;
+  cpx<T> g[C::P];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) lrow[j + C::TN * m] = wrow[j + C::TN * m];
+  for (int m = 0; m < C::P; ++m) g[m] = wrow[j + C::TN * m];
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, g[m]);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < C::P; ++m) v[m] = unpack_at<T, NX>(lrow, j + C::TN * m);
@@ -153,22 +186,22 @@ __device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, cpx
 
 // Packed x-spectra in W -> real fields (a, b), optional periodic ghost ring.
 template <typename T, int NX, int TR>
-__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(const cpx<T>* __restrict__ W, T* __restrict__ a,
-                                                        T* __restrict__ b, long long pitch, long long istride,
-                                                        int ny, int rows_per_inst_tiles, int ghosts,
-                                                        const cpx<T>* __restrict__ tw) {
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
+    const cpx<T>* __restrict__ W, T* __restrict__ a, T* __restrict__ b, long long pitch, long long istride, int ny,
+    int rows_per_inst_tiles, int ghosts, const cpx<T>* __restrict__ tw) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
-  const int G = gridDim.x;
-  const int w = xcd_remap(blockIdx.x, G);
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / rows_per_inst_tiles;
   const int tile = w - inst * rows_per_inst_tiles;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
   const int y = tile * C::TR + rl;
   cpx<T>* lrow = lds + rl * C::LP;
+  TwCache<T, NX> twc;
+  load_twiddles(twc, j, tw);
   cpx<T> v[C::P];
   load_unpack<T, NX, TR>(W + ((long long)inst * ny + y) * NX, lrow, j, v);
-  line_fft<T, NX, +1, 1>(v, j, lrow, tw);
+  line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
   T* ra = a + inst * istride + (long long)y * pitch;
   T* rb = b ? b + inst * istride + (long long)y * pitch : nullptr;
 #pragma unroll
@@ -213,12 +246,13 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(const c
 // Heun exactly as moljax/core/stepping.py:414-425:
 //   R1 = R(y*), y~ = y* + dt R1, R2 = R(y~), y** = y* + (dt/2)(R1 + R2).
 template <typename T, int NX, int TR>
-__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W, const InstParams* __restrict__ ip, int ny,
-                                                int rows_per_inst_tiles, const cpx<T>* __restrict__ tw) {
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W,
+                                                                  const InstParams* __restrict__ ip, int ny,
+                                                                  int rows_per_inst_tiles,
+                                                                  const cpx<T>* __restrict__ tw) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
-  const int G = gridDim.x;
-  const int w = xcd_remap(blockIdx.x, G);
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / rows_per_inst_tiles;
   const int tile = w - inst * rows_per_inst_tiles;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
@@ -227,9 +261,11 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
   cpx<T>* wrow = W + ((long long)inst * ny + y) * NX;
   const T F = (T)ip[inst].F, kk = (T)ip[inst].k, dt = (T)ip[inst].dt;
   const T hdt = (T)(0.5 * ip[inst].dt);
+  TwCache<T, NX> twc;
+  load_twiddles(twc, j, tw);
   cpx<T> v[C::P];
   load_unpack<T, NX, TR>(wrow, lrow, j, v);
-  line_fft<T, NX, +1, 1>(v, j, lrow, tw);
+  line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
     const T u0 = v[m].re, v0 = v[m].im;
@@ -239,9 +275,9 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
     gs_react(ut, vt, F, kk, r2u, r2v);
     v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
   }
-  line_fft<T, NX, -1, 1>(v, j, lrow, tw);
+  line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) lrow[j + C::TN * m] = v[m];
+  for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
@@ -259,16 +295,16 @@ struct ColCfg {
   static constexpr int BT = TC * TN;  // threads per workgroup
 };
 
-template <typename T, int NY, int TC_, bool LAM2D>
-__global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(cpx<T>* __restrict__ W, int nx, const T* __restrict__ lamx,
-                                                   const T* __restrict__ lamy, const T* __restrict__ lam2d,
-                                                   const InstParams* __restrict__ ip, int nrep, T s,
-                                                   int col_tiles, const cpx<T>* __restrict__ tw) {
+template <typename T, int NY, int TC_, bool LAM2D, int NREP>
+__global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
+    cpx<T>* __restrict__ W, int nx, const T* __restrict__ lamx, const T* __restrict__ lamy,
+    const T* __restrict__ lam2d, const InstParams* __restrict__ ip, int nrep, T s, int col_tiles,
+    const cpx<T>* __restrict__ tw) {
   using C = ColCfg<NY, TC_>;
   constexpr int TC = C::TC;
   __shared__ cpx<T> lds[NY * TC];
-  const int G = gridDim.x;
-  const int w = xcd_remap(blockIdx.x, G);
+  __shared__ T lds_lamy[LAM2D ? 1 : NY];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / col_tiles;
   const int tile = w - inst * col_tiles;
   const int c = threadIdx.x % TC, j = threadIdx.x / TC;
@@ -279,66 +315,75 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(cpx<T>* __r
   cpx<T> v[C::P];
 #pragma unroll
   for (int m = 0; m < C::P; ++m) v[m] = col[(long long)(j + C::TN * m) * nx];
-  line_fft<T, NY, -1, TC>(v, j, lds + c, tw);
+  if constexpr (!LAM2D) {
+    for (int i = threadIdx.x; i < NY; i += C::BT) lds_lamy[i] = lamy[i];
+  }
+  TwCache<T, NY> twc;
+  load_twiddles(twc, j, tw);
+  line_fft<T, NY, -1, TC, 0>(v, j, lds + c, twc);  // its barriers also publish lds_lamy
 
-  const T alpha_a = (T)ip[inst].alpha_a, alpha_b = (T)ip[inst].alpha_b;
-  const T beta_a = (T)ip[inst].beta_a, beta_b = (T)ip[inst].beta_b;
   const bool fb = p >= half;
   const int kx = fb ? p - half : p;
-  const T alpha = fb ? alpha_b : alpha_a, beta = fb ? beta_b : beta_a;
-  const bool special_tile = (p0 == 0) || (p0 == half);
-  if (special_tile) {
-#pragma unroll
-    for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v[m];
-    __syncthreads();
-  }
-  if (special_tile && (p == 0 || p == half)) {
-    // Two real sequences with multipliers at kx = 0 (ma) and kx = nx/2 (mb).
-#pragma unroll
-    for (int m = 0; m < C::P; ++m) {
-      const int ky = j + C::TN * m;
-      const int kym = (NY - ky) & (NY - 1);
-      const cpx<T> zm = conjg(lds[kym * TC + c]);
-      T la, lb;
-      if constexpr (LAM2D) {
-        la = lam2d[(long long)ky * nx + 0];
-        lb = lam2d[(long long)ky * nx + half];
-      } else {
-        la = lamx[0] + lamy[ky];
-        lb = lamx[half] + lamy[ky];
-      }
-      const T ma = spec_mult<T>(la, alpha, beta, nrep, s);
-      const T mb = spec_mult<T>(lb, alpha, beta, nrep, s);
-      const T cp = (T)0.5 * (ma + mb), cm = (T)0.5 * (ma - mb);
-      v[m] = mk(cp * v[m].re + cm * zm.re, cp * v[m].im + cm * zm.im);
-    }
-  } else {
+  const T alpha = (T)(fb ? ip[inst].alpha_b : ip[inst].alpha_a);
+  const T beta = (T)(fb ? ip[inst].beta_b : ip[inst].beta_a);
+  auto multiply_plain = [&]() {
     const T lx = LAM2D ? (T)0 : lamx[kx];
 #pragma unroll
     for (int m = 0; m < C::P; ++m) {
       const int ky = j + C::TN * m;
       T lam;
-      if constexpr (LAM2D) {
-        lam = lam2d[(long long)ky * nx + kx];
-      } else {
-        lam = lx + lamy[ky];
-      }
-      v[m] = scale(v[m], spec_mult<T>(lam, alpha, beta, nrep, s));
+      if constexpr (LAM2D) lam = lam2d[(long long)ky * nx + kx];
+      else lam = lx + lds_lamy[ky];
+      v[m] = scale(v[m], spec_mult<T, NREP>(lam, alpha, beta, nrep, s));
     }
+  };
+  if (p0 == 0 || p0 == half) {
+    // This tile holds packed column 0 or nx/2: two real sequences with the
+    // multipliers of kx = 0 (ma) and kx = nx/2 (mb),
+    // out(ky) = (ma+mb)/2 Z(ky) + (ma-mb)/2 conj(Z(-ky)).
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v[m];
+    __syncthreads();
+    if (c == 0) {
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) {
+        const int ky = j + C::TN * m;
+        const int kym = (NY - ky) & (NY - 1);
+        const cpx<T> zm = conjg(lds[kym * TC + c]);
+        T la, lb;
+        if constexpr (LAM2D) {
+          la = lam2d[(long long)ky * nx + 0];
+          lb = lam2d[(long long)ky * nx + half];
+        } else {
+          la = lamx[0] + lds_lamy[ky];
+          lb = lamx[half] + lds_lamy[ky];
+        }
+        const T ma = spec_mult<T, NREP>(la, alpha, beta, nrep, s);
+        const T mb = spec_mult<T, NREP>(lb, alpha, beta, nrep, s);
+        const T cp = (T)0.5 * (ma + mb), cm = (T)0.5 * (ma - mb);
+        v[m] = mk(cp * v[m].re + cm * zm.re, cp * v[m].im + cm * zm.im);
+      }
+    } else {
+      multiply_plain();
+    }
+    __syncthreads();
+  } else {
+    multiply_plain();
   }
-  if (special_tile) __syncthreads();
-  line_fft<T, NY, +1, TC>(v, j, lds + c, tw);
+  line_fft<T, NY, +1, TC, 0>(v, j, lds + c, twc);
 #pragma unroll
   for (int m = 0; m < C::P; ++m) col[(long long)(j + C::TN * m) * nx] = v[m];
 }
 
 // ------------------------------------------------------------- dispatch
-// Lines per workgroup: about 256 threads, at least one full line, and never
-// more lines than the grid has (rows: ny, column tile: nx/2 so that the two
-// packed special columns 0 and nx/2 fall at tile starts).
-template <int N>
+// Lines per workgroup: about 256 threads in FP32 and 128 in FP64 (LDS), at
+// least one full line, and never more lines than the grid has (rows: ny,
+// column tile: nx/2 so that the packed special columns 0 and nx/2 fall at
+// tile starts).
+template <typename T, int N>
 constexpr int lines_default() {
-  return (N / plan_p(N)) >= 256 ? 1 : 256 / (N / plan_p(N));
+  constexpr int bt = sizeof(T) == 8 ? 128 : 256;
+  return (N / plan_p(N)) >= bt ? 1 : bt / (N / plan_p(N));
 }
 
 inline int pick_lines(int def, int limit) { return def < limit ? def : limit; }
@@ -369,24 +414,33 @@ struct RowLaunch {
 
 template <typename T, int N, int L>
 struct ColLaunch {
+  template <bool LAM2D, int NREP>
+  static void go(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
+                 int nrep, T s, hipStream_t st) {
+    const int tiles = nx / L;
+    hipLaunchKernelGGL((k_col_apply<T, N, L, LAM2D, NREP>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W,
+                       nx, lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>());
+  }
   static int apply(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
                    int nrep, T s, hipStream_t st) {
-    const int tiles = nx / L;
-    if (lam2d)
-      hipLaunchKernelGGL((k_col_apply<T, N, L, true>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W, nx,
-                         lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>());
-    else
-      hipLaunchKernelGGL((k_col_apply<T, N, L, false>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W, nx,
-                         lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>());
+    if (lam2d) {
+      if (nrep == 1) go<true, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      else if (nrep == 2) go<true, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      else go<true, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+    } else {
+      if (nrep == 1) go<false, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      else if (nrep == 2) go<false, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      else go<false, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+    }
     return check_launch("col_apply");
   }
 };
 
 // Calls F<L>::run(args...) for the runtime line count l (a power of two no
-// larger than lines_default<N>()).
-template <int N, template <int> class F, typename... A>
+// larger than lines_default<T, N>()).
+template <typename T, int N, template <int> class F, typename... A>
 int with_lines(int l, A... args) {
-  constexpr int D = lines_default<N>();
+  constexpr int D = lines_default<T, N>();
   if constexpr (D >= 64) if (l == 64) return F<64>::run(args...);
   if constexpr (D >= 32) if (l == 32) return F<32>::run(args...);
   if constexpr (D >= 16) if (l == 16) return F<16>::run(args...);
@@ -446,7 +500,7 @@ struct Launch {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<N, FwdPack<N>::template F>(pick_lines(lines_default<N>(), ny), B, ny, a, b, pitch, istride, W, st);
+    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, a, b, pitch, istride, W, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -458,7 +512,7 @@ struct Launch {
     switch (nx) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
-    return with_lines<N, InvUnpack<N>::template F>(pick_lines(lines_default<N>(), ny), B, ny, W, a, b, pitch, \
+    return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, a, b, pitch, \
                                                    istride, ghosts, st);
       MJX_FOR_SIZES(X)
 #undef X
@@ -470,7 +524,7 @@ struct Launch {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<N, Gs<N>::template F>(pick_lines(lines_default<N>(), ny), B, ny, W, ip, st);
+    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, ip, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -483,7 +537,7 @@ struct Launch {
     switch (ny) {
 #define X(N)                                                                                                 \
   case N:                                                                                                    \
-    return with_lines<N, Col<N>::template F>(pick_lines(lines_default<N>(), nx / 2), B, nx, W, lamx, lamy, lam2d, \
+    return with_lines<T, N, Col<N>::template F>(pick_lines(lines_default<T, N>(), nx / 2), B, nx, W, lamx, lamy, lam2d, \
                                              ip, nrep, s, st);
       MJX_FOR_SIZES(X)
 #undef X
