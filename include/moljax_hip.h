@@ -90,6 +90,36 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                        long long istride, int fill_ghosts, const void* lamx, const void* lamy,
                        const void* lam2d, const double* inst, int n_steps, void* work, void* stream);
 
+/* Non-periodic 2-D Helmholtz / Poisson solve on B instances of ny x nx
+ * interior points:  kind 0 homogeneous Dirichlet (DST-I in x and y, vertex
+ * grid; ny+1, nx+1 powers of two in [16, 1024]), kind 1 homogeneous Neumann
+ * (orthonormal DCT-II; ny, nx powers of two in [16, 1024]).
+ *   u = T^-1[ m(kx, ky) T[rhs] ],  mmode 0: m = (1 + alpha lam)/(1 - beta lam),
+ *   mmode 1 (Poisson): m = 1/lam (0 where |lam| < 1e-14),  lam = lamx + lamy
+ * with lamx (nx) / lamy (ny) the 1-D symbols indexed like the transform
+ * coefficients (Dirichlet k = 1..N at [k-1], Neumann k = 0..N-1 at [k]);
+ * beta = inst[8b + 0], alpha = inst[8b + 2] per instance b.  work: B*ny*nx
+ * reals.  Extends moljax/core/fft_nonperiodic.py:247-355
+ * (solve_{poisson,helmholtz}_{dirichlet,neumann}, 1-D there) to 2-D. */
+int mjx_r2r_solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
+                    long long in_istride, void* out, long long out_pitch, long long out_istride,
+                    const void* lamx, const void* lamy, const double* inst, int mmode, void* work,
+                    void* stream);
+
+/* The same solve along nlines independent 1-D lines of n points (line
+ * strides in elements), one parameter row inst[0..7] for all lines.
+ * Replaces moljax/core/fft_nonperiodic.py:247-355 for 1-D input. */
+int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long long in_stride, void* out,
+                    long long out_stride, const void* lam, const double* inst, int mmode, void* stream);
+
+/* Raw 1-D transforms of nlines lines, times `scale`:  kind 0: DST-I
+ * X_k = sum_n x_n sin(pi (k+1)(n+1)/(n_len+1)) (its own inverse up to
+ * 2/(n_len+1));  kind 1: unnormalised DCT-II (inverse = 0) or DCT-III
+ * (inverse != 0).  Replaces fft_nonperiodic.py:97-133 (dst_I_fast,
+ * idst_I_fast). */
+int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, const void* in, long long in_stride,
+                        void* out, long long out_stride, double scale, void* stream);
+
 /* Opt-in timing of the fused IMEX chain's hot kernels (kind 0: the fused
  * Gray-Scott row pass, kind 1: the column pass).  While enabled every such
  * launch is bracketed by HIP events on its stream.  Enabling resets the

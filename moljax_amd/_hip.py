@@ -36,6 +36,9 @@ PROTOTYPES = {
     "mjx_fd_op2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _i, _d, _d, _p, _ll, _ll, _d, _p]),
     "mjx_bc_fill2d": (_i, [_i, _i, _i, _i, _p, _ll, _ll, _i, _dp, _d, _d, _p]),
     "mjx_gs_reaction": (_i, [_i, _ll, _p, _p, _p, _p, _d, _d, _p]),
+    "mjx_r2r_solve2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _p, _p, _p, _i, _p, _p]),
+    "mjx_r2r_solve1d": (_i, [_i, _i, _i, _i, _p, _ll, _p, _ll, _p, _p, _i, _p]),
+    "mjx_r2r_transform1d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _p, _ll, _d, _p]),
     "mjx_kernel_timing_enable": (_i, [_i]),
     "mjx_kernel_timing": (_i, [_i, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 }

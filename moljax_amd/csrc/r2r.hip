@@ -1,0 +1,124 @@
+// C-ABI entry points of the non-periodic (DST-I / DCT-II) solves.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "mjx_common.h"
+#include "r2r_kernels.h"
+
+namespace mjx {
+
+int r2r_run(int kind, int orient, int mode, int L, R2RArgs a, int B, hipStream_t st, float);
+int r2r_run(int kind, int orient, int mode, int L, R2RArgs a, int B, hipStream_t st, double);
+
+static int run(int dtype, int kind, int orient, int mode, int L, const R2RArgs& a, int B, hipStream_t st) {
+  if (dtype == MJX_F32) return r2r_run(kind, orient, mode, L, a, B, st, 0.0f);
+  return r2r_run(kind, orient, mode, L, a, B, st, 0.0);
+}
+
+static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Line length for a transform of n points: DST-I needs n+1, DCT n, a power of
+// two in [16, 1024].
+static int line_len(int kind, int n) {
+  const int m = kind == R2R_DST ? n + 1 : n;
+  if (!pow2(m) || m < 16 || m > 1024) return -1;
+  return 2 * m;
+}
+
+static int check_common(int dtype, int kind) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (kind != R2R_DST && kind != R2R_DCT) return fail(MJX_E_ARG, "kind must be 0 (Dirichlet/DST-I) or 1 (Neumann/DCT-II)");
+  return ensure_tables(nullptr);
+}
+
+static R2RArgs make(const void* in, long long in_es, long long in_ls, long long in_is, void* out, long long out_es,
+                    long long out_ls, long long out_is, int nlines) {
+  R2RArgs a{};
+  a.in = in;
+  a.in_es = in_es;
+  a.in_ls = in_ls;
+  a.in_is = in_is;
+  a.out = out;
+  a.out_es = out_es;
+  a.out_ls = out_ls;
+  a.out_is = out_is;
+  a.nlines = nlines;
+  a.scale = 1.0;
+  return a;
+}
+
+}  // namespace mjx
+
+using namespace mjx;
+
+extern "C" {
+
+int mjx_r2r_solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
+                    long long in_istride, void* out, long long out_pitch, long long out_istride, const void* lamx,
+                    const void* lamy, const double* inst, int mmode, void* work, void* stream) {
+  int rc = check_common(dtype, kind);
+  if (rc) return rc;
+  const int Lx = line_len(kind, nx), Ly = line_len(kind, ny);
+  if (Lx < 0 || Ly < 0)
+    return fail(MJX_E_SIZE, std::string("grid ") + std::to_string(ny) + "x" + std::to_string(nx) +
+                                (kind == R2R_DST ? " not supported: ny+1 and nx+1 must be powers of two in [16, 1024]"
+                                                 : " not supported: ny and nx must be powers of two in [16, 1024]"));
+  if (B < 1 || !rhs || !out || !lamx || !lamy || !inst || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON) return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz) or 1 (Poisson)");
+  hipStream_t st = (hipStream_t)stream;
+  const long long plane = (long long)ny * nx;
+  // rows: forward transform along x into the work array
+  R2RArgs a = make(rhs, 1, in_pitch, in_istride, work, 1, nx, plane, ny);
+  rc = run(dtype, kind, R2R_ROW, R2R_FWD, Lx, a, B, st);
+  if (rc) return rc;
+  // columns: transform along y, multiply, inverse along y (in place)
+  R2RArgs c = make(work, nx, 1, plane, work, nx, 1, plane, nx);
+  c.lam_line = lamy;
+  c.lam_cross = lamx;
+  c.n_cross = kind == R2R_DCT ? nx : 0;
+  c.inst = inst;
+  c.mmode = mmode;
+  rc = run(dtype, kind, R2R_COL, R2R_APPLY, Ly, c, B, st);
+  if (rc) return rc;
+  // rows: inverse transform along x into the output
+  R2RArgs r = make(work, 1, nx, plane, out, 1, out_pitch, out_istride, ny);
+  r.scale = kind == R2R_DST ? 2.0 / (nx + 1) : 1.0;
+  return run(dtype, kind, R2R_ROW, R2R_INV, Lx, r, B, st);
+}
+
+int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long long in_stride, void* out,
+                    long long out_stride, const void* lam, const double* inst, int mmode, void* stream) {
+  int rc = check_common(dtype, kind);
+  if (rc) return rc;
+  const int L = line_len(kind, n);
+  if (L < 0)
+    return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) +
+                                (kind == R2R_DST ? " not supported: n+1 must be a power of two in [16, 1024]"
+                                                 : " not supported: n must be a power of two in [16, 1024]"));
+  if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON) return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz) or 1 (Poisson)");
+  R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
+  a.lam_line = lam;
+  a.inst = inst;
+  a.mmode = mmode;
+  return run(dtype, kind, R2R_ROW, R2R_APPLY, L, a, 1, (hipStream_t)stream);
+}
+
+int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, const void* in, long long in_stride,
+                        void* out, long long out_stride, double scale, void* stream) {
+  int rc = check_common(dtype, kind);
+  if (rc) return rc;
+  const int L = line_len(kind, n);
+  if (L < 0) return fail(MJX_E_SIZE, "unsupported transform length " + std::to_string(n));
+  if (nlines < 1 || !in || !out) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
+  R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
+  a.scale = scale;
+  const int mode = (kind == R2R_DCT && inverse) ? R2R_INV : R2R_FWD;
+  return run(dtype, kind, R2R_ROW, mode, L, a, 1, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,10 @@
+// double instantiation of the DST-I / DCT line kernels.
+#include "r2r_kernels.h"
+
+namespace mjx {
+
+int r2r_run(int kind, int orient, int mode, int L, R2RArgs a, int B, hipStream_t st, double) {
+  return r2r_dispatch<double>(kind, orient, mode, L, a, B, st);
+}
+
+}  // namespace mjx

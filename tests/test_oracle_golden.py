@@ -174,3 +174,55 @@ def test_fixed_dt_step_count_quirk():
     """stepping.py:1402: int((t_end - t0)/dt) + 1 steps."""
     assert gray_scott.fixed_dt_n_steps(0.0, 10.0, 1.0) == 11
     assert gray_scott.fixed_dt_n_steps(0.0, 1.0, 0.3) == 4
+
+
+# ------------------------------------------------------- DST-I / DCT-II
+def test_dst_matches_library_definition():
+    """dst_I_fast restates fft_nonperiodic.py:97-122; scipy's DST-I is the
+    same sum times 2."""
+    from oracle import nonperiodic as onp
+    import scipy.fft as sf
+    x = np.random.default_rng(4).standard_normal((3, 31))
+    assert np.max(np.abs(onp.dst_I_fast(x) - sf.dst(x, type=1) / 2)) < 1e-13
+    assert np.max(np.abs(onp.idst_I_fast(onp.dst_I_fast(x)) - x)) < 1e-13
+
+
+@pytest.mark.parametrize("kind", ["dirichlet", "neumann"])
+def test_nonperiodic_solves_fd_residual(kind):
+    """The 2-D DST-I (vertex grid, zero boundary) and DCT-II (cell grid,
+    mirrored ghosts) solves invert the 5-point operator exactly; the
+    symbols are its eigenvalues."""
+    from oracle import nonperiodic as onp
+    rng = np.random.default_rng(5)
+    if kind == "dirichlet":
+        ny, nx = 31, 63
+        dy, dx = 1.0 / (ny + 1), 2.0 / (nx + 1)
+        ly, lx = onp.laplacian_symbol_dirichlet(ny, dy), onp.laplacian_symbol_dirichlet(nx, dx)
+        lap = onp.dirichlet_laplacian_2d
+    else:
+        ny, nx = 32, 16
+        dy, dx = 1.0 / ny, 0.5 / nx
+        ly, lx = onp.laplacian_symbol_neumann(ny, dy), onp.laplacian_symbol_neumann(nx, dx)
+        lap = onp.neumann_laplacian_2d
+    rhs = rng.standard_normal((ny, nx))
+    dt, D = 0.3, 0.02
+    u = onp.solve_helmholtz_2d(kind, rhs, ly, lx, dt, D)
+    assert np.max(np.abs(u - dt * D * lap(u, dy, dx) - rhs)) < 1e-12
+    if kind == "dirichlet":
+        p = onp.solve_poisson_2d(kind, rhs, ly, lx)
+        assert np.max(np.abs(lap(p, dy, dx) - rhs)) < 1e-9
+    else:
+        r0 = rhs - rhs.mean()
+        p = onp.solve_poisson_2d(kind, r0, ly, lx)
+        assert np.max(np.abs(lap(p, dy, dx) - r0)) < 1e-9 and abs(p.mean()) < 1e-12
+
+
+def test_poisson_dirichlet_sine_mode():
+    """test_fft_nonperiodic.py:90-106 intent: Poisson with f = -pi^2 sin(pi x)
+    on a vertex grid gives sin(pi x) to second order."""
+    from oracle import nonperiodic as onp
+    N = 63
+    dx = 1.0 / (N + 1)
+    x = dx * np.arange(1, N + 1)
+    u = onp.solve_poisson_dirichlet(-np.pi ** 2 * np.sin(np.pi * x), onp.laplacian_symbol_dirichlet(N, dx))
+    assert np.max(np.abs(u - np.sin(np.pi * x))) < 5e-2 * 0.1
