@@ -21,6 +21,10 @@
  *   ((1 + alpha_f*lam) / (1 - beta_f*lam))^nrep / (nx*ny),
  * with lam = lamx[kx] + lamy[ky] (or lam2d[ky*nx + kx]) and |1 - beta*lam|
  * clamped below at 1e-14 as in moljax/core/fft_solvers.py:280-283.
+ * Two further multipliers are selected by negative nrep:
+ *   nrep = -1:  alpha_f * lam / (nx*ny)        (apply D*lap, alpha = D)
+ *   nrep = -2:  exp(beta_f * lam) / (nx*ny)    (apply exp(dt D lap), beta = dt D)
+ * (moljax/core/fft_operators.py:110-149 matvec / exp_matvec).
  */
 #ifndef MOLJAX_HIP_H
 #define MOLJAX_HIP_H
@@ -119,6 +123,28 @@ int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long
  * idst_I_fast). */
 int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, const void* in, long long in_stride,
                         void* out, long long out_stride, double scale, void* stream);
+
+/* ETDRK4 coefficient tables for periodic Gray-Scott: six real tables
+ * E, E2, Q, f1, f2, f3 of ny*(nx+2) elements each (dtype), in the separated
+ * spectral layout (row ky, column s = kx for u, nx/2+1+kx for v), evaluated
+ * in FP64 by the M = 32 contour mean of benchmarks/benchmark_gray_scott.py:
+ * 336-356 with z = dt * D_f * (lamx[kx] + lamy[ky]); lamx, lamy are device
+ * float64 tables.  mjx_etdrk4_coeff_elems gives the element count. */
+long long mjx_etdrk4_coeff_elems(int ny, int nx);
+int mjx_etdrk4_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
+                      double dt, void* coef, void* stream);
+
+/* Batched coupled Gray-Scott ETDRK4 (Cox-Matthews), n_steps in place on
+ * (u, v) of B square n x n instances (n in {32, ..., 1024}); per-instance F, k
+ * from inst rows; coef from mjx_etdrk4_coeffs.  Four row passes (inverse
+ * x-FFT, N(u,v), forward x-FFT) and four column passes (y-FFT, stage
+ * combination, inverse y-FFT) per step.  Replaces the coupled ETDRK4 of
+ * benchmarks/benchmark_gray_scott.py:369-423 and serves
+ * moljax/core/fft_integrators.py:252-341 (etdrk4_step) for Gray-Scott.
+ * work: mjx_etdrk4_workspace_bytes(dtype, B, n, n). */
+long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx);
+int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                  int fill_ghosts, const void* coef, const double* inst, int n_steps, void* work, void* stream);
 
 /* Opt-in timing of the fused IMEX chain's hot kernels (kind 0: the fused
  * Gray-Scott row pass, kind 1: the column pass).  While enabled every such

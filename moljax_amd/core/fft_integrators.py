@@ -1,0 +1,170 @@
+"""Exponential time differencing on the MI355X kernels
+(moljax/core/fft_integrators.py).
+
+ETDRK4 (Cox-Matthews) for coupled Gray-Scott runs as one batched kernel
+chain (mjx_gs_etdrk4) with FP64 contour-integral coefficients computed once
+per (grid, dt, Du, Dv) on the device (mjx_etdrk4_coeffs, the Kassam-Trefethen
+form of benchmarks/benchmark_gray_scott.py:336-356).  The reference library's
+etdrk4_step evaluates the nonlinearity one field at a time (fft_integrators.py
+:311,319,327) and cannot couple u and v; here the coupled form is served and
+other nonlinearities raise NotImplementedError.
+"""
+from typing import Callable, Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from .. import _hip
+from ._arrays import as_device, field_dtype, inst_table, view2d
+from .fft_operators import DiffusionOperator
+from .state import StateDict
+
+
+def _fd_symbol(n: int, d: float) -> np.ndarray:
+    k = 2.0 * np.pi * np.fft.fftfreq(n, d=d)
+    return (2.0 * np.cos(k * d) - 2.0) / (d * d)
+
+
+def spectral_symbol_1d(n: int, L: float = 1.0) -> np.ndarray:
+    """-(2 pi fftfreq(n, L/n))^2, the pseudo-spectral symbol of the ETDRK4
+    benchmark (benchmarks/benchmark_gray_scott.py:52-57)."""
+    k = np.fft.fftfreq(n, L / n) * 2 * np.pi
+    return -(k ** 2)
+
+
+_COEF_CACHE: Dict[tuple, torch.Tensor] = {}
+
+
+def etdrk4_coefficients(lamx: np.ndarray, lamy: np.ndarray, Du: float, Dv: float, dt: float,
+                        dtype=torch.float32) -> torch.Tensor:
+    """(6, ny, nx+2) device tables E, E2, Q, f1, f2, f3 for both fields in the
+    separated spectral layout, FP64 contour means rounded once to ``dtype``.
+    Cached per (symbol, Du, Dv, dt, dtype)."""
+    lamx = np.ascontiguousarray(lamx, dtype=np.float64)
+    lamy = np.ascontiguousarray(lamy, dtype=np.float64)
+    key = (lamx.tobytes(), lamy.tobytes(), float(Du), float(Dv), float(dt), dtype)
+    hit = _COEF_CACHE.get(key)
+    if hit is not None:
+        return hit
+    lib = _hip.lib()
+    ny, nx = lamy.size, lamx.size
+    lx = torch.from_numpy(lamx).cuda()
+    ly = torch.from_numpy(lamy).cuda()
+    coef = torch.empty((6, ny, nx + 2), dtype=dtype, device="cuda")
+    rc = lib.mjx_etdrk4_coeffs(_hip.dtype_code(dtype), ny, nx, _hip.ptr(lx), _hip.ptr(ly), float(Du), float(Dv),
+                               float(dt), _hip.ptr(coef), _hip.stream_ptr())
+    _hip.check(rc, "mjx_etdrk4_coeffs")
+    _COEF_CACHE[key] = coef
+    return coef
+
+
+def gs_etdrk4_inplace(u: torch.Tensor, v: torch.Tensor, coef: torch.Tensor, rows, n_steps: int,
+                      fill_ghosts: bool = False, work: Optional[torch.Tensor] = None):
+    """Advance (..., n, n) interior arrays (or padded ones with
+    ``fill_ghosts``: pass the padded tensors) by n_steps ETDRK4 steps.
+    ``rows``: per-instance [*, *, *, *, F, k, dt] lists or a (B, 8) table."""
+    tu, pitch, istr, B = view2d(u)
+    tv, pv, iv, _ = view2d(v)
+    if (pv, iv) != (pitch, istr) or tu.dtype != tv.dtype:
+        raise ValueError("u and v must share dtype and layout")
+    if coef.dtype != tu.dtype:
+        raise ValueError("coefficient dtype must match the state dtype")
+    off = 1 if fill_ghosts else 0
+    ny, nx = tu.shape[-2] - 2 * off, tu.shape[-1] - 2 * off
+    inst = rows if isinstance(rows, torch.Tensor) else inst_table(rows, tu.device)
+    lib = _hip.lib()
+    if work is None:
+        nbytes = lib.mjx_etdrk4_workspace_bytes(_hip.dtype_code(tu.dtype), B, ny, nx)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=tu.device)
+    pu = tu[:, 1:, 1:] if off else tu
+    pvv = tv[:, 1:, 1:] if off else tv
+    rc = lib.mjx_gs_etdrk4(_hip.dtype_code(tu.dtype), B, ny, nx, _hip.ptr(pu), _hip.ptr(pvv), pitch, istr,
+                           int(bool(fill_ghosts)), _hip.ptr(coef), _hip.ptr(inst), int(n_steps), _hip.ptr(work),
+                           _hip.stream_ptr())
+    _hip.check(rc, "mjx_gs_etdrk4")
+
+
+def gray_scott_nonlinear_rhs(F: float, k: float) -> Callable[[StateDict, float], StateDict]:
+    """N(u, v) = (-u v^2 + F(1-u), u v^2 - (F+k) v) as a nonlinear_rhs callable
+    the ETD drivers recognise."""
+    from .operators import gray_scott_reaction
+
+    def nonlinear_rhs(state: StateDict, t: float) -> StateDict:
+        ru, rv = gray_scott_reaction(state["u"], state["v"], F, k)
+        return {"u": ru, "v": rv}
+
+    nonlinear_rhs._mjx_gray_scott = (float(F), float(k))
+    return nonlinear_rhs
+
+
+def _gray_scott_params(nonlinear_rhs) -> Tuple[float, float]:
+    tag = getattr(nonlinear_rhs, "_mjx_gray_scott", None)
+    if tag is not None:
+        return tag
+    owner = getattr(nonlinear_rhs, "__self__", None)
+    if owner is not None and getattr(owner, "metadata", {}).get("mjx_kernel") == "gray_scott":
+        return float(owner.params.get("F", 0.04)), float(owner.params.get("k", 0.06))
+    raise NotImplementedError("ETDRK4 on the MI355X path serves the coupled Gray-Scott nonlinearity "
+                              "(gray_scott_nonlinear_rhs or a Gray-Scott model's nonlinear_rhs)")
+
+
+def _linear_symbols(linear_ops) -> Tuple[np.ndarray, np.ndarray, float, float]:
+    try:
+        ou, ov = linear_ops["u"], linear_ops["v"]
+    except KeyError as e:
+        raise NotImplementedError("ETDRK4 needs linear operators for both 'u' and 'v'") from e
+    for op in (ou, ov):
+        if not isinstance(op, DiffusionOperator):
+            raise NotImplementedError("ETDRK4 linear operators must be DiffusionOperator instances")
+    g = ou.grid
+    if ov.grid != g:
+        raise ValueError("u and v operators must share the grid")
+    return _fd_symbol(g.nx, g.dx), _fd_symbol(g.ny, g.dy), float(ou.D), float(ov.D)
+
+
+def etdrk4_step(state: StateDict, t: float, dt: float, linear_ops: Dict[str, DiffusionOperator],
+                nonlinear_rhs: Callable[[StateDict, float], StateDict]) -> StateDict:
+    """fft_integrators.py:252-341, coupled: one ETDRK4 step of (u, v) interior
+    arrays with L = D lap_h (FD symbol) per field."""
+    F, k = _gray_scott_params(nonlinear_rhs)
+    lamx, lamy, Du, Dv = _linear_symbols(linear_ops)
+    u = as_device(state["u"])
+    dtype = field_dtype(u)
+    u = u.to(dtype).clone().contiguous()
+    v = as_device(state["v"], dtype).clone().contiguous()
+    coef = etdrk4_coefficients(lamx, lamy, Du, Dv, dt, dtype)
+    B = view2d(u)[3]
+    gs_etdrk4_inplace(u, v, coef, [[0, 0, 0, 0, F, k, dt]] * B, 1)
+    return {"u": u, "v": v}
+
+
+def etd_integrate(u0: StateDict, t_span: Tuple[float, float], dt: float, linear_ops: Dict[str, DiffusionOperator],
+                  nonlinear_rhs: Callable[[StateDict, float], StateDict], method: str = "etdrk4",
+                  save_every: int = 1):
+    """fft_integrators.py:344-393: int((t_end - t_start)/dt) steps, returns
+    (t_history, state_history) with the initial state first."""
+    if method != "etdrk4":
+        if method in ("etd1", "etd2"):
+            raise NotImplementedError(f"{method} is not on the MI355X path; use method='etdrk4'")
+        raise ValueError(f"Unknown method: {method}. Use 'etd1', 'etd2', or 'etdrk4'")
+    F, k = _gray_scott_params(nonlinear_rhs)
+    lamx, lamy, Du, Dv = _linear_symbols(linear_ops)
+    t0, t1 = t_span
+    n_steps = int((t1 - t0) / dt)
+    u = as_device(u0["u"])
+    dtype = field_dtype(u)
+    u = u.to(dtype).clone().contiguous()
+    v = as_device(u0["v"], dtype).clone().contiguous()
+    coef = etdrk4_coefficients(lamx, lamy, Du, Dv, dt, dtype)
+    rows = inst_table([[0, 0, 0, 0, F, k, dt]] * view2d(u)[3], u.device)
+    t_hist = [t0]
+    hist = [{"u": u.clone(), "v": v.clone()}]
+    done = 0
+    while done < n_steps:
+        chunk = min(save_every, n_steps - done)
+        gs_etdrk4_inplace(u, v, coef, rows, chunk)
+        done += chunk
+        if chunk == save_every:
+            t_hist.append(t0 + done * dt)
+            hist.append({"u": u.clone(), "v": v.clone()})
+    return torch.tensor(t_hist, dtype=torch.float64), hist

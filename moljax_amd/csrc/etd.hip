@@ -1,0 +1,62 @@
+// C-ABI entry points of the batched Gray-Scott ETDRK4 path.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "mjx_common.h"
+#include "mjx_fft.h"
+
+namespace mjx {
+int etd_run(int B, int n, float* u, float* v, long long pitch, long long istride, int ghosts, const float* coef,
+            const InstParams* ip, int n_steps, cpx<float>* work, hipStream_t st);
+int etd_run(int B, int n, double* u, double* v, long long pitch, long long istride, int ghosts, const double* coef,
+            const InstParams* ip, int n_steps, cpx<double>* work, hipStream_t st);
+int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, float* coef,
+               hipStream_t st);
+int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, double* coef,
+               hipStream_t st);
+
+static bool etd_size(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024; }
+}  // namespace mjx
+
+using namespace mjx;
+
+extern "C" {
+
+long long mjx_etdrk4_coeff_elems(int ny, int nx) { return 6LL * ny * (nx + 2); }
+
+long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx) {
+  const long long es = dtype == MJX_F64 ? 16 : 8;
+  return es * (long long)B * (3LL * ny * nx + 3LL * ny * (nx + 2));
+}
+
+int mjx_etdrk4_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
+                      double dt, void* coef, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (ny < 2 || nx < 2 || (nx & 1)) return fail(MJX_E_ARG, "grid must be at least 2x2 with even nx");
+  if (!lamx || !lamy || !coef) return fail(MJX_E_ARG, "null pointer argument");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == MJX_F32) return etd_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (float*)coef, st);
+  return etd_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (double*)coef, st);
+}
+
+int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                  int fill_ghosts, const void* coef, const double* inst, int n_steps, void* work, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (ny != nx || !etd_size(nx)) return fail(MJX_E_SIZE, "ETDRK4 grid must be square with n in {32,...,1024}");
+  if (B < 1 || !u || !v || !coef || !inst || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
+  if (pitch < nx || (fill_ghosts && pitch < nx + 2)) return fail(MJX_E_ARG, "pitch too small");
+  if (n_steps == 0) return MJX_OK;
+  int rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (dtype == MJX_F32)
+    return etd_run(B, nx, (float*)u, (float*)v, pitch, istride, fill_ghosts, (const float*)coef, ip, n_steps,
+                   (cpx<float>*)work, st);
+  return etd_run(B, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)coef, ip, n_steps,
+                 (cpx<double>*)work, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+// ETDRK4 (Cox-Matthews) for periodic Gray-Scott, batched, with FP64
+// contour-integral coefficients.
+//
+// Data per instance (complex, precision T):
+//   Ws  packed x-spectra of the state (layout of spectral_kernels.h)
+//   Wg  packed x-spectra of the current stage value (a, b or c)
+//   Wn  packed x-spectra of N(.) of the last stage
+//   Uh, Nn, Fh  full 2-D spectra in the separated layout: ny rows of nx+2
+//       columns, s = kx (0..nx/2) for u and s = nx/2+1+kx for v, so that
+//       every column is one field at one kx (the packed special columns 0
+//       and nx/2 are split into their two real-sequence spectra).
+// Coefficients E, E2, Q, f1, f2, f3 are six real (ny, nx+2) tables in the
+// same separated layout, shared by all instances with the same (dt, Du, Dv).
+//
+// One step (benchmarks/benchmark_gray_scott.py:369-417, Kassam-Trefethen form):
+//   row_nl(Ws -> Wn)             N_n
+//   col stage 1: Uh = F2 Ws, Nn = F2 Wn, a^ = E2 Uh + Q Nn      -> Wg
+//   row_nl(Wg -> Wn)             N_a
+//   col stage 2: b^ = E2 Uh + Q Na^, Fh = E Uh + f1 Nn + f2 Na^  -> Wg
+//   row_nl(Wg -> Wn)             N_b
+//   col stage 3: c^ = E2 a^ + Q (2 Nb^ - Nn), a^ = E2 Uh + Q Nn,
+//                Fh += f2 Nb^                                     -> Wg
+//   row_nl(Wg -> Wn)             N_c
+//   col stage 4: u^new = Fh + f3 Nc^                              -> Ws
+// ("F2 W" is the y-FFT of packed x-spectra.)  Every packed x-spectral buffer
+// holds the x-DFT divided by nx (the state is packed with scale 1/nx and N is
+// scaled by 1/nx before its x-FFT), so all spectra share one normalisation
+// and the stage outputs, which go back through the inverse y-FFT, carry 1/ny.
+#pragma once
+#include "spectral_kernels.h"
+
+namespace mjx {
+
+// Row pass: packed x-spectra -> physical (u, v) -> Gray-Scott reaction
+// N(u, v) = (-u v^2 + F(1-u), u v^2 - (F+k) v) -> packed x-spectra.
+template <typename T, int NX, int TR>
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_nl(const cpx<T>* __restrict__ Win,
+                                                                  cpx<T>* __restrict__ Wout,
+                                                                  const InstParams* __restrict__ ip, int ny,
+                                                                  int rows_per_inst_tiles,
+                                                                  const cpx<T>* __restrict__ tw) {
+  using C = RowCfg<NX, TR>;
+  __shared__ cpx<T> lds[C::TR * C::LP];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int inst = w / rows_per_inst_tiles;
+  const int tile = w - inst * rows_per_inst_tiles;
+  const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
+  const int y = tile * C::TR + rl;
+  cpx<T>* lrow = lds + rl * C::LP;
+  const long long roff = ((long long)inst * ny + y) * NX;
+  const T F = (T)ip[inst].F, kk = (T)ip[inst].k;
+  const T inv_nx = (T)(1.0 / NX);
+  TwCache<T, NX> twc;
+  load_twiddles(twc, j, tw);
+  cpx<T> v[C::P];
+  load_unpack<T, NX, TR>(Win + roff, lrow, j, v);
+  line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    T ru, rv;
+    gs_react(v[m].re, v[m].im, F, kk, ru, rv);
+    v[m] = mk(ru * inv_nx, rv * inv_nx);
+  }
+  line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
+  __syncthreads();
+  cpx<T>* wrow = Wout + roff;
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int p = j + C::TN * m;
+    wrow[p] = pack_at<T, NX>(lrow, p);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> caxpy(T a, cpx<T> x, cpx<T> y) {  // a*x + y
+  return mk(a * x.re + y.re, a * x.im + y.im);
+}
+
+// Stage arithmetic on one separated-layout spectral value at (s, ky).
+// x is the y-FFT of the stage input (STAGE 1: state and N_n in x, x2).
+template <typename T, int STAGE>
+__device__ __forceinline__ cpx<T> etd_stage(long long si, long long ci, long long tstride, const T* __restrict__ coef,
+                                            cpx<T> x, cpx<T> x2, cpx<T>* __restrict__ Uh, cpx<T>* __restrict__ Nn,
+                                            cpx<T>* __restrict__ Fh, T s) {
+  const T E = coef[ci], E2 = coef[ci + tstride], Q = coef[ci + 2 * tstride];
+  if constexpr (STAGE == 1) {
+    Uh[si] = x;
+    Nn[si] = x2;
+    return scale(caxpy(E2, x, scale(x2, Q)), s);
+  } else if constexpr (STAGE == 2) {
+    const T f1 = coef[ci + 3 * tstride], f2 = coef[ci + 4 * tstride];
+    const cpx<T> u = Uh[si], nn = Nn[si];
+    Fh[si] = caxpy(E, u, caxpy(f1, nn, scale(x, f2)));
+    return scale(caxpy(E2, u, scale(x, Q)), s);
+  } else if constexpr (STAGE == 3) {
+    const T f2 = coef[ci + 4 * tstride];
+    const cpx<T> u = Uh[si], nn = Nn[si];
+    const cpx<T> a = caxpy(E2, u, scale(nn, Q));
+    const cpx<T> d = mk((T)2 * x.re - nn.re, (T)2 * x.im - nn.im);
+    Fh[si] = caxpy(f2, x, Fh[si]);
+    return scale(caxpy(E2, a, scale(d, Q)), s);
+  } else {
+    const T f3 = coef[ci + 5 * tstride];
+    return scale(caxpy(f3, x, Fh[si]), s);
+  }
+}
+
+template <typename T, int NY, int TC_, int STAGE>
+__global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
+    const cpx<T>* __restrict__ Win, const cpx<T>* __restrict__ Win2, cpx<T>* __restrict__ Uh, cpx<T>* __restrict__ Nn,
+    cpx<T>* __restrict__ Fh, const T* __restrict__ coef, cpx<T>* __restrict__ Wout, int nx, int col_tiles,
+    const cpx<T>* __restrict__ tw) {
+  using C = ColCfg<NY, TC_>;
+  constexpr int TC = C::TC;
+  __shared__ cpx<T> lds[NY * TC];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int inst = w / col_tiles;
+  const int tile = w - inst * col_tiles;
+  const int c = threadIdx.x % TC, j = threadIdx.x / TC;
+  const int p0 = tile * TC;
+  const int p = p0 + c;
+  const int half = nx >> 1;
+  const int sw = nx + 2;  // separated-layout row length
+  const long long tstride = (long long)NY * sw;
+  const T s = (T)(1.0 / (double)NY);
+  const long long woff = (long long)inst * NY * nx + p;
+  cpx<T> v[C::P], v2[C::P];
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) v[m] = Win[woff + (long long)(j + C::TN * m) * nx];
+#pragma unroll
+  for (int m = 0; m < C::P; ++m)
+    v2[m] = STAGE == 1 ? Win2[woff + (long long)(j + C::TN * m) * nx] : mk((T)0, (T)0);
+  TwCache<T, NY> twc;
+  load_twiddles(twc, j, tw);
+  line_fft<T, NY, -1, TC, 0>(v, j, lds + c, twc);
+  if constexpr (STAGE == 1) line_fft<T, NY, -1, TC, 0>(v2, j, lds + c, twc);
+
+  const long long sbase = (long long)inst * tstride;
+  if (p0 == 0 || p0 == half) {
+    // Split packed column 0 (u at kx = 0, nx/2) or nx/2 (v at kx = 0, nx/2)
+    // into its two Hermitian spectra: A = (Z + conj Z(-ky))/2,
+    // B = (Z - conj Z(-ky))/(2i); recombine the stage outputs as Ra + i Rb.
+    cpx<T> zm[C::P], zm2[C::P];
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) zm2[m] = mk((T)0, (T)0);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) zm[m] = conjg(lds[((NY - (j + C::TN * m)) & (NY - 1)) * TC + c]);
+    if constexpr (STAGE == 1) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v2[m];
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) zm2[m] = conjg(lds[((NY - (j + C::TN * m)) & (NY - 1)) * TC + c]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int ky = j + C::TN * m;
+      const long long crow = (long long)ky * sw;
+      if (c == 0) {
+        const int sa = p0 == 0 ? 0 : half + 1;
+        const int sb = p0 == 0 ? half : nx + 1;
+        const cpx<T> A = mk((v[m].re + zm[m].re) * (T)0.5, (v[m].im + zm[m].im) * (T)0.5);
+        const cpx<T> Bv = mk((v[m].im - zm[m].im) * (T)0.5, -(v[m].re - zm[m].re) * (T)0.5);
+        cpx<T> A2 = mk((T)0, (T)0), B2 = A2;
+        if constexpr (STAGE == 1) {
+          A2 = mk((v2[m].re + zm2[m].re) * (T)0.5, (v2[m].im + zm2[m].im) * (T)0.5);
+          B2 = mk((v2[m].im - zm2[m].im) * (T)0.5, -(v2[m].re - zm2[m].re) * (T)0.5);
+        }
+        const cpx<T> ra = etd_stage<T, STAGE>(sbase + crow + sa, crow + sa, tstride, coef, A, A2, Uh, Nn, Fh, s);
+        const cpx<T> rb = etd_stage<T, STAGE>(sbase + crow + sb, crow + sb, tstride, coef, Bv, B2, Uh, Nn, Fh, s);
+        v[m] = mk(ra.re - rb.im, ra.im + rb.re);
+      } else {
+        const int sc = p < half ? p : p + 1;
+        v[m] = etd_stage<T, STAGE>(sbase + crow + sc, crow + sc, tstride, coef, v[m], v2[m], Uh, Nn, Fh, s);
+      }
+    }
+  } else {
+    const int sc = p < half ? p : p + 1;
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const long long crow = (long long)(j + C::TN * m) * sw;
+      v[m] = etd_stage<T, STAGE>(sbase + crow + sc, crow + sc, tstride, coef, v[m], v2[m], Uh, Nn, Fh, s);
+    }
+  }
+  line_fft<T, NY, +1, TC, 0>(v, j, lds + c, twc);
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) Wout[woff + (long long)(j + C::TN * m) * nx] = v[m];
+}
+
+// ------------------------------------------------- contour coefficients
+// Kassam-Trefethen contour means over M = 32 points r_j = exp(i pi (j-1/2)/M)
+// (benchmarks/benchmark_gray_scott.py:336-356), in FP64 for one table entry:
+// z = dt * D_f * lam(kx, ky), LR = z + r_j,
+//   E = e^z, E2 = e^{z/2}, Q = dt <(e^{LR/2} - 1)/LR>,
+//   f1 = dt <(-4 - LR + e^{LR}(4 - 3 LR + LR^2))/LR^3>,
+//   f2 = 2 dt <(2 + LR + e^{LR}(-2 + LR))/LR^3>  (the Cox-Matthews weight
+//        of N_a + N_b, dt(2 phi2 - 4 phi3); the benchmark multiplies by the
+//        mean without Kassam-Trefethen's factor 2, an inconsistent scheme),
+//   f3 = dt <(-4 - 3 LR - LR^2 + e^{LR}(4 - LR))/LR^3>   (real parts).
+struct Cd {
+  double re, im;
+};
+__device__ __forceinline__ Cd cd(double r, double i) { return Cd{r, i}; }
+__device__ __forceinline__ Cd cadd(Cd a, Cd b) { return cd(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ Cd cmuld(Cd a, Cd b) { return cd(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+__device__ __forceinline__ Cd cdivd(Cd a, Cd b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return cd((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+__device__ __forceinline__ Cd cexpd(Cd a) {
+  double s, c;
+  sincos(a.im, &s, &c);
+  const double e = exp(a.re);
+  return cd(e * c, e * s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_etd_coeffs(const double* __restrict__ lamx, const double* __restrict__ lamy,
+                                                    int ny, int nx, double Du, double Dv, double dt,
+                                                    T* __restrict__ coef) {
+  const int sw = nx + 2, half = nx / 2;
+  const long long n = (long long)ny * sw;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int ky = (int)(g / sw), s = (int)(g % sw);
+  const bool fu = s <= half;
+  const int kx = fu ? s : s - half - 1;
+  const double z = dt * (fu ? Du : Dv) * (lamx[kx] + lamy[ky]);
+  const int M = 32;
+  Cd q = cd(0, 0), a1 = q, a2 = q, a3 = q;
+  for (int jj = 1; jj <= M; ++jj) {
+    double sn, cs;
+    sincospi((jj - 0.5) / M, &sn, &cs);
+    const Cd LR = cd(z + cs, sn);
+    const Cd eLR = cexpd(LR), eLR2 = cexpd(cd(0.5 * LR.re, 0.5 * LR.im));
+    const Cd LR2 = cmuld(LR, LR), LR3 = cmuld(LR2, LR);
+    q = cadd(q, cdivd(cd(eLR2.re - 1.0, eLR2.im), LR));
+    const Cd t1 = cadd(cd(-4.0 - LR.re, -LR.im), cmuld(eLR, cd(4.0 - 3.0 * LR.re + LR2.re, -3.0 * LR.im + LR2.im)));
+    a1 = cadd(a1, cdivd(t1, LR3));
+    const Cd t2 = cadd(cd(2.0 + LR.re, LR.im), cmuld(eLR, cd(-2.0 + LR.re, LR.im)));
+    a2 = cadd(a2, cdivd(t2, LR3));
+    const Cd t3 = cadd(cd(-4.0 - 3.0 * LR.re - LR2.re, -3.0 * LR.im - LR2.im), cmuld(eLR, cd(4.0 - LR.re, -LR.im)));
+    a3 = cadd(a3, cdivd(t3, LR3));
+  }
+  coef[g] = (T)exp(z);
+  coef[g + n] = (T)exp(0.5 * z);
+  coef[g + 2 * n] = (T)(dt * q.re / M);
+  coef[g + 3 * n] = (T)(dt * a1.re / M);
+  coef[g + 4 * n] = (T)(2.0 * dt * a2.re / M);
+  coef[g + 5 * n] = (T)(dt * a3.re / M);
+}
+
+// ------------------------------------------------------------- launch
+template <typename T>
+struct EtdLaunch {
+  template <int N, int L>
+  static int row_nl(int B, int ny, const cpx<T>* Win, cpx<T>* Wout, const InstParams* ip, hipStream_t st) {
+    const int tiles = ny / L;
+    hipLaunchKernelGGL((k_row_nl<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip, ny, tiles,
+                       twiddles<T>());
+    return check_launch("row_nl");
+  }
+  template <int N, int L, int STAGE>
+  static int col(int B, int nx, const cpx<T>* Win, const cpx<T>* Win2, cpx<T>* Uh, cpx<T>* Nn, cpx<T>* Fh,
+                 const T* coef, cpx<T>* Wout, hipStream_t st) {
+    const int tiles = nx / L;
+    hipLaunchKernelGGL((k_col_etd<T, N, L, STAGE>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, Win, Win2, Uh,
+                       Nn, Fh, coef, Wout, nx, tiles, twiddles<T>());
+    return check_launch("col_etd");
+  }
+};
+
+}  // namespace mjx

@@ -106,7 +106,7 @@ int mjx_spectral_pair(int dtype, int B, int ny, int nx, const void* a_in, const 
   if (!a_in || !a_out || !work || !inst) return fail(MJX_E_ARG, "null pointer argument");
   if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
   if (b_in && !b_out) return fail(MJX_E_ARG, "b_out required when b_in is given");
-  if (nrep < 0) return fail(MJX_E_ARG, "nrep must be >= 0");
+  if (nrep < -2) return fail(MJX_E_ARG, "nrep must be >= -2");
   if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
   if (fill_ghosts && out_pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs out_pitch >= nx + 2");
   rc = ensure_tables(nullptr);

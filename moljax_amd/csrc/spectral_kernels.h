@@ -90,7 +90,8 @@ __device__ __forceinline__ T recip(T d) {
 
 // s * ((1 + alpha*lam) / (1 - beta*lam))^nrep with |1 - beta*lam| clamped
 // below at 1e-14 (moljax/core/fft_solvers.py:280-283).  NREP > 0 fixes the
-// power at compile time; NREP == 0 takes the runtime nrep.
+// power at compile time; NREP == 0 takes the runtime nrep, where nrep = -1
+// selects s * alpha * lam and nrep = -2 selects s * exp(beta * lam).
 template <typename T, int NREP>
 __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
   T den = (T)1 - beta * lam;
@@ -104,6 +105,8 @@ __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
   } else if constexpr (NREP == 2) {
     return (g * g) * s;
   } else {
+    if (nrep == -1) return alpha * lam * s;
+    if (nrep == -2) return exp(beta * lam) * s;
     T m = (T)1, b = g;
     int n = nrep;
     while (n > 0) {
@@ -137,7 +140,7 @@ struct RowCfg {
 template <typename T, int NX, int TR>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(
     const T* __restrict__ a, const T* __restrict__ b, long long pitch, long long istride, cpx<T>* __restrict__ W,
-    int ny, int rows_per_inst_tiles, const cpx<T>* __restrict__ tw) {
+    int ny, int rows_per_inst_tiles, T in_scale, const cpx<T>* __restrict__ tw) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -152,7 +155,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
     const int x = j + C::TN * m;
-    v[m] = mk(ra[x], rb ? rb[x] : (T)0);
+    v[m] = mk(ra[x] * in_scale, rb ? rb[x] * in_scale : (T)0);
   }
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
@@ -391,10 +394,10 @@ inline int pick_lines(int def, int limit) { return def < limit ? def : limit; }
 template <typename T, int N, int L>
 struct RowLaunch {
   static int fwd_pack(int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
-                      hipStream_t st) {
+                      T in_scale, hipStream_t st) {
     const int tiles = ny / L;
     hipLaunchKernelGGL((k_row_fwd_pack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, a, b, pitch,
-                       istride, W, ny, tiles, twiddles<T>());
+                       istride, W, ny, tiles, in_scale, twiddles<T>());
     return check_launch("row_fwd_pack");
   }
   static int inv_unpack(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
@@ -460,8 +463,8 @@ struct Launch {
     template <int L>
     struct F {
       static int run(int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
-                     hipStream_t st) {
-        return RowLaunch<T, N, L>::fwd_pack(B, ny, a, b, pitch, istride, W, st);
+                     T in_scale, hipStream_t st) {
+        return RowLaunch<T, N, L>::fwd_pack(B, ny, a, b, pitch, istride, W, in_scale, st);
       }
     };
   };
@@ -496,11 +499,11 @@ struct Launch {
   };
 
   static int row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,
-                          cpx<T>* W, hipStream_t st) {
+                          cpx<T>* W, hipStream_t st, T in_scale = (T)1) {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, a, b, pitch, istride, W, st);
+    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, a, b, pitch, istride, W, in_scale, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:

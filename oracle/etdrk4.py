@@ -1,0 +1,84 @@
+"""Coupled Gray-Scott ETDRK4 with contour-integral coefficients, float64 /
+complex128 NumPy restatement (test infrastructure).
+
+Follows benchmarks/benchmark_gray_scott.py:336-417 (Kassam-Trefethen
+coefficients with M = 32 points on the upper unit semicircle, four stages,
+nine FFTs per field per step), with the symbol passed as 1-D tables
+lam(kx, ky) = lamx[kx] + lamy[ky] (axis 0 = y) so that both the benchmark's
+pseudo-spectral -k^2 and the library FD symbol can be used.
+The benchmark's final combination multiplies (N_a + N_b) by the
+Kassam-Trefethen mean f2 = dt(phi2 - 2 phi3) without Kassam-Trefethen's factor
+2 (benchmark_gray_scott.py:414-415), so its weights sum to 2/3 dt for a
+constant N and the scheme is not consistent.  ``cox_matthews=True`` (the
+default) applies the Cox-Matthews weight 2 f2 = dt(2 phi2 - 4 phi3), the same
+b2 as the library's etdrk4_step (moljax/core/fft_integrators.py:110-115,
+330); ``cox_matthews=False`` restates the benchmark as written.
+Pinned by the z -> 0 limits (Q -> dt/2, f1, f2, f3 -> dt/6), the direct
+phi-function formulas at moderate |z|, linear exactness (N = 0 gives E u) and
+fourth-order convergence of the Cox-Matthews form (tests/test_oracle_golden.py).
+"""
+import numpy as np
+
+
+def contour_coeffs(L, dt, M=32):
+    """benchmark_gray_scott.py:336-356; returns real (E, E2, Q, f1, f2, f3)."""
+    r = np.exp(1j * np.pi * (np.arange(1, M + 1) - 0.5) / M)
+    LR = dt * L[..., np.newaxis] + r
+    E = np.exp(dt * L)
+    E2 = np.exp(dt * L / 2)
+    Q = dt * np.mean((np.exp(LR / 2) - 1) / LR, axis=-1)
+    f1 = dt * np.mean((-4 - LR + np.exp(LR) * (4 - 3 * LR + LR ** 2)) / LR ** 3, axis=-1)
+    f2 = dt * np.mean((2 + LR + np.exp(LR) * (-2 + LR)) / LR ** 3, axis=-1)
+    f3 = dt * np.mean((-4 - 3 * LR - LR ** 2 + np.exp(LR) * (4 - LR)) / LR ** 3, axis=-1)
+    return E, E2, Q.real, f1.real, f2.real, f3.real
+
+
+def spectral_symbol_1d(n, L=1.0):
+    """-(2 pi fftfreq(n, L/n))^2 (benchmark_gray_scott.py:52-57)."""
+    k = np.fft.fftfreq(n, L / n) * 2 * np.pi
+    return -(k ** 2)
+
+
+def gs_coeffs(lamx, lamy, Du, Dv, dt):
+    lam = lamy[:, None] + lamx[None, :]
+    return contour_coeffs(Du * lam, dt), contour_coeffs(Dv * lam, dt)
+
+
+def gs_etdrk4_step(u, v, cu, cv, F, k, cox_matthews=True):
+    """benchmark_gray_scott.py:369-417 (see the module note on f2)."""
+    Eu, E2u, Qu, f1u, f2u, f3u = cu
+    Ev, E2v, Qv, f1v, f2v, f3v = cv
+    if cox_matthews:
+        f2u, f2v = 2 * f2u, 2 * f2v
+
+    def Nu(a, b):
+        return -a * b * b + F * (1 - a)
+
+    def Nv(a, b):
+        return a * b * b - (F + k) * b
+
+    fft2, ifft2 = np.fft.fft2, np.fft.ifft2
+    uh, vh = fft2(u), fft2(v)
+    Nnu, Nnv = fft2(Nu(u, v)), fft2(Nv(u, v))
+    au = E2u * uh + Qu * Nnu
+    av = E2v * vh + Qv * Nnv
+    ar, br_ = np.real(ifft2(au)), np.real(ifft2(av))
+    Nau, Nav = fft2(Nu(ar, br_)), fft2(Nv(ar, br_))
+    bu = E2u * uh + Qu * Nau
+    bv = E2v * vh + Qv * Nav
+    bur, bvr = np.real(ifft2(bu)), np.real(ifft2(bv))
+    Nbu, Nbv = fft2(Nu(bur, bvr)), fft2(Nv(bur, bvr))
+    cu_ = E2u * au + Qu * (2 * Nbu - Nnu)
+    cv_ = E2v * av + Qv * (2 * Nbv - Nnv)
+    cur, cvr = np.real(ifft2(cu_)), np.real(ifft2(cv_))
+    Ncu, Ncv = fft2(Nu(cur, cvr)), fft2(Nv(cur, cvr))
+    un = Eu * uh + f1u * Nnu + f2u * (Nau + Nbu) + f3u * Ncu
+    vn = Ev * vh + f1v * Nnv + f2v * (Nav + Nbv) + f3v * Ncv
+    return np.real(ifft2(un)), np.real(ifft2(vn))
+
+
+def gs_etdrk4(u, v, lamx, lamy, Du, Dv, F, k, dt, n_steps, cox_matthews=True):
+    cu, cv = gs_coeffs(lamx, lamy, Du, Dv, dt)
+    for _ in range(n_steps):
+        u, v = gs_etdrk4_step(u, v, cu, cv, F, k, cox_matthews)
+    return u, v

@@ -1,0 +1,108 @@
+"""Coupled Gray-Scott ETDRK4 and FFT operators on the GPU vs the float64
+oracle.  Tolerances (relative L2): coefficients 1e-12; float64 trajectories
+1e-12; float32 trajectories 1e-5 over 10 steps."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import etdrk4 as oe
+from oracle import gray_scott as ogs
+from oracle import spectral as osp
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().double().cpu().numpy()
+
+
+def _ic(n, seed):
+    u, v = ogs.initial_condition(n, n, seed, square=max(2, n * 20 // 256))["u"], None
+    y = ogs.initial_condition(n, n, seed, square=max(2, n * 20 // 256))
+    return y["u"][1:-1, 1:-1].copy(), y["v"][1:-1, 1:-1].copy()
+
+
+@pytest.mark.parametrize("n", [32, 256])
+def test_coefficients_match_oracle(gpu, n):
+    from moljax_amd.core.fft_integrators import etdrk4_coefficients, spectral_symbol_1d
+    lam = spectral_symbol_1d(n)
+    coef = _np(etdrk4_coefficients(lam, lam, 2e-5, 1e-5, 1.0, torch.float64))
+    cu, cv = oe.gs_coeffs(lam, lam, 2e-5, 1e-5, 1.0)
+    h = n // 2
+    for t in range(6):
+        w = 2.0 if t == 4 else 1.0  # f2 holds the Cox-Matthews weight 2 f2
+        assert ogs.rel_l2(coef[t][:, :h + 1], w * np.real(cu[t])[:, :h + 1]) < 1e-12
+        assert ogs.rel_l2(coef[t][:, h + 1:], w * np.real(cv[t])[:, :h + 1]) < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+def test_gs_etdrk4_matches_oracle(gpu, n, dtype, tol):
+    from moljax_amd.core.fft_integrators import etdrk4_coefficients, gs_etdrk4_inplace, spectral_symbol_1d
+    lam = spectral_symbol_1d(n)
+    F, k, Du, Dv, dt, steps = 0.035, 0.065, 2e-5, 1e-5, 1.0, 10
+    u0, v0 = _ic(n, 0)
+    U = torch.from_numpy(u0).to("cuda", dtype)
+    V = torch.from_numpy(v0).to("cuda", dtype)
+    coef = etdrk4_coefficients(lam, lam, Du, Dv, dt, dtype)
+    gs_etdrk4_inplace(U, V, coef, [[0, 0, 0, 0, F, k, dt]], steps)
+    ru, rv = oe.gs_etdrk4(u0, v0, lam, lam, Du, Dv, F, k, dt, steps)
+    eu, ev = ogs.rel_l2(_np(U), ru), ogs.rel_l2(_np(V), rv)
+    print(f"n={n} {dtype} rel_l2 u={eu:.2e} v={ev:.2e}", flush=True)
+    assert eu <= tol and ev <= tol
+
+
+def test_batched_per_instance_params_and_invariance(gpu):
+    from moljax_amd.core.fft_integrators import etdrk4_coefficients, gs_etdrk4_inplace, spectral_symbol_1d
+    n, steps = 64, 6
+    lam = spectral_symbol_1d(n)
+    params = [(0.03, 0.06), (0.05, 0.062), (0.04, 0.058)]
+    ics = [_ic(n, s) for s in range(3)]
+    U = torch.from_numpy(np.stack([a for a, _ in ics])).cuda()
+    V = torch.from_numpy(np.stack([b for _, b in ics])).cuda()
+    coef = etdrk4_coefficients(lam, lam, 2e-5, 1e-5, 1.0, torch.float64)
+    rows = [[0, 0, 0, 0, F, k, 1.0] for F, k in params]
+    u1, v1 = U[1].clone(), V[1].clone()
+    gs_etdrk4_inplace(U, V, coef, rows, steps)
+    gs_etdrk4_inplace(u1, v1, coef, rows[1:2], steps)
+    assert torch.equal(U[1], u1) and torch.equal(V[1], v1)
+    for i, (F, k) in enumerate(params):
+        ru, rv = oe.gs_etdrk4(ics[i][0], ics[i][1], lam, lam, 2e-5, 1e-5, F, k, 1.0, steps)
+        assert ogs.rel_l2(_np(U[i]), ru) < 1e-12 and ogs.rel_l2(_np(V[i]), rv) < 1e-12
+
+
+def test_etdrk4_step_api_fd_symbol(gpu):
+    """etdrk4_step / etd_integrate with DiffusionOperator (library FD symbol)."""
+    from moljax_amd.core import Grid2D
+    from moljax_amd.core.fft_integrators import etd_integrate, etdrk4_step, gray_scott_nonlinear_rhs
+    from moljax_amd.core.fft_operators import DiffusionOperator
+    n = 64
+    grid = Grid2D.uniform(n, n, 0.0, 2.5, 0.0, 2.5)
+    ops = {"u": DiffusionOperator(grid, 2e-5), "v": DiffusionOperator(grid, 1e-5)}
+    nl = gray_scott_nonlinear_rhs(0.035, 0.065)
+    u0, v0 = _ic(n, 1)
+    st = {"u": torch.from_numpy(u0).cuda(), "v": torch.from_numpy(v0).cuda()}
+    one = etdrk4_step(st, 0.0, 1.0, ops, nl)
+    lam = osp.laplacian_symbol_1d(n, grid.dx)
+    ru, rv = oe.gs_etdrk4(u0, v0, lam, lam, 2e-5, 1e-5, 0.035, 0.065, 1.0, 1)
+    assert ogs.rel_l2(_np(one["u"]), ru) < 1e-12
+    t, hist = etd_integrate(st, (0.0, 4.0), 1.0, ops, nl, save_every=2)
+    assert len(hist) == 3 and np.allclose(t.numpy(), [0, 2, 4])
+    ru4, _ = oe.gs_etdrk4(u0, v0, lam, lam, 2e-5, 1e-5, 0.035, 0.065, 1.0, 4)
+    assert ogs.rel_l2(_np(hist[-1]["u"]), ru4) < 1e-12
+
+
+def test_diffusion_operator(gpu):
+    from moljax_amd.core import Grid2D
+    from moljax_amd.core.fft_operators import DiffusionOperator
+    grid = Grid2D.uniform(64, 32, 0.0, 1.0, 0.0, 2.0)
+    op = DiffusionOperator(grid, 0.01)
+    u = np.random.default_rng(8).standard_normal((32, 64))
+    lam = osp.laplacian_symbol_2d(32, 64, grid.dy, grid.dx)
+    t = torch.from_numpy(u).cuda()
+    assert ogs.rel_l2(_np(op.matvec(t)), 0.01 * osp.periodic_fd_laplacian(u, grid.dy, grid.dx)) < 1e-12
+    assert ogs.rel_l2(_np(op.solve(t, 0.3)), osp.solve_helmholtz_2d(u, lam, 0.3, 0.01)) < 1e-13
+    ex = np.fft.ifft2(np.fft.fft2(u) * np.exp(0.3 * 0.01 * lam)).real
+    assert ogs.rel_l2(_np(op.exp_matvec(t, 0.3)), ex) < 1e-13
+    with pytest.raises(ValueError):
+        DiffusionOperator(grid, 0.0)
