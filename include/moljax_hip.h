@@ -146,6 +146,51 @@ long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx);
 int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
                   int fill_ghosts, const void* coef, const double* inst, int n_steps, void* work, void* stream);
 
+/* Implicit reaction-diffusion step by Newton-Krylov on an interior (ny, nx)
+ * array: bc 0 homogeneous Dirichlet on a vertex grid (zero outside), bc 1
+ * periodic; u_t = D lap_h u + f(u), f = reaction[0] u^3 + reaction[1] u^2 +
+ * reaction[2] u + reaction[3];
+ *   R(u) = u - u_old - theta dt (D lap u + f(u)) - fold_term   (fold_term may
+ *   be NULL; for Crank-Nicolson it is (1 - theta) dt F(u_old)),
+ *   J v = v - theta dt (D lap v + f'(u) v)   (hand-written JVP).
+ * u (float64, in/out) holds the initial guess and receives the solution.
+ * Newton as moljax/core/newton_krylov.py:171-359 (convergence tested before
+ * the update, max_backtrack residual trials, fallback damped step);
+ * left-preconditioned restarted GMRES(restart) with FP64 CGS2
+ * orthogonalisation on deterministic wave-shuffle dot products and FP64
+ * Givens rotations; one GMRES iteration = one Arnoldi step, convergence when
+ * the Givens residual <= krylov_tol * ||M b||, max_krylov restart cycles.
+ * nk_iparams = {operator dtype (MJX_F32/MJX_F64), precond (0 none, 1 exact
+ *   (I - theta dt D lap)^-1 by DST-I or FFT, 2 Jacobi), jacobi sweeps,
+ *   max_newton, max_krylov, restart, max_backtrack};
+ * nk_params = {newton_tol, krylov_tol, damping, backtrack_factor,
+ *   min_residual_decrease, jacobi_omega};
+ * lam_tables = {lamx f64, lamy f64, lamx f32, lamy f32} device symbol tables
+ *   of the exact preconditioner (Dirichlet: DST-I eigenvalues, periodic: FD).
+ * stats_out = {converged, newton_iters, gmres_iters, final ||R||}.  Synchronous
+ * (the host reads Krylov scalars each Arnoldi step).
+ * work: mjx_nk_workspace_bytes(ny, nx, restart) bytes. */
+long long mjx_nk_workspace_bytes(int ny, int nx, int restart);
+int mjx_rd_newton_krylov(int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
+                         const double* reaction, double* u, const double* u_old, const double* fold_term,
+                         const void* const* lam_tables, const double* nk_params, const int* nk_iparams,
+                         double* stats_out, void* work, void* stream);
+
+/* The residual R(u) (float64) and the JVP J(u) v (v, w in dtype, u float64)
+ * of the problem above, as single kernels. */
+int mjx_rd_residual(int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
+                    const double* reaction, const double* u, const double* u_old, const double* fold_term,
+                    double* r, void* stream);
+int mjx_rd_jvp(int dtype, int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
+               const double* reaction, const double* u, const void* v, void* w, void* stream);
+
+/* Deterministic FP64 dot products out[i] = <V_i, w> of k vectors of length n
+ * (rows of V, stride n), wave64 shuffle + LDS reduction over a fixed block
+ * partition; partial: mjx_dot_partial_elems(k) doubles of scratch.
+ * Replaces the reductions of moljax/core/state.py:115-189. */
+long long mjx_dot_partial_elems(int k);
+int mjx_dot_f64(long long n, int k, const double* V, const double* w, double* out, double* partial, void* stream);
+
 /* Opt-in timing of the fused IMEX chain's hot kernels (kind 0: the fused
  * Gray-Scott row pass, kind 1: the column pass).  While enabled every such
  * launch is bracketed by HIP events on its stream.  Enabling resets the

@@ -1,0 +1,136 @@
+"""Scalar reaction-diffusion models u_t = D lap_h u + f(u) served by the
+native Newton-Krylov driver, with f a cubic polynomial
+f(u) = a3 u^3 + a2 u^2 + a1 u + a0 (Allen-Cahn type f = u - u^3 is
+(-1, 0, 1, 0)).
+
+The unknown is an interior array: homogeneous Dirichlet boundaries live on a
+vertex grid of n interior points with h = L/(n+1) (zero boundary values, the
+grid on which DST-I diagonalises the 5-point Laplacian; the vertex-grid
+Dirichlet problem of moljax/core/fft_nonperiodic.py), periodic boundaries on
+n cells with h = L/n.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from .. import _hip
+from .newton_krylov import NKParams, NKStats, create_implicit_residual, newton_krylov_solve
+from .state import StateDict
+
+
+def _fd_symbol(n: int, d: float) -> np.ndarray:
+    k = 2.0 * np.pi * np.fft.fftfreq(n, d=d)
+    return (2.0 * np.cos(k * d) - 2.0) / (d * d)
+
+
+def _dirichlet_symbol(n: int, d: float) -> np.ndarray:
+    k = np.arange(1, n + 1, dtype=np.float64)
+    return -4.0 / (d * d) * np.sin(np.pi * k / (2 * (n + 1))) ** 2
+
+
+@dataclass(frozen=True)
+class ReactionDiffusionModel:
+    ny: int
+    nx: int
+    dy: float
+    dx: float
+    bc: str
+    D: float
+    reaction: Tuple[float, float, float, float]
+    params: dict = field(default_factory=dict)
+    mjx_kernel: str = "reaction_diffusion"
+
+    @property
+    def bc_code(self) -> int:
+        return 0 if self.bc == "dirichlet" else 1
+
+    @property
+    def grid(self):
+        return self
+
+    def as_field(self, u) -> torch.Tensor:
+        t = torch.as_tensor(u).to("cuda", torch.float64).contiguous()
+        if t.shape != (self.ny, self.nx):
+            raise ValueError(f"expected an interior ({self.ny}, {self.nx}) array, got {tuple(t.shape)}")
+        return t
+
+    def rhs_field(self, u) -> torch.Tensor:
+        """F(u) = D lap_h u + f(u) (the residual kernel with u_old = u,
+        dt = theta = 1 returns -F(u) exactly)."""
+        u = self.as_field(u)
+        r = torch.empty_like(u)
+        reac = (ctypes.c_double * 4)(*self.reaction)
+        rc = _hip.lib().mjx_rd_residual(self.ny, self.nx, self.bc_code, self.dy, self.dx, self.D, 1.0, 1.0, reac,
+                                        _hip.ptr(u), _hip.ptr(u), 0, _hip.ptr(r), _hip.stream_ptr())
+        _hip.check(rc, "mjx_rd_residual")
+        return -r
+
+    def rhs(self, y: StateDict, t: float = 0.0) -> StateDict:
+        return {"u": self.rhs_field(y["u"])}
+
+    def precond_tables(self):
+        cached = self.params.get("_tables")
+        if cached is None:
+            if self.bc == "dirichlet":
+                lx, ly = _dirichlet_symbol(self.nx, self.dx), _dirichlet_symbol(self.ny, self.dy)
+            else:
+                lx, ly = _fd_symbol(self.nx, self.dx), _fd_symbol(self.ny, self.dy)
+            t64x, t64y = torch.from_numpy(lx).cuda(), torch.from_numpy(ly).cuda()
+            cached = (t64x, t64y, t64x.float(), t64y.float())
+            self.params["_tables"] = cached
+        return cached
+
+
+def create_reaction_diffusion_model(n: int, L: float = 1.0, D: float = 1e-3,
+                                    reaction: Tuple[float, float, float, float] = (-1.0, 0.0, 1.0, 0.0),
+                                    bc: str = "dirichlet", ny: Optional[int] = None) -> ReactionDiffusionModel:
+    if bc not in ("dirichlet", "periodic"):
+        raise ValueError("bc must be 'dirichlet' or 'periodic'")
+    ny = n if ny is None else ny
+    if bc == "dirichlet":
+        dx, dy = L / (n + 1), L / (ny + 1)
+    else:
+        dx, dy = L / n, L / ny
+    return ReactionDiffusionModel(ny=ny, nx=n, dy=dy, dx=dx, bc=bc, D=float(D),
+                                  reaction=tuple(float(c) for c in reaction), params={"D": float(D)})
+
+
+def euler_step(model: ReactionDiffusionModel, y: StateDict, t: float, dt: float) -> StateDict:
+    """stepping.py:64-85: y + dt F(y)."""
+    u = model.as_field(y["u"])
+    return {"u": u + dt * model.rhs_field(u)}
+
+
+def be_step(model: ReactionDiffusionModel, y: StateDict, t: float, dt: float, preconditioner=None,
+            nk_params: Optional[NKParams] = None) -> Tuple[StateDict, NKStats]:
+    """stepping.py:170-214: Newton-Krylov from the explicit Euler guess."""
+    res = create_implicit_residual(model, y, t + dt, dt, "be")
+    out = newton_krylov_solve(res, euler_step(model, y, t, dt), model, model.params, preconditioner, nk_params, dt)
+    return out.solution, out.stats
+
+
+def cn_step(model: ReactionDiffusionModel, y: StateDict, t: float, dt: float, preconditioner=None,
+            nk_params: Optional[NKParams] = None) -> Tuple[StateDict, NKStats]:
+    """stepping.py:217-260."""
+    res = create_implicit_residual(model, y, t + dt, dt, "cn")
+    out = newton_krylov_solve(res, euler_step(model, y, t, dt), model, model.params, preconditioner, nk_params, dt)
+    return out.solution, out.stats
+
+
+def sine_mode_ic(n: int, seed: int = 0, modes: int = 8, kmax: int = 6) -> np.ndarray:
+    """Sum of ``modes`` sine modes a sin(pi p x) sin(pi q y), a ~ N(0, 0.5),
+    p, q uniform in 1..kmax, on interior vertices x_i = i/(n+1) (seeded
+    numpy default_rng)."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n + 1)
+    x = h * np.arange(1, n + 1)
+    X, Y = np.meshgrid(x, x)
+    u = np.zeros((n, n))
+    for _ in range(modes):
+        a = rng.normal(0.0, 0.5)
+        p, q = rng.integers(1, kmax + 1, size=2)
+        u += a * np.sin(np.pi * p * X) * np.sin(np.pi * q * Y)
+    return u

@@ -1,0 +1,137 @@
+"""Newton-Krylov (residual, hand JVP, GMRES, preconditioners) on the GPU vs
+the float64 oracle.  With a float64 Krylov operator the Newton and GMRES
+iteration counts equal the oracle's exactly and solutions agree to 1e-10
+(relative L2); with a float32 operator the Newton residual still converges
+below newton_tol (it is evaluated in float64)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import gray_scott as ogs
+from oracle import newton_krylov as onk
+
+pytestmark = pytest.mark.gpu
+
+AC = (-1.0, 0.0, 1.0, 0.0)  # f(u) = u - u^3
+
+
+def _np(t):
+    return t.detach().double().cpu().numpy()
+
+
+def test_jvp_golden_norm_on_gpu(gpu):
+    """benchmarks/results/jvp_vs_fd_sweep.json:3 through the JVP kernel."""
+    import json, os
+    from moljax_amd.core.newton_krylov import RDResidual
+    from moljax_amd.core.reaction_diffusion import create_reaction_diffusion_model
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "nk_golden.json")))["jvp_vs_fd"]
+    N, K = g["N"], g["reaction_K"]
+    np.random.seed(g["seed"])
+    u0 = np.abs(np.random.randn(N, N)) + 0.5
+    v = np.random.randn(N, N)
+    model = create_reaction_diffusion_model(N, L=1.0, D=g["D"], reaction=(-K, K, 0.0, 0.0), bc="periodic")
+    res = RDResidual(model, {"u": torch.from_numpy(u0)}, g["dt"], 1.0)
+    Jv = res.jvp(torch.from_numpy(u0), torch.from_numpy(v))
+    assert float(torch.linalg.norm(Jv)) == pytest.approx(g["ad_jvp_norm"], rel=1e-12)
+    Jv32 = res.jvp(torch.from_numpy(u0), torch.from_numpy(v).float())
+    assert float(torch.linalg.norm(Jv32.double())) == pytest.approx(g["ad_jvp_norm"], rel=1e-5)
+
+
+@pytest.mark.parametrize("bc,n", [("dirichlet", 31), ("periodic", 32)])
+def test_residual_kernel(gpu, bc, n):
+    from moljax_amd.core.newton_krylov import create_implicit_residual
+    from moljax_amd.core.reaction_diffusion import create_reaction_diffusion_model
+    model = create_reaction_diffusion_model(n, D=2e-3, reaction=(-1.0, 0.3, 1.0, 0.1), bc=bc)
+    rng = np.random.default_rng(1)
+    u, uo = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    for method, theta in (("be", 1.0), ("cn", 0.5)):
+        res = create_implicit_residual(model, {"u": torch.from_numpy(uo)}, 0.1, 0.05, method)
+        r = _np(res({"u": torch.from_numpy(u)})["u"])
+        F_old = onk.rhs(uo, 2e-3, model.reaction, model.dy, model.dx, bc)
+        ref = onk.residual(u, uo, 0.05, theta, 2e-3, model.reaction, model.dy, model.dx, bc, F_old)
+        assert np.max(np.abs(r - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("precond", ["dst", "jacobi", "none"])
+@pytest.mark.parametrize("bc,n", [("dirichlet", 63), ("periodic", 64)])
+def test_be_step_counts_match_oracle(gpu, precond, bc, n):
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import (DSTDiffusionPreconditioner, DiffusionPreconditioner,
+                                                 FFTDiffusionPreconditioner, IdentityPreconditioner)
+    from moljax_amd.core.reaction_diffusion import be_step, create_reaction_diffusion_model, sine_mode_ic
+    D, dt = 1e-3, 1e-2
+    model = create_reaction_diffusion_model(n, D=D, reaction=AC, bc=bc)
+    u0 = sine_mode_ic(n, seed=0)
+    if precond == "dst":
+        P = DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"}) if bc == "dirichlet" else \
+            FFTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"}, fft_cache=True)
+    elif precond == "jacobi":
+        P = DiffusionPreconditioner(field_diffusivities={"u": "D"})
+    else:
+        P = IdentityPreconditioner()
+    nkp = NKParams(krylov_tol=1e-8, gmres_restart=30)
+    y, st = be_step(model, {"u": torch.from_numpy(u0)}, 0.0, dt, P, nkp)
+    guess = u0 + dt * onk.rhs(u0, D, AC, model.dy, model.dx, bc)
+    ref = onk.newton_krylov(guess, u0, dt, 1.0, D, AC, model.dy, model.dx, bc, precond=precond,
+                            krylov_tol=1e-8)
+    print(f"{bc} {precond}: newton {st.newton_iters}/{ref.newton_iters} gmres {st.lin_iters}/{ref.gmres_iters}",
+          flush=True)
+    assert st.converged == ref.converged and st.newton_iters == ref.newton_iters
+    assert st.lin_iters == ref.gmres_iters
+    assert ogs.rel_l2(_np(y["u"]), ref.u) < 1e-10
+    assert st.final_res_norm < 1e-8
+
+
+def test_mixed_precision_operator_converges(gpu):
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import DSTDiffusionPreconditioner
+    from moljax_amd.core.reaction_diffusion import be_step, create_reaction_diffusion_model, sine_mode_ic
+    n, D, dt = 127, 1e-3, 1e-2
+    model = create_reaction_diffusion_model(n, D=D, reaction=AC)
+    u0 = sine_mode_ic(n, seed=3)
+    P = DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"})
+    y64, s64 = be_step(model, {"u": torch.from_numpy(u0)}, 0.0, dt, P, NKParams(krylov_tol=1e-8))
+    y32, s32 = be_step(model, {"u": torch.from_numpy(u0)}, 0.0, dt, P,
+                       NKParams(krylov_tol=1e-8, operator_dtype=torch.float32))
+    print(f"fp64 op: newton {s64.newton_iters} gmres {s64.lin_iters}; fp32 op: newton {s32.newton_iters} "
+          f"gmres {s32.lin_iters} res {s32.final_res_norm:.2e}", flush=True)
+    assert s32.converged and s32.final_res_norm < 1e-8
+    assert ogs.rel_l2(_np(y32["u"]), _np(y64["u"])) < 1e-9
+
+
+def test_cn_step_matches_oracle(gpu):
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import DSTDiffusionPreconditioner
+    from moljax_amd.core.reaction_diffusion import cn_step, create_reaction_diffusion_model, sine_mode_ic
+    n, D, dt = 31, 5e-3, 2e-2
+    model = create_reaction_diffusion_model(n, D=D, reaction=AC)
+    u0 = sine_mode_ic(n, seed=5)
+    y, st = cn_step(model, {"u": torch.from_numpy(u0)}, 0.0, dt,
+                    DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"}), NKParams(krylov_tol=1e-8))
+    F0 = onk.rhs(u0, D, AC, model.dy, model.dx, "dirichlet")
+    ref = onk.newton_krylov(u0 + dt * F0, u0, dt, 0.5, D, AC, model.dy, model.dx, "dirichlet", precond="dst",
+                            F_old=F0)
+    assert st.newton_iters == ref.newton_iters and st.lin_iters == ref.gmres_iters
+    assert ogs.rel_l2(_np(y["u"]), ref.u) < 1e-10
+
+
+def test_dot_kernel_deterministic_and_exact(gpu):
+    import ctypes
+    from moljax_amd import _hip
+    rng = np.random.default_rng(11)
+    n, k = 100003, 11
+    V = rng.standard_normal((k, n))
+    w = rng.standard_normal(n)
+    Vt, wt = torch.from_numpy(V).cuda(), torch.from_numpy(w).cuda()
+    part = torch.empty(_hip.lib().mjx_dot_partial_elems(k), dtype=torch.float64, device="cuda")
+    outs = []
+    for _ in range(2):
+        out = torch.empty(k, dtype=torch.float64, device="cuda")
+        _hip.check(_hip.lib().mjx_dot_f64(n, k, _hip.ptr(Vt), _hip.ptr(wt), _hip.ptr(out), _hip.ptr(part),
+                                          _hip.stream_ptr()), "dot")
+        outs.append(out.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    exact = np.array([math.fsum(V[i] * w) for i in range(k)])
+    assert np.max(np.abs(outs[0] - exact)) < 1e-12 * np.sqrt(n)

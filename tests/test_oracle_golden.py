@@ -279,3 +279,59 @@ def test_etdrk4_linear_exact_and_fourth_order():
     bad = [np.max(np.abs(oe.gs_etdrk4(uu, vv, lam, lam, 2e-3, 1e-3, 0.035, 0.065, T / s, s,
                                       cox_matthews=False)[0] - ref[0])) for s in (8, 16)]
     assert math.log2(bad[0] / bad[1]) < 1.5
+
+
+# --------------------------------------------------------- Newton-Krylov
+def test_hand_jvp_matches_golden_ad_norm():
+    """benchmarks/results/jvp_vs_fd_sweep.json:3: AD-JVP norm of the BE
+    residual u - u_n - dt (D lap u + 10 u^2 (1-u)) on a periodic 64^2 grid
+    (dx = 1/64, D = dt = 0.01, numpy legacy seed 42: u0 = |randn| + 0.5,
+    v = randn); the hand JVP gives the same vector."""
+    from oracle import newton_krylov as onk
+    g = _golden("nk_golden.json")["jvp_vs_fd"]
+    N = g["N"]
+    np.random.seed(g["seed"])
+    u0 = np.abs(np.random.randn(N, N)) + 0.5
+    v = np.random.randn(N, N)
+    dx = 1.0 / N
+    K = g["reaction_K"]
+    Jv = onk.jvp(u0, v, g["dt"], 1.0, g["D"], (-K, K, 0.0, 0.0), dx, dx, "periodic")
+    assert np.linalg.norm(Jv) == pytest.approx(g["ad_jvp_norm"], rel=1e-13)
+    # central finite differences at eps = 1e-5 agree to ~1e-10
+    R = lambda u: onk.residual(u, u0, g["dt"], 1.0, g["D"], (-K, K, 0.0, 0.0), dx, dx, "periodic")
+    eps = 1e-5
+    fd = (R(u0 + eps * v) - R(u0 - eps * v)) / (2 * eps)
+    assert np.linalg.norm(fd - Jv) / np.linalg.norm(Jv) < 1e-9
+
+
+def test_gmres_matches_dense_solve_and_precond_exact():
+    from oracle import newton_krylov as onk
+    n = 15
+    h = 1.0 / (n + 1)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, n))
+    args = (0.05, 1.0, 0.02, (-1.0, 0.0, 1.0, 0.0), h, h, "dirichlet")
+    A = lambda v: onk.jvp(u, v, *args)
+    Ad = np.stack([A(e.reshape(n, n)).ravel() for e in np.eye(n * n)], axis=1)
+    xd = np.linalg.solve(Ad, b.ravel()).reshape(n, n)
+    M = lambda v: onk.helmholtz_precond(v, 0.05 * 0.02, h, h, "dirichlet")
+    x, it, conv, res = onk.gmres(A, b, M, rtol=1e-10, restart=30, maxiter=20)
+    assert conv and np.max(np.abs(x - xd)) < 1e-8 * np.max(np.abs(xd))
+    x2, it2, conv2, _ = onk.gmres(A, b, None, rtol=1e-10, restart=10, maxiter=200)
+    assert conv2 and np.max(np.abs(x2 - xd)) < 1e-7 * np.max(np.abs(xd)) and it2 > it
+    r = rng.standard_normal((n, n))
+    p = M(r)
+    assert np.max(np.abs(p - 0.05 * 0.02 * onk.lap(p, h, h, "dirichlet") - r)) < 1e-12
+
+
+def test_newton_converges_on_allen_cahn_step():
+    from oracle import newton_krylov as onk
+    n = 31
+    h = 1.0 / (n + 1)
+    u_old = onk.sine_mode_ic(n, seed=0)
+    res = onk.newton_krylov(u_old, u_old, 1e-2, 1.0, 1e-3, (-1.0, 0.0, 1.0, 0.0), h, h, "dirichlet")
+    assert res.converged and res.res_norm < 1e-8
+    r = onk.residual(res.u, u_old, 1e-2, 1.0, 1e-3, (-1.0, 0.0, 1.0, 0.0), h, h, "dirichlet")
+    assert np.linalg.norm(r) < 1e-8
+    assert 0 < res.gmres_iters < 40
