@@ -94,6 +94,60 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                        long long istride, int fill_ghosts, const void* lamx, const void* lamy,
                        const void* lam2d, const double* inst, int n_steps, void* work, void* stream);
 
+/* Gray-Scott IMEX with a choice of scheme, n_steps fixed steps in place:
+ * scheme 0 Strang (identical to mjx_gs_imex_strang), 1 IMEX-Euler
+ *   y_new = (I - dt D lap)^-1 (y + dt R(y))      (stepping.py:317-367),
+ * 2 IMEX-SSPRK2  y_new = y/2 + E(E(y))/2 with E the IMEX-Euler map
+ *   (stepping.py:438-495).  inst beta columns: dt/2*D for Strang, dt*D for
+ * Euler and SSPRK2.  Replaces moljax/core/stepping.py:317-495 for the model
+ * of moljax/core/model.py:529-584.  work as mjx_gs_imex_strang. */
+int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch,
+                long long istride, int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d,
+                const double* inst, int n_steps, void* work, void* stream);
+
+/* Per-instance adaptive-integrator state: MJX_AS_STRIDE doubles per
+ * instance, device memory.  The caller initialises T = t0,
+ * DT = min(dt0, dt_cfl), TEND, DTCFL, PREV_ERR = PREVPREV_ERR = 1,
+ * WRITE_IDX = 1, STATUS = 1 (running), the rest 0. */
+enum {
+  MJX_AS_T = 0, MJX_AS_DT = 1, MJX_AS_TEND = 2, MJX_AS_DTCFL = 3, MJX_AS_PREV_ERR = 4,
+  MJX_AS_PREVPREV_ERR = 5, MJX_AS_N_ACCEPTED = 6, MJX_AS_N_REJECTED = 7, MJX_AS_WRITE_IDX = 8,
+  MJX_AS_STATUS = 9, MJX_AS_ERR = 10, MJX_AS_ACCEPTED = 11, MJX_AS_SLOT = 12, MJX_AS_ATTEMPTS = 13,
+  MJX_AS_DT_ATTEMPT = 14, MJX_AS_STRIDE = 16
+};
+
+long long mjx_gs_adaptive_workspace_bytes(int dtype, int B, int ny, int nx);
+
+/* Adaptive IMEX integration of B Gray-Scott instances on the periodic grid,
+ * each instance with its own accept/reject and dt sequence, in place on
+ * (u, v).  Every attempt takes one step of dt_c = min(dt, t_end - t) and two
+ * of dt_c/2 (step doubling), e = max(RMS over both interior fields of
+ * (y_new - y_full)/(atol + rtol |y_new|), 1e-10) in FP64, accepts when
+ * e <= 1 and y_new is finite, and proposes the next dt by the PI controller
+ * (reject: never grows, then halved), limited by DTCFL; status codes
+ * 0 success, 1 running, 2 max steps, 3 dt too small, 4 non-finite.
+ * scheme 0 Strang (order 2), 1 IMEX-Euler (order 1).
+ * model: device, B x 4 doubles {Du, Dv, F, k}.
+ * ctl: host, 11 doubles {atol, rtol, safety, kI, kP, dt_min, dt_max,
+ *   max_factor, min_factor, order, save_every}.
+ * t_hist, dt_hist: device, B x max_saves doubles (t_hist[b][0] = t0 set by
+ *   the caller); y_hist: NULL or device (B, 2, max_saves, ny, nx) of dtype
+ *   (slot 0 = the initial interior set by the caller).  An accepted attempt
+ *   with n_accepted % save_every == 0 stores t, dt_c and y_new at WRITE_IDX.
+ * Runs until no instance is running, or max_attempts attempts (> 0) have
+ * been made (the state then resumes with another call).  attempts_done may
+ * be NULL.  fill_ghosts != 0 fills the periodic ghost ring at the end.
+ * Replaces: moljax/core/stepping.py:1146-1371 (adaptive_integrate_imex) with
+ *   :566-608 (estimate_error_imex_doubling), moljax/core/state.py:312-343
+ *   (scaled_error_norm), moljax/core/dt_policy.py:178-268,544-610
+ *   (pid_controller_dt, handle_rejected_step, propose_dt_imex).
+ * work: mjx_gs_adaptive_workspace_bytes(dtype, B, ny, nx) bytes. */
+int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch,
+                         long long istride, int fill_ghosts, const void* lamx, const void* lamy,
+                         const void* lam2d, const double* model, const double* ctl, double* state,
+                         double* t_hist, double* dt_hist, void* y_hist, int max_saves,
+                         long long max_attempts, long long* attempts_done, void* work, void* stream);
+
 /* Non-periodic 2-D Helmholtz / Poisson solve on B instances of ny x nx
  * interior points:  kind 0 homogeneous Dirichlet (DST-I in x and y, vertex
  * grid; ny+1, nx+1 powers of two in [16, 1024]), kind 1 homogeneous Neumann

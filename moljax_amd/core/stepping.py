@@ -11,9 +11,11 @@ Models are served by kernel: ``model.metadata['mjx_kernel']`` names the fused
 path.  Models without one are rejected with NotImplementedError; there is no
 generic per-op fallback.
 """
+import ctypes
 from enum import IntEnum
-from typing import Dict, Optional, Tuple
+from typing import Dict, NamedTuple, Optional, Tuple
 
+import numpy as np
 import torch
 
 from .. import _hip
@@ -47,14 +49,23 @@ def _require_gray_scott(model, diffusivities: Dict[str, float]):
         raise ValueError(f"diffusivities name unknown fields {sorted(extra)}")
 
 
-def _gs_rows(model, dt: float, diffusivities: Dict[str, float], B: int):
+GS_STRANG, GS_EULER, GS_SSPRK2 = 0, 1, 2
+
+
+def _gs_coeffs(model, diffusivities: Dict[str, float]):
+    """(Du, Dv, F, k) with diffusion switched off for D <= 1e-14 as
+    apply_diffusion_inverse_fft does (fft_solvers.py:436-441)."""
     Du = float(diffusivities.get("u", 0.0))
     Dv = float(diffusivities.get("v", 0.0))
-    bu = 0.5 * dt * Du if Du > 1e-14 else 0.0
-    bv = 0.5 * dt * Dv if Dv > 1e-14 else 0.0
-    F = float(model.params.get("F", 0.04))
-    k = float(model.params.get("k", 0.06))
-    return [[bu, bv, 0.0, 0.0, F, k, float(dt)]] * B
+    return (Du if Du > 1e-14 else 0.0, Dv if Dv > 1e-14 else 0.0,
+            float(model.params.get("F", 0.04)), float(model.params.get("k", 0.06)))
+
+
+def _gs_rows(model, dt: float, diffusivities: Dict[str, float], B: int, scheme: int = GS_STRANG):
+    """Instance rows: Strang solves with dt/2, Euler and SSPRK2 with dt."""
+    Du, Dv, F, k = _gs_coeffs(model, diffusivities)
+    h = 0.5 * dt if scheme == GS_STRANG else float(dt)
+    return [[h * Du, h * Dv, 0.0, 0.0, F, k, float(dt)]] * B
 
 
 def gs_imex_strang_inplace(u: torch.Tensor, v: torch.Tensor, lam, rows, n_steps: int,
@@ -63,6 +74,13 @@ def gs_imex_strang_inplace(u: torch.Tensor, v: torch.Tensor, lam, rows, n_steps:
     IMEX-Strang steps.  ``rows`` is a per-instance list of
     [beta_u, beta_v, 0, 0, F, k, dt] (beta = dt/2 * D) or an already built
     (B, 8) float64 device table; ``lam`` a symbol from laplacian_symbol_2d."""
+    gs_imex_inplace(u, v, lam, rows, n_steps, GS_STRANG, fill_ghosts, work)
+
+
+def gs_imex_inplace(u: torch.Tensor, v: torch.Tensor, lam, rows, n_steps: int, scheme: int = GS_STRANG,
+                    fill_ghosts: bool = True, work: Optional[torch.Tensor] = None):
+    """As gs_imex_strang_inplace for scheme GS_STRANG, GS_EULER or
+    GS_SSPRK2 (rows from _gs_rows with the same scheme)."""
     from .fft_solvers import _tables
     tu, pitch, istr, B = view2d(u)
     tv, pitch_v, istr_v, _ = view2d(v)
@@ -75,11 +93,11 @@ def gs_imex_strang_inplace(u: torch.Tensor, v: torch.Tensor, lam, rows, n_steps:
         raise ValueError("one parameter row per instance is required")
     if work is None:
         work = spectral_work(tu.dtype, B, ny, nx, tu.device)
-    rc = _hip.lib().mjx_gs_imex_strang(
-        _hip.dtype_code(tu.dtype), B, ny, nx, _hip.ptr(tu[:, 1:, 1:]), _hip.ptr(tv[:, 1:, 1:]), pitch, istr,
-        int(bool(fill_ghosts)), _hip.ptr(tabs.lamx), _hip.ptr(tabs.lamy), _hip.ptr(tabs.lam2d), _hip.ptr(inst),
-        int(n_steps), _hip.ptr(work), _hip.stream_ptr())
-    _hip.check(rc, "mjx_gs_imex_strang")
+    rc = _hip.lib().mjx_gs_imex(
+        _hip.dtype_code(tu.dtype), int(scheme), B, ny, nx, _hip.ptr(tu[:, 1:, 1:]), _hip.ptr(tv[:, 1:, 1:]), pitch,
+        istr, int(bool(fill_ghosts)), _hip.ptr(tabs.lamx), _hip.ptr(tabs.lamy), _hip.ptr(tabs.lam2d),
+        _hip.ptr(inst), int(n_steps), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "mjx_gs_imex")
 
 
 def _copy_state(y: StateDict):
@@ -103,6 +121,129 @@ def imex_strang_step(model, y: StateDict, t: float, dt: float, fft_cache,
     return {"u": u, "v": v}
 
 
+def imex_euler_step(model, y: StateDict, t: float, dt: float, fft_cache,
+                    diffusivities: Dict[str, float]) -> StateDict:
+    """stepping.py:317-367: y_new = (I - dt D lap)^-1 (y + dt N(y))."""
+    _require_gray_scott(model, diffusivities)
+    u, v = _copy_state(y)
+    B = view2d(u)[3]
+    gs_imex_inplace(u, v, fft_cache.laplacian_symbol, _gs_rows(model, dt, diffusivities, B, GS_EULER), 1, GS_EULER)
+    return {"u": u, "v": v}
+
+
+def imex_ssprk2_step(model, y: StateDict, t: float, dt: float, fft_cache,
+                     diffusivities: Dict[str, float]) -> StateDict:
+    """stepping.py:438-495: y_new = y/2 + E(E(y))/2, E the IMEX-Euler map."""
+    _require_gray_scott(model, diffusivities)
+    u, v = _copy_state(y)
+    B = view2d(u)[3]
+    gs_imex_inplace(u, v, fft_cache.laplacian_symbol, _gs_rows(model, dt, diffusivities, B, GS_SSPRK2), 1,
+                    GS_SSPRK2)
+    return {"u": u, "v": v}
+
+
+def estimate_error_imex_doubling(model, y: StateDict, t: float, dt: float, fft_cache,
+                                 diffusivities: Dict[str, float],
+                                 use_strang: bool = True) -> Tuple[StateDict, StateDict]:
+    """stepping.py:566-608: (two half steps, their difference from one full
+    step)."""
+    step = imex_strang_step if use_strang else imex_euler_step
+    y_full = step(model, y, t, dt, fft_cache, diffusivities)
+    y_half = step(model, step(model, y, t, dt / 2.0, fft_cache, diffusivities), t + dt / 2.0, dt / 2.0, fft_cache,
+                  diffusivities)
+    return y_half, {n: y_half[n] - y_full[n] for n in y_half}
+
+
+class AdaptiveResult(NamedTuple):
+    """stepping.py:710-720.  For a batch of instances (leading dimension B
+    on the state) every field gains that leading dimension."""
+    t_final: object
+    y_final: StateDict
+    t_history: torch.Tensor
+    y_history: Dict[str, torch.Tensor]
+    dt_history: torch.Tensor
+    n_steps: object
+    n_accepted: object
+    n_rejected: object
+    status: object
+
+
+def adaptive_integrate_imex(model, y0: StateDict, t0: float, t_end: float, dt0: float, fft_cache,
+                            diffusivities: Dict[str, float], use_strang: bool = True, max_steps: int = 10000,
+                            cfl_params=None, pid_params=None, save_every: int = 1, save_states: bool = True,
+                            max_attempts: int = 0) -> AdaptiveResult:
+    """stepping.py:1146-1371 on the device: step-doubling IMEX (Strang,
+    order 2, or Euler, order 1) with the PI controller running per instance
+    inside mjx_gs_imex_adaptive.
+
+    y0 holds padded fields (ny+2, nx+2) or a batch (B, ny+2, nx+2) of
+    independent instances, each with its own dt sequence.  y_history holds
+    interior states (max_steps // save_every + 1 slots) like the reference;
+    save_states=False skips it (t_history and dt_history are always kept).
+    max_attempts > 0 bounds the number of attempts (status stays RUNNING
+    when it cuts the run short)."""
+    from .dt_policy import CFLParams, PIDParams, imex_cfl_dt
+    _require_gray_scott(model, diffusivities)
+    if save_every < 1:
+        raise ValueError("save_every must be >= 1")
+    cfl = cfl_params or CFLParams()
+    pid = pid_params or PIDParams()
+    order = 2 if use_strang else 1
+    scheme = GS_STRANG if use_strang else GS_EULER
+    u, v = _copy_state(y0)
+    batched = u.dim() == 3
+    tu, pitch, istr, B = view2d(u)
+    tv = view2d(v)[0]
+    ny, nx = tu.shape[-2] - 2, tu.shape[-1] - 2
+    dev, dt_ = u.device, u.dtype
+    dt_cfl = imex_cfl_dt(model.grid, model.params, cfl)
+    dt_init = min(float(dt0), dt_cfl)
+    max_saves = max_steps // save_every + 1
+    lib = _hip.lib()
+    st = np.zeros((B, 16))
+    st[:, 0], st[:, 1], st[:, 2], st[:, 3] = t0, dt_init, t_end, dt_cfl
+    st[:, 4] = st[:, 5] = 1.0
+    st[:, 8], st[:, 9] = 1.0, 1.0
+    state = torch.from_numpy(st).to(dev)
+    coeffs = torch.tensor([_gs_coeffs(model, diffusivities)] * B, dtype=torch.float64, device=dev)
+    t_hist = torch.zeros((B, max_saves), dtype=torch.float64, device=dev)
+    t_hist[:, 0] = t0
+    dt_hist = torch.zeros_like(t_hist)
+    y_hist = None
+    if save_states:
+        y_hist = torch.zeros((B, 2, max_saves, ny, nx), dtype=dt_, device=dev)
+        y_hist[:, 0, 0] = tu[:, 1:-1, 1:-1]
+        y_hist[:, 1, 0] = tv[:, 1:-1, 1:-1]
+    from .fft_solvers import _tables
+    tabs = _tables(fft_cache.laplacian_symbol, dt_)
+    work = torch.empty(lib.mjx_gs_adaptive_workspace_bytes(_hip.dtype_code(dt_), B, ny, nx), dtype=torch.uint8,
+                       device=dev)
+    ctl = (ctypes.c_double * 11)(pid.atol, pid.rtol, pid.safety, pid.kI, pid.kP, pid.dt_min, pid.dt_max,
+                                 pid.max_factor, pid.min_factor, float(order), float(save_every))
+    done = ctypes.c_longlong(0)
+    rc = lib.mjx_gs_imex_adaptive(
+        _hip.dtype_code(dt_), scheme, B, ny, nx, _hip.ptr(tu[:, 1:, 1:]), _hip.ptr(tv[:, 1:, 1:]), pitch, istr, 1,
+        _hip.ptr(tabs.lamx), _hip.ptr(tabs.lamy), _hip.ptr(tabs.lam2d), _hip.ptr(coeffs), ctl, _hip.ptr(state),
+        _hip.ptr(t_hist), _hip.ptr(dt_hist), _hip.ptr(y_hist), max_saves, int(max_attempts), ctypes.byref(done),
+        _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "mjx_gs_imex_adaptive")
+    s = state.cpu().numpy()
+    if y_hist is not None:
+        yh = {"u": y_hist[:, 0], "v": y_hist[:, 1]}
+    else:
+        yh = {}
+    if not batched:
+        yh = {k: h[0] for k, h in yh.items()}
+        return AdaptiveResult(t_final=float(s[0, 0]), y_final={"u": u, "v": v}, t_history=t_hist[0], y_history=yh,
+                              dt_history=dt_hist[0], n_steps=int(s[0, 8]), n_accepted=int(s[0, 6]),
+                              n_rejected=int(s[0, 7]), status=int(s[0, 9]))
+    return AdaptiveResult(t_final=torch.from_numpy(s[:, 0].copy()), y_final={"u": u, "v": v}, t_history=t_hist,
+                          y_history=yh, dt_history=dt_hist, n_steps=torch.from_numpy(s[:, 8].astype(np.int64)),
+                          n_accepted=torch.from_numpy(s[:, 6].astype(np.int64)),
+                          n_rejected=torch.from_numpy(s[:, 7].astype(np.int64)),
+                          status=torch.from_numpy(s[:, 9].astype(np.int64)))
+
+
 def integrate_imex_fixed_dt(model, y0: StateDict, t0: float, t_end: float, dt: float, fft_cache,
                             diffusivities: Dict[str, float], use_strang: bool = True, save_every: int = 1,
                             n_steps: Optional[int] = None) -> Tuple[torch.Tensor, Dict[str, torch.Tensor], StateDict]:
@@ -112,15 +253,14 @@ def integrate_imex_fixed_dt(model, y0: StateDict, t0: float, t_end: float, dt: f
     kept) unless ``n_steps`` is given.  Returns (t_history, y_history, y_final)
     where the histories hold the padded state after every ``save_every``-th
     step (the reference stacks every step; save_every=1 gives its shapes)."""
-    if not use_strang:
-        raise NotImplementedError("IMEX-Euler is not on the MI355X path; use use_strang=True")
     _require_gray_scott(model, diffusivities)
+    scheme = GS_STRANG if use_strang else GS_EULER
     if save_every < 1:
         raise ValueError("save_every must be >= 1")
     n = int((t_end - t0) / dt) + 1 if n_steps is None else int(n_steps)
     u, v = _copy_state(y0)
     tu, _, _, B = view2d(u)
-    rows = inst_table(_gs_rows(model, dt, diffusivities, B), u.device)
+    rows = inst_table(_gs_rows(model, dt, diffusivities, B, scheme), u.device)
     work = spectral_work(u.dtype, B, tu.shape[-2] - 2, tu.shape[-1] - 2, u.device)
     n_saves = n // save_every
     t_hist = torch.empty(n_saves, dtype=torch.float64)
@@ -129,7 +269,7 @@ def integrate_imex_fixed_dt(model, y0: StateDict, t0: float, t_end: float, dt: f
     done, k = 0, 0
     while done < n:
         chunk = min(save_every, n - done)
-        gs_imex_strang_inplace(u, v, fft_cache.laplacian_symbol, rows, chunk, work=work)
+        gs_imex_inplace(u, v, fft_cache.laplacian_symbol, rows, chunk, scheme, work=work)
         done += chunk
         if chunk == save_every:
             hist_u[k].copy_(u)

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "mjx_common.h"
+#include "gs_chain.h"
 #include "spectral_api.h"
 
 namespace mjx {
@@ -144,6 +145,34 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                             (const float*)lamy, (const float*)lam2d, ip, n_steps, (cpx<float>*)work, st);
   return gs_imex_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)lamx,
                            (const double*)lamy, (const double*)lam2d, ip, n_steps, (cpx<double>*)work, st);
+}
+
+int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d, const double* inst,
+                int n_steps, void* work, void* stream) {
+  if (scheme == GS_STRANG)
+    return mjx_gs_imex_strang(dtype, B, ny, nx, u, v, pitch, istride, fill_ghosts, lamx, lamy, lam2d, inst,
+                              n_steps, work, stream);
+  if (scheme != GS_EULER && scheme != GS_SSPRK2) return fail(MJX_E_ARG, "scheme must be 0 (Strang), 1 (Euler) or 2 (SSPRK2)");
+  int rc = check_grid(dtype, B, ny, nx);
+  if (rc) return rc;
+  if (!u || !v || !inst || !work) return fail(MJX_E_ARG, "null pointer argument");
+  if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
+  if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
+  if (pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (fill_ghosts && pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= nx + 2");
+  if (n_steps == 0) return MJX_OK;
+  rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (dtype == MJX_F32)
+    return gs_chain_t<float>(scheme, B, ny, nx, (float*)u, (float*)v, pitch, istride, (float*)u, (float*)v, pitch,
+                             istride, fill_ghosts, (const float*)lamx, (const float*)lamy, (const float*)lam2d, ip,
+                             n_steps, (cpx<float>*)work, st);
+  return gs_chain_t<double>(scheme, B, ny, nx, (double*)u, (double*)v, pitch, istride, (double*)u, (double*)v, pitch,
+                            istride, fill_ghosts, (const double*)lamx, (const double*)lamy, (const double*)lam2d, ip,
+                            n_steps, (cpx<double>*)work, st);
 }
 
 int mjx_kernel_timing_enable(int on) {

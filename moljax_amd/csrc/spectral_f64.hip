@@ -7,15 +7,15 @@ namespace mjx {
 using T = double;
 
 int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
-                      hipStream_t st) {
-  return Launch<T>::row_fwd_pack(nx, B, ny, a, b, pitch, istride, W, st);
+                      hipStream_t st, double in_scale) {
+  return Launch<T>::row_fwd_pack(nx, B, ny, a, b, pitch, istride, W, st, (T)in_scale);
 }
 int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
-                        int ghosts, hipStream_t st) {
-  return Launch<T>::row_inv_unpack(nx, B, ny, W, a, b, pitch, istride, ghosts, st);
+                        int ghosts, hipStream_t st, double keep) {
+  return Launch<T>::row_inv_unpack(nx, B, ny, W, a, b, pitch, istride, ghosts, st, (T)keep);
 }
-int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
-  return Launch<T>::row_gs(nx, B, ny, W, ip, st);
+int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn) {
+  return Launch<T>::row_gs(nx, B, ny, W, ip, st, rxn);
 }
 int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
                    const InstParams* ip, int nrep, hipStream_t st) {

@@ -22,6 +22,7 @@
 
 #include "mjx_common.h"
 #include "mjx_fft.h"
+#include "spectral_api.h"
 
 namespace mjx {
 
@@ -188,10 +189,12 @@ __device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, cpx
 }
 
 // Packed x-spectra in W -> real fields (a, b), optional periodic ghost ring.
+// keep != 0 blends with the destination's previous content:
+// out = keep * old + (1 - keep) * new (the averaging stage of IMEX-SSPRK2).
 template <typename T, int NX, int TR>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
     const cpx<T>* __restrict__ W, T* __restrict__ a, T* __restrict__ b, long long pitch, long long istride, int ny,
-    int rows_per_inst_tiles, int ghosts, const cpx<T>* __restrict__ tw) {
+    int rows_per_inst_tiles, int ghosts, T keep, const cpx<T>* __restrict__ tw) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -207,6 +210,15 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
   line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
   T* ra = a + inst * istride + (long long)y * pitch;
   T* rb = b ? b + inst * istride + (long long)y * pitch : nullptr;
+  if (keep != (T)0) {
+    const T fresh = (T)1 - keep;
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int x = j + C::TN * m;
+      v[m].re = keep * ra[x] + fresh * v[m].re;
+      if (rb) v[m].im = keep * rb[x] + fresh * v[m].im;
+    }
+  }
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
     const int x = j + C::TN * m;
@@ -244,11 +256,13 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
   }
 }
 
-// Fused Gray-Scott row pass: packed spectra -> physical (u, v) -> Heun
+// Fused Gray-Scott row pass: packed spectra -> physical (u, v) -> explicit
 // reaction step -> packed spectra, in place.
-// Heun exactly as moljax/core/stepping.py:414-425:
+// RXN = RXN_HEUN (IMEX-Strang), as moljax/core/stepping.py:414-425:
 //   R1 = R(y*), y~ = y* + dt R1, R2 = R(y~), y** = y* + (dt/2)(R1 + R2).
-template <typename T, int NX, int TR>
+// RXN = RXN_EULER (IMEX-Euler / SSPRK2 stages, stepping.py:351-360,472-489):
+//   y + dt R(y).
+template <typename T, int NX, int TR, int RXN>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W,
                                                                   const InstParams* __restrict__ ip, int ny,
                                                                   int rows_per_inst_tiles,
@@ -275,8 +289,12 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
     T r1u, r1v, r2u, r2v;
     gs_react(u0, v0, F, kk, r1u, r1v);
     const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
-    gs_react(ut, vt, F, kk, r2u, r2v);
-    v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
+    if constexpr (RXN == RXN_EULER) {
+      v[m] = mk(ut, vt);
+    } else {
+      gs_react(ut, vt, F, kk, r2u, r2v);
+      v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
+    }
   }
   line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
@@ -401,16 +419,20 @@ struct RowLaunch {
     return check_launch("row_fwd_pack");
   }
   static int inv_unpack(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
-                        int ghosts, hipStream_t st) {
+                        int ghosts, T keep, hipStream_t st) {
     const int tiles = ny / L;
     hipLaunchKernelGGL((k_row_inv_unpack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, a, b,
-                       pitch, istride, ny, tiles, ghosts, twiddles<T>());
+                       pitch, istride, ny, tiles, ghosts, keep, twiddles<T>());
     return check_launch("row_inv_unpack");
   }
-  static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
+  static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, hipStream_t st) {
     const int tiles = ny / L;
-    hipLaunchKernelGGL((k_row_gs<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip, ny, tiles,
-                       twiddles<T>());
+    if (rxn == RXN_EULER)
+      hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
+                         ny, tiles, twiddles<T>());
+    else
+      hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
+                         ny, tiles, twiddles<T>());
     return check_launch("row_gs");
   }
 };
@@ -473,8 +495,8 @@ struct Launch {
     template <int L>
     struct F {
       static int run(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride, int ghosts,
-                     hipStream_t st) {
-        return RowLaunch<T, N, L>::inv_unpack(B, ny, W, a, b, pitch, istride, ghosts, st);
+                     T keep, hipStream_t st) {
+        return RowLaunch<T, N, L>::inv_unpack(B, ny, W, a, b, pitch, istride, ghosts, keep, st);
       }
     };
   };
@@ -482,8 +504,8 @@ struct Launch {
   struct Gs {
     template <int L>
     struct F {
-      static int run(int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
-        return RowLaunch<T, N, L>::gs(B, ny, W, ip, st);
+      static int run(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, hipStream_t st) {
+        return RowLaunch<T, N, L>::gs(B, ny, W, ip, rxn, st);
       }
     };
   };
@@ -511,23 +533,23 @@ struct Launch {
     }
   }
   static int row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,
-                            long long istride, int ghosts, hipStream_t st) {
+                            long long istride, int ghosts, hipStream_t st, T keep = (T)0) {
     switch (nx) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
     return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, a, b, pitch, \
-                                                   istride, ghosts, st);
+                                                   istride, ghosts, keep, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
         return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
     }
   }
-  static int row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st) {
+  static int row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = RXN_HEUN) {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, ip, st);
+    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, ip, rxn, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:

@@ -1,0 +1,184 @@
+"""Adaptive IMEX driver for Gray-Scott, float64 NumPy restatement (test
+infrastructure).
+
+Follows adaptive_integrate_imex (moljax/core/stepping.py:1146-1371) with
+step-doubling error estimation (stepping.py:566-608), scaled_error_norm
+(moljax/core/state.py:312-343), and the PI controller of
+moljax/core/dt_policy.py (CFLParams :28-51, PIDParams :54-83,
+pid_controller_dt :178-219, handle_rejected_step :222-268, imex_cfl_dt
+:430-502, propose_dt_imex :544-610).  Scalars are Python floats (IEEE
+float64), so the accept/reject sequence of an FP64 state is reproduced
+decision for decision.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from . import gray_scott as ogs
+
+SUCCESS, RUNNING, MAX_STEPS_REACHED, DT_TOO_SMALL, NON_FINITE_VALUES = 0, 1, 2, 3, 4
+
+
+@dataclass(frozen=True)
+class CFLParams:
+    """dt_policy.py:28-51."""
+    safety: float = 0.9
+    dt_min: float = 1e-12
+    dt_max: float = 1.0
+    cfl_advection: float = 0.5
+    cfl_diffusion: float = 0.25
+    cfl_wave: float = 0.5
+
+
+@dataclass(frozen=True)
+class PIDParams:
+    """dt_policy.py:54-83."""
+    safety: float = 0.9
+    kI: float = 0.7
+    kP: float = 0.4
+    dt_min: float = 1e-12
+    dt_max: float = 1.0
+    max_factor: float = 5.0
+    min_factor: float = 0.2
+    atol: float = 1e-6
+    rtol: float = 1e-4
+
+
+def _nanmax(a, b):
+    """jnp.maximum: NaN propagates."""
+    if math.isnan(a) or math.isnan(b):
+        return math.nan
+    return max(a, b)
+
+
+def _nanmin(a, b):
+    if math.isnan(a) or math.isnan(b):
+        return math.nan
+    return min(a, b)
+
+
+def _clip(x, lo, hi):
+    """jnp.clip = minimum(maximum(x, lo), hi)."""
+    return _nanmin(_nanmax(x, lo), hi)
+
+
+def imex_cfl_dt(params, cfl: CFLParams, min_dx: float) -> float:
+    """dt_policy.py:430-502 (advection, wave and reaction limits; diffusion
+    is implicit and does not limit)."""
+    dt = cfl.dt_max
+    v_max = abs(params.get("vx", 0.0)) + abs(params.get("vy", 0.0))
+    dt = min(dt, cfl.cfl_advection * min_dx / v_max if v_max > 1e-14 else cfl.dt_max)
+    K, rho = params.get("K", 0.0), params.get("rho", 1.0)
+    c = max(params.get("wave_speed", 0.0), math.sqrt(K / rho) if K > 1e-14 else 0.0)
+    dt = min(dt, cfl.cfl_wave * min_dx / c if c > 1e-14 else cfl.dt_max)
+    rate = max(params.get("F", 0.0) + params.get("k", 0.0), params.get("r", 0.0))
+    dt = min(dt, 0.5 / rate if rate > 1e-14 else cfl.dt_max)
+    return _clip(cfl.safety * dt, cfl.dt_min, cfl.dt_max)
+
+
+def pid_controller_dt(dt_old, err, prev_err, pid: PIDParams, order):
+    """dt_policy.py:178-219; returns (dt_new, new prev_err)."""
+    kI, kP = pid.kI / order, pid.kP / order
+    e = _nanmax(err, 1e-10)
+    pe = _nanmax(prev_err, 1e-10)
+    factor = (1.0 / e) ** kI * (pe / e) ** kP
+    factor = _clip(factor * pid.safety, pid.min_factor, pid.max_factor)
+    return _clip(dt_old * factor, pid.dt_min, pid.dt_max), err
+
+
+def handle_rejected_step(dt_old, err, pid: PIDParams, order):
+    """dt_policy.py:222-268 (never grows)."""
+    kI = pid.kI / order
+    e = _nanmax(err, 1.0)
+    factor = _clip(pid.safety * (1.0 / e) ** kI, pid.min_factor, 1.0)
+    return _clip(dt_old * factor, pid.dt_min, pid.dt_max)
+
+
+def scaled_error_norm(err, y, atol, rtol, ng=1):
+    """state.py:312-343: sqrt(mean over interiors of all fields of
+    (err / (atol + rtol |y|))^2)."""
+    tot, n = 0.0, 0
+    for name in err:
+        e = err[name][ng:-ng, ng:-ng]
+        s = atol + rtol * np.abs(y[name][ng:-ng, ng:-ng])
+        tot += float(np.sum((e / s) ** 2))
+        n += e.size
+    return math.sqrt(tot / n)
+
+
+@dataclass
+class AdaptiveResult:
+    t_final: float
+    y_final: dict
+    t_history: np.ndarray
+    dt_history: np.ndarray
+    y_saved: List[dict]
+    n_steps: int
+    n_accepted: int
+    n_rejected: int
+    status: int
+    attempts: List[tuple] = field(default_factory=list)  # (t, dt_clamped, err_ratio, accepted)
+
+
+def adaptive_integrate_imex(y0, t0, t_end, dt0, F, k, Du, Dv, lam, min_dx, use_strang=True, max_steps=10000,
+                            cfl: CFLParams = None, pid: PIDParams = None, save_every=1, ng=1,
+                            keep_states=True, max_attempts=None):
+    """stepping.py:1146-1371 for the Gray-Scott model (params F, k)."""
+    cfl = cfl or CFLParams()
+    pid = pid or PIDParams()
+    order = 2 if use_strang else 1
+    step = ogs.imex_strang_step if use_strang else ogs.imex_euler_step
+    dt_cfl = imex_cfl_dt({"F": F, "k": k}, cfl, min_dx)
+    dt = _nanmin(float(dt0), dt_cfl)
+    max_saves = max_steps // save_every + 1
+    t_hist = np.zeros(max_saves)
+    dt_hist = np.zeros(max_saves)
+    t_hist[0] = t0
+    saved = [{n: v[ng:-ng, ng:-ng].copy() for n, v in y0.items()}] if keep_states else []
+    y = {n: v.copy() for n, v in y0.items()}
+    t, prev_err, write_idx, n_acc, n_rej, status = float(t0), 1.0, 1, 0, 0, RUNNING
+    attempts = []
+    while status == RUNNING and t < t_end and write_idx < max_saves:
+        if max_attempts is not None and len(attempts) >= max_attempts:
+            break
+        dtc = _nanmin(dt, t_end - t)
+        y_full = step(y, dtc, F, k, Du, Dv, lam, ng)
+        y_h = step(y, dtc / 2.0, F, k, Du, Dv, lam, ng)
+        y_new = step(y_h, dtc / 2.0, F, k, Du, Dv, lam, ng)
+        err = {n: y_new[n] - y_full[n] for n in y_new}
+        e = _nanmax(scaled_error_norm(err, y_new, pid.atol, pid.rtol, ng), 1e-10)
+        finite = all(bool(np.all(np.isfinite(v))) for v in y_new.values())
+        accept = (e <= 1.0) and finite
+        if e <= 1.0:
+            dt_ad, new_prev = pid_controller_dt(dtc, e, prev_err, pid, order)
+        else:
+            dt_ad, new_prev = handle_rejected_step(dtc, e, pid, order), prev_err
+        dt_new = _clip(_nanmin(dt_ad, dt_cfl), pid.dt_min, pid.dt_max)
+        attempts.append((t, dtc, e, accept))
+        if accept:
+            if n_acc % save_every == 0:
+                t_hist[write_idx] = t + dtc
+                if keep_states:
+                    saved.append({n: v[ng:-ng, ng:-ng].copy() for n, v in y_new.items()})
+                dt_hist[write_idx - 1] = dtc
+                write_idx += 1
+            t += dtc
+            y = y_new
+            dt = dt_new
+            prev_err = new_prev
+            n_acc += 1
+        else:
+            dt = _nanmax(dt_new * 0.5, pid.dt_min)
+            if dt <= pid.dt_min * 1.1:
+                status = DT_TOO_SMALL
+            n_rej += 1
+        if not finite:
+            status = NON_FINITE_VALUES
+    if t >= t_end:
+        status = SUCCESS
+    elif write_idx >= max_saves:
+        status = MAX_STEPS_REACHED
+    return AdaptiveResult(t_final=t, y_final=y, t_history=t_hist, dt_history=dt_hist, y_saved=saved,
+                          n_steps=write_idx, n_accepted=n_acc, n_rejected=n_rej, status=status, attempts=attempts)

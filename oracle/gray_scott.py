@@ -54,6 +54,24 @@ def imex_strang_step(y, dt, F, k, Du, Dv, lam, ng=1):
     return apply_bcs(y_new, ng)
 
 
+def imex_euler_step(y, dt, F, k, Du, Dv, lam, ng=1):
+    """stepping.py:317-367: y_new = (I - dt D lap)^-1 (y + dt N(y))."""
+    diff = {"u": Du, "v": Dv}
+    y = apply_bcs(y, ng)
+    N = reaction(y, F, k)
+    rhs = {n: y[n] + dt * N[n] for n in y}
+    return apply_bcs(spectral.apply_diffusion_inverse_fft(rhs, ng, dt, diff, lam), ng)
+
+
+def imex_ssprk2_step(y, dt, F, k, Du, Dv, lam, ng=1):
+    """stepping.py:438-495: y1 = E(y), y2 = E(y1), y_new = y/2 + y2/2 with E
+    the IMEX-Euler map above."""
+    y = apply_bcs(y, ng)
+    y1 = imex_euler_step(y, dt, F, k, Du, Dv, lam, ng)
+    y2 = imex_euler_step(y1, dt, F, k, Du, Dv, lam, ng)
+    return apply_bcs({n: 0.5 * y[n] + 0.5 * y2[n] for n in y}, ng)
+
+
 def integrate(y0, dt, n_steps, F, k, Du, Dv, lam, ng=1, record=None):
     """Fixed-dt loop (stepping.py:1414-1434 body).  ``record`` is an optional
     set of step counts at which the state is saved."""
