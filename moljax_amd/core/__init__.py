@@ -13,4 +13,18 @@ from .fft_solvers import (FFTCache1D, FFTCache2D, build_wavenumbers_1d, build_wa
                           embed_interior, solve_diffusion_fft_field, apply_diffusion_inverse_fft, fft_cn_step)
 from .model import (MOLModel, create_gray_scott_model, create_gray_scott_periodic_fft, get_fft_cache,
                     get_diffusivities)
-from .stepping import IntegratorType, imex_strang_step, integrate_imex_fixed_dt
+from .stepping import (IntegratorType, AdaptiveResult, imex_strang_step, imex_euler_step, imex_ssprk2_step,
+                       estimate_error_imex_doubling, integrate_imex_fixed_dt, adaptive_integrate_imex)
+from .dt_policy import (CFLParams, PIDParams, ControllerState, create_initial_controller_state, pid_controller_dt,
+                        handle_rejected_step, heisenberg_cfl_dt, imex_cfl_dt, implicit_robustness_dt, propose_dt,
+                        propose_dt_imex, compute_error_order)
+from .fft_nonperiodic import (BCType as NonperiodicBCType, dst_I_fast, idst_I_fast, solve_helmholtz_dirichlet,
+                              solve_helmholtz_neumann, solve_poisson_dirichlet, solve_poisson_neumann)
+from .fft_operators import DiffusionOperator
+from .fft_integrators import etdrk4_step, etd_integrate
+from .newton_krylov import NKParams, NKStats, NKResult, newton_krylov_solve, create_implicit_residual
+from .preconditioners import (PrecondContext, IdentityPreconditioner, DiffusionPreconditioner,
+                              FFTDiffusionPreconditioner, DSTDiffusionPreconditioner,
+                              create_gray_scott_preconditioner, create_gray_scott_fft_preconditioner)
+from .reaction_diffusion import (ReactionDiffusionModel, create_reaction_diffusion_model, be_step, cn_step,
+                                 euler_step)
