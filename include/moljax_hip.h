@@ -105,6 +105,20 @@ int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, 
                 long long istride, int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d,
                 const double* inst, int n_steps, void* work, void* stream);
 
+/* Explicit steps of the periodic Gray-Scott model on the 5-point FD
+ * right-hand side F = D lap_h y + R(y), n_steps in place on (u, v):
+ * method 0 forward Euler, 1 SSPRK3 (Shu-Osher), 2 classical RK4, each stage
+ * one fused kernel (stencil of both fields + reaction + stage update).
+ * inst rows: alpha_a = Du, alpha_b = Dv, F, k, dt.  Any grid size.
+ * Replaces: moljax/core/stepping.py:60-167 (euler_step, ssprk3_step,
+ * rk4_step) with model.rhs of moljax/core/model.py:76-118 for the model of
+ * model.py:213-292 (periodic), and integrate_fixed_dt (stepping.py:1024-1143).
+ * work: mjx_gs_explicit_workspace_bytes(dtype, B, ny, nx) bytes. */
+long long mjx_gs_explicit_workspace_bytes(int dtype, int B, int ny, int nx);
+int mjx_gs_explicit(int dtype, int method, int B, int ny, int nx, void* u, void* v, long long pitch,
+                    long long istride, int fill_ghosts, double dy, double dx, const double* inst, int n_steps,
+                    void* work, void* stream);
+
 /* Per-instance adaptive-integrator state: MJX_AS_STRIDE doubles per
  * instance, device memory.  The caller initialises T = t0,
  * DT = min(dt0, dt_cfl), TEND, DTCFL, PREV_ERR = PREVPREV_ERR = 1,
@@ -116,7 +130,7 @@ enum {
   MJX_AS_DT_ATTEMPT = 14, MJX_AS_STRIDE = 16
 };
 
-long long mjx_gs_adaptive_workspace_bytes(int dtype, int B, int ny, int nx);
+long long mjx_gs_adaptive_workspace_bytes(int dtype, int scheme, int B, int ny, int nx);
 
 /* Adaptive IMEX integration of B Gray-Scott instances on the periodic grid,
  * each instance with its own accept/reject and dt sequence, in place on
@@ -126,10 +140,14 @@ long long mjx_gs_adaptive_workspace_bytes(int dtype, int B, int ny, int nx);
  * e <= 1 and y_new is finite, and proposes the next dt by the PI controller
  * (reject: never grows, then halved), limited by DTCFL; status codes
  * 0 success, 1 running, 2 max steps, 3 dt too small, 4 non-finite.
- * scheme 0 Strang (order 2), 1 IMEX-Euler (order 1).
+ * scheme 0 IMEX-Strang, 1 IMEX-Euler (spectral chains, power-of-two grids),
+ * 3 forward Euler, 4 SSPRK3, 5 RK4 (explicit on the 5-point FD right-hand
+ * side, periodic, any grid size; the explicit branch of stepping.py:723-1021).
  * model: device, B x 4 doubles {Du, Dv, F, k}.
- * ctl: host, 11 doubles {atol, rtol, safety, kI, kP, dt_min, dt_max,
- *   max_factor, min_factor, order, save_every}.
+ * ctl: host, 13 doubles {atol, rtol, safety, kI, kP, dt_min, dt_max,
+ *   max_factor, min_factor, order, save_every, dy, dx}; order is the error
+ *   order (compute_error_order, dt_policy.py:411-423), DTCFL carries
+ *   imex_cfl_dt for IMEX schemes and heisenberg_cfl_dt for explicit ones.
  * t_hist, dt_hist: device, B x max_saves doubles (t_hist[b][0] = t0 set by
  *   the caller); y_hist: NULL or device (B, 2, max_saves, ny, nx) of dtype
  *   (slot 0 = the initial interior set by the caller).  An accepted attempt
@@ -141,12 +159,33 @@ long long mjx_gs_adaptive_workspace_bytes(int dtype, int B, int ny, int nx);
  *   :566-608 (estimate_error_imex_doubling), moljax/core/state.py:312-343
  *   (scaled_error_norm), moljax/core/dt_policy.py:178-268,544-610
  *   (pid_controller_dt, handle_rejected_step, propose_dt_imex).
- * work: mjx_gs_adaptive_workspace_bytes(dtype, B, ny, nx) bytes. */
+ * work: mjx_gs_adaptive_workspace_bytes(dtype, scheme, B, ny, nx) bytes. */
 int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch,
                          long long istride, int fill_ghosts, const void* lamx, const void* lamy,
                          const void* lam2d, const double* model, const double* ctl, double* state,
                          double* t_hist, double* dt_hist, void* y_hist, int max_saves,
                          long long max_attempts, long long* attempts_done, void* work, void* stream);
+
+/* Reductions over B strided (ny, nx) interiors, FP64 accumulation in a
+ * fixed order (bitwise reproducible), one result per instance in out[b]
+ * (device doubles):
+ *   MJX_RED_SUMSQ      sum a^2                      (tree_norm2 before sqrt)
+ *   MJX_RED_DOT        sum a b                      (tree_vdot)
+ *   MJX_RED_MAXABS     max |a|, NaN propagates      (tree_norm_inf)
+ *   MJX_RED_SCALED_SQ  sum (a / (atol + rtol |b|))^2 (scaled_error_norm
+ *                      before mean and sqrt)
+ *   MJX_RED_NONFINITE  count of non-finite a        (is_finite)
+ * accumulate != 0 combines with the existing out[b] (sum, or max for
+ * MAXABS), so several fields reduce into one value with successive calls.
+ * work: mjx_reduce_workspace_elems(B) doubles.
+ * Replaces: moljax/core/state.py:115-189 (tree_norm2, tree_norm_inf,
+ * tree_vdot), state.py:312-343 (scaled_error_norm), moljax/core/utils.py:186-197
+ * (is_finite). */
+enum { MJX_RED_SUMSQ = 0, MJX_RED_DOT = 1, MJX_RED_MAXABS = 2, MJX_RED_SCALED_SQ = 3, MJX_RED_NONFINITE = 4 };
+long long mjx_reduce_workspace_elems(int B);
+int mjx_reduce2d(int dtype, int op, int B, int ny, int nx, const void* a, long long a_pitch, long long a_istride,
+                 const void* b, long long b_pitch, long long b_istride, double atol, double rtol, double* out,
+                 int accumulate, double* work, void* stream);
 
 /* Non-periodic 2-D Helmholtz / Poisson solve on B instances of ny x nx
  * interior points:  kind 0 homogeneous Dirichlet (DST-I in x and y, vertex

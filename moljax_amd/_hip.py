@@ -34,9 +34,13 @@ PROTOTYPES = {
     "mjx_spectral_pair": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_imex_strang": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_imex": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
-    "mjx_gs_adaptive_workspace_bytes": (_ll, [_i, _i, _i, _i]),
+    "mjx_gs_adaptive_workspace_bytes": (_ll, [_i, _i, _i, _i, _i]),
+    "mjx_gs_explicit_workspace_bytes": (_ll, [_i, _i, _i, _i]),
+    "mjx_gs_explicit": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _d, _d, _p, _i, _p, _p]),
     "mjx_gs_imex_adaptive": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _dp, _p, _p, _p, _p,
                                   _i, _ll, ctypes.POINTER(ctypes.c_longlong), _p, _p]),
+    "mjx_reduce_workspace_elems": (_ll, [_i]),
+    "mjx_reduce2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _d, _d, _p, _i, _p, _p]),
     "mjx_fd_op2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _i, _d, _d, _p, _ll, _ll, _d, _p]),
     "mjx_bc_fill2d": (_i, [_i, _i, _i, _i, _p, _ll, _ll, _i, _dp, _d, _d, _p]),
     "mjx_gs_reaction": (_i, [_i, _ll, _p, _p, _p, _p, _d, _d, _p]),

@@ -48,22 +48,72 @@ def _interior(f: torch.Tensor, grid, interior_only: bool) -> torch.Tensor:
     return f[..., grid.interior_slice]
 
 
+RED_SUMSQ, RED_DOT, RED_MAXABS, RED_SCALED_SQ, RED_NONFINITE = 0, 1, 2, 3, 4
+
+
+def _rview(f: torch.Tensor):
+    """(B, ny, nx) strided view with pitch and instance stride for the
+    reduction kernel (1-D fields become one row)."""
+    from ._arrays import view2d
+    if f.dim() == 1:
+        f = f.unsqueeze(0)
+    return view2d(f)
+
+
+def _reduce(op: int, fields_a, fields_b=None, atol: float = 0.0, rtol: float = 0.0) -> torch.Tensor:
+    """Per-instance FP64 reduction over several fields (mjx_reduce2d, fixed
+    order, bitwise reproducible); returns a (B,) float64 device tensor."""
+    from .. import _hip
+    lib = _hip.lib()
+    out = work = None
+    for i, fa in enumerate(fields_a):
+        ta, ap, ai, B = _rview(fa)
+        tb, bp, bi = None, 0, 0
+        if fields_b is not None:
+            fb = fields_b[i].to(ta.dtype)
+            tb, bp, bi, Bb = _rview(fb)
+            if Bb != B or tb.shape != ta.shape:
+                raise ValueError("fields of a and b must have the same shape")
+        if out is None:
+            out = torch.zeros(B, dtype=torch.float64, device=ta.device)
+            work = torch.empty(lib.mjx_reduce_workspace_elems(B), dtype=torch.float64, device=ta.device)
+        elif out.numel() != B:
+            raise ValueError("all fields must have the same batch shape")
+        rc = lib.mjx_reduce2d(_hip.dtype_code(ta.dtype), op, B, ta.shape[-2], ta.shape[-1], _hip.ptr(ta), ap, ai,
+                              _hip.ptr(tb), bp, bi, float(atol), float(rtol), _hip.ptr(out), int(i > 0),
+                              _hip.ptr(work), _hip.stream_ptr())
+        _hip.check(rc, "mjx_reduce2d")
+    return out
+
+
+def _device_fields(fields):
+    from ._arrays import as_device, field_dtype
+    out = []
+    for f in fields:
+        t = as_device(f)
+        out.append(t.to(field_dtype(t)))
+    return out
+
+
 def tree_norm2(state: StateDict, grid, interior_only: bool = True) -> torch.Tensor:
-    """state.py:115-137, float64 accumulation."""
-    tot = sum(torch.sum(_interior(f, grid, interior_only).double() ** 2) for f in state.values())
-    return torch.sqrt(tot)
+    """state.py:115-137: sqrt of the sum of squares over all fields, float64
+    accumulation."""
+    f = _device_fields([_interior(v, grid, interior_only) for v in state.values()])
+    return torch.sqrt(torch.sum(_reduce(RED_SUMSQ, f)))
 
 
 def tree_norm_inf(state: StateDict, grid, interior_only: bool = True) -> torch.Tensor:
-    """state.py:140-160."""
-    return torch.stack([torch.max(torch.abs(_interior(f, grid, interior_only))).double()
-                        for f in state.values()]).max()
+    """state.py:140-160: max |x| over all fields (NaN propagates)."""
+    f = _device_fields([_interior(v, grid, interior_only) for v in state.values()])
+    return torch.max(_reduce(RED_MAXABS, f))
 
 
 def tree_vdot(a: StateDict, b: StateDict, grid, interior_only: bool = True) -> torch.Tensor:
-    """state.py:163-189, float64 accumulation."""
-    return sum(torch.sum(_interior(a[k], grid, interior_only).double() * _interior(b[k], grid, interior_only).double())
-               for k in a)
+    """state.py:163-189: sum over fields of sum(a*b), float64 accumulation."""
+    keys = list(a.keys())
+    fa = _device_fields([_interior(a[k], grid, interior_only) for k in keys])
+    fb = _device_fields([_interior(b[k], grid, interior_only) for k in keys])
+    return torch.sum(_reduce(RED_DOT, fa, fb))
 
 
 def state_interior_size(state: StateDict, grid) -> int:
@@ -80,13 +130,16 @@ def get_interior_state(state: StateDict, grid) -> StateDict:
 def scaled_error_norm(error: StateDict, y: StateDict, grid, atol: float, rtol: float) -> torch.Tensor:
     """state.py:312-343: sqrt(mean((err / (atol + rtol |y|))^2)) over interior
     points of all fields, accumulated in float64."""
-    tot = None
-    for k in error:
-        e = _interior(error[k], grid, True).double()
-        s = atol + rtol * torch.abs(_interior(y[k], grid, True).double())
-        term = torch.sum((e / s) ** 2)
-        tot = term if tot is None else tot + term
-    return torch.sqrt(tot / state_interior_size(error, grid))
+    keys = list(error.keys())
+    fe = _device_fields([_interior(error[k], grid, True) for k in keys])
+    fy = _device_fields([_interior(y[k], grid, True) for k in keys])
+    return torch.sqrt(torch.sum(_reduce(RED_SCALED_SQ, fe, fy, atol, rtol)) / state_interior_size(error, grid))
+
+
+def is_finite(state: StateDict) -> bool:
+    """utils.py:186-197: every value of every field finite (ghosts included)."""
+    f = _device_fields(list(state.values()))
+    return bool(torch.sum(_reduce(RED_NONFINITE, f)).item() == 0)
 
 
 def flatten_to_vec(state: StateDict) -> Tuple[torch.Tensor, Callable[[torch.Tensor], StateDict]]:

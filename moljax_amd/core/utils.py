@@ -20,3 +20,9 @@ def get_interior(field: torch.Tensor, grid) -> torch.Tensor:
         sy, sx = grid.interior_slice
         return field[..., sy, sx]
     return field[..., grid.interior_slice]
+
+
+def is_finite(state) -> bool:
+    """moljax/core/utils.py:186-197 (native reduction)."""
+    from .state import is_finite as _is_finite
+    return _is_finite(state)

@@ -15,6 +15,9 @@
 //                   attempt
 //   k_accept        y <- yh (interior) and the saved-state slot, for the
 //                   instances that accepted
+// Explicit schemes (forward Euler, SSPRK3, RK4 on the FD right-hand side,
+// the explicit branch of moljax/core/stepping.py:723-1021) take the same
+// attempt with explicit.hip's stage kernels in place of the IMEX chains.
 // Each instance follows its own dt sequence; an instance that has left the
 // loop (status != RUNNING) is skipped by the error, controller and accept
 // kernels.  The host reads the per-instance state only between chunks of
@@ -26,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "explicit_api.h"
 #include "gs_chain.h"
 #include "mjx_common.h"
 #include "moljax_hip.h"
@@ -42,8 +46,12 @@ enum {
 };
 constexpr double RUNNING = 1, SUCCESS = 0, MAX_STEPS_REACHED = 2, DT_TOO_SMALL = 3, NON_FINITE_VALUES = 4;
 
+// Attempt schemes: the IMEX chains of gs_chain.h, then the explicit methods.
+enum { AD_IMEX_STRANG = 0, AD_IMEX_EULER = 1, AD_EXPL_EULER = 3, AD_EXPL_SSPRK3 = 4, AD_EXPL_RK4 = 5 };
+__host__ __device__ inline bool is_explicit(int scheme) { return scheme >= AD_EXPL_EULER; }
+
 struct Ctl {
-  double atol, rtol, safety, kI, kP, dt_min, dt_max, max_factor, min_factor, order;
+  double atol, rtol, safety, kI, kP, dt_min, dt_max, max_factor, min_factor, order, dy, dx;
   long long save_every;
   int max_saves;
   int scheme;
@@ -70,14 +78,21 @@ __device__ void prepare_rows(const double* S, const double* model, InstParams* f
   *dtc_out = dtc;
   const double Du = model[0], Dv = model[1], F = model[2], k = model[3];
   const double hf = dtc / 2.0;
-  // Strang solves with dt/2 (stepping.py:399), Euler with dt (stepping.py:363).
-  const double df = scheme == GS_STRANG ? hf : dtc;
-  const double dh = scheme == GS_STRANG ? hf / 2.0 : hf;
-  full->beta_a = df * Du;
-  full->beta_b = df * Dv;
-  half->beta_a = dh * Du;
-  half->beta_b = dh * Dv;
-  full->alpha_a = full->alpha_b = half->alpha_a = half->alpha_b = 0.0;
+  if (is_explicit(scheme)) {
+    // explicit stages read D from the alpha columns
+    full->beta_a = full->beta_b = half->beta_a = half->beta_b = 0.0;
+    full->alpha_a = half->alpha_a = Du;
+    full->alpha_b = half->alpha_b = Dv;
+  } else {
+    // Strang solves with dt/2 (stepping.py:399), Euler with dt (stepping.py:363).
+    const double df = scheme == GS_STRANG ? hf : dtc;
+    const double dh = scheme == GS_STRANG ? hf / 2.0 : hf;
+    full->beta_a = df * Du;
+    full->beta_b = df * Dv;
+    half->beta_a = dh * Du;
+    half->beta_b = dh * Dv;
+    full->alpha_a = full->alpha_b = half->alpha_a = half->alpha_b = 0.0;
+  }
   full->F = half->F = F;
   full->k = half->k = k;
   full->dt = dtc;
@@ -264,11 +279,11 @@ __global__ __launch_bounds__(256) void k_accept(const T* __restrict__ yh, T* __r
 }
 
 struct Layout {
-  long long W, yf, yh, full, half, partial, total;
+  long long W, yf, yh, full, half, partial, xs, ymid, total;
   int nblk;
 };
 
-Layout layout(int dtype, int B, int ny, int nx) {
+Layout layout(int dtype, int scheme, int B, int ny, int nx) {
   const long long es = dtype == MJX_F64 ? 8 : 4;
   const long long n = (long long)ny * nx;
   auto al = [](long long x) { return (x + 255) / 256 * 256; };
@@ -280,7 +295,9 @@ Layout layout(int dtype, int B, int ny, int nx) {
   L.full = L.yh + al((long long)B * 2 * n * es);
   L.half = L.full + al((long long)B * sizeof(InstParams));
   L.partial = L.half + al((long long)B * sizeof(InstParams));
-  L.total = L.partial + al((long long)B * L.nblk * 2 * 8);
+  L.xs = L.partial + al((long long)B * L.nblk * 2 * 8);
+  L.ymid = L.xs + (is_explicit(scheme) ? al(explicit_scratch_elems(B, ny, nx) * es) : 0);
+  L.total = L.ymid + (is_explicit(scheme) ? al((long long)B * 2 * n * es) : 0);
   return L;
 }
 
@@ -289,13 +306,15 @@ int adaptive_t(const Ctl& c, int B, int ny, int nx, T* u, T* v, long long pitch,
                int fill_ghosts, const T* lamx, const T* lamy, const T* lam2d, const double* model, double* state,
                double* t_hist, double* dt_hist, T* y_hist, long long max_attempts, char* work, hipStream_t st,
                long long* attempts_done) {
-  const Layout L = layout(sizeof(T) == 8 ? MJX_F64 : MJX_F32, B, ny, nx);
+  const Layout L = layout(sizeof(T) == 8 ? MJX_F64 : MJX_F32, c.scheme, B, ny, nx);
   cpx<T>* W = (cpx<T>*)(work + L.W);
   T* yf = (T*)(work + L.yf);
   T* yh = (T*)(work + L.yh);
   InstParams* full = (InstParams*)(work + L.full);
   InstParams* half = (InstParams*)(work + L.half);
   double* partial = (double*)(work + L.partial);
+  T* xs = (T*)(work + L.xs);
+  T* ymid = (T*)(work + L.ymid);
   const long long n = (long long)ny * nx;
   const int acc_blk = (int)std::min<long long>(64, (n + 1023) / 1024);
 
@@ -310,11 +329,20 @@ int adaptive_t(const Ctl& c, int B, int ny, int nx, T* u, T* v, long long pitch,
   for (;;) {
     for (long long i = 0; i < chunk; ++i) {
       if (max_attempts > 0 && done >= max_attempts) break;
-      rc = gs_chain_t<T>(c.scheme, B, ny, nx, u, v, pitch, istride, yf, yf + n, nx, 2 * n, 0, lamx, lamy, lam2d,
-                         full, 1, W, st);
-      if (rc) return rc;
-      rc = gs_chain_t<T>(c.scheme, B, ny, nx, u, v, pitch, istride, yh, yh + n, nx, 2 * n, 0, lamx, lamy, lam2d,
-                         half, 2, W, st);
+      if (is_explicit(c.scheme)) {
+        const int m = c.scheme - AD_EXPL_EULER;
+        const Layout2<T> ys{u, v, pitch, istride}, of{yf, yf + n, nx, 2 * n}, om{ymid, ymid + n, nx, 2 * n},
+            oh{yh, yh + n, nx, 2 * n};
+        rc = gs_explicit_step(m, B, ny, nx, ys, of, c.dy, c.dx, full, xs, st);
+        if (!rc) rc = gs_explicit_step(m, B, ny, nx, ys, om, c.dy, c.dx, half, xs, st);
+        if (!rc) rc = gs_explicit_step(m, B, ny, nx, om, oh, c.dy, c.dx, half, xs, st);
+      } else {
+        rc = gs_chain_t<T>(c.scheme, B, ny, nx, u, v, pitch, istride, yf, yf + n, nx, 2 * n, 0, lamx, lamy, lam2d,
+                           full, 1, W, st);
+        if (!rc)
+          rc = gs_chain_t<T>(c.scheme, B, ny, nx, u, v, pitch, istride, yh, yh + n, nx, 2 * n, 0, lamx, lamy,
+                             lam2d, half, 2, W, st);
+      }
       if (rc) return rc;
       hipLaunchKernelGGL((k_err_partial<T>), dim3(B * L.nblk), dim3(ERR_THREADS), 0, st, yf, yh, n, L.nblk, c.atol,
                          c.rtol, state, partial);
@@ -361,9 +389,9 @@ using namespace mjx;
 
 extern "C" {
 
-long long mjx_gs_adaptive_workspace_bytes(int dtype, int B, int ny, int nx) {
+long long mjx_gs_adaptive_workspace_bytes(int dtype, int scheme, int B, int ny, int nx) {
   if (B < 1 || ny < 1 || nx < 1) return 0;
-  return layout(dtype, B, ny, nx).total;
+  return layout(dtype, scheme, B, ny, nx).total;
 }
 
 int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch,
@@ -372,15 +400,17 @@ int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, 
                          void* y_hist, int max_saves, long long max_attempts, long long* attempts_done, void* work,
                          void* stream) {
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
-  if (scheme != GS_STRANG && scheme != GS_EULER)
-    return fail(MJX_E_ARG, "adaptive scheme must be 0 (Strang) or 1 (Euler)");
+  if (scheme != AD_IMEX_STRANG && scheme != AD_IMEX_EULER && !(scheme >= AD_EXPL_EULER && scheme <= AD_EXPL_RK4))
+    return fail(MJX_E_ARG, "adaptive scheme must be 0/1 (IMEX Strang/Euler) or 3/4/5 (explicit Euler/SSPRK3/RK4)");
+  if (B > 65535) return fail(MJX_E_LIMIT, "at most 65535 instances per call");
   if (B < 1) return fail(MJX_E_ARG, "batch must be >= 1");
-  if (!is_pow2_len(nx) || !is_pow2_len(ny))
+  if (!is_explicit(scheme) && (!is_pow2_len(nx) || !is_pow2_len(ny)))
     return fail(MJX_E_SIZE, "grid " + std::to_string(ny) + "x" + std::to_string(nx) +
                                 " not supported: nx and ny must be powers of two in [16, 4096]");
+  if (ny < 1 || nx < 1) return fail(MJX_E_ARG, "empty grid");
   if (!u || !v || !model || !ctl || !state || !t_hist || !dt_hist || !work)
     return fail(MJX_E_ARG, "null pointer argument");
-  if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
+  if (!is_explicit(scheme) && !lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
   if (pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
   if (fill_ghosts && pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= nx + 2");
   if (max_saves < 1) return fail(MJX_E_ARG, "max_saves must be >= 1");
@@ -396,6 +426,9 @@ int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, 
   c.min_factor = ctl[8];
   c.order = ctl[9];
   c.save_every = (long long)ctl[10];
+  c.dy = ctl[11];
+  c.dx = ctl[12];
+  if (is_explicit(scheme) && !(c.dx > 0 && c.dy > 0)) return fail(MJX_E_ARG, "explicit schemes need dx, dy > 0");
   c.max_saves = max_saves;
   c.scheme = scheme;
   if (c.save_every < 1) return fail(MJX_E_ARG, "save_every must be >= 1");

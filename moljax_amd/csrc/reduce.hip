@@ -1,0 +1,142 @@
+// Deterministic FP64-accumulated reductions over strided 2-D interiors
+// (the state norms of moljax/core/state.py:115-189,312-343 and is_finite of
+// moljax/core/utils.py:186-197).
+//
+// Two launches per call: k_reduce_partial gives every instance a fixed set
+// of RED_BLOCKS workgroups; each walks the instance's points in a fixed
+// order, reduces its 256 lanes by wave64 shuffles and LDS in a fixed tree,
+// and writes one partial.  k_reduce_final folds an instance's partials with
+// one wave, again in a fixed order, and combines the result with out[b]
+// when accumulating over fields.  The same inputs therefore give bitwise
+// the same result on every run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mjx_common.h"
+#include "moljax_hip.h"
+
+namespace mjx {
+namespace {
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 64;
+
+__device__ __forceinline__ double nanmax(double a, double b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+
+template <int OP>
+__device__ __forceinline__ double combine(double a, double b) {
+  if constexpr (OP == MJX_RED_MAXABS) return nanmax(a, b);
+  else return a + b;
+}
+
+template <int OP>
+__device__ __forceinline__ double wave_reduce(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = combine<OP>(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <typename T, int OP>
+__device__ __forceinline__ double term(T a, T b, double atol, double rtol) {
+  const double x = (double)a;
+  if constexpr (OP == MJX_RED_SUMSQ) return x * x;
+  else if constexpr (OP == MJX_RED_DOT) return x * (double)b;
+  else if constexpr (OP == MJX_RED_MAXABS) return fabs(x);
+  else if constexpr (OP == MJX_RED_SCALED_SQ) {
+    const double q = x / (atol + rtol * fabs((double)b));
+    return q * q;
+  } else return isfinite(x) ? 0.0 : 1.0;  // MJX_RED_NONFINITE
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const T* __restrict__ a, long long ap, long long ai,
+                                                                const T* __restrict__ b, long long bp, long long bi,
+                                                                int ny, int nx, double atol, double rtol,
+                                                                double* __restrict__ partial) {
+  const int inst = blockIdx.x / RED_BLOCKS, blk = blockIdx.x - inst * RED_BLOCKS;
+  __shared__ double lds[RED_THREADS / 64];
+  const T* A = a + inst * ai;
+  const T* Bp = b ? b + inst * bi : nullptr;
+  const long long n = (long long)ny * nx;
+  double acc = 0.0;  // identity of + and of max over |x|
+  for (long long i = (long long)blk * RED_THREADS + threadIdx.x; i < n; i += (long long)RED_BLOCKS * RED_THREADS) {
+    const int y = (int)(i / nx), x = (int)(i - (long long)y * nx);
+    const T av = A[(long long)y * ap + x];
+    const T bv = Bp ? Bp[(long long)y * bp + x] : (T)0;
+    acc = combine<OP>(acc, term<T, OP>(av, bv, atol, rtol));
+  }
+  acc = wave_reduce<OP>(acc);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = lds[0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) s = combine<OP>(s, lds[w]);
+    partial[(long long)inst * RED_BLOCKS + blk] = s;
+  }
+}
+
+template <int OP>
+__global__ __launch_bounds__(64) void k_reduce_final(const double* __restrict__ partial, double* __restrict__ out,
+                                                     int accumulate) {
+  const int inst = blockIdx.x;
+  double s = partial[(long long)inst * RED_BLOCKS + threadIdx.x];  // RED_BLOCKS == 64 lanes
+  s = wave_reduce<OP>(s);
+  if (threadIdx.x == 0) out[inst] = accumulate ? combine<OP>(out[inst], s) : s;
+}
+
+static_assert(RED_BLOCKS == 64, "k_reduce_final folds one partial per lane");
+
+template <typename T, int OP>
+int launch(int B, int ny, int nx, const T* a, long long ap, long long ai, const T* b, long long bp, long long bi,
+           double atol, double rtol, double* out, double* work, int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL((k_reduce_partial<T, OP>), dim3(B * RED_BLOCKS), dim3(RED_THREADS), 0, st, a, ap, ai, b, bp,
+                     bi, ny, nx, atol, rtol, work);
+  int rc = check_launch("reduce_partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_reduce_final<OP>), dim3(B), dim3(64), 0, st, work, out, accumulate);
+  return check_launch("reduce_final");
+}
+
+template <typename T>
+int dispatch(int op, int B, int ny, int nx, const T* a, long long ap, long long ai, const T* b, long long bp,
+             long long bi, double atol, double rtol, double* out, double* work, int acc, hipStream_t st) {
+  switch (op) {
+    case MJX_RED_SUMSQ: return launch<T, MJX_RED_SUMSQ>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
+    case MJX_RED_DOT: return launch<T, MJX_RED_DOT>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
+    case MJX_RED_MAXABS: return launch<T, MJX_RED_MAXABS>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
+    case MJX_RED_SCALED_SQ:
+      return launch<T, MJX_RED_SCALED_SQ>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
+    case MJX_RED_NONFINITE:
+      return launch<T, MJX_RED_NONFINITE>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
+    default: return fail(MJX_E_ARG, "unknown reduction op");
+  }
+}
+
+}  // namespace
+}  // namespace mjx
+
+using namespace mjx;
+
+extern "C" {
+
+long long mjx_reduce_workspace_elems(int B) { return B < 1 ? 0 : (long long)B * RED_BLOCKS; }
+
+int mjx_reduce2d(int dtype, int op, int B, int ny, int nx, const void* a, long long a_pitch, long long a_istride,
+                 const void* b, long long b_pitch, long long b_istride, double atol, double rtol, double* out,
+                 int accumulate, double* work, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (B < 1 || ny < 1 || nx < 1) return fail(MJX_E_ARG, "empty reduction");
+  if (!a || !out || !work) return fail(MJX_E_ARG, "null pointer argument");
+  if ((op == MJX_RED_DOT || op == MJX_RED_SCALED_SQ) && !b) return fail(MJX_E_ARG, "this reduction needs b");
+  if (a_pitch < nx || (b && b_pitch < nx)) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if ((long long)B * RED_BLOCKS > 0x7fffffffLL) return fail(MJX_E_LIMIT, "batch too large for one launch");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == MJX_F32)
+    return dispatch<float>(op, B, ny, nx, (const float*)a, a_pitch, a_istride, (const float*)b, b_pitch, b_istride,
+                           atol, rtol, out, work, accumulate, st);
+  return dispatch<double>(op, B, ny, nx, (const double*)a, a_pitch, a_istride, (const double*)b, b_pitch,
+                          b_istride, atol, rtol, out, work, accumulate, st);
+}
+
+}  // extern "C"

@@ -78,6 +78,20 @@ def imex_cfl_dt(params, cfl: CFLParams, min_dx: float) -> float:
     return _clip(cfl.safety * dt, cfl.dt_min, cfl.dt_max)
 
 
+def heisenberg_cfl_dt(params, cfl: CFLParams, min_dx: float) -> float:
+    """dt_policy.py:104-175: advection, diffusion (dx^2 / D_max) and wave
+    limits."""
+    dt = cfl.dt_max
+    v_max = abs(params.get("vx", 0.0)) + abs(params.get("vy", 0.0))
+    dt = min(dt, cfl.cfl_advection * min_dx / v_max if v_max > 1e-14 else cfl.dt_max)
+    D_max = max(params.get("D", 0.0), params.get("Du", 0.0), params.get("Dv", 0.0))
+    dt = min(dt, cfl.cfl_diffusion * min_dx ** 2 / D_max if D_max > 1e-14 else cfl.dt_max)
+    K, rho = params.get("K", 0.0), params.get("rho", 1.0)
+    c = max(params.get("wave_speed", 0.0), math.sqrt(K / rho) if K > 1e-14 else 0.0)
+    dt = min(dt, cfl.cfl_wave * min_dx / c if c > 1e-14 else cfl.dt_max)
+    return _clip(cfl.safety * dt, cfl.dt_min, cfl.dt_max)
+
+
 def pid_controller_dt(dt_old, err, prev_err, pid: PIDParams, order):
     """dt_policy.py:178-219; returns (dt_new, new prev_err)."""
     kI, kP = pid.kI / order, pid.kP / order
@@ -127,10 +141,31 @@ def adaptive_integrate_imex(y0, t0, t_end, dt0, F, k, Du, Dv, lam, min_dx, use_s
                             keep_states=True, max_attempts=None):
     """stepping.py:1146-1371 for the Gray-Scott model (params F, k)."""
     cfl = cfl or CFLParams()
-    pid = pid or PIDParams()
-    order = 2 if use_strang else 1
     step = ogs.imex_strang_step if use_strang else ogs.imex_euler_step
-    dt_cfl = imex_cfl_dt({"F": F, "k": k}, cfl, min_dx)
+    return _adaptive(y0, t0, t_end, dt0, lambda y, h: step(y, h, F, k, Du, Dv, lam, ng),
+                     imex_cfl_dt({"F": F, "k": k}, cfl, min_dx), 2 if use_strang else 1, max_steps, pid,
+                     save_every, ng, keep_states, max_attempts)
+
+
+EXPLICIT_ORDER = {"euler": 1, "ssprk3": 3, "rk4": 4}  # dt_policy.py:411-423
+
+
+def adaptive_integrate_explicit(y0, t0, t_end, dt0, F, k, Du, Dv, dy, dx, method="rk4", max_steps=10000,
+                                cfl: CFLParams = None, pid: PIDParams = None, save_every=1, ng=1,
+                                keep_states=True, max_attempts=None):
+    """The explicit branch of adaptive_integrate (stepping.py:723-1021):
+    step doubling with euler/ssprk3/rk4 (stepping.py:502-536), dt limited by
+    heisenberg_cfl_dt."""
+    cfl = cfl or CFLParams()
+    step = {"euler": ogs.euler_step, "ssprk3": ogs.ssprk3_step, "rk4": ogs.rk4_step}[method]
+    dt_cfl = heisenberg_cfl_dt({"Du": Du, "Dv": Dv, "F": F, "k": k}, cfl, min(dx, dy))
+    return _adaptive(y0, t0, t_end, dt0, lambda y, h: step(y, h, F, k, Du, Dv, dy, dx, ng), dt_cfl,
+                     EXPLICIT_ORDER[method], max_steps, pid, save_every, ng, keep_states, max_attempts)
+
+
+def _adaptive(y0, t0, t_end, dt0, stepper, dt_cfl, order, max_steps, pid, save_every, ng, keep_states,
+              max_attempts):
+    pid = pid or PIDParams()
     dt = _nanmin(float(dt0), dt_cfl)
     max_saves = max_steps // save_every + 1
     t_hist = np.zeros(max_saves)
@@ -144,9 +179,8 @@ def adaptive_integrate_imex(y0, t0, t_end, dt0, F, k, Du, Dv, lam, min_dx, use_s
         if max_attempts is not None and len(attempts) >= max_attempts:
             break
         dtc = _nanmin(dt, t_end - t)
-        y_full = step(y, dtc, F, k, Du, Dv, lam, ng)
-        y_h = step(y, dtc / 2.0, F, k, Du, Dv, lam, ng)
-        y_new = step(y_h, dtc / 2.0, F, k, Du, Dv, lam, ng)
+        y_full = stepper(y, dtc)
+        y_new = stepper(stepper(y, dtc / 2.0), dtc / 2.0)
         err = {n: y_new[n] - y_full[n] for n in y_new}
         e = _nanmax(scaled_error_norm(err, y_new, pid.atol, pid.rtol, ng), 1e-10)
         finite = all(bool(np.all(np.isfinite(v))) for v in y_new.values())

@@ -72,6 +72,51 @@ def imex_ssprk2_step(y, dt, F, k, Du, Dv, lam, ng=1):
     return apply_bcs({n: 0.5 * y[n] + 0.5 * y2[n] for n in y}, ng)
 
 
+def laplacian_padded(f, dy, dx, ng=1):
+    """operators.py:159-322: d2_central axis 0 + axis 1 on the interior of a
+    padded array, zeros in the ghost ring."""
+    out = np.zeros_like(f)
+    c = f[ng:-ng, ng:-ng]
+    d2y = (f[ng + 1:f.shape[0] - ng + 1, ng:-ng] - 2.0 * c + f[ng - 1:f.shape[0] - ng - 1, ng:-ng]) / dy ** 2
+    d2x = (f[ng:-ng, ng + 1:f.shape[1] - ng + 1] - 2.0 * c + f[ng:-ng, ng - 1:f.shape[1] - ng - 1]) / dx ** 2
+    out[ng:-ng, ng:-ng] = d2y + d2x
+    return out
+
+
+def rhs(y, F, k, Du, Dv, dy, dx, ng=1):
+    """model.py:100-118 for the periodic Gray-Scott model: apply_bcs, then
+    D lap (interior) + reaction (padded)."""
+    y = apply_bcs(y, ng)
+    R = reaction(y, F, k)
+    return {"u": Du * laplacian_padded(y["u"], dy, dx, ng) + R["u"],
+            "v": Dv * laplacian_padded(y["v"], dy, dx, ng) + R["v"]}
+
+
+def euler_step(y, dt, F, k, Du, Dv, dy, dx, ng=1):
+    """stepping.py:64-85."""
+    f = rhs(y, F, k, Du, Dv, dy, dx, ng)
+    return {n: y[n] + dt * f[n] for n in y}
+
+
+def ssprk3_step(y, dt, F, k, Du, Dv, dy, dx, ng=1):
+    """stepping.py:88-124."""
+    f0 = rhs(y, F, k, Du, Dv, dy, dx, ng)
+    y1 = {n: y[n] + dt * f0[n] for n in y}
+    f1 = rhs(y1, F, k, Du, Dv, dy, dx, ng)
+    y2 = {n: (0.75 * y[n] + 0.25 * y1[n]) + (0.25 * dt) * f1[n] for n in y}
+    f2 = rhs(y2, F, k, Du, Dv, dy, dx, ng)
+    return {n: ((1.0 / 3.0) * y[n] + (2.0 / 3.0) * y2[n]) + (2.0 / 3.0 * dt) * f2[n] for n in y}
+
+
+def rk4_step(y, dt, F, k, Du, Dv, dy, dx, ng=1):
+    """stepping.py:127-167."""
+    k1 = rhs(y, F, k, Du, Dv, dy, dx, ng)
+    k2 = rhs({n: y[n] + 0.5 * dt * k1[n] for n in y}, F, k, Du, Dv, dy, dx, ng)
+    k3 = rhs({n: y[n] + 0.5 * dt * k2[n] for n in y}, F, k, Du, Dv, dy, dx, ng)
+    k4 = rhs({n: y[n] + dt * k3[n] for n in y}, F, k, Du, Dv, dy, dx, ng)
+    return {n: y[n] + dt / 6.0 * (((k1[n] + 2.0 * k2[n]) + 2.0 * k3[n]) + k4[n]) for n in y}
+
+
 def integrate(y0, dt, n_steps, F, k, Du, Dv, lam, ng=1, record=None):
     """Fixed-dt loop (stepping.py:1414-1434 body).  ``record`` is an optional
     set of step counts at which the state is saved."""

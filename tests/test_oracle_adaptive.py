@@ -160,3 +160,60 @@ def test_status_codes():
     bad["u"][5, 5] = np.nan
     r = oad.adaptive_integrate_imex(bad, 0.0, 1.0, 0.1, 0.04, 0.06, 0.16, 0.08, lam, h)
     assert r.status == oad.NON_FINITE_VALUES and r.n_accepted == 0
+
+
+@pytest.mark.parametrize("method,poly", [
+    ("euler", lambda z: 1 + z),
+    ("ssprk3", lambda z: 1 + z + z * z / 2 + z ** 3 / 6),
+    ("rk4", lambda z: 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24)])
+def test_explicit_steps_mode_amplification(method, poly):
+    """Pure diffusion of one Fourier mode: an explicit RK step multiplies the
+    mode by its stability polynomial at z = dt D lam_m (lam_m the 5-point
+    eigenvalue), independently of the stage arithmetic."""
+    N, L, m, a, Du, dt = 32, 2.5, 3, 0.3, 0.16, 0.02
+    h = L / N
+    y0, x = _mode_state(N, L, m, a)
+    lm = (2.0 * np.cos(2 * np.pi * m / N) - 2.0) / h ** 2
+    step = {"euler": ogs.euler_step, "ssprk3": ogs.ssprk3_step, "rk4": ogs.rk4_step}[method]
+    y = step(y0, dt, 0.0, 0.0, Du, 0.0, h, h)
+    c = np.cos(2 * np.pi * m * x / L)[None, :]
+    assert np.max(np.abs(y["u"][1:-1, 1:-1] - (1.0 + a * poly(dt * Du * lm) * c))) < 1e-14
+    # reaction only (uniform state): the stage recurrences on the scalar ODE
+    y1 = {"u": np.full((N + 2, N + 2), 0.7), "v": np.full((N + 2, N + 2), 0.2)}
+    yr = step(y1, dt, 0.04, 0.06, Du, 0.08, h, h)
+
+    def f(u, v):
+        return -u * v * v + 0.04 * (1 - u), u * v * v - 0.1 * v
+    u, v = 0.7, 0.2
+    if method == "euler":
+        fu, fv = f(u, v)
+        ue, ve = u + dt * fu, v + dt * fv
+    elif method == "ssprk3":
+        f0 = f(u, v)
+        u1, v1 = u + dt * f0[0], v + dt * f0[1]
+        f1 = f(u1, v1)
+        u2, v2 = 0.75 * u + 0.25 * (u1 + dt * f1[0]), 0.75 * v + 0.25 * (v1 + dt * f1[1])
+        f2 = f(u2, v2)
+        ue, ve = u / 3 + 2 / 3 * (u2 + dt * f2[0]), v / 3 + 2 / 3 * (v2 + dt * f2[1])
+    else:
+        k1 = f(u, v)
+        k2 = f(u + dt / 2 * k1[0], v + dt / 2 * k1[1])
+        k3 = f(u + dt / 2 * k2[0], v + dt / 2 * k2[1])
+        k4 = f(u + dt * k3[0], v + dt * k3[1])
+        ue = u + dt / 6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
+        ve = v + dt / 6 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1])
+    assert np.max(np.abs(yr["u"][1:-1, 1:-1] - ue)) < 1e-15
+    assert np.max(np.abs(yr["v"][1:-1, 1:-1] - ve)) < 1e-15
+
+
+def test_heisenberg_cfl_and_explicit_adaptive_loop():
+    N, L = 32, 2.5
+    h = L / N
+    c = oad.CFLParams()
+    assert oad.heisenberg_cfl_dt({"Du": 0.16, "Dv": 0.08}, c, h) == pytest.approx(0.9 * 0.25 * h * h / 0.16)
+    y0 = ogs.initial_condition(N, N, seed=2, square=8)
+    r = oad.adaptive_integrate_explicit(y0, 0.0, 2.0, 0.5, 0.04, 0.06, 0.16, 0.08, h, h, method="rk4",
+                                        pid=oad.PIDParams(atol=1e-5, rtol=1e-4))
+    assert r.status == oad.SUCCESS and r.t_final == pytest.approx(2.0)
+    # every accepted dt respects the diffusion CFL cap
+    assert np.all(r.dt_history[:r.n_steps - 1] <= oad.heisenberg_cfl_dt({"Du": 0.16, "Dv": 0.08}, c, h) + 1e-15)

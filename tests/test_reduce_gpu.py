@@ -1,0 +1,75 @@
+"""FP64-accumulated reductions (tree_norm2, tree_vdot, tree_norm_inf,
+scaled_error_norm, is_finite) on the GPU, pinned by math.fsum brute force on
+the same values (moljax/core/state.py:115-189,312-343)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+class _G:
+    """Minimal 2-D grid: one ghost ring."""
+    def __init__(self, ny, nx):
+        self.ny, self.nx = ny, nx
+        self.interior_slice = (slice(1, ny + 1), slice(1, nx + 1))
+
+
+def _state(rng, ny, nx, dtype):
+    return {k: torch.from_numpy(rng.standard_normal((ny + 2, nx + 2)) * s).to("cuda", dtype)
+            for k, s in (("u", 1.0), ("v", 1e-3))}
+
+
+def _np(t):
+    return t.detach().cpu().double().numpy()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("shape", [(1, 7), (33, 65), (256, 256)])
+def test_norms_match_fsum(gpu, dtype, shape):
+    from moljax_amd.core.state import scaled_error_norm, tree_norm2, tree_norm_inf, tree_vdot
+    ny, nx = shape
+    rng = np.random.default_rng(ny * 1000 + nx)
+    g = _G(ny, nx)
+    a, b = _state(rng, ny, nx, dtype), _state(rng, ny, nx, dtype)
+    ia = {k: _np(v)[1:-1, 1:-1] for k, v in a.items()}
+    ib = {k: _np(v)[1:-1, 1:-1] for k, v in b.items()}
+    n2 = math.sqrt(math.fsum(np.concatenate([(x * x).ravel() for x in ia.values()])))
+    assert float(tree_norm2(a, g)) == pytest.approx(n2, rel=1e-14)
+    dot = math.fsum(np.concatenate([(ia[k] * ib[k]).ravel() for k in ia]))
+    assert abs(float(tree_vdot(a, b, g)) - dot) <= 1e-14 * math.fsum(
+        np.concatenate([np.abs(ia[k] * ib[k]).ravel() for k in ia]))
+    assert float(tree_norm_inf(a, g)) == max(np.max(np.abs(x)) for x in ia.values())
+    full = math.sqrt(math.fsum(np.concatenate([(_np(x) ** 2).ravel() for x in a.values()])))
+    assert float(tree_norm2(a, g, interior_only=False)) == pytest.approx(full, rel=1e-14)
+    atol, rtol = 1e-6, 1e-4
+    terms = np.concatenate([((ia[k] / (atol + rtol * np.abs(ib[k]))) ** 2).ravel() for k in ia])
+    sen = math.sqrt(math.fsum(terms) / terms.size)
+    assert float(scaled_error_norm(a, b, g, atol, rtol)) == pytest.approx(sen, rel=1e-13)
+
+
+def test_bitwise_reproducible_and_nonfinite(gpu):
+    from moljax_amd.core.state import is_finite, tree_norm2, tree_norm_inf
+    rng = np.random.default_rng(7)
+    g = _G(300, 200)
+    a = _state(rng, 300, 200, torch.float64)
+    r = [float(tree_norm2(a, g)) for _ in range(3)]
+    assert r[0] == r[1] == r[2]
+    assert is_finite(a)
+    a["v"][5, 7] = float("nan")
+    assert not is_finite(a)
+    assert math.isnan(float(tree_norm_inf(a, g)))
+    a["v"][5, 7] = float("inf")
+    assert not is_finite(a)
+
+
+def test_batched_reduction_per_instance(gpu):
+    from moljax_amd.core.state import RED_SUMSQ, _reduce
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.standard_normal((5, 34, 66))).cuda()
+    inner = x[:, 1:-1, 1:-1]
+    out = _reduce(RED_SUMSQ, [inner]).cpu().numpy()
+    for b in range(5):
+        assert out[b] == pytest.approx(math.fsum((inner[b].cpu().numpy() ** 2).ravel()), rel=1e-14)
