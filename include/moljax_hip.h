@@ -166,6 +166,41 @@ int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, 
                          double* t_hist, double* dt_hist, void* y_hist, int max_saves,
                          long long max_attempts, long long* attempts_done, void* work, void* stream);
 
+/* Batched launch of Gray-Scott instances of uneven square sizes (no
+ * reference counterpart; the nearest reference code batches fields of one
+ * grid, moljax/core/jit_kernels.py:228-262,356-531).  A plan groups the
+ * instances by size: group g has B[g] instances of n[g] x n[g] in (u[g],
+ * v[g]) with row pitch pitch[g] and instance stride istride[g], parameter
+ * rows inst[g] (B[g] x 8 doubles, F, k, dt and for Strang beta = dt/2 D),
+ * tables tables[2g], tables[2g+1] = {ETDRK4 coefficients (from
+ * mjx_etdrk4_coeffs), NULL} or {lamx, lamy} for Strang, and workspace
+ * work[g] (mjx_etdrk4_workspace_bytes / mjx_spectral_workspace_bytes).
+ * scheme 0 ETDRK4 (n in 32..1024), 1 IMEX-Strang (n in 16..4096).
+ * mjx_batch_run advances every instance by n_steps: the steps run as
+ * replays of one hipGraph of steps_per_graph steps whose branches are the
+ * groups (one forked stream per group), the remainder as direct launches.
+ * Each instance's result is bitwise that of the instance advanced alone.
+ * A plan is used from one thread at a time; destroy it before freeing the
+ * buffers it names. */
+int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const int* B, void* const* u,
+                     void* const* v, const long long* pitch, const long long* istride,
+                     const void* const* tables, const double* const* inst, void* const* work,
+                     int steps_per_graph, void** plan);
+int mjx_batch_run(void* plan, int n_steps, int fill_ghosts, void* stream);
+int mjx_batch_destroy(void* plan);
+
+/* Autotuner (no reference counterpart): times the candidate tiles (lines per
+ * workgroup) of the scheme's row and column kernels on scratch data of B
+ * instances of n x n (reps timed repetitions each, HIP events), keeps the
+ * fastest in the process-wide tile table, and returns them in chosen[0]
+ * (row) and chosen[1] (column).  scheme 0 ETDRK4, 1 IMEX-Strang.
+ * mjx_tune_get / mjx_tune_set read and write that table (kind 0 spectral
+ * row, 1 spectral column, 2 ETDRK4 row, 3 ETDRK4 column; lines 0 = default),
+ * so callers can persist tuned tiles. */
+int mjx_autotune(int dtype, int scheme, int n, int B, int reps, int* chosen, void* stream);
+int mjx_tune_get(int kind, int dtype, int n);
+int mjx_tune_set(int kind, int dtype, int n, int lines);
+
 /* Reductions over B strided (ny, nx) interiors, FP64 accumulation in a
  * fixed order (bitwise reproducible), one result per instance in out[b]
  * (device doubles):

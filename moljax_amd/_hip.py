@@ -41,6 +41,12 @@ PROTOTYPES = {
                                   _i, _ll, ctypes.POINTER(ctypes.c_longlong), _p, _p]),
     "mjx_reduce_workspace_elems": (_ll, [_i]),
     "mjx_reduce2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _d, _d, _p, _i, _p, _p]),
+    "mjx_batch_create": (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
+    "mjx_batch_run": (_i, [_p, _i, _i, _p]),
+    "mjx_batch_destroy": (_i, [_p]),
+    "mjx_autotune": (_i, [_i, _i, _i, _i, _i, ctypes.POINTER(ctypes.c_int), _p]),
+    "mjx_tune_get": (_i, [_i, _i, _i]),
+    "mjx_tune_set": (_i, [_i, _i, _i, _i]),
     "mjx_fd_op2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _i, _d, _d, _p, _ll, _ll, _d, _p]),
     "mjx_bc_fill2d": (_i, [_i, _i, _i, _i, _p, _ll, _ll, _i, _dp, _d, _d, _p]),
     "mjx_gs_reaction": (_i, [_i, _ll, _p, _p, _p, _p, _d, _d, _p]),

@@ -1,0 +1,160 @@
+"""Many Gray-Scott instances of uneven grid sizes advanced together on one
+GPU (build-only; the nearest reference code batches fields of one grid,
+moljax/core/jit_kernels.py:228-262,356-531 and
+moljax/core/fft_integrators.py:473-510).
+
+Instances are grouped by size.  Each group keeps its state resident in one
+contiguous (B_g, n, n) pair of device arrays, its parameter rows, its
+coefficient or symbol tables and its workspace; mjx_batch_run advances every
+group by replaying one hipGraph whose branches are the groups.  Tiles come
+from the autotuner (mjx_autotune), run once per (precision, scheme, size) in
+a process.  Every instance's result is bitwise that of the same instance
+advanced alone.
+"""
+import ctypes
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _hip
+from .core._arrays import inst_table
+
+SCHEME_ETDRK4, SCHEME_STRANG = 0, 1
+_TUNED: Dict[tuple, Tuple[int, int]] = {}
+
+
+def spectral_symbol_1d(n: int, L: float = 1.0) -> np.ndarray:
+    """-(2 pi fftfreq(n, L/n))^2 (benchmarks/benchmark_gray_scott.py:52-57)."""
+    k = np.fft.fftfreq(n, L / n) * 2 * np.pi
+    return -(k ** 2)
+
+
+def fd_symbol_1d(n: int, L: float = 1.0) -> np.ndarray:
+    """(2 cos(k h) - 2) / h^2 with h = L / n (the 5-point Laplacian)."""
+    h = L / n
+    k = 2.0 * np.pi * np.fft.fftfreq(n, d=h)
+    return (2.0 * np.cos(k * h) - 2.0) / (h * h)
+
+
+def autotune(dtype: torch.dtype, scheme: int, n: int, B: int, reps: int = 3) -> Tuple[int, int]:
+    """Tile lines (row kernels, column kernels) chosen by timing on the
+    device; cached per process."""
+    key = (dtype, scheme, n)
+    hit = _TUNED.get(key)
+    if hit is not None:
+        return hit
+    chosen = (ctypes.c_int * 2)()
+    _hip.check(_hip.lib().mjx_autotune(_hip.dtype_code(dtype), scheme, n, int(B), int(reps), chosen,
+                                       _hip.stream_ptr()), "mjx_autotune")
+    _TUNED[key] = (chosen[0], chosen[1])
+    return _TUNED[key]
+
+
+class GrayScottBatch:
+    """Uneven-size Gray-Scott instances advanced by one batched plan.
+
+    sizes[i] is instance i's square grid size; F[i], k[i] its feed and kill
+    rates; Du, Dv, dt are shared (one coefficient table per size).
+    scheme 'etdrk4' (Cox-Matthews, FP64 contour coefficients) or 'strang'
+    (IMEX-Strang with implicit diffusion half-steps).  symbol 'spectral'
+    (-k^2, the ETDRK4 benchmark's) or 'fd' (5-point eigenvalues); the domain
+    is [0, L)^2 for every instance."""
+
+    def __init__(self, sizes: Sequence[int], F: Sequence[float], k: Sequence[float], Du: float, Dv: float,
+                 dt: float, scheme: str = "etdrk4", dtype: torch.dtype = torch.float32, L: float = 1.0,
+                 symbol: str = "spectral", steps_per_graph: int = 16, tune: bool = True):
+        if scheme not in ("etdrk4", "strang"):
+            raise ValueError("scheme must be 'etdrk4' or 'strang'")
+        if len(F) != len(sizes) or len(k) != len(sizes):
+            raise ValueError("one F and one k per instance")
+        self.scheme = SCHEME_ETDRK4 if scheme == "etdrk4" else SCHEME_STRANG
+        self.dtype = dtype
+        self.sizes = [int(s) for s in sizes]
+        self.order = sorted(set(self.sizes))
+        sym = spectral_symbol_1d if symbol == "spectral" else fd_symbol_1d
+        self._slot: List[Tuple[int, int]] = []  # instance -> (group, index)
+        members: Dict[int, List[int]] = {n: [] for n in self.order}
+        for i, n in enumerate(self.sizes):
+            self._slot.append((self.order.index(n), len(members[n])))
+            members[n].append(i)
+        self.members = members
+        lib = _hip.lib()
+        self.u, self.v, self.rows, self.tables, self.work = [], [], [], [], []
+        for n in self.order:
+            idx = members[n]
+            B = len(idx)
+            self.u.append(torch.zeros((B, n, n), dtype=dtype, device="cuda"))
+            self.v.append(torch.zeros((B, n, n), dtype=dtype, device="cuda"))
+            lam = sym(n, L)
+            if self.scheme == SCHEME_ETDRK4:
+                from .core.fft_integrators import etdrk4_coefficients
+                coef = etdrk4_coefficients(lam, lam, Du, Dv, dt, dtype)
+                self.tables.append((coef, None))
+                rows = [[0, 0, 0, 0, F[i], k[i], dt] for i in idx]
+                nbytes = lib.mjx_etdrk4_workspace_bytes(_hip.dtype_code(dtype), B, n, n)
+            else:
+                t = torch.from_numpy(lam).to("cuda", dtype)
+                self.tables.append((t, t))
+                rows = [[0.5 * dt * Du, 0.5 * dt * Dv, 0, 0, F[i], k[i], dt] for i in idx]
+                nbytes = lib.mjx_spectral_workspace_bytes(_hip.dtype_code(dtype), B, n, n)
+            self.rows.append(inst_table(rows))
+            self.work.append(torch.empty(nbytes, dtype=torch.uint8, device="cuda"))
+            if tune:
+                autotune(dtype, self.scheme, n, B)
+        G = len(self.order)
+        arr = lambda ct, vals: (ct * G)(*vals)
+        tabs = []
+        for t0, t1 in self.tables:
+            tabs += [_hip.ptr(t0), _hip.ptr(t1)]
+        self._plan = ctypes.c_void_p()
+        rc = lib.mjx_batch_create(
+            _hip.dtype_code(dtype), self.scheme, G, arr(ctypes.c_int, self.order),
+            arr(ctypes.c_int, [len(members[n]) for n in self.order]),
+            arr(ctypes.c_void_p, [_hip.ptr(x) for x in self.u]), arr(ctypes.c_void_p, [_hip.ptr(x) for x in self.v]),
+            arr(ctypes.c_longlong, self.order), arr(ctypes.c_longlong, [n * n for n in self.order]),
+            (ctypes.c_void_p * (2 * G))(*tabs), arr(ctypes.c_void_p, [_hip.ptr(r) for r in self.rows]),
+            arr(ctypes.c_void_p, [_hip.ptr(w) for w in self.work]), int(steps_per_graph), ctypes.byref(self._plan))
+        _hip.check(rc, "mjx_batch_create")
+
+    @property
+    def n_points(self) -> int:
+        return sum(n * n for n in self.sizes)
+
+    def state_bytes(self) -> int:
+        es = 8 if self.dtype == torch.float64 else 4
+        return 2 * es * self.n_points
+
+    def set_state(self, i: int, u, v):
+        g, j = self._slot[i]
+        self.u[g][j].copy_(torch.as_tensor(u))
+        self.v[g][j].copy_(torch.as_tensor(v))
+
+    def state(self, i: int) -> Tuple[torch.Tensor, torch.Tensor]:
+        g, j = self._slot[i]
+        return self.u[g][j], self.v[g][j]
+
+    def run(self, n_steps: int):
+        _hip.check(_hip.lib().mjx_batch_run(self._plan, int(n_steps), 0, _hip.stream_ptr()), "mjx_batch_run")
+
+    def close(self):
+        if self._plan:
+            _hip.lib().mjx_batch_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def c4_workload(n_instances: int = 1024, seed: int = 0, sizes=(64, 128, 256, 512)):
+    """Sizes, F, k of the uneven batched ETDRK4 workload: sizes uniform over
+    ``sizes``, F ~ U(0.02, 0.06), k ~ U(0.055, 0.068) from
+    numpy.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    sz = rng.choice(np.asarray(sizes), size=n_instances)
+    F = rng.uniform(0.02, 0.06, size=n_instances)
+    k = rng.uniform(0.055, 0.068, size=n_instances)
+    return [int(s) for s in sz], F.tolist(), k.tolist()

@@ -1,0 +1,347 @@
+// Batched launch of many Gray-Scott instances of uneven grid sizes.
+//
+// A plan holds G groups; group g is B_g instances of one square size n_g,
+// with its own state, parameter rows, coefficient/symbol tables and
+// workspace (all caller-owned device memory).  run(n_steps):
+//   1. pack every group into its spectral workspace (direct launches);
+//   2. replay a hipGraph that advances every group by K steps: the capture
+//      forks one stream per group off the plan's capture stream, so the
+//      groups' kernels are independent branches of one graph and small
+//      groups run alongside large ones instead of behind them;
+//   3. the remainder (n_steps mod K) and the unpack run as direct launches.
+// Kernel tiles (lines per workgroup) come from the autotuner table, which
+// mjx_autotune fills by timing the candidate tiles on scratch data of the
+// group's shape.  Results are bitwise those of the same instances advanced
+// alone: every kernel is batch-invariant and the graph only changes the
+// order of independent launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mjx_common.h"
+#include "mjx_fft.h"
+#include "spectral_api.h"
+
+namespace mjx {
+
+int etd_pack(int B, int n, const float* u, const float* v, long long pitch, long long istride, cpx<float>* work,
+             hipStream_t st);
+int etd_pack(int B, int n, const double* u, const double* v, long long pitch, long long istride,
+             cpx<double>* work, hipStream_t st);
+int etd_unpack(int B, int n, float* u, float* v, long long pitch, long long istride, int ghosts, cpx<float>* work,
+               hipStream_t st);
+int etd_unpack(int B, int n, double* u, double* v, long long pitch, long long istride, int ghosts,
+               cpx<double>* work, hipStream_t st);
+int etd_advance(int B, int n, const float* coef, const InstParams* ip, int n_steps, cpx<float>* work,
+                hipStream_t st);
+int etd_advance(int B, int n, const double* coef, const InstParams* ip, int n_steps, cpx<double>* work,
+                hipStream_t st);
+
+namespace {
+
+enum { SCHEME_ETDRK4 = 0, SCHEME_STRANG = 1 };
+
+struct Group {
+  int n, B;
+  void *u, *v;
+  long long pitch, istride;
+  const void *t0, *t1;  // ETDRK4: coefficients; Strang: lamx, lamy
+  const InstParams* ip;
+  void* work;
+};
+
+template <typename T>
+struct Ops {
+  static int pack(const Group& g, hipStream_t st) {
+    if (g.t1 == nullptr)
+      return etd_pack(g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
+    int rc = spec_row_fwd_pack(g.n, g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
+    if (rc) return rc;
+    return spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, 1, st);
+  }
+  // Strang: `steps` middle steps (row_gs, col(H^2)); the last step's
+  // trailing half-solve is applied by finish().
+  static int advance(const Group& g, int steps, hipStream_t st) {
+    if (g.t1 == nullptr) return etd_advance(g.B, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
+    for (int s = 0; s < steps; ++s) {
+      int rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, RXN_HEUN);
+      if (!rc) rc = spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, 2, st);
+      if (rc) return rc;
+    }
+    return MJX_OK;
+  }
+  static int finish(const Group& g, int ghosts, hipStream_t st) {
+    if (g.t1 == nullptr)
+      return etd_unpack(g.B, g.n, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts, (cpx<T>*)g.work, st);
+    int rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, RXN_HEUN);
+    if (!rc) rc = spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, 1, st);
+    if (!rc)
+      rc = spec_row_inv_unpack(g.n, g.B, g.n, (const cpx<T>*)g.work, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts, st);
+    return rc;
+  }
+};
+
+}  // namespace
+
+struct BatchPlan {
+  int dtype = MJX_F32, scheme = SCHEME_ETDRK4, K = 1;
+  std::vector<Group> groups;
+  hipStream_t cap = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;  // ev[0] fork, ev[1 + g] join
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  ~BatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    for (auto s : side) (void)hipStreamDestroy(s);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+
+  int advance_all(int steps, hipStream_t st) {
+    for (auto& g : groups) {
+      const int rc = dtype == MJX_F32 ? Ops<float>::advance(g, steps, st) : Ops<double>::advance(g, steps, st);
+      if (rc) return rc;
+    }
+    return MJX_OK;
+  }
+
+  // One graph advancing every group by K steps, one branch per group.
+  int capture() {
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess)
+      return fail(MJX_E_HIP, "batch: hipStreamCreate failed");
+    const size_t G = groups.size();
+    side.resize(G, nullptr);
+    ev.resize(G + 1, nullptr);
+    for (size_t g = 0; g < G; ++g)
+      if (hipStreamCreateWithFlags(&side[g], hipStreamNonBlocking) != hipSuccess)
+        return fail(MJX_E_HIP, "batch: hipStreamCreate failed");
+    for (auto& e : ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return fail(MJX_E_HIP, "batch: hipEventCreate failed");
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail(MJX_E_HIP, "batch: hipStreamBeginCapture failed");
+    int rc = MJX_OK;
+    if (hipEventRecord(ev[0], cap) != hipSuccess) rc = fail(MJX_E_HIP, "batch: event record in capture failed");
+    for (size_t g = 0; g < G && rc == MJX_OK; ++g) {
+      if (hipStreamWaitEvent(side[g], ev[0], 0) != hipSuccess) {
+        rc = fail(MJX_E_HIP, "batch: stream wait in capture failed");
+        break;
+      }
+      rc = dtype == MJX_F32 ? Ops<float>::advance(groups[g], K, side[g]) : Ops<double>::advance(groups[g], K, side[g]);
+      if (rc == MJX_OK && (hipEventRecord(ev[1 + g], side[g]) != hipSuccess ||
+                           hipStreamWaitEvent(cap, ev[1 + g], 0) != hipSuccess))
+        rc = fail(MJX_E_HIP, "batch: join in capture failed");
+    }
+    hipGraph_t gph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cap, &gph);
+    if (rc) {
+      if (gph) (void)hipGraphDestroy(gph);
+      return rc;
+    }
+    if (ec != hipSuccess || !gph) return fail(MJX_E_HIP, "batch: hipStreamEndCapture failed");
+    graph = gph;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      return fail(MJX_E_HIP, "batch: hipGraphInstantiate failed");
+    return MJX_OK;
+  }
+
+  int run(int n_steps, int ghosts, hipStream_t st) {
+    int rc = MJX_OK;
+    for (auto& g : groups) {
+      rc = dtype == MJX_F32 ? Ops<float>::pack(g, st) : Ops<double>::pack(g, st);
+      if (rc) return rc;
+    }
+    // Strang: the first step's leading half-solve is in pack(), the last
+    // step's row pass and trailing half-solve in finish().
+    const int middle = scheme == SCHEME_STRANG ? n_steps - 1 : n_steps;
+    if (middle > 0) {
+      const int reps = middle / K;
+      if (reps > 0 && !exec) {
+        rc = capture();
+        if (rc) return rc;
+      }
+      for (int r = 0; r < reps; ++r)
+        if (hipGraphLaunch(exec, st) != hipSuccess) return fail(MJX_E_HIP, "batch: hipGraphLaunch failed");
+      rc = advance_all(middle - reps * K, st);
+      if (rc) return rc;
+    }
+    for (auto& g : groups) {
+      rc = dtype == MJX_F32 ? Ops<float>::finish(g, ghosts, st) : Ops<double>::finish(g, ghosts, st);
+      if (rc) return rc;
+    }
+    return MJX_OK;
+  }
+};
+
+// ------------------------------------------------------------ autotuner
+namespace {
+
+template <typename T>
+int time_candidates(int kind, int n, int B, int reps, const std::vector<int>& cands, hipStream_t st,
+                    const std::function<int()>& body, int* best_out, std::vector<float>* times) {
+  constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(MJX_E_HIP, "autotune: hipEventCreate failed");
+  float best = 1e30f;
+  int best_l = 0, rc = MJX_OK;
+  for (int L : cands) {
+    set_tuned_lines(kind, dt_, n, L);
+    rc = body();  // warm-up (and first-launch code loading)
+    if (rc) break;
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && rc == MJX_OK; ++r) rc = body();
+    (void)hipEventRecord(e1, st);
+    if (rc) break;
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(MJX_E_HIP, "autotune: hipEventSynchronize failed");
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (times) times->push_back(ms / reps);
+    if (ms < best) {
+      best = ms;
+      best_l = L;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_tuned_lines(kind, dt_, n, rc ? 0 : best_l);
+  *best_out = rc ? 0 : best_l;
+  (void)B;
+  return rc;
+}
+
+// Default lines per workgroup (lines_default<T, N> at run time).
+int lines_default_rt(int dtype, int n) {
+  const int bt = dtype == MJX_F64 ? 128 : 256;
+  const int tn = n / plan_p(n);
+  return tn >= bt ? 1 : bt / tn;
+}
+
+std::vector<int> candidates(int dflt, int cap, int count) {
+  std::vector<int> c;
+  for (int L = std::min(dflt, cap); L >= 1 && (int)c.size() < count; L /= 2) c.push_back(L);
+  return c;
+}
+
+template <typename T>
+int autotune_t(int scheme, int n, int B, int reps, hipStream_t st, int* chosen) {
+  const long long pts = (long long)B * n * n;
+  T* u = nullptr;
+  InstParams* ip = nullptr;
+  void* work = nullptr;
+  T* tab = nullptr;
+  const long long work_bytes = (long long)sizeof(T) * 2 * B * (3LL * n * n + 3LL * n * (n + 2));
+  const long long tab_elems = 6LL * n * (n + 2);
+  int rc = MJX_OK;
+  if (hipMalloc(&u, 2 * pts * sizeof(T)) != hipSuccess || hipMalloc(&ip, B * sizeof(InstParams)) != hipSuccess ||
+      hipMalloc(&work, work_bytes) != hipSuccess || hipMalloc(&tab, tab_elems * sizeof(T)) != hipSuccess)
+    rc = fail(MJX_E_HIP, "autotune: scratch allocation failed");
+  if (!rc && (hipMemsetAsync(u, 0, 2 * pts * sizeof(T), st) != hipSuccess ||
+              hipMemsetAsync(ip, 0, B * sizeof(InstParams), st) != hipSuccess ||
+              hipMemsetAsync(work, 0, work_bytes, st) != hipSuccess ||
+              hipMemsetAsync(tab, 0, tab_elems * sizeof(T), st) != hipSuccess))
+    rc = fail(MJX_E_HIP, "autotune: scratch clear failed");
+  const int dflt = lines_default_rt(sizeof(T) == 8 ? MJX_F64 : MJX_F32, n);
+  if (!rc) {
+    if (scheme == SCHEME_ETDRK4) {
+      Group g{n, B, u, u + pts, n, (long long)n * n, tab, nullptr, ip, work};
+      auto body = [&]() { return Ops<T>::advance(g, 1, st); };
+      rc = time_candidates<T>(TK_ETD_ROW, n, B, reps, candidates(dflt, n, 3), st, body, &chosen[0], nullptr);
+      if (!rc) rc = time_candidates<T>(TK_ETD_COL, n, B, reps, candidates(dflt, n / 2, 3), st, body, &chosen[1], nullptr);
+    } else {
+      cpx<T>* W = (cpx<T>*)work;
+      auto row = [&]() { return spec_row_gs(n, B, n, W, ip, st, RXN_HEUN); };
+      auto col = [&]() { return spec_col_apply(n, B, n, W, tab, tab, nullptr, ip, 2, st); };
+      rc = time_candidates<T>(TK_ROW, n, B, reps, candidates(dflt, n, 4), st, row, &chosen[0], nullptr);
+      if (!rc) rc = time_candidates<T>(TK_COL, n, B, reps, candidates(dflt, n / 2, 4), st, col, &chosen[1], nullptr);
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  if (u) (void)hipFree(u);
+  if (ip) (void)hipFree(ip);
+  if (work) (void)hipFree(work);
+  if (tab) (void)hipFree(tab);
+  return rc;
+}
+
+}  // namespace
+}  // namespace mjx
+
+using namespace mjx;
+
+extern "C" {
+
+int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const int* B, void* const* u, void* const* v,
+                     const long long* pitch, const long long* istride, const void* const* tables,
+                     const double* const* inst, void* const* work, int steps_per_graph, void** plan) {
+  if (!plan) return fail(MJX_E_ARG, "null plan pointer");
+  *plan = nullptr;
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG) return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4) or 1 (Strang)");
+  if (n_groups < 1 || !n || !B || !u || !v || !pitch || !istride || !tables || !inst || !work)
+    return fail(MJX_E_ARG, "null pointer argument or no groups");
+  if (steps_per_graph < 1 || steps_per_graph > 256) return fail(MJX_E_ARG, "steps_per_graph must be in [1, 256]");
+  int rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  std::unique_ptr<BatchPlan> p(new BatchPlan());
+  p->dtype = dtype;
+  p->scheme = scheme;
+  p->K = steps_per_graph;
+  for (int g = 0; g < n_groups; ++g) {
+    const int ng = n[g];
+    const bool ok = scheme == SCHEME_ETDRK4 ? (ng == 32 || ng == 64 || ng == 128 || ng == 256 || ng == 512 || ng == 1024)
+                                            : is_pow2_len(ng);
+    if (!ok) return fail(MJX_E_SIZE, "batch group " + std::to_string(g) + ": unsupported size " + std::to_string(ng));
+    if (B[g] < 1 || !u[g] || !v[g] || !inst[g] || !work[g] || !tables[2 * g] ||
+        (scheme == SCHEME_STRANG && !tables[2 * g + 1]))
+      return fail(MJX_E_ARG, "batch group " + std::to_string(g) + ": null pointer or empty");
+    if (pitch[g] < ng) return fail(MJX_E_ARG, "batch group " + std::to_string(g) + ": pitch smaller than n");
+    Group gr{ng, B[g], u[g], v[g], pitch[g], istride[g], tables[2 * g],
+             scheme == SCHEME_STRANG ? tables[2 * g + 1] : nullptr, reinterpret_cast<const InstParams*>(inst[g]),
+             work[g]};
+    p->groups.push_back(gr);
+  }
+  *plan = p.release();
+  return MJX_OK;
+}
+
+int mjx_batch_run(void* plan, int n_steps, int fill_ghosts, void* stream) {
+  if (!plan) return fail(MJX_E_ARG, "null plan");
+  if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
+  if (n_steps == 0) return MJX_OK;
+  BatchPlan* p = static_cast<BatchPlan*>(plan);
+  if (fill_ghosts)
+    for (auto& g : p->groups)
+      if (g.pitch < g.n + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= n + 2");
+  return p->run(n_steps, fill_ghosts, (hipStream_t)stream);
+}
+
+int mjx_batch_destroy(void* plan) {
+  delete static_cast<BatchPlan*>(plan);
+  return MJX_OK;
+}
+
+int mjx_autotune(int dtype, int scheme, int n, int B, int reps, int* chosen, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG) return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4) or 1 (Strang)");
+  if (!is_pow2_len(n) || (scheme == SCHEME_ETDRK4 && (n < 32 || n > 1024)))
+    return fail(MJX_E_SIZE, "unsupported size " + std::to_string(n));
+  if (B < 1 || reps < 1 || !chosen) return fail(MJX_E_ARG, "B, reps >= 1 and chosen required");
+  int rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return dtype == MJX_F32 ? autotune_t<float>(scheme, n, B, reps, st, chosen)
+                          : autotune_t<double>(scheme, n, B, reps, st, chosen);
+}
+
+}  // extern "C"

@@ -277,4 +277,88 @@ struct EtdLaunch {
   }
 };
 
+// Runtime tile dispatch for the ETDRK4 kernels: the lines per workgroup
+// is the default or the autotuned value among {D, D/2, D/4} (D the default,
+// capped by the grid), instantiated per transform length.
+template <typename T, int N>
+struct EtdTiles {
+  static constexpr int D = lines_default<T, N>();
+  static constexpr int DR = D < N ? D : N;          // rows per block
+  static constexpr int DC = D < N / 2 ? D : N / 2;  // columns per block
+  static int pick(int tuned, int dmax) {
+    if (tuned == dmax || tuned == dmax / 2 || tuned == dmax / 4) return tuned > 0 ? tuned : dmax;
+    return dmax;
+  }
+  static int row_nl(int L, int B, int ny, const cpx<T>* Win, cpx<T>* Wout, const InstParams* ip, hipStream_t st) {
+    if constexpr (DR >= 4) if (L == DR / 4) return EtdLaunch<T>::template row_nl<N, DR / 4>(B, ny, Win, Wout, ip, st);
+    if constexpr (DR >= 2) if (L == DR / 2) return EtdLaunch<T>::template row_nl<N, DR / 2>(B, ny, Win, Wout, ip, st);
+    return EtdLaunch<T>::template row_nl<N, DR>(B, ny, Win, Wout, ip, st);
+  }
+  template <int STAGE>
+  static int col(int L, int B, int nx, const cpx<T>* Win, const cpx<T>* Win2, cpx<T>* Uh, cpx<T>* Nn, cpx<T>* Fh,
+                 const T* coef, cpx<T>* Wout, hipStream_t st) {
+    if constexpr (DC >= 4)
+      if (L == DC / 4) return EtdLaunch<T>::template col<N, DC / 4, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+    if constexpr (DC >= 2)
+      if (L == DC / 2) return EtdLaunch<T>::template col<N, DC / 2, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+    return EtdLaunch<T>::template col<N, DC, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+  }
+};
+
+// n_steps ETDRK4 steps on the packed state Ws of B square N x N instances
+// (workspace planes as laid out by etd_planes), without the pack/unpack.
+template <typename T>
+struct EtdPlanes {
+  cpx<T>*Ws, *Wg, *Wn, *Uh, *Nn, *Fh;
+};
+
+template <typename T>
+inline EtdPlanes<T> etd_planes(cpx<T>* work, int B, int n) {
+  const long long plane = (long long)B * n * n;
+  const long long splane = (long long)B * n * (n + 2);
+  EtdPlanes<T> p;
+  p.Ws = work;
+  p.Wg = p.Ws + plane;
+  p.Wn = p.Wg + plane;
+  p.Uh = p.Wn + plane;
+  p.Nn = p.Uh + splane;
+  p.Fh = p.Nn + splane;
+  return p;
+}
+
+template <typename T, int N>
+int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
+                    hipStream_t st) {
+  using E = EtdTiles<T, N>;
+  constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
+  const int LR = E::pick(tuned_lines(TK_ETD_ROW, dt_, N), E::DR);
+  const int LC = E::pick(tuned_lines(TK_ETD_COL, dt_, N), E::DC);
+  int rc = MJX_OK;
+  for (int s = 0; s < n_steps && rc == 0; ++s) {
+    rc = E::row_nl(LR, B, N, p.Ws, p.Wn, ip, st);
+    if (!rc) rc = E::template col<1>(LC, B, N, p.Ws, p.Wn, p.Uh, p.Nn, p.Fh, coef, p.Wg, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.Wg, p.Wn, ip, st);
+    if (!rc) rc = E::template col<2>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Wg, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.Wg, p.Wn, ip, st);
+    if (!rc) rc = E::template col<3>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Wg, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.Wg, p.Wn, ip, st);
+    if (!rc) rc = E::template col<4>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Ws, st);
+  }
+  return rc;
+}
+
+template <typename T>
+int etd_steps(int B, int n, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
+              hipStream_t st) {
+  switch (n) {
+    case 32: return etd_steps_sized<T, 32>(B, p, coef, ip, n_steps, st);
+    case 64: return etd_steps_sized<T, 64>(B, p, coef, ip, n_steps, st);
+    case 128: return etd_steps_sized<T, 128>(B, p, coef, ip, n_steps, st);
+    case 256: return etd_steps_sized<T, 256>(B, p, coef, ip, n_steps, st);
+    case 512: return etd_steps_sized<T, 512>(B, p, coef, ip, n_steps, st);
+    case 1024: return etd_steps_sized<T, 1024>(B, p, coef, ip, n_steps, st);
+    default: return fail(MJX_E_SIZE, "ETDRK4 grid " + std::to_string(n) + " not supported: square n in 32..1024");
+  }
+}
+
 }  // namespace mjx

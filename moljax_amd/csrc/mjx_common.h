@@ -25,6 +25,13 @@ int fail(int code, const std::string& msg);
 int check_launch(const char* what);
 int ensure_tables(hipStream_t stream);
 
+// Tile shapes chosen by the autotuner: lines (rows or columns) per
+// workgroup for a kernel family, precision and transform length; 0 means the
+// built-in default.  Set by mjx_autotune_* or mjx_tune_set, read at launch.
+enum { TK_ROW = 0, TK_COL = 1, TK_ETD_ROW = 2, TK_ETD_COL = 3, TK_NUM = 4 };
+int tuned_lines(int kind, int dtype, int n);
+void set_tuned_lines(int kind, int dtype, int n, int lines);
+
 inline bool is_pow2_len(int n) {
   return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 ||
          n == 4096;

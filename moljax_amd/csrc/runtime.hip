@@ -1,6 +1,7 @@
 // Library runtime: error reporting and the per-device FFT twiddle tables.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -81,7 +82,43 @@ const cpx<double>* twiddles_f64() {
 
 }  // namespace mjx
 
+namespace mjx {
+
+// [kind][dtype][log2 n], lock-free: a launch reads one entry.
+static std::atomic<int> g_tuned[TK_NUM][2][13];
+
+static int log2_len(int n) {
+  int k = 0;
+  while ((1 << k) < n && k < 12) ++k;
+  return (1 << k) == n ? k : -1;
+}
+
+int tuned_lines(int kind, int dtype, int n) {
+  const int k = log2_len(n);
+  if (kind < 0 || kind >= TK_NUM || (dtype != MJX_F32 && dtype != MJX_F64) || k < 0) return 0;
+  return g_tuned[kind][dtype][k].load(std::memory_order_relaxed);
+}
+
+void set_tuned_lines(int kind, int dtype, int n, int lines) {
+  const int k = log2_len(n);
+  if (kind < 0 || kind >= TK_NUM || (dtype != MJX_F32 && dtype != MJX_F64) || k < 0) return;
+  g_tuned[kind][dtype][k].store(lines, std::memory_order_relaxed);
+}
+
+}  // namespace mjx
+
 extern "C" {
+
+int mjx_tune_get(int kind, int dtype, int n) { return mjx::tuned_lines(kind, dtype, n); }
+
+int mjx_tune_set(int kind, int dtype, int n, int lines) {
+  if (kind < 0 || kind >= mjx::TK_NUM) return mjx::fail(MJX_E_ARG, "unknown tuning kind");
+  if (dtype != MJX_F32 && dtype != MJX_F64) return mjx::fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (!mjx::is_pow2_len(n) && n != 8) return mjx::fail(MJX_E_ARG, "n must be a power of two in [16, 4096]");
+  if (lines < 0 || lines > 64 || (lines & (lines - 1))) return mjx::fail(MJX_E_ARG, "lines must be 0 or a power of two <= 64");
+  mjx::set_tuned_lines(kind, dtype, n, lines);
+  return MJX_OK;
+}
 
 int mjx_version(void) { return 100; }
 

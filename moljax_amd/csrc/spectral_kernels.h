@@ -409,6 +409,15 @@ constexpr int lines_default() {
 
 inline int pick_lines(int def, int limit) { return def < limit ? def : limit; }
 
+// Tuned lines per workgroup for a kernel family, else the default; always a
+// power of two no larger than the default (what with_lines instantiates).
+template <typename T, int N>
+inline int lines_for(int kind) {
+  const int t = tuned_lines(kind, sizeof(T) == 8 ? MJX_F64 : MJX_F32, N);
+  const int d = lines_default<T, N>();
+  return (t > 0 && t <= d) ? t : d;
+}
+
 template <typename T, int N, int L>
 struct RowLaunch {
   static int fwd_pack(int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
@@ -525,7 +534,7 @@ struct Launch {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, a, b, pitch, istride, W, in_scale, st);
+    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, a, b, pitch, istride, W, in_scale, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -537,7 +546,7 @@ struct Launch {
     switch (nx) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
-    return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, a, b, pitch, \
+    return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, a, b, pitch, \
                                                    istride, ghosts, keep, st);
       MJX_FOR_SIZES(X)
 #undef X
@@ -549,7 +558,7 @@ struct Launch {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_default<T, N>(), ny), B, ny, W, ip, rxn, st);
+    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, ip, rxn, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -562,7 +571,7 @@ struct Launch {
     switch (ny) {
 #define X(N)                                                                                                 \
   case N:                                                                                                    \
-    return with_lines<T, N, Col<N>::template F>(pick_lines(lines_default<T, N>(), nx / 2), B, nx, W, lamx, lamy, lam2d, \
+    return with_lines<T, N, Col<N>::template F>(pick_lines(lines_for<T, N>(TK_COL), nx / 2), B, nx, W, lamx, lamy, lam2d, \
                                              ip, nrep, s, st);
       MJX_FOR_SIZES(X)
 #undef X
