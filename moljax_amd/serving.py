@@ -1,0 +1,301 @@
+"""Continuous-batching request loop for mixed diffusion / Gray-Scott solves
+on one GPU (build-only; the reference has no serving path).
+
+Request kinds (all periodic or homogeneous-BC squares, states resident on the
+device from admission to retirement):
+  'fft_cn'   periodic Crank-Nicolson diffusion, spectral symbol on [0, 1)^2
+             (fft_solvers.fft_cn_step semantics);
+  'dst'      homogeneous Dirichlet backward-Euler diffusion steps on n - 1
+             interior vertices of [0, 1]^2 (DST-I Helmholtz solves);
+  'dct'      homogeneous Neumann backward-Euler diffusion steps on n cells of
+             [0, 1]^2 (DCT-II Helmholtz solves);
+  'imex_gs'  Gray-Scott IMEX-Strang on [0, 2.5]^2 (FD symbol), Du = D,
+             Dv = D / 2, F = 0.035, k = 0.065.
+
+Scheduling:
+  * admission is FIFO by arrival time and by HBM footprint: a request enters
+    when its state and working buffers fit the remaining budget;
+  * every request advances in chunks of ``chunk`` steps counted from its own
+    first step (the last chunk is the remainder), so its arithmetic does not
+    depend on what else is running;
+  * each iteration runs one chunk of every active request; requests of the
+    same (kind, size, chunk length) form one batched launch with per-request
+    parameters in the instance rows, and the launches of different groups go
+    to different streams so they overlap on the device;
+  * requests that arrive while others run are admitted at the next
+    iteration; finished ones retire and release their footprint.
+Because every kernel is batch-invariant, a request's result is bitwise the
+same alone and in any mix.
+"""
+import ctypes
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _hip
+from .core._arrays import inst_table
+
+KINDS = ("fft_cn", "dst", "dct", "imex_gs")
+GS_F, GS_K, GS_L = 0.035, 0.065, 2.5
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str
+    n: int
+    D: float
+    dt: float
+    steps: int
+    seed: int
+    arrival: float = 0.0
+    # filled by the server
+    admitted: float = math.nan
+    finished: float = math.nan
+    done_steps: int = 0
+    state: List[torch.Tensor] = field(default_factory=list)
+
+    @property
+    def grid(self) -> int:
+        """Interior points per side (DST-I runs on n - 1 vertices)."""
+        return self.n - 1 if self.kind == "dst" else self.n
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+    @property
+    def points(self) -> int:
+        return self.grid * self.grid * (2 if self.kind == "imex_gs" else 1)
+
+
+def initial_state(req: Request, dtype: torch.dtype) -> List[torch.Tensor]:
+    """Seeded random-mode initial condition (numpy default_rng(seed))."""
+    rng = np.random.default_rng(req.seed)
+    m = req.grid
+    modes = [(rng.normal(0.0, 1.0), int(rng.integers(1, 6)), int(rng.integers(1, 6)), rng.uniform(0, 2 * np.pi))
+             for _ in range(4)]
+    if req.kind == "dst":
+        x = np.arange(1, m + 1) / (m + 1)
+        X, Y = np.meshgrid(x, x)
+        f = sum(a * np.sin(np.pi * p * X) * np.sin(np.pi * q * Y) for a, p, q, _ in modes)
+        fields = [f]
+    elif req.kind == "dct":
+        x = (np.arange(m) + 0.5) / m
+        X, Y = np.meshgrid(x, x)
+        f = sum(a * np.cos(np.pi * p * X) * np.cos(np.pi * q * Y) for a, p, q, _ in modes)
+        fields = [f]
+    else:
+        x = np.arange(m) / m
+        X, Y = np.meshgrid(x, x)
+        f = sum(a * np.cos(2 * np.pi * (p * X + q * Y) + ph) for a, p, q, ph in modes)
+        if req.kind == "fft_cn":
+            fields = [f]
+        else:
+            g = (f - f.min()) / max(f.max() - f.min(), 1e-12)
+            fields = [1.0 - 0.5 * g, 0.25 * g]
+    return [torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype) for a in fields]
+
+
+def _spectral(n: int, L: float) -> np.ndarray:
+    k = np.fft.fftfreq(n, L / n) * 2 * np.pi
+    return -(k ** 2)
+
+
+def _fd(n: int, L: float) -> np.ndarray:
+    h = L / n
+    k = 2.0 * np.pi * np.fft.fftfreq(n, d=h)
+    return (2.0 * np.cos(k * h) - 2.0) / (h * h)
+
+
+def _dirichlet(m: int) -> np.ndarray:
+    h = 1.0 / (m + 1)
+    k = np.arange(1, m + 1, dtype=np.float64)
+    return -4.0 / (h * h) * np.sin(np.pi * k / (2 * (m + 1))) ** 2
+
+
+def _neumann(m: int) -> np.ndarray:
+    h = 1.0 / m
+    k = np.arange(m, dtype=np.float64)
+    return -4.0 / (h * h) * np.sin(np.pi * k / (2 * m)) ** 2
+
+
+class _Symbols:
+    def __init__(self):
+        self._c: Dict[tuple, torch.Tensor] = {}
+
+    def get(self, kind: str, m: int, dtype: torch.dtype) -> torch.Tensor:
+        key = (kind, m, dtype)
+        t = self._c.get(key)
+        if t is None:
+            lam = {"fft_cn": lambda: _spectral(m, 1.0), "imex_gs": lambda: _fd(m, GS_L),
+                   "dst": lambda: _dirichlet(m), "dct": lambda: _neumann(m)}[kind]()
+            t = torch.from_numpy(lam).to("cuda", dtype)
+            self._c[key] = t
+        return t
+
+
+def advance(kind: str, batch: List[torch.Tensor], rows: torch.Tensor, steps: int, lam: torch.Tensor,
+            st: int):
+    """Advance a (B, m, m) batch (one or two fields) by ``steps`` steps in
+    place on stream handle ``st``."""
+    lib = _hip.lib()
+    a = batch[0]
+    B, m = a.shape[0], a.shape[-1]
+    dc = _hip.dtype_code(a.dtype)
+    if kind == "fft_cn":
+        work = torch.empty(lib.mjx_spectral_workspace_bytes(dc, B, m, m), dtype=torch.uint8, device=a.device)
+        rc = lib.mjx_spectral_pair(dc, B, m, m, _hip.ptr(a), None, m, m * m, _hip.ptr(a), None, m, m * m, 0,
+                                   _hip.ptr(lam), _hip.ptr(lam), None, _hip.ptr(rows), int(steps), _hip.ptr(work),
+                                   st)
+        _hip.check(rc, "mjx_spectral_pair")
+    elif kind == "imex_gs":
+        b = batch[1]
+        work = torch.empty(lib.mjx_spectral_workspace_bytes(dc, B, m, m), dtype=torch.uint8, device=a.device)
+        rc = lib.mjx_gs_imex_strang(dc, B, m, m, _hip.ptr(a), _hip.ptr(b), m, m * m, 0, _hip.ptr(lam),
+                                    _hip.ptr(lam), None, _hip.ptr(rows), int(steps), _hip.ptr(work), st)
+        _hip.check(rc, "mjx_gs_imex_strang")
+    else:
+        kcode = 0 if kind == "dst" else 1
+        other = torch.empty_like(a)
+        work = torch.empty_like(a)
+        src, dst = a, other
+        for _ in range(steps):
+            rc = lib.mjx_r2r_solve2d(dc, kcode, B, m, m, _hip.ptr(src), m, m * m, _hip.ptr(dst), m, m * m,
+                                     _hip.ptr(lam), _hip.ptr(lam), _hip.ptr(rows), 0, _hip.ptr(work), st)
+            _hip.check(rc, "mjx_r2r_solve2d")
+            src, dst = dst, src
+        if src is not a:
+            a.copy_(src)
+
+
+def _rows(req: Request) -> List[float]:
+    if req.kind == "fft_cn":
+        h = 0.5 * req.dt * req.D
+        return [h, 0.0, h, 0.0, 0.0, 0.0, req.dt]
+    if req.kind == "imex_gs":
+        return [0.5 * req.dt * req.D, 0.25 * req.dt * req.D, 0.0, 0.0, GS_F, GS_K, req.dt]
+    return [req.dt * req.D, 0.0, 0.0, 0.0, 0.0, 0.0, req.dt]
+
+
+class Server:
+    """The continuous-batching loop.  ``hbm_budget`` bounds the bytes of
+    resident request state plus working buffers (footprint = 4x state)."""
+
+    def __init__(self, dtype: torch.dtype = torch.float32, hbm_budget: int = 16 << 30, chunk: int = 32,
+                 n_streams: int = 4):
+        self.dtype = dtype
+        self.budget = int(hbm_budget)
+        self.chunk = int(chunk)
+        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        self.symbols = _Symbols()
+        self.used = 0
+        self.peak_used = 0
+        self.iterations = 0
+        self.peak_active = 0
+        self.max_group = 0
+
+    def footprint(self, req: Request) -> int:
+        es = 8 if self.dtype == torch.float64 else 4
+        return 4 * req.points * es
+
+    def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
+        """Serve ``requests`` (arrival times in seconds from the start);
+        returns them with admitted/finished times and final states."""
+        for r in requests:
+            if self.footprint(r) > self.budget:
+                raise ValueError(f"request {r.rid} needs {self.footprint(r)} bytes, above the HBM budget")
+        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        active: List[Request] = []
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+        while pending or active:
+            now = clock()
+            while pending and pending[0].arrival <= now and self.used + self.footprint(pending[0]) <= self.budget:
+                r = pending.popleft()
+                r.state = initial_state(r, self.dtype)
+                r.admitted = time.perf_counter() - t0
+                self.used += self.footprint(r)
+                self.peak_used = max(self.peak_used, self.used)
+                active.append(r)
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.002)))
+                continue
+            self.peak_active = max(self.peak_active, len(active))
+            groups: Dict[tuple, List[Request]] = {}
+            for r in active:
+                n_chunk = min(self.chunk, r.steps - r.done_steps)
+                groups.setdefault((r.kind, r.grid, n_chunk), []).append(r)
+            cur = torch.cuda.current_stream()
+            for gi, (key, reqs) in enumerate(sorted(groups.items())):
+                kind, m, n_chunk = key
+                s = self.streams[gi % len(self.streams)]
+                s.wait_stream(cur)
+                with torch.cuda.stream(s):
+                    nf = len(reqs[0].state)
+                    batch = [torch.stack([r.state[f] for r in reqs]) for f in range(nf)]
+                    rows = inst_table([_rows(r) for r in reqs])
+                    advance(kind, batch, rows, n_chunk, self.symbols.get(kind, m, self.dtype), s.cuda_stream)
+                    for i, r in enumerate(reqs):
+                        for f in range(nf):
+                            r.state[f].copy_(batch[f][i])
+                            r.state[f].record_stream(s)
+                self.max_group = max(self.max_group, len(reqs))
+            for s in self.streams:
+                cur.wait_stream(s)
+            torch.cuda.synchronize()
+            self.iterations += 1
+            t_done = time.perf_counter() - t0
+            still = []
+            for r in active:
+                r.done_steps += min(self.chunk, r.steps - r.done_steps)
+                if r.done_steps >= r.steps:
+                    r.finished = t_done
+                    self.used -= self.footprint(r)
+                else:
+                    still.append(r)
+            active = still
+        return requests
+
+
+def poisson_stream(n_requests: int, rate: float = 200.0, seed: int = 0, sizes=(64, 128, 256, 512, 1024),
+                   kinds=KINDS, steps=(10, 500)) -> List[Request]:
+    """Seeded open-loop workload: exponential inter-arrival times at ``rate``
+    per second; kind, size uniform; D ~ logU(1e-4, 1e-1); dt ~ logU(1e-3,
+    1e-1); steps ~ U{steps[0]..steps[1]}."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        out.append(Request(rid=i, kind=str(rng.choice(kinds)), n=int(rng.choice(sizes)),
+                           D=float(10 ** rng.uniform(-4, -1)), dt=float(10 ** rng.uniform(-3, -1)),
+                           steps=int(rng.integers(steps[0], steps[1] + 1)), seed=int(rng.integers(1 << 31)),
+                           arrival=t))
+    return out
+
+
+def latency_stats(reqs: List[Request]) -> Dict[str, float]:
+    lat = np.array([r.latency for r in reqs])
+    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3)}
+
+
+def sequential(reqs: List[Request], dtype: torch.dtype = torch.float64) -> float:
+    """Seconds to run the same requests one at a time, all steps in one
+    launch sequence each (no batching), in ``dtype``."""
+    sym = _Symbols()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in reqs:
+        st = initial_state(r, dtype)
+        batch = [s.unsqueeze(0) for s in st]
+        advance(r.kind, batch, inst_table([_rows(r)]), r.steps, sym.get(r.kind, r.grid, dtype),
+                torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
