@@ -7,7 +7,11 @@ k=0.065, dt=1.0; B instances with seeded ICs (seeds 0..B-1), FP32 state,
 resident in HBM before the timed region.  Also reported in the same line:
 the FP64 single-instance GPU form, the FP32-vs-FP64 relative L2 error curve,
 per-step latency percentiles, the roofline of the dominant kernel (HIP events
-on the launch stream) and the CPU oracle timed on this host.
+on the launch stream) and the CPU oracle timed on this host.  Secondary
+workloads in the same line: the uneven-size batched ETDRK4 plan (1024
+instances of {64,128,256,512}^2), backward-Euler Newton-Krylov on a 511^2
+Dirichlet grid (iteration counts per step), and the serving loop (Poisson
+arrivals at 200/s, p50/p99 request latency, throughput vs sequential FP64).
 
 Prints ONE JSON line.  Only the cpu_baseline leg touches the oracle.
 """
@@ -68,13 +72,14 @@ def time_form(dtype, B, steps, warmup, kernel_timing=False):
     run(U, V, lam, rows, work, steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kt = None
+    kt = kts = None
     if kernel_timing:
         kt = {k: _hip.kernel_timing(k) for k in (KT_ROW_GS, KT_COL)}
+        kts = {k: _hip.kernel_timing_samples(k) for k in (KT_ROW_GS, KT_COL)}
         _hip.lib().mjx_kernel_timing_enable(0)
     finite = bool(torch.isfinite(U).all() and torch.isfinite(V).all())
     del U, V, work
-    return t1 - t0, kt, finite
+    return t1 - t0, kt, finite, kts
 
 
 def error_curve(checkpoints, seed=0):
@@ -114,6 +119,107 @@ def cpu_baseline(target_s=10.0):
             "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count()}
 
 
+def step_latency(kt_samples):
+    """Per-step device latency of the fused chain (row_gs + col_apply launch
+    durations of the same step, HIP events) and its percentiles."""
+    row, col = kt_samples[KT_ROW_GS], kt_samples[KT_COL]
+    steps = np.array(row[:len(col)]) + np.array(col[:len(row)])
+    return {"row_gs_avg_ms": float(np.mean(row)), "col_apply_avg_ms": float(np.mean(col)),
+            "step_p50_ms": float(np.percentile(steps, 50)), "step_p99_ms": float(np.percentile(steps, 99)),
+            "step_max_ms": float(np.max(steps)), "samples": int(len(steps))}
+
+
+def measured_traffic(kernel, B):
+    """HBM bytes per launch of ``kernel`` at batch B from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, copy-kernel
+    calibrated), or None when the summary has no matching line."""
+    path = os.path.join(ROOT, "profiles", "gray_scott_256_imex_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == f"B={B}":
+                return float(f[3])
+    except OSError:
+        pass
+    return None
+
+
+def c4_batched_etdrk4(n_inst, steps, seed=0):
+    """Uneven-size batched ETDRK4 (1024 instances of {64..512}^2, FP32 state,
+    FP64 contour coefficients, hipGraph replay, autotuned tiles)."""
+    import torch
+    from moljax_amd.batch import GrayScottBatch, c4_workload
+    from moljax_amd.workloads import gray_scott_ic
+    sizes, F, k = c4_workload(n_inst, seed)
+    t0 = time.perf_counter()
+    b = GrayScottBatch(sizes, F, k, 2e-5, 1e-5, 1.0, scheme="etdrk4", dtype=torch.float32, steps_per_graph=16)
+    setup_s = time.perf_counter() - t0
+    for i, n in enumerate(sizes):
+        u, v = gray_scott_ic(n, n, i, square=max(2, round(20 * n / 256)))
+        b.set_state(i, torch.from_numpy(u[1:-1, 1:-1]), torch.from_numpy(v[1:-1, 1:-1]))
+    b.run(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(t).all()) for t in b.u + b.v)
+    from moljax_amd import _hip
+    tiles = {str(n): [_hip.lib().mjx_tune_get(2, 0, n), _hip.lib().mjx_tune_get(3, 0, n)] for n in b.order}
+    out = {"value": b.n_points * steps / el, "unit": "grid-point-steps/s", "instances": n_inst,
+           "points": b.n_points, "state_bytes": b.state_bytes(), "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "sizes": {str(n): len(b.members[n]) for n in b.order}, "tiles_row_col": tiles,
+           "setup_s": setup_s, "finite": finite,
+           "note": "2000-step workload timed on its first steps (per-step cost is constant)"}
+    b.close()
+    return out
+
+
+def c3_newton_krylov(n, steps, op_dtype):
+    """Backward-Euler Newton-Krylov steps of u_t = D lap u - u^3 + u on an
+    n x n Dirichlet vertex grid, GMRES(30) rtol 1e-8, DST preconditioner."""
+    import torch
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import DSTDiffusionPreconditioner
+    from moljax_amd.core.reaction_diffusion import be_step, create_reaction_diffusion_model, sine_mode_ic
+    model = create_reaction_diffusion_model(n, D=1e-3, reaction=(-1.0, 0.0, 1.0, 0.0), bc="dirichlet")
+    y = {"u": torch.from_numpy(sine_mode_ic(n, seed=0)).cuda()}
+    P = DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"})
+    nkp = NKParams(krylov_tol=1e-8, gmres_restart=30, operator_dtype=op_dtype)
+    be_step(model, y, 0.0, 1e-2, P, nkp)  # warm-up
+    torch.cuda.synchronize()
+    newton, gmres, res = [], [], []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        y, st = be_step(model, y, s * 1e-2, 1e-2, P, nkp)
+        newton.append(st.newton_iters)
+        gmres.append(st.lin_iters)
+        res.append(st.final_res_norm)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"ms_per_step": 1e3 * el / steps, "newton_iters": newton, "gmres_iters": gmres,
+            "max_final_residual": max(res), "operator_dtype": str(op_dtype).replace("torch.", "")}
+
+
+def c5_serving(n_req, rate, seed=0):
+    import torch
+    from moljax_amd.serving import Server, latency_stats, poisson_stream, sequential
+    reqs = poisson_stream(n_req, rate=rate, seed=seed)
+    srv = Server(dtype=torch.float32, chunk=32)
+    srv.run(poisson_stream(8, rate=rate, seed=seed + 1, sizes=(64, 128)), realtime=True)  # warm-up
+    srv = Server(dtype=torch.float32, chunk=32)
+    t0 = time.perf_counter()
+    srv.run(reqs, realtime=True)
+    makespan = time.perf_counter() - t0
+    gps = sum(r.points * r.steps for r in reqs)
+    seq = sequential(reqs, torch.float64)
+    return {"requests": n_req, "rate_per_s": rate, **latency_stats(reqs), "makespan_s": makespan,
+            "throughput_gps": gps / makespan, "sequential_fp64_s": seq, "sequential_fp64_gps": gps / seq,
+            "served_vs_sequential_fp64": seq / makespan, "iterations": srv.iterations,
+            "peak_active": srv.peak_active, "max_group": srv.max_group,
+            "mix": {k: sum(r.kind == k for r in reqs) for k in ("fft_cn", "dst", "dct", "imex_gs")}}
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
@@ -137,6 +243,11 @@ def main():
     ap.add_argument("--sweep", type=str, default="", help="comma-separated batch sizes to time as well")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (byte-counter calibration)")
+    ap.add_argument("--c4-steps", type=int, default=100, help="steps of the uneven batched ETDRK4 workload (0: skip)")
+    ap.add_argument("--c4-instances", type=int, default=1024)
+    ap.add_argument("--c3-steps", type=int, default=10, help="Newton-Krylov backward-Euler steps (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="serving-loop requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         if int(os.environ.get("RANK", "0")) == 0:
@@ -158,12 +269,12 @@ def main():
         del src, dst
 
     log(f"fp32 batched B={B} steps={K}")
-    t32, kt, fin32 = time_form(torch.float32, B, K, W, kernel_timing=True)
+    t32, kt, fin32, kts = time_form(torch.float32, B, K, W, kernel_timing=True)
     gps32 = B * N * N * K / t32
     ms_step = 1e3 * t32 / K
 
     log(f"fp64 single steps={args.fp64_steps}")
-    t64, _, fin64 = time_form(torch.float64, 1, args.fp64_steps, W)
+    t64, _, fin64, _ = time_form(torch.float64, 1, args.fp64_steps, W)
     gps64 = N * N * args.fp64_steps / t64
 
     # roofline of the dominant kernel: the step's compulsory 16 B/GPS (FP32
@@ -174,18 +285,32 @@ def main():
     avg_s = ms_dom / n_dom / 1e3
     algo_bytes = 8.0 * B * N * N
     achieved = algo_bytes / avg_s
-    step_latency = {"row_gs_avg_ms": ms_gs / n_gs, "col_apply_avg_ms": ms_col / n_col}
+    lat = step_latency(kts)
+    traffic = measured_traffic(dom, B)
 
     sweep = {}
     for b in [int(x) for x in args.sweep.split(",") if x]:
         log(f"sweep B={b}")
-        t, _, _ = time_form(torch.float32, b, K, W)
+        t, _, _, _ = time_form(torch.float32, b, K, W)
         sweep[str(b)] = b * N * N * K / t
 
     curve = None
     if not args.no_curve:
         log("error curve")
         curve = error_curve([100, 1000, 2000, 3000, 5000, 10000])
+
+    c4 = c3 = c5 = None
+    if args.c4_steps > 0:
+        log(f"c4 batched uneven ETDRK4 ({args.c4_instances} instances, {args.c4_steps} steps)")
+        c4 = c4_batched_etdrk4(args.c4_instances, args.c4_steps)
+    if args.c3_steps > 0:
+        log(f"c3 Newton-Krylov 511^2 Dirichlet, {args.c3_steps} BE steps")
+        c3 = {"grid": [511, 511], "h": 1.0 / 512, "dt": 1e-2,
+              "fp32_operator": c3_newton_krylov(511, args.c3_steps, torch.float32),
+              "fp64_operator": c3_newton_krylov(511, args.c3_steps, torch.float64)}
+    if args.serve_requests > 0:
+        log(f"c5 serving {args.serve_requests} requests at {args.serve_rate}/s")
+        c5 = c5_serving(args.serve_requests, args.serve_rate)
 
     cpu = None
     if not args.no_cpu:
@@ -218,12 +343,16 @@ def main():
         "rel_l2_fp32_vs_fp64": curve,
         "steps_within_1e-5": horizon,
         "finite": fin32 and fin64,
-        "step_latency_ms": step_latency,
+        "step_latency_ms": lat,
+        "p99_step_latency_ms": lat["step_p99_ms"],
         "batch_sweep_gps": sweep or None,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dom,
                      "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_s * 1e3,
                      "step_frac": 16.0 * gps32 / HBM_PEAK},
+        "batched_uneven_etdrk4": c4,
+        "newton_krylov_511": c3,
+        "serving": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

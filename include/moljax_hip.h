@@ -24,6 +24,9 @@
  * Two further multipliers are selected by negative nrep:
  *   nrep = -1:  alpha_f * lam / (nx*ny)        (apply D*lap, alpha = D)
  *   nrep = -2:  exp(beta_f * lam) / (nx*ny)    (apply exp(dt D lap), beta = dt D)
+ *   nrep = -3:  the first form with the power taken per instance from the
+ *               instance row's aux column (a non-negative integer), so one
+ *               launch advances each instance by its own number of steps
  * (moljax/core/fft_operators.py:110-149 matvec / exp_matvec).
  */
 #ifndef MOLJAX_HIP_H
@@ -222,12 +225,25 @@ int mjx_reduce2d(int dtype, int op, int B, int ny, int nx, const void* a, long l
                  const void* b, long long b_pitch, long long b_istride, double atol, double rtol, double* out,
                  int accumulate, double* work, void* stream);
 
+/* IMEX-Strang in place where instance b takes n_b = inst[8b + 7] steps
+ * (0 <= n_b <= max_steps) in one batched chain: the column and row passes of
+ * step s leave an instance with s >= n_b untouched, so every instance's
+ * arithmetic is that of mjx_gs_imex_strang with n_steps = n_b (instances
+ * with n_b = 0 hold unspecified values afterwards).  For continuous
+ * batching of requests with different remaining step counts (no reference
+ * counterpart; serves moljax/core/stepping.py:370-435 per request). */
+int mjx_gs_imex_strang_var(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch,
+                           long long istride, int fill_ghosts, const void* lamx, const void* lamy,
+                           const void* lam2d, const double* inst, int max_steps, void* work, void* stream);
+
 /* Non-periodic 2-D Helmholtz / Poisson solve on B instances of ny x nx
  * interior points:  kind 0 homogeneous Dirichlet (DST-I in x and y, vertex
  * grid; ny+1, nx+1 powers of two in [16, 1024]), kind 1 homogeneous Neumann
  * (orthonormal DCT-II; ny, nx powers of two in [16, 1024]).
  *   u = T^-1[ m(kx, ky) T[rhs] ],  mmode 0: m = (1 + alpha lam)/(1 - beta lam),
- *   mmode 1 (Poisson): m = 1/lam (0 where |lam| < 1e-14),  lam = lamx + lamy
+ *   mmode 1 (Poisson): m = 1/lam (0 where |lam| < 1e-14),  lam = lamx + lamy,
+ *   mmode 2: the mmode-0 multiplier raised to the power inst[8b + 7] per
+ *   instance (that many backward-Euler steps in one solve)
  * with lamx (nx) / lamy (ny) the 1-D symbols indexed like the transform
  * coefficients (Dirichlet k = 1..N at [k-1], Neumann k = 0..N-1 at [k]);
  * beta = inst[8b + 0], alpha = inst[8b + 2] per instance b.  work: B*ny*nx
@@ -328,6 +344,10 @@ int mjx_dot_f64(long long n, int k, const double* V, const double* w, double* ou
  * block_until_ready, benchmarks/benchmark_utils.py:112-149). */
 int mjx_kernel_timing_enable(int on);
 int mjx_kernel_timing(int kind, double* total_ms, long long* count);
+/* Per-launch durations (ms, in launch order) of kernel `kind` since timing
+ * was enabled: copies up to cap values into out (may be NULL) and returns
+ * the number recorded, or a negative error code. */
+long long mjx_kernel_timing_samples(int kind, double* out, long long cap);
 
 /* Finite-difference operator on ghost-padded arrays; writes the whole padded
  * output (interior = stencil, ghost ring = 0).  in/out point at the first

@@ -33,6 +33,7 @@ PROTOTYPES = {
     "mjx_spectral_workspace_bytes": (_ll, [_i, _i, _i, _i]),
     "mjx_spectral_pair": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_imex_strang": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
+    "mjx_gs_imex_strang_var": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_imex": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_adaptive_workspace_bytes": (_ll, [_i, _i, _i, _i, _i]),
     "mjx_gs_explicit_workspace_bytes": (_ll, [_i, _i, _i, _i]),
@@ -66,6 +67,7 @@ PROTOTYPES = {
     "mjx_dot_f64": (_i, [_ll, _i, _p, _p, _p, _p, _p]),
     "mjx_kernel_timing_enable": (_i, [_i]),
     "mjx_kernel_timing": (_i, [_i, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "mjx_kernel_timing_samples": (_ll, [_i, _dp, _ll]),
 }
 
 _lib = None
@@ -128,6 +130,16 @@ def stream_ptr(device=None) -> int:
 
 def ptr(t) -> int:
     return 0 if t is None else t.data_ptr()
+
+
+def kernel_timing_samples(kind: int):
+    """Per-launch durations (ms) of a hot kernel since timing was enabled."""
+    n = lib().mjx_kernel_timing_samples(kind, None, 0)
+    if n < 0:
+        check(int(n), "mjx_kernel_timing_samples")
+    buf = (ctypes.c_double * max(int(n), 1))()
+    lib().mjx_kernel_timing_samples(kind, buf, n)
+    return list(buf[:n])
 
 
 def kernel_timing(kind: int):

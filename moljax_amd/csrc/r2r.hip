@@ -66,7 +66,8 @@ int mjx_r2r_solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs,
                                                  : " not supported: ny and nx must be powers of two in [16, 1024]"));
   if (B < 1 || !rhs || !out || !lamx || !lamy || !inst || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
-  if (mmode != R2R_HELM && mmode != R2R_POISSON) return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz) or 1 (Poisson)");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
+    return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
   hipStream_t st = (hipStream_t)stream;
   const long long plane = (long long)ny * nx;
   // rows: forward transform along x into the work array
@@ -99,7 +100,8 @@ int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long
                                                  : " not supported: n must be a power of two in [16, 1024]"));
   if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
-  if (mmode != R2R_HELM && mmode != R2R_POISSON) return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz) or 1 (Poisson)");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
+    return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.lam_line = lam;
   a.inst = inst;

@@ -33,7 +33,7 @@ namespace mjx {
 enum { R2R_DST = 0, R2R_DCT = 1 };
 enum { R2R_FWD = 0, R2R_INV = 1, R2R_APPLY = 2 };
 enum { R2R_ROW = 0, R2R_COL = 1 };
-enum { R2R_HELM = 0, R2R_POISSON = 1 };
+enum { R2R_HELM = 0, R2R_POISSON = 1, R2R_HELM_POW = 2 };
 
 const cpx<float>* twiddles_f32();
 const cpx<double>* twiddles_f64();
@@ -70,14 +70,24 @@ __device__ __forceinline__ cpx<T> dct_rot(const cpx<T>* __restrict__ tw, int k) 
   return SGN > 0 ? conjg(w) : w;
 }
 
+// Helmholtz ((1 + alpha lam) / (1 - beta lam)), raised to `power` in
+// R2R_HELM_POW mode (n backward-Euler / Crank-Nicolson steps in one solve),
+// or Poisson 1 / lam.
 template <typename T>
-__device__ __forceinline__ T r2r_mult(T lam, T alpha, T beta, int mmode) {
+__device__ __forceinline__ T r2r_mult(T lam, T alpha, T beta, int mmode, int power) {
   if (mmode == R2R_POISSON) {
     if (fabs(lam) < (T)1e-14) return (T)0;
     if constexpr (std::is_same<T, double>::value) return 1.0 / lam;
     else return 1.0f / lam;
   }
-  return ((T)1 + alpha * lam) / ((T)1 - beta * lam);
+  const T g = ((T)1 + alpha * lam) / ((T)1 - beta * lam);
+  if (mmode != R2R_HELM_POW) return g;
+  T m = (T)1, b = g;
+  for (int n = power; n > 0; n >>= 1) {
+    if (n & 1) m *= b;
+    b *= b;
+  }
+  return m;
 }
 
 template <typename T, int KIND, int L, int ORIENT, int MODE, int TL>
@@ -157,6 +167,7 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
     if constexpr (MODE == R2R_APPLY) {
       const double* ip = a.inst + 8 * inst;
       const T beta = (T)ip[0], alpha = (T)ip[2];
+      const int power = a.mmode == R2R_HELM_POW ? (int)ip[7] : 1;
       const T* lam_line = (const T*)a.lam_line;
       const T* lam_cross = (const T*)a.lam_cross;
       const int ga = line0 + 2 * t, gb = ga + 1;
@@ -176,8 +187,8 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
           const int k = e <= M ? e : L - e;
           if (k >= 1 && k <= N) {
             const T ll = lam_line[k - 1];
-            const T ma = r2r_mult<T>(ll + ca, alpha, beta, a.mmode);
-            const T mb = r2r_mult<T>(ll + cb, alpha, beta, a.mmode);
+            const T ma = r2r_mult<T>(ll + ca, alpha, beta, a.mmode, power);
+            const T mb = r2r_mult<T>(ll + cb, alpha, beta, a.mmode, power);
             v[m] = mk(mb * v[m].re, ma * v[m].im);
           } else {
             v[m] = mk((T)0, (T)0);
@@ -190,8 +201,8 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
             const cpx<T> q = e < N ? cmul(dct_rot<T, N, -1>(tw, k), v[m]) : cmul(dct_rot<T, N, +1>(tw, k), v[m]);
             const T f2 = (T)((k == 0 ? 1.0 : 2.0) / N);
             const T ll = lam_line[k];
-            const T ma = r2r_mult<T>(ll + ca, alpha, beta, a.mmode) * f2 * fa;
-            const T mb = r2r_mult<T>(ll + cb, alpha, beta, a.mmode) * f2 * fb;
+            const T ma = r2r_mult<T>(ll + ca, alpha, beta, a.mmode, power) * f2 * fa;
+            const T mb = r2r_mult<T>(ll + cb, alpha, beta, a.mmode, power) * f2 * fb;
             // Q'/1 with the 1/2 of Q = e^{-i th} Z / 2 and the doubled DC
             // term folded in: Z' = e^{+i th} Q' (e < N), 2 Q'_0 at e = 0.
             const cpx<T> qp = mk((T)0.5 * ma * q.re, (T)0.5 * mb * q.im);

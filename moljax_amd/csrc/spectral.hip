@@ -25,6 +25,23 @@ static bool g_kt_on = false;
 static std::vector<Timed> g_kt_pending;
 static double g_kt_ms[KT_NUM] = {0, 0};
 static long long g_kt_n[KT_NUM] = {0, 0};
+static std::vector<double> g_kt_samples[KT_NUM];
+
+// Resolve the pending event pairs into totals and per-launch samples.
+static int kt_resolve() {
+  for (auto& t : g_kt_pending) {
+    if (hipEventSynchronize(t.e1) != hipSuccess) return fail(MJX_E_HIP, "hipEventSynchronize failed");
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.e0, t.e1) != hipSuccess) return fail(MJX_E_HIP, "hipEventElapsedTime failed");
+    g_kt_ms[t.kind] += ms;
+    g_kt_n[t.kind] += 1;
+    g_kt_samples[t.kind].push_back(ms);
+    (void)hipEventDestroy(t.e0);
+    (void)hipEventDestroy(t.e1);
+  }
+  g_kt_pending.clear();
+  return MJX_OK;
+}
 
 template <class Fn>
 static int timed_launch(int kind, hipStream_t st, Fn&& fn) {
@@ -87,6 +104,24 @@ static int gs_imex_t(int B, int ny, int nx, T* u, T* v, long long pitch, long lo
   return spec_row_inv_unpack(nx, B, ny, W, u, v, pitch, istride, ghosts, st);
 }
 
+// Strang chain whose instances take their own number of steps (instance row
+// aux, 0..max_steps): the masked column and row passes leave an instance
+// untouched once its steps are done, so each instance's arithmetic is that of
+// a chain of its own length.
+template <typename T>
+static int gs_imex_var_t(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                         const T* lamx, const T* lamy, const T* lam2d, const InstParams* inst, int max_steps,
+                         cpx<T>* W, hipStream_t st) {
+  int rc = spec_row_fwd_pack(nx, B, ny, u, v, pitch, istride, W, st);
+  if (!rc) rc = spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, 1, st);
+  for (int s = 0; s < max_steps && !rc; ++s) {
+    rc = spec_row_gs(nx, B, ny, W, inst, st, RXN_HEUN, s);
+    if (!rc) rc = spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, 0, st, s);
+  }
+  if (rc) return rc;
+  return spec_row_inv_unpack(nx, B, ny, W, u, v, pitch, istride, ghosts, st);
+}
+
 }  // namespace mjx
 
 using namespace mjx;
@@ -107,7 +142,7 @@ int mjx_spectral_pair(int dtype, int B, int ny, int nx, const void* a_in, const 
   if (!a_in || !a_out || !work || !inst) return fail(MJX_E_ARG, "null pointer argument");
   if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
   if (b_in && !b_out) return fail(MJX_E_ARG, "b_out required when b_in is given");
-  if (nrep < -2) return fail(MJX_E_ARG, "nrep must be >= -2");
+  if (nrep < NREP_PER_INSTANCE) return fail(MJX_E_ARG, "nrep must be >= -3");
   if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
   if (fill_ghosts && out_pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs out_pitch >= nx + 2");
   rc = ensure_tables(nullptr);
@@ -145,6 +180,28 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                             (const float*)lamy, (const float*)lam2d, ip, n_steps, (cpx<float>*)work, st);
   return gs_imex_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)lamx,
                            (const double*)lamy, (const double*)lam2d, ip, n_steps, (cpx<double>*)work, st);
+}
+
+int mjx_gs_imex_strang_var(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                           int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d,
+                           const double* inst, int max_steps, void* work, void* stream) {
+  int rc = check_grid(dtype, B, ny, nx);
+  if (rc) return rc;
+  if (!u || !v || !inst || !work) return fail(MJX_E_ARG, "null pointer argument");
+  if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
+  if (max_steps < 0) return fail(MJX_E_ARG, "max_steps must be >= 0");
+  if (pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (fill_ghosts && pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= nx + 2");
+  if (max_steps == 0) return MJX_OK;
+  rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (dtype == MJX_F32)
+    return gs_imex_var_t<float>(B, ny, nx, (float*)u, (float*)v, pitch, istride, fill_ghosts, (const float*)lamx,
+                                (const float*)lamy, (const float*)lam2d, ip, max_steps, (cpx<float>*)work, st);
+  return gs_imex_var_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)lamx,
+                               (const double*)lamy, (const double*)lam2d, ip, max_steps, (cpx<double>*)work, st);
 }
 
 int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
@@ -187,6 +244,7 @@ int mjx_kernel_timing_enable(int on) {
     for (int i = 0; i < KT_NUM; ++i) {
       g_kt_ms[i] = 0;
       g_kt_n[i] = 0;
+      g_kt_samples[i].clear();
     }
   }
   return MJX_OK;
@@ -195,19 +253,22 @@ int mjx_kernel_timing_enable(int on) {
 int mjx_kernel_timing(int kind, double* total_ms, long long* count) {
   if (kind < 0 || kind >= KT_NUM || !total_ms || !count) return fail(MJX_E_ARG, "bad kernel timing query");
   std::lock_guard<std::mutex> lk(g_kt_mu);
-  for (auto& t : g_kt_pending) {
-    if (hipEventSynchronize(t.e1) != hipSuccess) return fail(MJX_E_HIP, "hipEventSynchronize failed");
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, t.e0, t.e1) != hipSuccess) return fail(MJX_E_HIP, "hipEventElapsedTime failed");
-    g_kt_ms[t.kind] += ms;
-    g_kt_n[t.kind] += 1;
-    (void)hipEventDestroy(t.e0);
-    (void)hipEventDestroy(t.e1);
-  }
-  g_kt_pending.clear();
+  const int rc = kt_resolve();
+  if (rc) return rc;
   *total_ms = g_kt_ms[kind];
   *count = g_kt_n[kind];
   return MJX_OK;
+}
+
+long long mjx_kernel_timing_samples(int kind, double* out, long long cap) {
+  if (kind < 0 || kind >= KT_NUM) return fail(MJX_E_ARG, "bad kernel timing query");
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  const int rc = kt_resolve();
+  if (rc) return rc;
+  const auto& v = g_kt_samples[kind];
+  const long long n = (long long)v.size();
+  for (long long i = 0; out && i < n && i < cap; ++i) out[i] = v[(size_t)i];
+  return n;
 }
 
 }  // extern "C"

@@ -10,15 +10,18 @@ namespace mjx {
 
 // Explicit reaction stage of the fused row pass (k_row_gs).
 enum { RXN_HEUN = 0, RXN_EULER = 1 };
+// Column-pass power taken per instance from the instance row's aux column.
+constexpr int NREP_PER_INSTANCE = -3;
 
 #define MJX_SPEC_DECL(T)                                                                                     \
   int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,   \
                         cpx<T>* W, hipStream_t st, double in_scale = 1.0);                                   \
   int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,              \
                           long long istride, int ghosts, hipStream_t st, double keep = 0.0);                 \
-  int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = 0);     \
+  int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = 0,      \
+                  int mstep = -1);                                                                           \
   int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,        \
-                     const InstParams* ip, int nrep, hipStream_t st);
+                     const InstParams* ip, int nrep, hipStream_t st, int mstep = -1);
 
 MJX_SPEC_DECL(float)
 MJX_SPEC_DECL(double)

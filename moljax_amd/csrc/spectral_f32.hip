@@ -14,12 +14,12 @@ int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long
                         int ghosts, hipStream_t st, double keep) {
   return Launch<T>::row_inv_unpack(nx, B, ny, W, a, b, pitch, istride, ghosts, st, (T)keep);
 }
-int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn) {
-  return Launch<T>::row_gs(nx, B, ny, W, ip, st, rxn);
+int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn, int mstep) {
+  return Launch<T>::row_gs(nx, B, ny, W, ip, st, rxn, mstep);
 }
 int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
-                   const InstParams* ip, int nrep, hipStream_t st) {
-  return Launch<T>::col_apply(ny, B, nx, W, lamx, lamy, lam2d, ip, nrep, st);
+                   const InstParams* ip, int nrep, hipStream_t st, int mstep) {
+  return Launch<T>::col_apply(ny, B, nx, W, lamx, lamy, lam2d, ip, nrep, st, mstep);
 }
 
 }  // namespace mjx

@@ -108,6 +108,10 @@ __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
   } else {
     if (nrep == -1) return alpha * lam * s;
     if (nrep == -2) return exp(beta * lam) * s;
+    // the fixed-power forms, so a runtime power of 1 or 2 rounds exactly as
+    // the NREP = 1 / 2 instantiations do
+    if (nrep == 1) return g * s;
+    if (nrep == 2) return (g * g) * s;
     T m = (T)1, b = g;
     int n = nrep;
     while (n > 0) {
@@ -266,12 +270,14 @@ template <typename T, int NX, int TR, int RXN>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W,
                                                                   const InstParams* __restrict__ ip, int ny,
                                                                   int rows_per_inst_tiles,
-                                                                  const cpx<T>* __restrict__ tw) {
+                                                                  const cpx<T>* __restrict__ tw, int mstep) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / rows_per_inst_tiles;
   const int tile = w - inst * rows_per_inst_tiles;
+  // masked chain step: an instance past its own step count is left as is
+  if (mstep >= 0 && mstep >= (int)ip[inst].aux) return;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
   const int y = tile * C::TR + rl;
   cpx<T>* lrow = lds + rl * C::LP;
@@ -319,8 +325,8 @@ struct ColCfg {
 template <typename T, int NY, int TC_, bool LAM2D, int NREP>
 __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
     cpx<T>* __restrict__ W, int nx, const T* __restrict__ lamx, const T* __restrict__ lamy,
-    const T* __restrict__ lam2d, const InstParams* __restrict__ ip, int nrep, T s, int col_tiles,
-    const cpx<T>* __restrict__ tw) {
+    const T* __restrict__ lam2d, const InstParams* __restrict__ ip, int nrep_arg, T s, int col_tiles,
+    const cpx<T>* __restrict__ tw, int mstep) {
   using C = ColCfg<NY, TC_>;
   constexpr int TC = C::TC;
   __shared__ cpx<T> lds[NY * TC];
@@ -328,6 +334,20 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / col_tiles;
   const int tile = w - inst * col_tiles;
+  // Per-instance step counts (instance row aux): NREP_PER_INSTANCE raises the
+  // multiplier to that power; a masked chain step mstep >= 0 applies H^2
+  // before the instance's last step, H at it, and leaves the instance
+  // untouched after it (the whole workgroup belongs to one instance).
+  int nrep = nrep_arg;
+  if (mstep >= 0 || nrep_arg == NREP_PER_INSTANCE) {
+    const int ni = (int)ip[inst].aux;
+    if (mstep >= 0) {
+      if (mstep >= ni) return;
+      nrep = mstep == ni - 1 ? 1 : 2;
+    } else {
+      nrep = ni;
+    }
+  }
   const int c = threadIdx.x % TC, j = threadIdx.x / TC;
   const int p0 = tile * TC;
   const int p = p0 + c;
@@ -434,14 +454,14 @@ struct RowLaunch {
                        pitch, istride, ny, tiles, ghosts, keep, twiddles<T>());
     return check_launch("row_inv_unpack");
   }
-  static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, hipStream_t st) {
+  static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, int mstep, hipStream_t st) {
     const int tiles = ny / L;
     if (rxn == RXN_EULER)
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
-                         ny, tiles, twiddles<T>());
+                         ny, tiles, twiddles<T>(), mstep);
     else
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
-                         ny, tiles, twiddles<T>());
+                         ny, tiles, twiddles<T>(), mstep);
     return check_launch("row_gs");
   }
 };
@@ -450,21 +470,23 @@ template <typename T, int N, int L>
 struct ColLaunch {
   template <bool LAM2D, int NREP>
   static void go(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
-                 int nrep, T s, hipStream_t st) {
+                 int nrep, T s, int mstep, hipStream_t st) {
     const int tiles = nx / L;
     hipLaunchKernelGGL((k_col_apply<T, N, L, LAM2D, NREP>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W,
-                       nx, lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>());
+                       nx, lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>(), mstep);
   }
   static int apply(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
-                   int nrep, T s, hipStream_t st) {
+                   int nrep, T s, int mstep, hipStream_t st) {
+    // per-instance powers (masked steps, NREP_PER_INSTANCE) use the runtime power path
+    const int fixed = mstep >= 0 ? 0 : nrep;
     if (lam2d) {
-      if (nrep == 1) go<true, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
-      else if (nrep == 2) go<true, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
-      else go<true, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      if (fixed == 1) go<true, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else if (fixed == 2) go<true, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else go<true, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
     } else {
-      if (nrep == 1) go<false, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
-      else if (nrep == 2) go<false, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
-      else go<false, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+      if (fixed == 1) go<false, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else if (fixed == 2) go<false, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else go<false, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
     }
     return check_launch("col_apply");
   }
@@ -513,8 +535,8 @@ struct Launch {
   struct Gs {
     template <int L>
     struct F {
-      static int run(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, hipStream_t st) {
-        return RowLaunch<T, N, L>::gs(B, ny, W, ip, rxn, st);
+      static int run(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, int mstep, hipStream_t st) {
+        return RowLaunch<T, N, L>::gs(B, ny, W, ip, rxn, mstep, st);
       }
     };
   };
@@ -523,8 +545,8 @@ struct Launch {
     template <int L>
     struct F {
       static int run(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
-                     const InstParams* ip, int nrep, T s, hipStream_t st) {
-        return ColLaunch<T, N, L>::apply(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, st);
+                     const InstParams* ip, int nrep, T s, int mstep, hipStream_t st) {
+        return ColLaunch<T, N, L>::apply(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
       }
     };
   };
@@ -554,11 +576,12 @@ struct Launch {
         return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
     }
   }
-  static int row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = RXN_HEUN) {
+  static int row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = RXN_HEUN,
+                    int mstep = -1) {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, ip, rxn, st);
+    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, ip, rxn, mstep, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -566,13 +589,13 @@ struct Launch {
     }
   }
   static int col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
-                       const InstParams* ip, int nrep, hipStream_t st) {
+                       const InstParams* ip, int nrep, hipStream_t st, int mstep = -1) {
     const T s = (T)(1.0 / ((double)nx * (double)ny));
     switch (ny) {
 #define X(N)                                                                                                 \
   case N:                                                                                                    \
     return with_lines<T, N, Col<N>::template F>(pick_lines(lines_for<T, N>(TK_COL), nx / 2), B, nx, W, lamx, lamy, lam2d, \
-                                             ip, nrep, s, st);
+                                             ip, nrep, s, mstep, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:

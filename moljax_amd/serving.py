@@ -18,16 +18,17 @@ Scheduling:
   * every request advances in chunks of ``chunk`` steps counted from its own
     first step (the last chunk is the remainder), so its arithmetic does not
     depend on what else is running;
-  * each iteration runs one chunk of every active request; requests of the
-    same (kind, size, chunk length) form one batched launch with per-request
-    parameters in the instance rows, and the launches of different groups go
-    to different streams so they overlap on the device;
+  * requests of one (kind, size) live in the slots of a resident pool; each
+    iteration is ONE batched launch per pool in which every resident request
+    advances min(chunk, its remaining steps): the instance rows carry each
+    request's parameters and step count (per-instance spectral powers for
+    the linear kinds, a masked Strang chain for Gray-Scott), and the pools'
+    launches go to different streams so they overlap on the device;
   * requests that arrive while others run are admitted at the next
     iteration; finished ones retire and release their footprint.
 Because every kernel is batch-invariant, a request's result is bitwise the
 same alone and in any mix.
 """
-import ctypes
 import math
 import time
 from collections import deque
@@ -125,62 +126,109 @@ def _neumann(m: int) -> np.ndarray:
     return -4.0 / (h * h) * np.sin(np.pi * k / (2 * m)) ** 2
 
 
-class _Symbols:
-    def __init__(self):
-        self._c: Dict[tuple, torch.Tensor] = {}
+class _Pool:
+    """Resident slots of one (kind, grid size): fields (cap, m, m), instance
+    rows (cap, 8) with the step count of the next launch in the aux column."""
 
-    def get(self, kind: str, m: int, dtype: torch.dtype) -> torch.Tensor:
-        key = (kind, m, dtype)
-        t = self._c.get(key)
-        if t is None:
+    def __init__(self, kind: str, m: int, dtype: torch.dtype, cap: int = 8):
+        self.kind, self.m, self.dtype = kind, m, dtype
+        self.nf = 2 if kind == "imex_gs" else 1
+        self.cap = 0
+        self.slots: List[Optional[Request]] = []
+        self._alloc(cap)
+
+    def _alloc(self, cap: int):
+        old = getattr(self, "fields", None)
+        self.fields = [torch.zeros((cap, self.m, self.m), dtype=self.dtype, device="cuda") for _ in range(self.nf)]
+        if old is not None:
+            for f in range(self.nf):
+                self.fields[f][:self.cap].copy_(old[f])
+        rows = np.zeros((cap, 8))
+        if self.cap:
+            rows[:self.cap] = self.rows_np
+        self.rows_np = rows
+        self.rows_host = torch.zeros((cap, 8), dtype=torch.float64).pin_memory()
+        self.rows_dev = torch.zeros((cap, 8), dtype=torch.float64, device="cuda")
+        lib = _hip.lib()
+        dc = _hip.dtype_code(self.dtype)
+        nbytes = (lib.mjx_spectral_workspace_bytes(dc, cap, self.m, self.m) if self.kind in ("fft_cn", "imex_gs")
+                  else cap * self.m * self.m * (8 if self.dtype == torch.float64 else 4))
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        self.slots += [None] * (cap - self.cap)
+        self.cap = cap
+
+    def admit(self, r: Request, state: List[torch.Tensor]):
+        try:
+            i = self.slots.index(None)
+        except ValueError:
+            i = self.cap
+            self._alloc(2 * self.cap)
+        self.slots[i] = r
+        for f in range(self.nf):
+            self.fields[f][i].copy_(state[f])
+        self.rows_np[i] = _rows(r)
+        return i
+
+    def active(self) -> bool:
+        return any(x is not None for x in self.slots)
+
+    def launch(self, chunk: int, st: int) -> int:
+        """One batched launch over slots [0, hi): every resident request
+        advances min(chunk, its remaining steps); returns the launch's
+        maximum step count."""
+        hi = max(i for i, x in enumerate(self.slots) if x is not None) + 1
+        n = np.zeros(hi)
+        for i in range(hi):
+            r = self.slots[i]
+            if r is not None:
+                n[i] = min(chunk, r.steps - r.done_steps)
+        self.rows_np[:hi, 7] = n
+        self.rows_host[:hi].copy_(torch.from_numpy(self.rows_np[:hi]))
+        self.rows_dev[:hi].copy_(self.rows_host[:hi], non_blocking=True)
+        run_slots(self.kind, self.fields, hi, self.rows_dev, int(n.max()), self._lam(), self.work, st)
+        return int(n.max())
+
+    def _lam(self) -> torch.Tensor:
+        if not hasattr(self, "lam"):
+            m = self.m
             lam = {"fft_cn": lambda: _spectral(m, 1.0), "imex_gs": lambda: _fd(m, GS_L),
-                   "dst": lambda: _dirichlet(m), "dct": lambda: _neumann(m)}[kind]()
-            t = torch.from_numpy(lam).to("cuda", dtype)
-            self._c[key] = t
-        return t
+                   "dst": lambda: _dirichlet(m), "dct": lambda: _neumann(m)}[self.kind]()
+            self.lam = torch.from_numpy(lam).to("cuda", self.dtype)
+        return self.lam
 
 
-def advance(kind: str, batch: List[torch.Tensor], rows: torch.Tensor, steps: int, lam: torch.Tensor,
-            st: int):
-    """Advance a (B, m, m) batch (one or two fields) by ``steps`` steps in
-    place on stream handle ``st``."""
+def run_slots(kind: str, fields: List[torch.Tensor], B: int, rows: torch.Tensor, max_steps: int, lam: torch.Tensor,
+              work: torch.Tensor, st: int):
+    """Advance slots [0, B) of a pool in place; slot b takes rows[b, 7] steps
+    (per-instance powers for the linear kinds, a masked Strang chain for
+    Gray-Scott)."""
     lib = _hip.lib()
-    a = batch[0]
-    B, m = a.shape[0], a.shape[-1]
+    a = fields[0]
+    m = a.shape[-1]
     dc = _hip.dtype_code(a.dtype)
+    if max_steps <= 0:
+        return
     if kind == "fft_cn":
-        work = torch.empty(lib.mjx_spectral_workspace_bytes(dc, B, m, m), dtype=torch.uint8, device=a.device)
         rc = lib.mjx_spectral_pair(dc, B, m, m, _hip.ptr(a), None, m, m * m, _hip.ptr(a), None, m, m * m, 0,
-                                   _hip.ptr(lam), _hip.ptr(lam), None, _hip.ptr(rows), int(steps), _hip.ptr(work),
-                                   st)
+                                   _hip.ptr(lam), _hip.ptr(lam), None, _hip.ptr(rows), -3, _hip.ptr(work), st)
         _hip.check(rc, "mjx_spectral_pair")
     elif kind == "imex_gs":
-        b = batch[1]
-        work = torch.empty(lib.mjx_spectral_workspace_bytes(dc, B, m, m), dtype=torch.uint8, device=a.device)
-        rc = lib.mjx_gs_imex_strang(dc, B, m, m, _hip.ptr(a), _hip.ptr(b), m, m * m, 0, _hip.ptr(lam),
-                                    _hip.ptr(lam), None, _hip.ptr(rows), int(steps), _hip.ptr(work), st)
-        _hip.check(rc, "mjx_gs_imex_strang")
+        rc = lib.mjx_gs_imex_strang_var(dc, B, m, m, _hip.ptr(a), _hip.ptr(fields[1]), m, m * m, 0, _hip.ptr(lam),
+                                        _hip.ptr(lam), None, _hip.ptr(rows), int(max_steps), _hip.ptr(work), st)
+        _hip.check(rc, "mjx_gs_imex_strang_var")
     else:
-        kcode = 0 if kind == "dst" else 1
-        other = torch.empty_like(a)
-        work = torch.empty_like(a)
-        src, dst = a, other
-        for _ in range(steps):
-            rc = lib.mjx_r2r_solve2d(dc, kcode, B, m, m, _hip.ptr(src), m, m * m, _hip.ptr(dst), m, m * m,
-                                     _hip.ptr(lam), _hip.ptr(lam), _hip.ptr(rows), 0, _hip.ptr(work), st)
-            _hip.check(rc, "mjx_r2r_solve2d")
-            src, dst = dst, src
-        if src is not a:
-            a.copy_(src)
+        rc = lib.mjx_r2r_solve2d(dc, 0 if kind == "dst" else 1, B, m, m, _hip.ptr(a), m, m * m, _hip.ptr(a), m,
+                                 m * m, _hip.ptr(lam), _hip.ptr(lam), _hip.ptr(rows), 2, _hip.ptr(work), st)
+        _hip.check(rc, "mjx_r2r_solve2d")
 
 
 def _rows(req: Request) -> List[float]:
     if req.kind == "fft_cn":
         h = 0.5 * req.dt * req.D
-        return [h, 0.0, h, 0.0, 0.0, 0.0, req.dt]
+        return [h, 0.0, h, 0.0, 0.0, 0.0, req.dt, 0.0]
     if req.kind == "imex_gs":
-        return [0.5 * req.dt * req.D, 0.25 * req.dt * req.D, 0.0, 0.0, GS_F, GS_K, req.dt]
-    return [req.dt * req.D, 0.0, 0.0, 0.0, 0.0, 0.0, req.dt]
+        return [0.5 * req.dt * req.D, 0.25 * req.dt * req.D, 0.0, 0.0, GS_F, GS_K, req.dt, 0.0]
+    return [req.dt * req.D, 0.0, 0.0, 0.0, 0.0, 0.0, req.dt, 0.0]
 
 
 class Server:
@@ -193,7 +241,7 @@ class Server:
         self.budget = int(hbm_budget)
         self.chunk = int(chunk)
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
-        self.symbols = _Symbols()
+        self.pools: Dict[tuple, _Pool] = {}
         self.used = 0
         self.peak_used = 0
         self.iterations = 0
@@ -204,6 +252,13 @@ class Server:
         es = 8 if self.dtype == torch.float64 else 4
         return 4 * req.points * es
 
+    def _pool(self, r: Request) -> _Pool:
+        key = (r.kind, r.grid)
+        p = self.pools.get(key)
+        if p is None:
+            p = self.pools[key] = _Pool(r.kind, r.grid, self.dtype)
+        return p
+
     def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
         """Serve ``requests`` (arrival times in seconds from the start);
         returns them with admitted/finished times and final states."""
@@ -211,56 +266,47 @@ class Server:
             if self.footprint(r) > self.budget:
                 raise ValueError(f"request {r.rid} needs {self.footprint(r)} bytes, above the HBM budget")
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        active: List[Request] = []
+        n_active = 0
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
-        while pending or active:
+        while pending or n_active:
             now = clock()
             while pending and pending[0].arrival <= now and self.used + self.footprint(pending[0]) <= self.budget:
                 r = pending.popleft()
-                r.state = initial_state(r, self.dtype)
+                self._pool(r).admit(r, initial_state(r, self.dtype))
                 r.admitted = time.perf_counter() - t0
                 self.used += self.footprint(r)
                 self.peak_used = max(self.peak_used, self.used)
-                active.append(r)
-            if not active:
+                n_active += 1
+            if not n_active:
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.002)))
                 continue
-            self.peak_active = max(self.peak_active, len(active))
-            groups: Dict[tuple, List[Request]] = {}
-            for r in active:
-                n_chunk = min(self.chunk, r.steps - r.done_steps)
-                groups.setdefault((r.kind, r.grid, n_chunk), []).append(r)
+            self.peak_active = max(self.peak_active, n_active)
             cur = torch.cuda.current_stream()
-            for gi, (key, reqs) in enumerate(sorted(groups.items())):
-                kind, m, n_chunk = key
+            live = [p for p in self.pools.values() if p.active()]
+            for gi, p in enumerate(live):
                 s = self.streams[gi % len(self.streams)]
                 s.wait_stream(cur)
                 with torch.cuda.stream(s):
-                    nf = len(reqs[0].state)
-                    batch = [torch.stack([r.state[f] for r in reqs]) for f in range(nf)]
-                    rows = inst_table([_rows(r) for r in reqs])
-                    advance(kind, batch, rows, n_chunk, self.symbols.get(kind, m, self.dtype), s.cuda_stream)
-                    for i, r in enumerate(reqs):
-                        for f in range(nf):
-                            r.state[f].copy_(batch[f][i])
-                            r.state[f].record_stream(s)
-                self.max_group = max(self.max_group, len(reqs))
+                    p.launch(self.chunk, s.cuda_stream)
+                self.max_group = max(self.max_group, sum(x is not None for x in p.slots))
             for s in self.streams:
                 cur.wait_stream(s)
             torch.cuda.synchronize()
             self.iterations += 1
             t_done = time.perf_counter() - t0
-            still = []
-            for r in active:
-                r.done_steps += min(self.chunk, r.steps - r.done_steps)
-                if r.done_steps >= r.steps:
-                    r.finished = t_done
-                    self.used -= self.footprint(r)
-                else:
-                    still.append(r)
-            active = still
+            for p in live:
+                for i, r in enumerate(p.slots):
+                    if r is None:
+                        continue
+                    r.done_steps += min(self.chunk, r.steps - r.done_steps)
+                    if r.done_steps >= r.steps:
+                        r.finished = t_done
+                        r.state = [f[i].clone() for f in p.fields]
+                        p.slots[i] = None
+                        self.used -= self.footprint(r)
+                        n_active -= 1
         return requests
 
 
@@ -287,15 +333,16 @@ def latency_stats(reqs: List[Request]) -> Dict[str, float]:
 
 
 def sequential(reqs: List[Request], dtype: torch.dtype = torch.float64) -> float:
-    """Seconds to run the same requests one at a time, all steps in one
-    launch sequence each (no batching), in ``dtype``."""
-    sym = _Symbols()
+    """Seconds to run the same requests one at a time, each in one launch
+    sequence of all its steps (no batching), in ``dtype``."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in reqs:
-        st = initial_state(r, dtype)
-        batch = [s.unsqueeze(0) for s in st]
-        advance(r.kind, batch, inst_table([_rows(r)]), r.steps, sym.get(r.kind, r.grid, dtype),
-                torch.cuda.current_stream().cuda_stream)
+        p = _Pool(r.kind, r.grid, dtype, cap=1)
+        p.admit(r, initial_state(r, dtype))
+        done = r.done_steps
+        p.launch(r.steps, torch.cuda.current_stream().cuda_stream)
+        p.slots[0] = None
+        r.done_steps = done
     torch.cuda.synchronize()
     return time.perf_counter() - t0

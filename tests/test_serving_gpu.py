@@ -101,3 +101,35 @@ def test_admission_by_footprint(gpu):
     assert st["p99_ms"] >= st["p50_ms"] > 0
     again = poisson_stream(16, rate=2000.0, seed=3, sizes=(32, 64), steps=(5, 40))
     assert [(a.kind, a.n, a.steps, a.seed) for a in again] == [(a.kind, a.n, a.steps, a.seed) for a in reqs]
+
+
+def test_masked_chain_equals_fixed_chains(gpu):
+    """mjx_gs_imex_strang_var: each instance's result is bitwise that of the
+    fixed-length chain with its own step count."""
+    from moljax_amd import _hip
+    from moljax_amd.core._arrays import inst_table, spectral_work
+    from moljax_amd.serving import _fd, GS_L
+    m, steps = 64, [3, 5, 1, 4]
+    B = len(steps)
+    lib = _hip.lib()
+    lam = torch.from_numpy(_fd(m, GS_L)).cuda()
+    rng = np.random.default_rng(0)
+    u0 = torch.from_numpy(1.0 - 0.5 * rng.random((B, m, m))).cuda()
+    v0 = torch.from_numpy(0.25 * rng.random((B, m, m))).cuda()
+    rows = [[0.5 * 0.3 * 1e-3 * (b + 1), 0.25 * 0.3 * 1e-3, 0, 0, 0.035, 0.065, 0.3, steps[b]] for b in range(B)]
+    u, v = u0.clone(), v0.clone()
+    # every device buffer stays referenced until its kernels have run
+    inst, work = inst_table(rows), spectral_work(torch.float64, B, m, m)
+    rc = lib.mjx_gs_imex_strang_var(1, B, m, m, _hip.ptr(u), _hip.ptr(v), m, m * m, 0, _hip.ptr(lam), _hip.ptr(lam),
+                                    None, _hip.ptr(inst), max(steps), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "var")
+    for b in range(B):
+        ub, vb = u0[b:b + 1].clone(), v0[b:b + 1].clone()
+        ib, wb = inst_table([rows[b]]), spectral_work(torch.float64, 1, m, m)
+        rc = lib.mjx_gs_imex_strang(1, 1, m, m, _hip.ptr(ub), _hip.ptr(vb), m, m * m, 0, _hip.ptr(lam), _hip.ptr(lam),
+                                    None, _hip.ptr(ib), steps[b], _hip.ptr(wb), _hip.stream_ptr())
+        _hip.check(rc, "fixed")
+        du = float((u[b] - ub[0]).abs().max())
+        dv = float((v[b] - vb[0]).abs().max())
+        print(f"instance {b} steps {steps[b]}: max |du| {du:.3e} |dv| {dv:.3e}", flush=True)
+        assert torch.equal(u[b], ub[0]) and torch.equal(v[b], vb[0]), b
