@@ -60,6 +60,7 @@ class Request:
     finished: float = math.nan
     done_steps: int = 0
     state: List[torch.Tensor] = field(default_factory=list)
+    ic: Optional[List[np.ndarray]] = None  # host input arriving with the request
 
     @property
     def grid(self) -> int:
@@ -75,8 +76,9 @@ class Request:
         return self.grid * self.grid * (2 if self.kind == "imex_gs" else 1)
 
 
-def initial_state(req: Request, dtype: torch.dtype) -> List[torch.Tensor]:
-    """Seeded random-mode initial condition (numpy default_rng(seed))."""
+def initial_fields(req: Request) -> List[np.ndarray]:
+    """Seeded random-mode initial condition on the host (numpy
+    default_rng(seed)), float64."""
     rng = np.random.default_rng(req.seed)
     m = req.grid
     modes = [(rng.normal(0.0, 1.0), int(rng.integers(1, 6)), int(rng.integers(1, 6)), rng.uniform(0, 2 * np.pi))
@@ -100,7 +102,23 @@ def initial_state(req: Request, dtype: torch.dtype) -> List[torch.Tensor]:
         else:
             g = (f - f.min()) / max(f.max() - f.min(), 1e-12)
             fields = [1.0 - 0.5 * g, 0.25 * g]
-    return [torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype) for a in fields]
+    return [np.ascontiguousarray(a) for a in fields]
+
+
+def initial_state(req: Request, dtype: torch.dtype) -> List[torch.Tensor]:
+    """The request's input on the device: its attached host arrays (``ic``)
+    or the seeded random-mode initial condition."""
+    fields = req.ic if req.ic is not None else initial_fields(req)
+    return [torch.from_numpy(a).to("cuda", dtype) for a in fields]
+
+
+def attach_inputs(reqs: List[Request], dtype: torch.dtype = torch.float32) -> List[Request]:
+    """Generate every request's input on the host ahead of the run (the data
+    a client would send with the request), in the serving precision."""
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    for r in reqs:
+        r.ic = [a.astype(npd) for a in initial_fields(r)]
+    return reqs
 
 
 def _spectral(n: int, L: float) -> np.ndarray:
