@@ -41,6 +41,7 @@ int etd_advance(int B, int n, const float* coef, const InstParams* ip, int n_ste
 int etd_advance(int B, int n, const double* coef, const InstParams* ip, int n_steps, cpx<double>* work,
                 hipStream_t st);
 
+
 namespace {
 
 enum { SCHEME_ETDRK4 = 0, SCHEME_STRANG = 1 };
@@ -257,7 +258,8 @@ int autotune_t(int scheme, int n, int B, int reps, hipStream_t st, int* chosen) 
       Group g{n, B, u, u + pts, n, (long long)n * n, tab, nullptr, ip, work};
       auto body = [&]() { return Ops<T>::advance(g, 1, st); };
       rc = time_candidates<T>(TK_ETD_ROW, n, B, reps, candidates(dflt, n, 3), st, body, &chosen[0], nullptr);
-      if (!rc) rc = time_candidates<T>(TK_ETD_COL, n, B, reps, candidates(dflt, n / 2, 3), st, body, &chosen[1], nullptr);
+      const int cm = etd_col_max(sizeof(T) == 8 ? MJX_F64 : MJX_F32, n);
+      if (!rc) rc = time_candidates<T>(TK_ETD_COL, n, B, reps, candidates(cm, cm, 4), st, body, &chosen[1], nullptr);
     } else {
       cpx<T>* W = (cpx<T>*)work;
       auto row = [&]() { return spec_row_gs(n, B, n, W, ip, st, RXN_HEUN); };

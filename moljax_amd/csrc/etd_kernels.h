@@ -277,17 +277,28 @@ struct EtdLaunch {
   }
 };
 
-// Runtime tile dispatch for the ETDRK4 kernels: the lines per workgroup
-// is the default or the autotuned value among {D, D/2, D/4} (D the default,
-// capped by the grid), instantiated per transform length.
+// Runtime tile dispatch for the ETDRK4 kernels.  Rows: the default lines
+// per workgroup or the autotuned value among {D, D/2, D/4}.  Columns: the
+// autotuned value among CM / {1, 2, 4, 8}, CM the widest tile that fits one
+// workgroup (<= 1024 threads, <= 64 KiB of LDS); wide column tiles turn the
+// strided column accesses of large grids into full 128-byte row segments.
+// The default column tile (lines_default, capped) is one of the candidates.
 template <typename T, int N>
 struct EtdTiles {
   static constexpr int D = lines_default<T, N>();
   static constexpr int DR = D < N ? D : N;          // rows per block
-  static constexpr int DC = D < N / 2 ? D : N / 2;  // columns per block
-  static int pick(int tuned, int dmax) {
-    if (tuned == dmax || tuned == dmax / 2 || tuned == dmax / 4) return tuned > 0 ? tuned : dmax;
-    return dmax;
+  static constexpr int DC = D < N / 2 ? D : N / 2;  // columns per block (default)
+  static constexpr int TN = N / plan_p(N);
+  static constexpr int CM0 = 1024 / TN < N / 2 ? 1024 / TN : N / 2;
+  static constexpr int CL = 65536 / (N * (int)sizeof(cpx<T>));
+  static constexpr int CM = CM0 < CL ? CM0 : CL;  // widest column tile
+  static int pick_row(int tuned) {
+    if (tuned == DR || tuned == DR / 2 || tuned == DR / 4) return tuned > 0 ? tuned : DR;
+    return DR;
+  }
+  static int pick_col(int tuned) {
+    if (tuned > 0 && (tuned == CM || tuned == CM / 2 || tuned == CM / 4 || tuned == CM / 8)) return tuned;
+    return DC;
   }
   static int row_nl(int L, int B, int ny, const cpx<T>* Win, cpx<T>* Wout, const InstParams* ip, hipStream_t st) {
     if constexpr (DR >= 4) if (L == DR / 4) return EtdLaunch<T>::template row_nl<N, DR / 4>(B, ny, Win, Wout, ip, st);
@@ -297,13 +308,17 @@ struct EtdTiles {
   template <int STAGE>
   static int col(int L, int B, int nx, const cpx<T>* Win, const cpx<T>* Win2, cpx<T>* Uh, cpx<T>* Nn, cpx<T>* Fh,
                  const T* coef, cpx<T>* Wout, hipStream_t st) {
-    if constexpr (DC >= 4)
-      if (L == DC / 4) return EtdLaunch<T>::template col<N, DC / 4, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
-    if constexpr (DC >= 2)
-      if (L == DC / 2) return EtdLaunch<T>::template col<N, DC / 2, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+    if (L == CM) return EtdLaunch<T>::template col<N, CM, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+    if constexpr (CM >= 2)
+      if (L == CM / 2) return EtdLaunch<T>::template col<N, CM / 2, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+    if constexpr (CM >= 4)
+      if (L == CM / 4) return EtdLaunch<T>::template col<N, CM / 4, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
+    if constexpr (CM >= 8)
+      if (L == CM / 8) return EtdLaunch<T>::template col<N, CM / 8, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
     return EtdLaunch<T>::template col<N, DC, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
   }
 };
+
 
 // n_steps ETDRK4 steps on the packed state Ws of B square N x N instances
 // (workspace planes as laid out by etd_planes), without the pack/unpack.
@@ -331,8 +346,8 @@ int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParam
                     hipStream_t st) {
   using E = EtdTiles<T, N>;
   constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
-  const int LR = E::pick(tuned_lines(TK_ETD_ROW, dt_, N), E::DR);
-  const int LC = E::pick(tuned_lines(TK_ETD_COL, dt_, N), E::DC);
+  const int LR = E::pick_row(tuned_lines(TK_ETD_ROW, dt_, N));
+  const int LC = E::pick_col(tuned_lines(TK_ETD_COL, dt_, N));
   int rc = MJX_OK;
   for (int s = 0; s < n_steps && rc == 0; ++s) {
     rc = E::row_nl(LR, B, N, p.Ws, p.Wn, ip, st);

@@ -290,4 +290,13 @@ __device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* 
   });
 }
 
+// Widest ETDRK4 column tile (columns per workgroup) for a precision and
+// size: at most 1024 threads and 64 KiB of LDS per workgroup.
+inline int etd_col_max(int dtype, int n) {
+  const int tn = n / plan_p(n);
+  const int cm0 = 1024 / tn < n / 2 ? 1024 / tn : n / 2;
+  const int cl = 65536 / (n * (dtype == 1 ? 16 : 8));
+  return cm0 < cl ? cm0 : cl;
+}
+
 }  // namespace mjx
