@@ -93,3 +93,67 @@ def test_autotune_and_tile_table(gpu):
     assert lib.mjx_tune_set(0, 0, 128, 4) == 0 and lib.mjx_tune_get(0, 0, 128) == 4
     assert lib.mjx_tune_set(0, 0, 128, 3) != 0  # not a power of two
     assert lib.mjx_tune_set(0, 0, 128, 0) == 0  # back to the default
+
+
+@pytest.mark.parametrize("n", [128, 512, 1024])
+def test_etdrk4_every_column_tile_matches_oracle(gpu, n):
+    """Each ETDRK4 column tile the autotuner may choose (widest that fits a
+    workgroup, /2, /4, /8) gives the oracle's result (FP64, 3 steps)."""
+    from moljax_amd import _hip
+    from moljax_amd.batch import spectral_symbol_1d
+    from moljax_amd.core.fft_integrators import etdrk4_coefficients, gs_etdrk4_inplace
+    lib = _hip.lib()
+    lam = spectral_symbol_1d(n)
+    u0, v0 = _ic(n, 3)
+    F, k, steps = 0.04, 0.06, 3
+    ru, rv = oetd.gs_etdrk4(u0, v0, lam, lam, DU, DV, F, k, DT, steps)
+    coef = etdrk4_coefficients(lam, lam, DU, DV, DT, torch.float64)
+    tn = n // {128: 16, 512: 8, 1024: 16}[n]
+    cm = min(1024 // tn, n // 2, 65536 // (n * 16))
+    try:
+        for L in sorted({cm, cm // 2, cm // 4, cm // 8} - {0}):
+            assert lib.mjx_tune_set(3, 1, n, L) == 0
+            U, V = torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()
+            gs_etdrk4_inplace(U, V, coef, [[0, 0, 0, 0, F, k, DT]], steps)
+            eu, ev = ogs.rel_l2(U.cpu().numpy(), ru), ogs.rel_l2(V.cpu().numpy(), rv)
+            print(f"n={n} column tile {L}: rel_l2 u={eu:.1e} v={ev:.1e}", flush=True)
+            assert eu < 1e-12 and ev < 1e-12
+    finally:
+        lib.mjx_tune_set(3, 1, n, 0)
+
+
+def test_strang_every_tile_matches_oracle(gpu):
+    """Each row / column tile of the Strang chain (lines per workgroup from 1
+    up to the default) gives the oracle's result (FP64, 256^2, 3 steps)."""
+    from moljax_amd import _hip
+    from moljax_amd.batch import fd_symbol_1d
+    from moljax_amd.core._arrays import inst_table, spectral_work
+    lib = _hip.lib()
+    n, steps, F, k = 256, 3, 0.035, 0.065
+    lam = fd_symbol_1d(n, 2.5)
+    u0, v0 = _ic(n, 4)
+    P = lambda a: ogs.apply_bc_periodic(np.pad(a, 1))
+    y = {"u": P(u0), "v": P(v0)}
+    for _ in range(steps):
+        y = ogs.imex_strang_step(y, DT, F, k, DU, DV, lam[:, None] + lam[None, :])
+    ref_u = y["u"][1:-1, 1:-1]
+    lt = torch.from_numpy(lam).cuda()
+    rows = inst_table([[0.5 * DT * DU, 0.5 * DT * DV, 0, 0, F, k, DT]])
+    work = spectral_work(torch.float64, 1, n, n)
+    try:
+        for kind in (0, 1):
+            for L in (1, 2, 4, 8):
+                lib.mjx_tune_set(0, 1, n, 0)
+                lib.mjx_tune_set(1, 1, n, 0)
+                assert lib.mjx_tune_set(kind, 1, n, L) == 0
+                U, V = torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()
+                rc = lib.mjx_gs_imex_strang(1, 1, n, n, _hip.ptr(U), _hip.ptr(V), n, n * n, 0, _hip.ptr(lt),
+                                            _hip.ptr(lt), None, _hip.ptr(rows), steps, _hip.ptr(work),
+                                            _hip.stream_ptr())
+                _hip.check(rc, "strang")
+                err = ogs.rel_l2(U.cpu().numpy(), ref_u)
+                print(f"{'row' if kind == 0 else 'col'} tile {L}: rel_l2={err:.1e}", flush=True)
+                assert err < 1e-12
+    finally:
+        lib.mjx_tune_set(0, 1, n, 0)
+        lib.mjx_tune_set(1, 1, n, 0)
