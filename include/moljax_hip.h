@@ -294,9 +294,11 @@ int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long 
  * array: bc 0 homogeneous Dirichlet on a vertex grid (zero outside), bc 1
  * periodic; u_t = D lap_h u + f(u), f = reaction[0] u^3 + reaction[1] u^2 +
  * reaction[2] u + reaction[3];
- *   R(u) = u - u_old - theta dt (D lap u + f(u)) - fold_term   (fold_term may
- *   be NULL; for Crank-Nicolson it is (1 - theta) dt F(u_old)),
- *   J v = v - theta dt (D lap v + f'(u) v)   (hand-written JVP).
+ *   R(u) = c u - u_old - theta dt (D lap u + f(u)) - fold_term   (fold_term
+ *   may be NULL; for Crank-Nicolson it is (1 - theta) dt F(u_old); for
+ *   variable-step BDF2 c = alpha0, theta dt = beta and u_old holds
+ *   -(alpha1 y_n + alpha2 y_{n-1}), newton_krylov.py:406-448),
+ *   J v = c v - theta dt (D lap v + f'(u) v)   (hand-written JVP).
  * u (float64, in/out) holds the initial guess and receives the solution.
  * Newton as moljax/core/newton_krylov.py:171-359 (convergence tested before
  * the update, max_backtrack residual trials, fallback damped step);
@@ -305,10 +307,12 @@ int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long 
  * Givens rotations; one GMRES iteration = one Arnoldi step, convergence when
  * the Givens residual <= krylov_tol * ||M b||, max_krylov restart cycles.
  * nk_iparams = {operator dtype (MJX_F32/MJX_F64), precond (0 none, 1 exact
- *   (I - theta dt D lap)^-1 by DST-I or FFT, 2 Jacobi), jacobi sweeps,
- *   max_newton, max_krylov, restart, max_backtrack};
+ *   (I - pdt D lap)^-1 by DST-I or FFT, 2 Jacobi on I - pdt D lap),
+ *   jacobi sweeps, max_newton, max_krylov, restart, max_backtrack};
  * nk_params = {newton_tol, krylov_tol, damping, backtrack_factor,
- *   min_residual_decrease, jacobi_omega};
+ *   min_residual_decrease, jacobi_omega, c, pdt}: c the residual's leading
+ *   coefficient (1 for BE/CN), pdt the preconditioner context's dt (the
+ *   step dt, as moljax/core/newton_krylov.py:214 passes it);
  * lam_tables = {lamx f64, lamy f64, lamx f32, lamy f32} device symbol tables
  *   of the exact preconditioner (Dirichlet: DST-I eigenvalues, periodic: FD).
  * stats_out = {converged, newton_iters, gmres_iters, final ||R||}.  Synchronous
@@ -323,10 +327,10 @@ int mjx_rd_newton_krylov(int ny, int nx, int bc, double dy, double dx, double D,
 /* The residual R(u) (float64) and the JVP J(u) v (v, w in dtype, u float64)
  * of the problem above, as single kernels. */
 int mjx_rd_residual(int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
-                    const double* reaction, const double* u, const double* u_old, const double* fold_term,
-                    double* r, void* stream);
+                    double lead, const double* reaction, const double* u, const double* u_old,
+                    const double* fold_term, double* r, void* stream);
 int mjx_rd_jvp(int dtype, int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
-               const double* reaction, const double* u, const void* v, void* w, void* stream);
+               double lead, const double* reaction, const double* u, const void* v, void* w, void* stream);
 
 /* Deterministic FP64 dot products out[i] = <V_i, w> of k vectors of length n
  * (rows of V, stride n), wave64 shuffle + LDS reduction over a fixed block

@@ -61,8 +61,8 @@ PROTOTYPES = {
     "mjx_nk_workspace_bytes": (_ll, [_i, _i, _i]),
     "mjx_rd_newton_krylov": (_i, [_i, _i, _i, _d, _d, _d, _d, _d, _dp, _p, _p, _p, _p, _dp,
                                   ctypes.POINTER(ctypes.c_int), _dp, _p, _p]),
-    "mjx_rd_residual": (_i, [_i, _i, _i, _d, _d, _d, _d, _d, _dp, _p, _p, _p, _p, _p]),
-    "mjx_rd_jvp": (_i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _dp, _p, _p, _p, _p]),
+    "mjx_rd_residual": (_i, [_i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _p, _p, _p, _p, _p]),
+    "mjx_rd_jvp": (_i, [_i, _i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _p, _p, _p, _p]),
     "mjx_dot_partial_elems": (_ll, [_i]),
     "mjx_dot_f64": (_i, [_ll, _i, _p, _p, _p, _p, _p]),
     "mjx_kernel_timing_enable": (_i, [_i]),

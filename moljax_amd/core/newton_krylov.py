@@ -55,18 +55,21 @@ class NKResult(NamedTuple):
 
 
 class RDResidual:
-    """R(u) = u - u_old - theta dt F(u) - (1 - theta) dt F(u_old) for a
-    ReactionDiffusionModel (newton_krylov.py:362-403, BE: theta = 1,
-    CN: theta = 1/2).  Callable on a state {'u': array} (evaluated by the
-    mjx_rd_residual kernel)."""
+    """R(u) = c u - g - b F(u) - fold for a ReactionDiffusionModel, evaluated
+    by the mjx_rd_residual kernel; callable on a state {'u': array}.
+      BE / CN (newton_krylov.py:362-403): c = 1, g = u_old, b = theta dt,
+        fold = (1 - theta) dt F(u_old)  (theta = 1 / 1/2);
+      BDF2 (newton_krylov.py:406-448): c = alpha0, g = -(alpha1 y_n +
+        alpha2 y_{n-1}), b = beta, no fold (see create_bdf2_residual)."""
 
-    def __init__(self, model, y_old: StateDict, dt: float, theta: float):
+    def __init__(self, model, y_old: StateDict, dt: float, theta: float, lead: float = 1.0, g=None):
         self.model = model
         self.dt = float(dt)
         self.theta = float(theta)
-        self.u_old = model.as_field(y_old["u"])
+        self.lead = float(lead)
+        self.u_old = model.as_field(y_old["u"]) if g is None else model.as_field(g)
         self.fold = None
-        if theta < 1.0:
+        if g is None and theta < 1.0:
             self.fold = (1.0 - theta) * self.dt * model.rhs_field(self.u_old)
 
     def field(self, u: torch.Tensor) -> torch.Tensor:
@@ -74,7 +77,7 @@ class RDResidual:
         u = m.as_field(u)
         r = torch.empty_like(u)
         reac = (ctypes.c_double * 4)(*m.reaction)
-        rc = _hip.lib().mjx_rd_residual(m.ny, m.nx, m.bc_code, m.dy, m.dx, m.D, self.dt, self.theta, reac,
+        rc = _hip.lib().mjx_rd_residual(m.ny, m.nx, m.bc_code, m.dy, m.dx, m.D, self.dt, self.theta, self.lead, reac,
                                         _hip.ptr(u), _hip.ptr(self.u_old), _hip.ptr(self.fold), _hip.ptr(r),
                                         _hip.stream_ptr())
         _hip.check(rc, "mjx_rd_residual")
@@ -94,7 +97,7 @@ class RDResidual:
         w = torch.empty_like(v)
         reac = (ctypes.c_double * 4)(*m.reaction)
         rc = _hip.lib().mjx_rd_jvp(_hip.dtype_code(v.dtype), m.ny, m.nx, m.bc_code, m.dy, m.dx, m.D, self.dt,
-                                   self.theta, reac, _hip.ptr(u), _hip.ptr(v), _hip.ptr(w), _hip.stream_ptr())
+                                   self.theta, self.lead, reac, _hip.ptr(u), _hip.ptr(v), _hip.ptr(w), _hip.stream_ptr())
         _hip.check(rc, "mjx_rd_jvp")
         return w
 
@@ -107,6 +110,32 @@ def create_implicit_residual(model, y_old: StateDict, t_new: float, dt: float, m
         raise NotImplementedError("implicit residuals on the MI355X path serve reaction-diffusion models "
                                   "(create_reaction_diffusion_model)")
     return RDResidual(model, y_old, dt, 1.0 if method == "be" else 0.5)
+
+
+def bdf2_coefficients(dt: float, dt_prev: float, consistent: bool = False):
+    """newton_krylov.py:436-440: (alpha0, alpha1, alpha2, beta) of the
+    variable-step BDF2 residual, omega = dt / dt_prev.  The reference pairs
+    unnormalised alphas with the normalised beta = dt (1+w)/(1+2w), which
+    integrates y' = F / alpha0; consistent=True uses beta = dt (the scheme
+    its docstring states)."""
+    omega = dt / dt_prev
+    alpha0 = (1.0 + 2.0 * omega) / (1.0 + omega)
+    alpha1 = -(1.0 + omega)
+    alpha2 = omega ** 2 / (1.0 + omega)
+    beta = dt if consistent else dt * (1.0 + omega) / (1.0 + 2.0 * omega)
+    return alpha0, alpha1, alpha2, beta
+
+
+def create_bdf2_residual(model, y_n: StateDict, y_nm1: StateDict, t_new: float, dt: float, dt_prev: float,
+                         consistent: bool = False):
+    """newton_krylov.py:406-448:
+    R = alpha0 y + alpha1 y_n + alpha2 y_{n-1} - beta F(y)."""
+    if getattr(model, "mjx_kernel", None) != "reaction_diffusion":
+        raise NotImplementedError("implicit residuals on the MI355X path serve reaction-diffusion models "
+                                  "(create_reaction_diffusion_model)")
+    a0, a1, a2, beta = bdf2_coefficients(dt, dt_prev, consistent)
+    g = -(a1 * model.as_field(y_n["u"]) + a2 * model.as_field(y_nm1["u"]))
+    return RDResidual(model, y_n, beta, 1.0, lead=a0, g=g)
 
 
 def _precond_kind(preconditioner) -> int:
@@ -136,8 +165,9 @@ def newton_krylov_solve(residual_fn: Callable[[StateDict], StateDict], x0: State
     work = torch.empty(lib.mjx_nk_workspace_bytes(m.ny, m.nx, restart), dtype=torch.uint8, device="cuda")
     tabs = m.precond_tables()
     lam = (ctypes.c_void_p * 4)(*[_hip.ptr(t) for t in tabs])
-    fp = (ctypes.c_double * 6)(nk_params.newton_tol, nk_params.krylov_tol, nk_params.damping,
-                               nk_params.backtrack_factor, nk_params.min_residual_decrease, jac_omega)
+    fp = (ctypes.c_double * 8)(nk_params.newton_tol, nk_params.krylov_tol, nk_params.damping,
+                               nk_params.backtrack_factor, nk_params.min_residual_decrease, jac_omega,
+                               residual_fn.lead, float(dt))
     ip = (ctypes.c_int * 7)(_hip.dtype_code(nk_params.operator_dtype), kind, jac_iters,
                             int(nk_params.max_newton_iters), int(nk_params.max_krylov_iters), restart,
                             int(nk_params.max_backtrack))

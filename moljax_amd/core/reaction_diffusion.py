@@ -63,7 +63,7 @@ class ReactionDiffusionModel:
         u = self.as_field(u)
         r = torch.empty_like(u)
         reac = (ctypes.c_double * 4)(*self.reaction)
-        rc = _hip.lib().mjx_rd_residual(self.ny, self.nx, self.bc_code, self.dy, self.dx, self.D, 1.0, 1.0, reac,
+        rc = _hip.lib().mjx_rd_residual(self.ny, self.nx, self.bc_code, self.dy, self.dx, self.D, 1.0, 1.0, 1.0, reac,
                                         _hip.ptr(u), _hip.ptr(u), 0, _hip.ptr(r), _hip.stream_ptr())
         _hip.check(rc, "mjx_rd_residual")
         return -r
@@ -120,6 +120,18 @@ def cn_step(model: ReactionDiffusionModel, y: StateDict, t: float, dt: float, pr
     return out.solution, out.stats
 
 
+def bdf2_step(model: ReactionDiffusionModel, y: StateDict, y_prev: StateDict, t: float, dt: float, dt_prev: float,
+              preconditioner=None, nk_params: Optional[NKParams] = None,
+              consistent: bool = False) -> Tuple[StateDict, NKStats]:
+    """stepping.py:263-310: variable-step BDF2 from the linear extrapolation
+    2 y - y_prev (consistent: see bdf2_coefficients)."""
+    from .newton_krylov import create_bdf2_residual
+    res = create_bdf2_residual(model, y, y_prev, t + dt, dt, dt_prev, consistent)
+    guess = {"u": 2.0 * model.as_field(y["u"]) - model.as_field(y_prev["u"])}
+    out = newton_krylov_solve(res, guess, model, model.params, preconditioner, nk_params, dt)
+    return out.solution, out.stats
+
+
 def sine_mode_ic(n: int, seed: int = 0, modes: int = 8, kmax: int = 6) -> np.ndarray:
     """Sum of ``modes`` sine modes a sin(pi p x) sin(pi q y), a ~ N(0, 0.5),
     p, q uniform in 1..kmax, on interior vertices x_i = i/(n+1) (seeded
@@ -134,3 +146,89 @@ def sine_mode_ic(n: int, seed: int = 0, modes: int = 8, kmax: int = 6) -> np.nda
         p, q = rng.integers(1, kmax + 1, size=2)
         u += a * np.sin(np.pi * p * X) * np.sin(np.pi * q * Y)
     return u
+
+
+def adaptive_integrate_implicit(model: ReactionDiffusionModel, y0: StateDict, t0: float, t_end: float, dt0: float,
+                                method: int = 3, max_steps: int = 10000, cfl_params=None, pid_params=None,
+                                preconditioner=None, nk_params: Optional[NKParams] = None, save_every: int = 1,
+                                save_states: bool = True):
+    """The implicit branch of adaptive_integrate (moljax/core/stepping.py:
+    723-1021) for a reaction-diffusion model, every attempt a native
+    Newton-Krylov solve:
+      BE (3)    err = dt F(y_be);
+      CN (4)    err = y_cn - y_be;
+      BDF2 (5)  err = y_bdf2 - y_be, one BE step first (the reference sends
+                the IMEX values here too; adaptive_integrate routes them to
+                the IMEX driver instead);
+    accept when the scaled error norm <= 1, the state is finite and Newton
+    converged; the next dt from propose_dt with the Newton statistics
+    (implicit_robustness_dt), reject -> dt_new / 2.  Histories as the
+    reference (t, dt, interior states every save_every accepted steps)."""
+    from .dt_policy import (NKStats as PolicyNKStats, PIDParams, compute_error_order,
+                            create_initial_controller_state, propose_dt)
+    from .state import RED_NONFINITE, RED_SCALED_SQ, _reduce
+    from .stepping import AdaptiveResult
+    m = int(method)
+    if m not in (3, 4, 5):
+        raise ValueError("method must be BE (3), CN (4) or BDF2 (5)")
+    if save_every < 1:
+        raise ValueError("save_every must be >= 1")
+    pid = pid_params or PIDParams()
+    order = compute_error_order(m)
+    u = model.as_field(y0["u"]).clone()
+    u_prev, dt_prev = u, float(dt0)
+    dt = float(dt0)
+    max_saves = max_steps // save_every + 1
+    t_hist = torch.zeros(max_saves, dtype=torch.float64)
+    dt_hist = torch.zeros(max_saves, dtype=torch.float64)
+    t_hist[0] = t0
+    hist = [u.clone()] if save_states else []
+    ctrl = create_initial_controller_state()
+    t, widx, n_acc, n_rej, status, step_count = float(t0), 1, 0, 0, 1, 0
+    while status == 1 and t < t_end and widx < max_saves:
+        dtc = min(dt, t_end - t)
+        y = {"u": u}
+        if m == 3 or (m == 5 and step_count < 1):
+            y_new, st = be_step(model, y, t, dtc, preconditioner, nk_params)
+            err = dtc * model.rhs_field(y_new["u"])
+        else:
+            if m == 4:
+                y_new, st = cn_step(model, y, t, dtc, preconditioner, nk_params)
+            else:
+                y_new, st = bdf2_step(model, y, {"u": u_prev}, t, dtc, dt_prev, preconditioner, nk_params)
+            y_be, _ = be_step(model, y, t, dtc, preconditioner, nk_params)
+            err = y_new["u"] - y_be["u"]
+        un = y_new["u"]
+        e = float(torch.sqrt(_reduce(RED_SCALED_SQ, [err], [un], pid.atol, pid.rtol)[0] / un.numel()))
+        e = max(e, 1e-10) if e == e else e
+        finite = float(_reduce(RED_NONFINITE, [un])[0]) == 0.0
+        accept = e <= 1.0 and finite and bool(st.converged)
+        nks = PolicyNKStats(bool(st.converged), int(st.newton_iters), int(st.lin_iters), float(st.final_res_norm))
+        dt_new, new_ctrl = propose_dt(m, model, model.params, y, t, dtc, e, ctrl, nks, cfl_params, pid, order)
+        if accept:
+            if n_acc % save_every == 0:
+                t_hist[widx] = t + dtc
+                dt_hist[widx - 1] = dtc
+                if save_states:
+                    hist.append(un.clone())
+                widx += 1
+            t += dtc
+            u_prev, u = u, un
+            dt_prev = dtc
+            dt = dt_new
+            ctrl = new_ctrl
+            n_acc += 1
+            step_count += 1
+        else:
+            dt = max(dt_new * 0.5, pid.dt_min) if dt_new == dt_new else dt_new
+            if dt <= pid.dt_min * 1.1:
+                status = 3
+            n_rej += 1
+        if not finite:
+            status = 4
+    if t >= t_end:
+        status = 0
+    elif widx >= max_saves:
+        status = 2
+    return AdaptiveResult(t_final=t, y_final={"u": u}, t_history=t_hist, y_history={"u": torch.stack(hist)} if hist else {},
+                          dt_history=dt_hist, n_steps=widx, n_accepted=n_acc, n_rejected=n_rej, status=status)

@@ -369,8 +369,8 @@ def adaptive_integrate(model, y0: StateDict, t0: float, t_end: float, dt0: float
                              stepping.py:864-879);
       ETDRK4                 fixed steps of dt0 (a final shorter step lands
                              on t_end) with the batched ETDRK4 kernel.
-    Implicit values (BE, CN, BDF2) raise NotImplementedError: the implicit
-    solves are served one step at a time by be_step / cn_step."""
+      BE / CN / BDF2         reaction-diffusion models: every attempt a native
+                             Newton-Krylov solve (adaptive_integrate_implicit)."""
     from .dt_policy import CFLParams, PIDParams, compute_error_order, heisenberg_cfl_dt
     m = int(method)
     cfl = cfl_params or CFLParams()
@@ -390,8 +390,14 @@ def adaptive_integrate(model, y0: StateDict, t0: float, t_end: float, dt0: float
                                        save_states=save_states, max_attempts=max_attempts)
     if m == IntegratorType.ETDRK4:
         return _etdrk4_fixed(model, y0, t0, t_end, dt0, max_steps, save_every)
-    raise NotImplementedError("adaptive implicit integration (BE/CN/BDF2) is not on the MI355X path; "
-                              "use be_step / cn_step with newton_krylov_solve")
+    if m in (IntegratorType.BE, IntegratorType.CN, IntegratorType.BDF2):
+        if getattr(model, "mjx_kernel", None) != "reaction_diffusion":
+            raise NotImplementedError("adaptive implicit integration on the MI355X path serves reaction-diffusion "
+                                      "models (create_reaction_diffusion_model)")
+        from .reaction_diffusion import adaptive_integrate_implicit
+        return adaptive_integrate_implicit(model, y0, t0, t_end, dt0, m, max_steps, cfl, pid, preconditioner,
+                                           nk_params, save_every, save_states)
+    raise ValueError(f"unknown integrator {method}")
 
 
 def _etdrk4_fixed(model, y0: StateDict, t0: float, t_end: float, dt: float, max_steps: int,

@@ -6,10 +6,13 @@
 // Problem class: unknown u on an interior (ny, nx) array, 5-point Laplacian
 // with homogeneous Dirichlet boundaries on a vertex grid (zero outside) or
 // periodic wrap, polynomial reaction f(u) = a3 u^3 + a2 u^2 + a1 u + a0:
-//   R(u) = u - u_old - theta dt (D lap u + f(u)) - (1 - theta) dt F_old
-//   J v  = v - theta dt (D lap v + f'(u) v)
-// (moljax/core/newton_krylov.py:362-403 residuals; the JVP replaces the
-// forward-mode AD of newton_krylov.py:61-80).
+//   R(u) = c u - u_old - b (D lap u + f(u)) - fold
+//   J v  = c v - b (D lap v + f'(u) v)
+// with (c, b, u_old, fold) = (1, theta dt, y_n, (1 - theta) dt F(y_n)) for
+// backward Euler / Crank-Nicolson (moljax/core/newton_krylov.py:362-403) and
+// (alpha0, beta, -(alpha1 y_n + alpha2 y_{n-1}), 0) for variable-step BDF2
+// (newton_krylov.py:406-448).  The JVP replaces the forward-mode AD of
+// newton_krylov.py:61-80.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +30,7 @@ struct RDArgs {
   int ny, nx, bc;  // bc 0: Dirichlet (vertex, zero outside), 1: periodic
   double dy2, dx2;
   double D, theta_dt, a3, a2, a1, a0;
+  double lead;  // coefficient c of u in the residual (1 except BDF2)
 };
 
 template <typename T>
@@ -48,7 +52,7 @@ __device__ __forceinline__ T lap5(const T* __restrict__ f, int y, int x, const R
   return yy + xx;
 }
 
-// r = u - u_old - theta dt (D lap u + f(u)) - [F_old term], FP64.
+// r = c u - u_old - b (D lap u + f(u)) - [fold term], FP64.
 __global__ __launch_bounds__(256) void k_rd_residual(const double* __restrict__ u, const double* __restrict__ u_old,
                                                      const double* __restrict__ fold_term, double* __restrict__ r,
                                                      RDArgs a) {
@@ -58,12 +62,12 @@ __global__ __launch_bounds__(256) void k_rd_residual(const double* __restrict__ 
   const int y = (int)(g / a.nx), x = (int)(g % a.nx);
   const double uu = u[g];
   const double f = ((a.a3 * uu + a.a2) * uu + a.a1) * uu + a.a0;
-  double res = uu - u_old[g] - a.theta_dt * (a.D * lap5(u, y, x, a) + f);
+  double res = a.lead * uu - u_old[g] - a.theta_dt * (a.D * lap5(u, y, x, a) + f);
   if (fold_term) res -= fold_term[g];
   r[g] = res;
 }
 
-// w = v - theta dt (D lap v + f'(u) v); u in FP64, v / w in T.
+// w = c v - b (D lap v + f'(u) v); u in FP64, v / w in T.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rd_jvp(const double* __restrict__ u, const T* __restrict__ v,
                                                 T* __restrict__ w, RDArgs a) {
@@ -74,7 +78,7 @@ __global__ __launch_bounds__(256) void k_rd_jvp(const double* __restrict__ u, co
   const double uu = u[g];
   const T fp = (T)((3.0 * a.a3 * uu + 2.0 * a.a2) * uu + a.a1);
   const T vv = v[g];
-  w[g] = vv - (T)a.theta_dt * ((T)a.D * lap5(v, y, x, a) + fp * vv);
+  w[g] = (T)a.lead * vv - (T)a.theta_dt * ((T)a.D * lap5(v, y, x, a) + fp * vv);
 }
 
 // Weighted Jacobi sweep for (I - beta lap) x = r (moljax/core/preconditioners.py:166-198):
@@ -183,7 +187,8 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 // ------------------------------------------------------------- driver
 struct NKConfig {
   int mv_dtype;        // MJX_F32 or MJX_F64: precision of the Krylov operator M A
-  int precond;         // 0 none, 1 exact (I - theta dt D lap)^-1 (DST-I / FFT), 2 Jacobi
+  int precond;         // 0 none, 1 exact (I - dt D lap)^-1 (DST-I / FFT), 2 Jacobi
+  double precond_dt;   // dt of the preconditioner context (preconditioners.py:29-43)
   int jacobi_iters;
   double jacobi_omega;
   int max_newton, max_krylov, restart, max_backtrack;
@@ -377,7 +382,7 @@ class NKSolver {
 
  private:
   int setup_inst() {
-    const double row[8] = {c_.precond == 1 ? a_.theta_dt * a_.D : 0.0, 0, 0, 0, 0, 0, 0, 0};
+    const double row[8] = {c_.precond == 1 ? c_.precond_dt * a_.D : 0.0, 0, 0, 0, 0, 0, 0, 0};
     if (hipMemcpyAsync(inst_, row, sizeof(row), hipMemcpyHostToDevice, st_) != hipSuccess)
       return fail(MJX_E_HIP, "nk: parameter upload failed");
     return hipStreamSynchronize(st_) == hipSuccess ? MJX_OK : fail(MJX_E_HIP, "nk: sync failed");
@@ -461,7 +466,7 @@ class NKSolver {
                                nullptr, inst_, 1, work, st_);
     }
     // Jacobi: x0 = r, n sweeps ping-ponging between out and tmp
-    const double beta = a_.theta_dt * a_.D;
+    const double beta = c_.precond_dt * a_.D;
     const double diag = 1.0 + beta * (2.0 / a_.dx2 + 2.0 / a_.dy2);
     const T* x = in;
     for (int i = 0; i < c_.jacobi_iters; ++i) {
@@ -536,6 +541,7 @@ int mjx_rd_newton_krylov(int ny, int nx, int bc, double dy, double dx, double D,
   a.a2 = reaction[1];
   a.a1 = reaction[2];
   a.a0 = reaction[3];
+  a.lead = nk_params[6];
   NKConfig c;
   c.mv_dtype = nk_iparams[0];
   c.precond = nk_iparams[1];
@@ -550,6 +556,8 @@ int mjx_rd_newton_krylov(int ny, int nx, int bc, double dy, double dx, double D,
   c.backtrack_factor = nk_params[3];
   c.min_decrease = nk_params[4];
   c.jacobi_omega = nk_params[5];
+  c.precond_dt = nk_params[7];
+  if (!(a.lead > 0)) return fail(MJX_E_ARG, "the residual's leading coefficient must be > 0");
   if (c.mv_dtype != MJX_F32 && c.mv_dtype != MJX_F64) return fail(MJX_E_ARG, "operator dtype must be F32 or F64");
   if (c.precond < 0 || c.precond > 2) return fail(MJX_E_ARG, "precond must be 0, 1 or 2");
   if (c.restart < 1 || c.restart > 200) return fail(MJX_E_LIMIT, "restart must be in [1, 200]");
@@ -567,21 +575,21 @@ int mjx_rd_newton_krylov(int ny, int nx, int bc, double dy, double dx, double D,
   return rc;
 }
 
-int mjx_rd_residual(int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
+int mjx_rd_residual(int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta, double lead,
                     const double* reaction, const double* u, const double* u_old, const double* fold_term, double* r,
                     void* stream) {
   if (!reaction || !u || !u_old || !r) return fail(MJX_E_ARG, "null pointer argument");
-  RDArgs a{ny, nx, bc, dy * dy, dx * dx, D, theta * dt, reaction[0], reaction[1], reaction[2], reaction[3]};
+  RDArgs a{ny, nx, bc, dy * dy, dx * dx, D, theta * dt, reaction[0], reaction[1], reaction[2], reaction[3], lead};
   const long long n = (long long)ny * nx;
   hipLaunchKernelGGL(k_rd_residual, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, u, u_old, fold_term, r, a);
   return check_launch("rd_residual");
 }
 
 int mjx_rd_jvp(int dtype, int ny, int nx, int bc, double dy, double dx, double D, double dt, double theta,
-               const double* reaction, const double* u, const void* v, void* w, void* stream) {
+               double lead, const double* reaction, const double* u, const void* v, void* w, void* stream) {
   if (!reaction || !u || !v || !w) return fail(MJX_E_ARG, "null pointer argument");
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
-  RDArgs a{ny, nx, bc, dy * dy, dx * dx, D, theta * dt, reaction[0], reaction[1], reaction[2], reaction[3]};
+  RDArgs a{ny, nx, bc, dy * dy, dx * dx, D, theta * dt, reaction[0], reaction[1], reaction[2], reaction[3], lead};
   const long long n = (long long)ny * nx;
   if (dtype == MJX_F64)
     hipLaunchKernelGGL(k_rd_jvp<double>, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, u, (const double*)v,

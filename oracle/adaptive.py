@@ -216,3 +216,88 @@ def _adaptive(y0, t0, t_end, dt0, stepper, dt_cfl, order, max_steps, pid, save_e
         status = MAX_STEPS_REACHED
     return AdaptiveResult(t_final=t, y_final=y, t_history=t_hist, dt_history=dt_hist, y_saved=saved,
                           n_steps=write_idx, n_accepted=n_acc, n_rejected=n_rej, status=status, attempts=attempts)
+
+
+def implicit_robustness_dt(dt_old, converged, newton_iters, dt_min=1e-12, dt_max=1.0, growth_factor=1.2,
+                           shrink_factor=0.5, fast_iters_threshold=3):
+    """dt_policy.py:279-321 (its own dt_max default of 1.0 applies)."""
+    dt_new = dt_old if converged else dt_old * shrink_factor
+    if converged and newton_iters <= fast_iters_threshold:
+        dt_new = _nanmin(dt_new * growth_factor, dt_max)
+    return _clip(dt_new, dt_min, dt_max)
+
+
+IMPLICIT_ORDER = {"be": 1, "cn": 2, "bdf2": 2}  # dt_policy.py:411-423
+
+
+def adaptive_integrate_implicit(u0, t0, t_end, dt0, D, coef, dy, dx, bc, method="be", precond="dst",
+                                max_steps=10000, pid: PIDParams = None, save_every=1, max_attempts=None,
+                                nk_kw=None):
+    """The implicit branch of adaptive_integrate (stepping.py:723-1021) for a
+    scalar reaction-diffusion problem on an interior array (every point is
+    interior): BE with err = dt F(y_be); CN and BDF2 with err = y - y_be (BDF2
+    starts with one BE step); accept when e <= 1, finite and Newton
+    converged; propose_dt for implicit methods (no CFL limit,
+    implicit_robustness_dt on the Newton count)."""
+    from . import newton_krylov as onk
+    pid = pid or PIDParams()
+    nk_kw = nk_kw or {}
+    order = IMPLICIT_ORDER[method]
+    dt = float(dt0)
+    max_saves = max_steps // save_every + 1
+    t_hist = np.zeros(max_saves)
+    dt_hist = np.zeros(max_saves)
+    t_hist[0] = t0
+    y, y_prev, dt_prev = u0.copy(), u0.copy(), float(dt0)
+    t, prev_err, write_idx, n_acc, n_rej, status, step_count = float(t0), 1.0, 1, 0, 0, RUNNING, 0
+    attempts = []
+    args = (D, coef, dy, dx, bc, precond)
+    while status == RUNNING and t < t_end and write_idx < max_saves:
+        if max_attempts is not None and len(attempts) >= max_attempts:
+            break
+        dtc = _nanmin(dt, t_end - t)
+        if method == "be" or (method == "bdf2" and step_count < 1):
+            st = onk.be_step(y, dtc, *args, **nk_kw)
+            y_new = st.u
+            err = dtc * onk.rhs(y_new, D, coef, dy, dx, bc)
+        else:
+            if method == "cn":
+                st = onk.cn_step(y, dtc, *args, **nk_kw)
+            else:
+                st = onk.bdf2_step(y, y_prev, dtc, dt_prev, *args, **nk_kw)
+            y_new = st.u
+            err = y_new - onk.be_step(y, dtc, *args, **nk_kw).u
+        e = _nanmax(math.sqrt(float(np.sum((err / (pid.atol + pid.rtol * np.abs(y_new))) ** 2)) / err.size), 1e-10)
+        finite = bool(np.all(np.isfinite(y_new)))
+        accept = (e <= 1.0) and finite and bool(st.converged)
+        if e <= 1.0:
+            dt_ad, new_prev = pid_controller_dt(dtc, e, prev_err, pid, order)
+        else:
+            dt_ad, new_prev = handle_rejected_step(dtc, e, pid, order), prev_err
+        dt_new = _clip(implicit_robustness_dt(dt_ad, bool(st.converged), st.newton_iters), pid.dt_min, pid.dt_max)
+        attempts.append((t, dtc, e, accept, st.newton_iters, st.gmres_iters))
+        if accept:
+            if n_acc % save_every == 0:
+                t_hist[write_idx] = t + dtc
+                dt_hist[write_idx - 1] = dtc
+                write_idx += 1
+            t += dtc
+            y_prev, y = y, y_new
+            dt_prev = dtc
+            dt = dt_new
+            prev_err = new_prev
+            n_acc += 1
+            step_count += 1
+        else:
+            dt = _nanmax(dt_new * 0.5, pid.dt_min)
+            if dt <= pid.dt_min * 1.1:
+                status = DT_TOO_SMALL
+            n_rej += 1
+        if not finite:
+            status = NON_FINITE_VALUES
+    if t >= t_end:
+        status = SUCCESS
+    elif write_idx >= max_saves:
+        status = MAX_STEPS_REACHED
+    return AdaptiveResult(t_final=t, y_final={"u": y}, t_history=t_hist, dt_history=dt_hist, y_saved=[],
+                          n_steps=write_idx, n_accepted=n_acc, n_rejected=n_rej, status=status, attempts=attempts)

@@ -54,15 +54,28 @@ def rhs(u, D, coef, dy, dx, bc):
     return D * lap(u, dy, dx, bc) + reaction(u, coef)
 
 
-def residual(u, u_old, dt, theta, D, coef, dy, dx, bc, F_old=None):
-    r = u - u_old - theta * dt * rhs(u, D, coef, dy, dx, bc)
+def residual(u, u_old, dt, theta, D, coef, dy, dx, bc, F_old=None, lead=1.0):
+    """lead * u - u_old - theta dt F(u) [- (1 - theta) dt F_old]."""
+    r = lead * u - u_old - theta * dt * rhs(u, D, coef, dy, dx, bc)
     if theta < 1.0:
         r = r - (1.0 - theta) * dt * F_old
     return r
 
 
-def jvp(u, v, dt, theta, D, coef, dy, dx, bc):
-    return v - theta * dt * (D * lap(v, dy, dx, bc) + reaction_prime(u, coef) * v)
+def jvp(u, v, dt, theta, D, coef, dy, dx, bc, lead=1.0):
+    return lead * v - theta * dt * (D * lap(v, dy, dx, bc) + reaction_prime(u, coef) * v)
+
+
+def bdf2_coefficients(dt, dt_prev, consistent=False):
+    """newton_krylov.py:436-440 (variable-step BDF2, omega = dt/dt_prev):
+    alpha0 = (1+2w)/(1+w), alpha1 = -(1+w), alpha2 = w^2/(1+w) and
+    beta = dt (1+w)/(1+2w).  That beta is the one of the alpha0-normalised
+    form while the alphas are not normalised, so the reference scheme
+    integrates y' = F/alpha0 (2/3 F at constant steps); consistent=True
+    gives beta = dt, the variable-step BDF2 of the docstring at :418."""
+    omega = dt / dt_prev
+    beta = dt if consistent else dt * (1.0 + omega) / (1.0 + 2.0 * omega)
+    return (1.0 + 2.0 * omega) / (1.0 + omega), -(1.0 + omega), omega ** 2 / (1.0 + omega), beta
 
 
 def helmholtz_precond(r, beta, dy, dx, bc):
@@ -161,12 +174,15 @@ class NKResult:
 
 def newton_krylov(u0, u_old, dt, theta, D, coef, dy, dx, bc, precond="dst", newton_tol=1e-8,
                   krylov_tol=1e-8, max_newton=10, max_krylov=50, restart=30, damping=1.0,
-                  backtrack_factor=0.5, max_backtrack=3, min_decrease=1e-4, F_old=None):
-    """newton_krylov.py:171-359 with the GMRES above."""
+                  backtrack_factor=0.5, max_backtrack=3, min_decrease=1e-4, F_old=None, lead=1.0, precond_dt=None):
+    """newton_krylov.py:171-359 with the GMRES above.  The preconditioner
+    approximates (I - precond_dt D lap)^-1 with precond_dt the dt of the
+    preconditioner context, the step dt (newton_krylov.py:214, passed by
+    stepping.py:211,257,307); by default it is dt."""
     def R(u):
-        return residual(u, u_old, dt, theta, D, coef, dy, dx, bc, F_old)
+        return residual(u, u_old, dt, theta, D, coef, dy, dx, bc, F_old, lead)
 
-    beta = theta * dt * D
+    beta = (dt if precond_dt is None else precond_dt) * D
     if precond == "dst":
         M = lambda v: helmholtz_precond(v, beta, dy, dx, bc)
     elif precond == "jacobi":
@@ -182,7 +198,7 @@ def newton_krylov(u0, u_old, dt, theta, D, coef, dy, dx, bc, precond="dst", newt
         r = R(x)
         rn = np.linalg.norm(r)
         converged = rn < newton_tol
-        A = lambda v, x=x: jvp(x, v, dt, theta, D, coef, dy, dx, bc)
+        A = lambda v, x=x: jvp(x, v, dt, theta, D, coef, dy, dx, bc, lead)
         dxv, lin, _, _ = gmres(A, -r, M, krylov_tol, restart, max_krylov)
         lin_total += lin
         history.append(lin)
@@ -217,3 +233,21 @@ def sine_mode_ic(n, seed=0, modes=8, kmax=6):
         p, q = rng.integers(1, kmax + 1, size=2)
         u += a * np.sin(np.pi * p * X) * np.sin(np.pi * q * Y)
     return u
+
+
+def be_step(u, dt, D, coef, dy, dx, bc, precond="dst", **kw):
+    """stepping.py:170-214: Newton-Krylov from the explicit Euler guess."""
+    return newton_krylov(u + dt * rhs(u, D, coef, dy, dx, bc), u, dt, 1.0, D, coef, dy, dx, bc, precond, **kw)
+
+
+def cn_step(u, dt, D, coef, dy, dx, bc, precond="dst", **kw):
+    """stepping.py:217-260."""
+    F0 = rhs(u, D, coef, dy, dx, bc)
+    return newton_krylov(u + dt * F0, u, dt, 0.5, D, coef, dy, dx, bc, precond, F_old=F0, **kw)
+
+
+def bdf2_step(u, u_prev, dt, dt_prev, D, coef, dy, dx, bc, precond="dst", consistent=False, **kw):
+    """stepping.py:263-310 with the residual of newton_krylov.py:406-448."""
+    a0, a1, a2, beta = bdf2_coefficients(dt, dt_prev, consistent)
+    g = -(a1 * u + a2 * u_prev)
+    return newton_krylov(2.0 * u - u_prev, g, beta, 1.0, D, coef, dy, dx, bc, precond, lead=a0, precond_dt=dt, **kw)

@@ -335,3 +335,35 @@ def test_newton_converges_on_allen_cahn_step():
     r = onk.residual(res.u, u_old, 1e-2, 1.0, 1e-3, (-1.0, 0.0, 1.0, 0.0), h, h, "dirichlet")
     assert np.linalg.norm(r) < 1e-8
     assert 0 < res.gmres_iters < 40
+
+
+def test_bdf2_coefficients_and_order():
+    """newton_krylov.py:406-448: at constant steps (omega = 1) the alphas are
+    (3/2, -2, 1/2) and beta = 2 dt / 3.  With f(u) = lam u and D = 0 every
+    point solves y' = lam y: the consistent form converges to exp(lam T) at
+    order 2; the reference form is BDF2 for y' = lam y / alpha0 = 2/3 lam y
+    and converges to exp(2/3 lam T), at first order because its
+    backward-Euler start step follows the unscaled equation."""
+    from oracle import newton_krylov as onk
+    a0, a1, a2, beta = onk.bdf2_coefficients(0.1, 0.1)
+    assert (a0, a1, a2) == (1.5, -2.0, 0.5) and beta == pytest.approx(0.2 / 3, rel=1e-15)
+    assert onk.bdf2_coefficients(0.1, 0.1, consistent=True)[3] == 0.1
+    lam, T = -2.0, 1.0
+
+    def run(n, consistent):
+        dt = T / n
+        u = np.full((3, 3), 1.0)
+        prev = u
+        u = onk.be_step(u, dt, 0.0, (0.0, 0.0, lam, 0.0), 0.25, 0.25, "dirichlet", precond="none",
+                        newton_tol=1e-14).u
+        for _ in range(n - 1):
+            nxt = onk.bdf2_step(u, prev, dt, dt, 0.0, (0.0, 0.0, lam, 0.0), 0.25, 0.25, "dirichlet",
+                                precond="none", consistent=consistent, newton_tol=1e-14).u
+            prev, u = u, nxt
+        return u[1, 1]
+    e1, e2 = abs(run(40, True) - np.exp(lam * T)), abs(run(80, True) - np.exp(lam * T))
+    assert 3.0 < e1 / e2 < 5.0, (e1, e2)
+    r1, r2 = run(40, False), run(80, False)
+    e1, e2 = abs(r1 - np.exp(2.0 / 3.0 * lam * T)), abs(r2 - np.exp(2.0 / 3.0 * lam * T))
+    assert 1.7 < e1 / e2 < 2.3 and e2 < 5e-3, (e1, e2)
+    assert abs(r2 - np.exp(lam * T)) > 0.1
