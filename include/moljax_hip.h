@@ -290,6 +290,29 @@ long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx);
 int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
                   int fill_ghosts, const void* coef, const double* inst, int n_steps, void* work, void* stream);
 
+/* Exponential Euler / exponential Adams-Bashforth-2 tables for periodic
+ * Gray-Scott: three real tables E = e^z, dt phi1(z), dt phi2(z) of ny*(nx+2)
+ * elements each, z = dt * D_f * (lamx[kx] + lamy[ky]) in the separated
+ * spectral layout of mjx_etdrk4_coeffs, FP64 with the series of
+ * moljax/core/fft_integrators.py:37-64 (_phi1, _phi2) for |z| < 1e-4.
+ * mjx_etd_phi_coeff_elems gives the element count. */
+long long mjx_etd_phi_coeff_elems(int ny, int nx);
+int mjx_etd_phi_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
+                       double dt, void* coef, void* stream);
+
+/* Batched coupled Gray-Scott ETD1 (order 1) or ETD2 (order 2), n_steps in
+ * place on (u, v) of B square n x n instances, one row pass and one column
+ * pass per step; coef from mjx_etd_phi_coeffs; work as for mjx_gs_etdrk4.
+ *   ETD1: u^ <- E u^ + dt phi1 N^            (fft_integrators.py:119-174)
+ *   ETD2: ... + dt phi2 (N^ - N^_{n-1})      (fft_integrators.py:177-249)
+ * prev (ETD2 only): 0 = no N_{n-1} (the first step is ETD1, as the
+ * reference's N_prev = None), 1 = the physical N_{n-1} of both fields in
+ * (nu_prev, nv_prev) with the state's pitch / istride / origin, 2 = the N^
+ * kept in work by the previous call on the same work and batch. */
+int mjx_gs_etd_exp(int dtype, int order, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                   int fill_ghosts, const void* coef, const double* inst, int prev, const void* nu_prev,
+                   const void* nv_prev, int n_steps, void* work, void* stream);
+
 /* Implicit reaction-diffusion step by Newton-Krylov on an interior (ny, nx)
  * array: bc 0 homogeneous Dirichlet on a vertex grid (zero outside), bc 1
  * periodic; u_t = D lap_h u + f(u), f = reaction[0] u^3 + reaction[1] u^2 +

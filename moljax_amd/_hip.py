@@ -58,6 +58,9 @@ PROTOTYPES = {
     "mjx_etdrk4_coeffs": (_i, [_i, _i, _i, _p, _p, _d, _d, _d, _p, _p]),
     "mjx_etdrk4_workspace_bytes": (_ll, [_i, _i, _i, _i]),
     "mjx_gs_etdrk4": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _i, _p, _p]),
+    "mjx_etd_phi_coeff_elems": (_ll, [_i, _i]),
+    "mjx_etd_phi_coeffs": (_i, [_i, _i, _i, _p, _p, _d, _d, _d, _p, _p]),
+    "mjx_gs_etd_exp": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _i, _p, _p, _i, _p, _p]),
     "mjx_nk_workspace_bytes": (_ll, [_i, _i, _i]),
     "mjx_rd_newton_krylov": (_i, [_i, _i, _i, _d, _d, _d, _d, _d, _dp, _p, _p, _p, _p, _dp,
                                   ctypes.POINTER(ctypes.c_int), _dp, _p, _p]),

@@ -21,7 +21,7 @@ from .dt_policy import (CFLParams, PIDParams, ControllerState, create_initial_co
 from .fft_nonperiodic import (BCType as NonperiodicBCType, dst_I_fast, idst_I_fast, solve_helmholtz_dirichlet,
                               solve_helmholtz_neumann, solve_poisson_dirichlet, solve_poisson_neumann)
 from .fft_operators import DiffusionOperator
-from .fft_integrators import etdrk4_step, etd_integrate
+from .fft_integrators import etd1_step, etd2_step, etdrk4_step, etd_integrate
 from .newton_krylov import NKParams, NKStats, NKResult, newton_krylov_solve, create_implicit_residual
 from .preconditioners import (PrecondContext, IdentityPreconditioner, DiffusionPreconditioner,
                               FFTDiffusionPreconditioner, DSTDiffusionPreconditioner,

@@ -8,6 +8,11 @@ form of benchmarks/benchmark_gray_scott.py:336-356).  The reference library's
 etdrk4_step evaluates the nonlinearity one field at a time (fft_integrators.py
 :311,319,327) and cannot couple u and v; here the coupled form is served and
 other nonlinearities raise NotImplementedError.
+
+ETD1 (exponential Euler) and ETD2 (exponential Adams-Bashforth-2) run as one
+row pass and one column pass per step (mjx_gs_etd_exp) with FP64 tables
+e^z, dt phi1(z), dt phi2(z) (mjx_etd_phi_coeffs); ETD2 keeps the previous
+step's spectral nonlinear term on the device between steps and calls.
 """
 from typing import Callable, Dict, Optional, Tuple
 
@@ -84,6 +89,71 @@ def gs_etdrk4_inplace(u: torch.Tensor, v: torch.Tensor, coef: torch.Tensor, rows
     _hip.check(rc, "mjx_gs_etdrk4")
 
 
+_PHI_CACHE: Dict[tuple, torch.Tensor] = {}
+
+
+def etd_phi_coefficients(lamx: np.ndarray, lamy: np.ndarray, Du: float, Dv: float, dt: float,
+                         dtype=torch.float32) -> torch.Tensor:
+    """(3, ny, nx+2) device tables e^z, dt phi1(z), dt phi2(z), z = dt D lam,
+    for both fields in the separated spectral layout (fft_integrators.py
+    :37-64 in FP64, rounded once to ``dtype``).  Cached."""
+    lamx = np.ascontiguousarray(lamx, dtype=np.float64)
+    lamy = np.ascontiguousarray(lamy, dtype=np.float64)
+    key = (lamx.tobytes(), lamy.tobytes(), float(Du), float(Dv), float(dt), dtype)
+    hit = _PHI_CACHE.get(key)
+    if hit is not None:
+        return hit
+    lib = _hip.lib()
+    ny, nx = lamy.size, lamx.size
+    lx = torch.from_numpy(lamx).cuda()
+    ly = torch.from_numpy(lamy).cuda()
+    coef = torch.empty((3, ny, nx + 2), dtype=dtype, device="cuda")
+    rc = lib.mjx_etd_phi_coeffs(_hip.dtype_code(dtype), ny, nx, _hip.ptr(lx), _hip.ptr(ly), float(Du), float(Dv),
+                                float(dt), _hip.ptr(coef), _hip.stream_ptr())
+    _hip.check(rc, "mjx_etd_phi_coeffs")
+    _PHI_CACHE[key] = coef
+    return coef
+
+
+def gs_etd_exp_inplace(u: torch.Tensor, v: torch.Tensor, coef: torch.Tensor, rows, order: int, n_steps: int,
+                       prev: int = 0, n_prev: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+                       fill_ghosts: bool = False, work: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """n_steps of ETD1 (order 1) or ETD2 (order 2) in place on (..., n, n)
+    arrays.  ETD2's N_{n-1}: prev 0 none (first step ETD1), 1 physical
+    ``n_prev`` = (Nu, Nv) laid out like (u, v), 2 the one kept in ``work`` by
+    the previous call with the same work.  Returns the workspace."""
+    tu, pitch, istr, B = view2d(u)
+    tv, pv, iv, _ = view2d(v)
+    if (pv, iv) != (pitch, istr) or tu.dtype != tv.dtype:
+        raise ValueError("u and v must share dtype and layout")
+    if coef.dtype != tu.dtype:
+        raise ValueError("coefficient dtype must match the state dtype")
+    off = 1 if fill_ghosts else 0
+    ny, nx = tu.shape[-2] - 2 * off, tu.shape[-1] - 2 * off
+    inst = rows if isinstance(rows, torch.Tensor) else inst_table(rows, tu.device)
+    lib = _hip.lib()
+    if work is None:
+        nbytes = lib.mjx_etdrk4_workspace_bytes(_hip.dtype_code(tu.dtype), B, ny, nx)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=tu.device)
+    pu = tu[:, 1:, 1:] if off else tu
+    pvv = tv[:, 1:, 1:] if off else tv
+    nu = nv = None
+    if prev == 1:
+        if n_prev is None:
+            raise ValueError("prev = 1 needs n_prev")
+        nu_t, npitch, nistr, _ = view2d(n_prev[0].to(tu.dtype))
+        nv_t, _, _, _ = view2d(n_prev[1].to(tu.dtype))
+        if (npitch, nistr) != (pitch, istr) or nu_t.shape != tu.shape or nv_t.shape != tu.shape:
+            raise ValueError("n_prev must be laid out like the state")
+        nu, nv = (nu_t[:, 1:, 1:], nv_t[:, 1:, 1:]) if off else (nu_t, nv_t)
+    rc = lib.mjx_gs_etd_exp(_hip.dtype_code(tu.dtype), int(order), B, ny, nx, _hip.ptr(pu), _hip.ptr(pvv), pitch,
+                            istr, int(bool(fill_ghosts)), _hip.ptr(coef), _hip.ptr(inst), int(prev),
+                            _hip.ptr(nu) if nu is not None else None, _hip.ptr(nv) if nv is not None else None,
+                            int(n_steps), _hip.ptr(work), _hip.stream_ptr())
+    _hip.check(rc, "mjx_gs_etd_exp")
+    return work
+
+
 def gray_scott_nonlinear_rhs(F: float, k: float) -> Callable[[StateDict, float], StateDict]:
     """N(u, v) = (-u v^2 + F(1-u), u v^2 - (F+k) v) as a nonlinear_rhs callable
     the ETD drivers recognise."""
@@ -138,14 +208,50 @@ def etdrk4_step(state: StateDict, t: float, dt: float, linear_ops: Dict[str, Dif
     return {"u": u, "v": v}
 
 
+def _exp_setup(state: StateDict, dt: float, linear_ops, nonlinear_rhs):
+    F, k = _gray_scott_params(nonlinear_rhs)
+    lamx, lamy, Du, Dv = _linear_symbols(linear_ops)
+    u = as_device(state["u"])
+    dtype = field_dtype(u)
+    u = u.to(dtype).clone().contiguous()
+    v = as_device(state["v"], dtype).clone().contiguous()
+    coef = etd_phi_coefficients(lamx, lamy, Du, Dv, dt, dtype)
+    return u, v, coef, [[0, 0, 0, 0, F, k, dt]] * view2d(u)[3], F, k
+
+
+def etd1_step(state: StateDict, t: float, dt: float, linear_ops: Dict[str, DiffusionOperator],
+              nonlinear_rhs: Callable[[StateDict, float], StateDict]) -> StateDict:
+    """fft_integrators.py:119-174: u^ <- e^z u^ + dt phi1(z) N^ for the
+    coupled Gray-Scott state, z = dt D lam per field."""
+    u, v, coef, rows, _, _ = _exp_setup(state, dt, linear_ops, nonlinear_rhs)
+    gs_etd_exp_inplace(u, v, coef, rows, 1, 1)
+    return {"u": u, "v": v}
+
+
+def etd2_step(state: StateDict, t: float, dt: float, linear_ops: Dict[str, DiffusionOperator],
+              nonlinear_rhs: Callable[[StateDict, float], StateDict],
+              N_prev: Optional[StateDict] = None) -> Tuple[StateDict, StateDict]:
+    """fft_integrators.py:177-249: exponential Adams-Bashforth-2; returns
+    (new state, N of the input state) — ETD1 when N_prev is None."""
+    from .operators import gray_scott_reaction
+    u, v, coef, rows, F, k = _exp_setup(state, dt, linear_ops, nonlinear_rhs)
+    nu, nv = gray_scott_reaction(u, v, F, k)
+    if N_prev is None:
+        gs_etd_exp_inplace(u, v, coef, rows, 2, 1)
+    else:
+        npu = as_device(N_prev["u"], u.dtype).contiguous()
+        npv = as_device(N_prev["v"], u.dtype).contiguous()
+        gs_etd_exp_inplace(u, v, coef, rows, 2, 1, prev=1, n_prev=(npu, npv))
+    return {"u": u, "v": v}, {"u": nu, "v": nv}
+
+
 def etd_integrate(u0: StateDict, t_span: Tuple[float, float], dt: float, linear_ops: Dict[str, DiffusionOperator],
-                  nonlinear_rhs: Callable[[StateDict, float], StateDict], method: str = "etdrk4",
+                  nonlinear_rhs: Callable[[StateDict, float], StateDict], method: str = "etd1",
                   save_every: int = 1):
-    """fft_integrators.py:344-393: int((t_end - t_start)/dt) steps, returns
+    """fft_integrators.py:344-393: int((t_end - t_start)/dt) steps of 'etd1',
+    'etd2' (N_prev carried on the device across saves) or 'etdrk4'; returns
     (t_history, state_history) with the initial state first."""
-    if method != "etdrk4":
-        if method in ("etd1", "etd2"):
-            raise NotImplementedError(f"{method} is not on the MI355X path; use method='etdrk4'")
+    if method not in ("etd1", "etd2", "etdrk4"):
         raise ValueError(f"Unknown method: {method}. Use 'etd1', 'etd2', or 'etdrk4'")
     F, k = _gray_scott_params(nonlinear_rhs)
     lamx, lamy, Du, Dv = _linear_symbols(linear_ops)
@@ -155,14 +261,22 @@ def etd_integrate(u0: StateDict, t_span: Tuple[float, float], dt: float, linear_
     dtype = field_dtype(u)
     u = u.to(dtype).clone().contiguous()
     v = as_device(u0["v"], dtype).clone().contiguous()
-    coef = etdrk4_coefficients(lamx, lamy, Du, Dv, dt, dtype)
+    if method == "etdrk4":
+        coef = etdrk4_coefficients(lamx, lamy, Du, Dv, dt, dtype)
+    else:
+        coef = etd_phi_coefficients(lamx, lamy, Du, Dv, dt, dtype)
     rows = inst_table([[0, 0, 0, 0, F, k, dt]] * view2d(u)[3], u.device)
     t_hist = [t0]
     hist = [{"u": u.clone(), "v": v.clone()}]
-    done = 0
+    done, work = 0, None
     while done < n_steps:
         chunk = min(save_every, n_steps - done)
-        gs_etdrk4_inplace(u, v, coef, rows, chunk)
+        if method == "etdrk4":
+            gs_etdrk4_inplace(u, v, coef, rows, chunk)
+        else:
+            order = 1 if method == "etd1" else 2
+            work = gs_etd_exp_inplace(u, v, coef, rows, order, chunk, prev=2 if (order == 2 and done) else 0,
+                                      work=work)
         done += chunk
         if chunk == save_every:
             t_hist.append(t0 + done * dt)

@@ -16,6 +16,17 @@ int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du
 int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, double* coef,
                hipStream_t st);
 
+int etd_exp_run(int B, int n, float* u, float* v, long long pitch, long long istride, int ghosts, const float* coef,
+                const InstParams* ip, int order, int prev, const float* nu, const float* nv, int n_steps,
+                cpx<float>* work, hipStream_t st);
+int etd_exp_run(int B, int n, double* u, double* v, long long pitch, long long istride, int ghosts,
+                const double* coef, const InstParams* ip, int order, int prev, const double* nu, const double* nv,
+                int n_steps, cpx<double>* work, hipStream_t st);
+int etd_phi_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt,
+                   float* coef, hipStream_t st);
+int etd_phi_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt,
+                   double* coef, hipStream_t st);
+
 static bool etd_size(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024; }
 }  // namespace mjx
 
@@ -57,6 +68,41 @@ int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long 
                    (cpx<float>*)work, st);
   return etd_run(B, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)coef, ip, n_steps,
                  (cpx<double>*)work, st);
+}
+
+long long mjx_etd_phi_coeff_elems(int ny, int nx) { return 3LL * ny * (nx + 2); }
+
+int mjx_etd_phi_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
+                       double dt, void* coef, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (ny < 2 || nx < 2 || (nx & 1)) return fail(MJX_E_ARG, "grid must be at least 2x2 with even nx");
+  if (!lamx || !lamy || !coef) return fail(MJX_E_ARG, "null pointer argument");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == MJX_F32) return etd_phi_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (float*)coef, st);
+  return etd_phi_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (double*)coef, st);
+}
+
+int mjx_gs_etd_exp(int dtype, int order, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                   int fill_ghosts, const void* coef, const double* inst, int prev, const void* nu_prev,
+                   const void* nv_prev, int n_steps, void* work, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (order != 1 && order != 2) return fail(MJX_E_ARG, "order must be 1 (ETD1) or 2 (ETD2)");
+  if (prev < 0 || prev > 2 || (order == 1 && prev != 0)) return fail(MJX_E_ARG, "prev must be 0, 1 or 2 (ETD2 only)");
+  if (ny != nx || !etd_size(nx)) return fail(MJX_E_SIZE, "ETD grid must be square with n in {32,...,1024}");
+  if (B < 1 || !u || !v || !coef || !inst || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (prev == 1 && (!nu_prev || !nv_prev)) return fail(MJX_E_ARG, "prev = 1 needs nu_prev and nv_prev");
+  if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
+  if (pitch < nx || (fill_ghosts && pitch < nx + 2)) return fail(MJX_E_ARG, "pitch too small");
+  if (n_steps == 0) return MJX_OK;
+  int rc = ensure_tables(nullptr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (dtype == MJX_F32)
+    return etd_exp_run(B, nx, (float*)u, (float*)v, pitch, istride, fill_ghosts, (const float*)coef, ip, order, prev,
+                       (const float*)nu_prev, (const float*)nv_prev, n_steps, (cpx<float>*)work, st);
+  return etd_exp_run(B, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)coef, ip, order, prev,
+                     (const double*)nu_prev, (const double*)nv_prev, n_steps, (cpx<double>*)work, st);
 }
 
 }  // extern "C"

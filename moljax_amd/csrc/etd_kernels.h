@@ -78,12 +78,37 @@ __device__ __forceinline__ cpx<T> caxpy(T a, cpx<T> x, cpx<T> y) {  // a*x + y
   return mk(a * x.re + y.re, a * x.im + y.im);
 }
 
+// Exponential Euler / exponential Adams-Bashforth-2 column stages
+// (moljax/core/fft_integrators.py:119-249), with the table slots
+// {E, dt phi1, dt phi2}:
+//   ETD1        u^ <- E u^ + dt phi1 N^
+//   ETD2_FIRST  ETD1, keeping N^ for the next step
+//   ETD2        u^ <- E u^ + dt phi1 N^ + dt phi2 (N^ - N^_prev), keeping N^
+//   STORE_N     N^_prev <- the y-FFT of the input (no output plane)
+enum { STAGE_ETD1 = 11, STAGE_ETD2_FIRST = 12, STAGE_ETD2 = 13, STAGE_STORE_N = 14 };
+template <int STAGE>
+constexpr bool two_inputs() { return STAGE == 1 || (STAGE >= STAGE_ETD1 && STAGE <= STAGE_ETD2); }
+
 // Stage arithmetic on one separated-layout spectral value at (s, ky).
 // x is the y-FFT of the stage input (STAGE 1: state and N_n in x, x2).
 template <typename T, int STAGE>
 __device__ __forceinline__ cpx<T> etd_stage(long long si, long long ci, long long tstride, const T* __restrict__ coef,
                                             cpx<T> x, cpx<T> x2, cpx<T>* __restrict__ Uh, cpx<T>* __restrict__ Nn,
                                             cpx<T>* __restrict__ Fh, T s) {
+  if constexpr (STAGE == STAGE_STORE_N) {
+    Nn[si] = x;
+    return x;
+  } else if constexpr (STAGE >= STAGE_ETD1) {
+    const T E = coef[ci], F1 = coef[ci + tstride];
+    cpx<T> out = caxpy(E, x, scale(x2, F1));
+    if constexpr (STAGE == STAGE_ETD2) {
+      const T F2 = coef[ci + 2 * tstride];
+      const cpx<T> np = Nn[si];
+      out = caxpy(F2, mk(x2.re - np.re, x2.im - np.im), out);
+    }
+    if constexpr (STAGE != STAGE_ETD1) Nn[si] = x2;
+    return scale(out, s);
+  }
   const T E = coef[ci], E2 = coef[ci + tstride], Q = coef[ci + 2 * tstride];
   if constexpr (STAGE == 1) {
     Uh[si] = x;
@@ -131,11 +156,11 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
   for (int m = 0; m < C::P; ++m) v[m] = Win[woff + (long long)(j + C::TN * m) * nx];
 #pragma unroll
   for (int m = 0; m < C::P; ++m)
-    v2[m] = STAGE == 1 ? Win2[woff + (long long)(j + C::TN * m) * nx] : mk((T)0, (T)0);
+    v2[m] = two_inputs<STAGE>() ? Win2[woff + (long long)(j + C::TN * m) * nx] : mk((T)0, (T)0);
   TwCache<T, NY> twc;
   load_twiddles(twc, j, tw);
   line_fft<T, NY, -1, TC, 0>(v, j, lds + c, twc);
-  if constexpr (STAGE == 1) line_fft<T, NY, -1, TC, 0>(v2, j, lds + c, twc);
+  if constexpr (two_inputs<STAGE>()) line_fft<T, NY, -1, TC, 0>(v2, j, lds + c, twc);
 
   const long long sbase = (long long)inst * tstride;
   if (p0 == 0 || p0 == half) {
@@ -150,7 +175,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < C::P; ++m) zm[m] = conjg(lds[((NY - (j + C::TN * m)) & (NY - 1)) * TC + c]);
-    if constexpr (STAGE == 1) {
+    if constexpr (two_inputs<STAGE>()) {
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v2[m];
@@ -169,7 +194,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
         const cpx<T> A = mk((v[m].re + zm[m].re) * (T)0.5, (v[m].im + zm[m].im) * (T)0.5);
         const cpx<T> Bv = mk((v[m].im - zm[m].im) * (T)0.5, -(v[m].re - zm[m].re) * (T)0.5);
         cpx<T> A2 = mk((T)0, (T)0), B2 = A2;
-        if constexpr (STAGE == 1) {
+        if constexpr (two_inputs<STAGE>()) {
           A2 = mk((v2[m].re + zm2[m].re) * (T)0.5, (v2[m].im + zm2[m].im) * (T)0.5);
           B2 = mk((v2[m].im - zm2[m].im) * (T)0.5, -(v2[m].re - zm2[m].re) * (T)0.5);
         }
@@ -189,6 +214,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
       v[m] = etd_stage<T, STAGE>(sbase + crow + sc, crow + sc, tstride, coef, v[m], v2[m], Uh, Nn, Fh, s);
     }
   }
+  if constexpr (STAGE == STAGE_STORE_N) return;
   line_fft<T, NY, +1, TC, 0>(v, j, lds + c, twc);
 #pragma unroll
   for (int m = 0; m < C::P; ++m) Wout[woff + (long long)(j + C::TN * m) * nx] = v[m];
@@ -255,6 +281,36 @@ __global__ __launch_bounds__(256) void k_etd_coeffs(const double* __restrict__ l
   coef[g + 3 * n] = (T)(dt * a1.re / M);
   coef[g + 4 * n] = (T)(2.0 * dt * a2.re / M);
   coef[g + 5 * n] = (T)(dt * a3.re / M);
+}
+
+// Exponential Euler / AB2 tables {E, dt phi1, dt phi2} of z = dt D lam in
+// FP64, with the series of fft_integrators.py:39-60 for |z| < 1e-4:
+//   phi1 = (e^z - 1)/z      ~ 1 + z/2 + z^2/6 + z^3/24
+//   phi2 = (e^z - 1 - z)/z^2 ~ 1/2 + z/6 + z^2/24 + z^3/120
+template <typename T>
+__global__ __launch_bounds__(256) void k_etd_phi_coeffs(const double* __restrict__ lamx,
+                                                        const double* __restrict__ lamy, int ny, int nx, double Du,
+                                                        double Dv, double dt, T* __restrict__ coef) {
+  const int sw = nx + 2, half = nx / 2;
+  const long long n = (long long)ny * sw;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int ky = (int)(g / sw), s = (int)(g % sw);
+  const bool fu = s <= half;
+  const int kx = fu ? s : s - half - 1;
+  const double z = dt * ((fu ? Du : Dv) * (lamx[kx] + lamy[ky]));
+  const double ez = exp(z);
+  double p1, p2;
+  if (fabs(z) < 1e-4) {
+    p1 = 1.0 + z / 2.0 + z * z / 6.0 + z * z * z / 24.0;
+    p2 = 0.5 + z / 6.0 + z * z / 24.0 + z * z * z / 120.0;
+  } else {
+    p1 = (ez - 1.0) / z;
+    p2 = (ez - 1.0 - z) / (z * z);
+  }
+  coef[g] = (T)ez;
+  coef[g + n] = (T)(dt * p1);
+  coef[g + 2 * n] = (T)(dt * p2);
 }
 
 // ------------------------------------------------------------- launch
@@ -360,6 +416,71 @@ int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParam
     if (!rc) rc = E::template col<4>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Ws, st);
   }
   return rc;
+}
+
+// n_steps of exponential Euler (order 1) or AB2 (order 2) on the packed
+// state in plane cur (Ws or Wg; each step writes the other one, and cur
+// names the plane holding the result).  AB2 keeps N^ of the previous step in
+// the separated plane Nn (have_prev: Nn already holds it, else the first step
+// is ETD1).
+template <typename T, int N>
+int etd_exp_steps_sized(int B, const EtdPlanes<T>& p, cpx<T>*& cur, const T* coef, const InstParams* ip, int order,
+                        int have_prev, int n_steps, hipStream_t st) {
+  using E = EtdTiles<T, N>;
+  constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
+  const int LR = E::pick_row(tuned_lines(TK_ETD_ROW, dt_, N));
+  const int LC = E::pick_col(tuned_lines(TK_ETD_COL, dt_, N));
+  int rc = MJX_OK;
+  for (int s = 0; s < n_steps && rc == 0; ++s) {
+    cpx<T>* next = cur == p.Ws ? p.Wg : p.Ws;
+    rc = E::row_nl(LR, B, N, cur, p.Wn, ip, st);
+    if (rc) break;
+    if (order == 1)
+      rc = E::template col<STAGE_ETD1>(LC, B, N, cur, p.Wn, p.Uh, p.Nn, p.Fh, coef, next, st);
+    else if (s == 0 && !have_prev)
+      rc = E::template col<STAGE_ETD2_FIRST>(LC, B, N, cur, p.Wn, p.Uh, p.Nn, p.Fh, coef, next, st);
+    else
+      rc = E::template col<STAGE_ETD2>(LC, B, N, cur, p.Wn, p.Uh, p.Nn, p.Fh, coef, next, st);
+    cur = next;
+  }
+  return rc;
+}
+
+// Nn <- the separated-layout spectrum of the packed plane W (a nonlinear
+// term packed by row_fwd_pack with the same 1/n scale as row_nl's output).
+template <typename T, int N>
+int etd_store_n_sized(int B, const EtdPlanes<T>& p, const cpx<T>* W, hipStream_t st) {
+  using E = EtdTiles<T, N>;
+  constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
+  const int LC = E::pick_col(tuned_lines(TK_ETD_COL, dt_, N));
+  return E::template col<STAGE_STORE_N>(LC, B, N, W, nullptr, p.Uh, p.Nn, p.Fh, nullptr, nullptr, st);
+}
+
+template <typename T>
+int etd_exp_steps(int B, int n, const EtdPlanes<T>& p, cpx<T>*& cur, const T* coef, const InstParams* ip, int order,
+                  int have_prev, int n_steps, hipStream_t st) {
+  switch (n) {
+    case 32: return etd_exp_steps_sized<T, 32>(B, p, cur, coef, ip, order, have_prev, n_steps, st);
+    case 64: return etd_exp_steps_sized<T, 64>(B, p, cur, coef, ip, order, have_prev, n_steps, st);
+    case 128: return etd_exp_steps_sized<T, 128>(B, p, cur, coef, ip, order, have_prev, n_steps, st);
+    case 256: return etd_exp_steps_sized<T, 256>(B, p, cur, coef, ip, order, have_prev, n_steps, st);
+    case 512: return etd_exp_steps_sized<T, 512>(B, p, cur, coef, ip, order, have_prev, n_steps, st);
+    case 1024: return etd_exp_steps_sized<T, 1024>(B, p, cur, coef, ip, order, have_prev, n_steps, st);
+    default: return fail(MJX_E_SIZE, "ETD grid " + std::to_string(n) + " not supported: square n in 32..1024");
+  }
+}
+
+template <typename T>
+int etd_store_n(int B, int n, const EtdPlanes<T>& p, const cpx<T>* W, hipStream_t st) {
+  switch (n) {
+    case 32: return etd_store_n_sized<T, 32>(B, p, W, st);
+    case 64: return etd_store_n_sized<T, 64>(B, p, W, st);
+    case 128: return etd_store_n_sized<T, 128>(B, p, W, st);
+    case 256: return etd_store_n_sized<T, 256>(B, p, W, st);
+    case 512: return etd_store_n_sized<T, 512>(B, p, W, st);
+    case 1024: return etd_store_n_sized<T, 1024>(B, p, W, st);
+    default: return fail(MJX_E_SIZE, "ETD grid " + std::to_string(n) + " not supported: square n in 32..1024");
+  }
 }
 
 template <typename T>

@@ -16,6 +16,14 @@ b2 as the library's etdrk4_step (moljax/core/fft_integrators.py:110-115,
 Pinned by the z -> 0 limits (Q -> dt/2, f1, f2, f3 -> dt/6), the direct
 phi-function formulas at moderate |z|, linear exactness (N = 0 gives E u) and
 fourth-order convergence of the Cox-Matthews form (tests/test_oracle_golden.py).
+
+Exponential Euler (ETD1) and exponential Adams-Bashforth-2 (ETD2) follow
+moljax/core/fft_integrators.py:37-64 (phi1, phi2 with the cubic series for
+|z| < 1e-4) and :119-249 (one nonlinear evaluation of the whole state per
+step, so u and v couple; ETD2 without N_prev is ETD1).  Pinned by the
+series / expm1 forms of phi1, phi2, linear exactness, the reduction to
+forward Euler / Adams-Bashforth-2 on spatially constant fields (lam = 0),
+and first / second-order convergence (tests/test_oracle_golden.py).
 """
 import numpy as np
 
@@ -82,3 +90,62 @@ def gs_etdrk4(u, v, lamx, lamy, Du, Dv, F, k, dt, n_steps, cox_matthews=True):
     for _ in range(n_steps):
         u, v = gs_etdrk4_step(u, v, cu, cv, F, k, cox_matthews)
     return u, v
+
+
+def phi1(z):
+    """fft_integrators.py:37-50."""
+    z = np.asarray(z, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        direct = (np.exp(z) - 1.0) / z
+    return np.where(np.abs(z) < 1e-4, 1.0 + z / 2.0 + z ** 2 / 6.0 + z ** 3 / 24.0, direct)
+
+
+def phi2(z):
+    """fft_integrators.py:53-64."""
+    z = np.asarray(z, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        direct = (np.exp(z) - 1.0 - z) / (z * z)
+    return np.where(np.abs(z) < 1e-4, 0.5 + z / 6.0 + z ** 2 / 24.0 + z ** 3 / 120.0, direct)
+
+
+def gs_nonlinear(u, v, F, k):
+    return -u * v * v + F * (1 - u), u * v * v - (F + k) * v
+
+
+def gs_etd1_step(u, v, lamx, lamy, Du, Dv, F, k, dt):
+    """fft_integrators.py:119-174 for the coupled Gray-Scott state."""
+    lam = lamy[:, None] + lamx[None, :]
+    out = []
+    Nu, Nv = gs_nonlinear(u, v, F, k)
+    for f, N, D in ((u, Nu, Du), (v, Nv, Dv)):
+        z = dt * D * lam
+        out.append(np.real(np.fft.ifft2(np.exp(z) * np.fft.fft2(f) + dt * phi1(z) * np.fft.fft2(N))))
+    return out[0], out[1]
+
+
+def gs_etd2_step(u, v, lamx, lamy, Du, Dv, F, k, dt, N_prev=None):
+    """fft_integrators.py:177-249; returns (u, v, (Nu, Nv) of the input)."""
+    N_curr = gs_nonlinear(u, v, F, k)
+    if N_prev is None:
+        un, vn = gs_etd1_step(u, v, lamx, lamy, Du, Dv, F, k, dt)
+        return un, vn, N_curr
+    lam = lamy[:, None] + lamx[None, :]
+    out = []
+    for f, Nn, Np, D in ((u, N_curr[0], N_prev[0], Du), (v, N_curr[1], N_prev[1], Dv)):
+        z = dt * D * lam
+        Nh, Nph = np.fft.fft2(Nn), np.fft.fft2(Np)
+        uh = np.exp(z) * np.fft.fft2(f) + dt * phi1(z) * Nh + dt * phi2(z) * (Nh - Nph)
+        out.append(np.real(np.fft.ifft2(uh)))
+    return out[0], out[1], N_curr
+
+
+def gs_etd(u, v, lamx, lamy, Du, Dv, F, k, dt, n_steps, order, N_prev=None):
+    """n_steps of ETD1 (order 1) or ETD2 (order 2, N_prev carried as in
+    fft_integrators.py:376-387); returns (u, v, N_prev after the last step)."""
+    for _ in range(n_steps):
+        if order == 1:
+            N_prev = gs_nonlinear(u, v, F, k)
+            u, v = gs_etd1_step(u, v, lamx, lamy, Du, Dv, F, k, dt)
+        else:
+            u, v, N_prev = gs_etd2_step(u, v, lamx, lamy, Du, Dv, F, k, dt, N_prev)
+    return u, v, N_prev

@@ -86,7 +86,7 @@ def test_etdrk4_step_api_fd_symbol(gpu):
     lam = osp.laplacian_symbol_1d(n, grid.dx)
     ru, rv = oe.gs_etdrk4(u0, v0, lam, lam, 2e-5, 1e-5, 0.035, 0.065, 1.0, 1)
     assert ogs.rel_l2(_np(one["u"]), ru) < 1e-12
-    t, hist = etd_integrate(st, (0.0, 4.0), 1.0, ops, nl, save_every=2)
+    t, hist = etd_integrate(st, (0.0, 4.0), 1.0, ops, nl, method="etdrk4", save_every=2)
     assert len(hist) == 3 and np.allclose(t.numpy(), [0, 2, 4])
     ru4, _ = oe.gs_etdrk4(u0, v0, lam, lam, 2e-5, 1e-5, 0.035, 0.065, 1.0, 4)
     assert ogs.rel_l2(_np(hist[-1]["u"]), ru4) < 1e-12

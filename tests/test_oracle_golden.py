@@ -281,6 +281,77 @@ def test_etdrk4_linear_exact_and_fourth_order():
     assert math.log2(bad[0] / bad[1]) < 1.5
 
 
+def test_etd_phi_functions():
+    """phi1, phi2 (fft_integrators.py:37-64): the cubic series below
+    |z| = 1e-4 and the direct form above agree with expm1(z)/z and with the
+    exponential series summed to 40 terms (closed forms independent of the
+    restatement); phi(0) = 1, 1/2."""
+    from oracle import etdrk4 as oe
+    z = np.array([0.0, 3e-5, -9e-5, -2e-3, 0.4, -1.0, -7.5, -40.0])
+
+    def series(z, j):
+        return sum(z ** m / math.factorial(m + j) for m in range(40))
+
+    p1, p2 = oe.phi1(z), oe.phi2(z)
+    assert p1[0] == 1.0 and p2[0] == 0.5
+    nz = z != 0
+    assert np.allclose(p1[nz], np.expm1(z[nz]) / z[nz], rtol=1e-12)
+    small = np.abs(z) < 1.0
+    assert np.allclose(p1[small], series(z[small], 1), rtol=1e-12)
+    # the direct phi2 loses digits to cancellation near |z| = 1e-4 (the
+    # reference's formula): ~eps/z^2 relative
+    assert np.allclose(p2[small], series(z[small], 2), rtol=1e-7)
+    big = z[np.abs(z) >= 0.4]
+    assert np.allclose(p2[np.abs(z) >= 0.4], (np.expm1(big) - big) / big ** 2, rtol=1e-12)
+
+
+def test_etd1_etd2_reductions_and_order():
+    """ETD1 / ETD2 (fft_integrators.py:119-249): exact for N = 0; on
+    spatially constant fields (only lam = 0 is excited) ETD1 is forward Euler
+    and ETD2 is Adams-Bashforth-2 on the Gray-Scott ODE; first / second order
+    on a smooth coupled problem."""
+    from oracle import etdrk4 as oe
+    n = 16
+    lam = oe.spectral_symbol_1d(n)
+    rng = np.random.default_rng(8)
+    u0 = rng.standard_normal((n, n))
+    for order in (1, 2):
+        u, v, _ = oe.gs_etd(u0, np.zeros((n, n)), lam, lam, 1e-3, 1e-3, 0.0, 0.0, 0.5, 3, order)
+        expect = np.fft.ifft2(np.fft.fft2(u0) * np.exp(1.5 * 1e-3 * (lam[None, :] + lam[:, None]))).real
+        assert np.max(np.abs(u - expect)) < 1e-13 and np.max(np.abs(v)) == 0.0
+    F, k, dt, steps = 0.035, 0.065, 0.4, 6
+    a, b = 0.7, 0.2
+
+    def f(a, b):
+        return -a * b * b + F * (1 - a), a * b * b - (F + k) * b
+
+    ea, eb = a, b
+    for _ in range(steps):
+        fa, fb = f(ea, eb)
+        ea, eb = ea + dt * fa, eb + dt * fb
+    u, v, _ = oe.gs_etd(np.full((n, n), a), np.full((n, n), b), lam, lam, 2e-3, 1e-3, F, k, dt, steps, 1)
+    assert np.max(np.abs(u - ea)) < 1e-14 and np.max(np.abs(v - eb)) < 1e-14
+    qa, qb = f(a, b)
+    ab = (a + dt * qa, b + dt * qb)  # the first step is ETD1 = Euler
+    for _ in range(steps - 1):
+        na, nb = f(*ab)
+        ab = (ab[0] + dt * (1.5 * na - 0.5 * qa), ab[1] + dt * (1.5 * nb - 0.5 * qb))
+        qa, qb = na, nb
+    u, v, _ = oe.gs_etd(np.full((n, n), a), np.full((n, n), b), lam, lam, 2e-3, 1e-3, F, k, dt, steps, 2)
+    assert np.max(np.abs(u - ab[0])) < 1e-14 and np.max(np.abs(v - ab[1])) < 1e-14
+    x = np.arange(n) / n
+    X, Y = np.meshgrid(x, x)
+    uu = 0.6 + 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    vv = 0.3 + 0.1 * np.cos(2 * np.pi * X)
+    T = 8.0
+    ref = oe.gs_etdrk4(uu, vv, lam, lam, 2e-3, 1e-3, F, k, T / 256, 256)
+    for order, lo in ((1, 0.85), (2, 1.8)):
+        errs = [np.max(np.abs(oe.gs_etd(uu, vv, lam, lam, 2e-3, 1e-3, F, k, T / s, s, order)[0] - ref[0]))
+                for s in (16, 32, 64)]
+        rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+        assert all(lo < r < order + 0.5 for r in rates), (order, rates)
+
+
 # --------------------------------------------------------- Newton-Krylov
 def test_hand_jvp_matches_golden_ad_norm():
     """benchmarks/results/jvp_vs_fd_sweep.json:3: AD-JVP norm of the BE
