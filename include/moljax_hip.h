@@ -239,7 +239,10 @@ int mjx_gs_imex_strang_var(int dtype, int B, int ny, int nx, void* u, void* v, l
 /* Non-periodic 2-D Helmholtz / Poisson solve on B instances of ny x nx
  * interior points:  kind 0 homogeneous Dirichlet (DST-I in x and y, vertex
  * grid; ny+1, nx+1 powers of two in [16, 1024]), kind 1 homogeneous Neumann
- * (orthonormal DCT-II; ny, nx powers of two in [16, 1024]).
+ * (orthonormal DCT-II; ny, nx powers of two in [16, 1024]), kind 2
+ * homogeneous Dirichlet on a cell-centred grid (DST-II; ghost = -interior as
+ * moljax/core/bc.py:237-258 fills it; ny, nx powers of two in [16, 1024];
+ * symbol -4/h^2 sin^2(pi k/(2N)), k = 1..N at [k-1]).
  *   u = T^-1[ m(kx, ky) T[rhs] ],  mmode 0: m = (1 + alpha lam)/(1 - beta lam),
  *   mmode 1 (Poisson): m = 1/lam (0 where |lam| < 1e-14),  lam = lamx + lamy,
  *   mmode 2: the mmode-0 multiplier raised to the power inst[8b + 7] per
@@ -254,16 +257,38 @@ int mjx_r2r_solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs,
                     const void* lamx, const void* lamy, const double* inst, int mmode, void* work,
                     void* stream);
 
+/* The same solve (kind 1 or 2, mmode 0 with alpha = 0, or mmode 1) with the
+ * inhomogeneous boundary conditions of moljax/core/bc.py:218-307 on the
+ * interior cells: side_values = {left, right, bottom, top} (host doubles)
+ * are Dirichlet values g (kind 2, ghost = 2 g - interior) or Neumann fluxes
+ * q (kind 1, ghost = interior -/+ 2 h q).  The ghosts' share of the 5-point
+ * Laplacian (2 g / h^2, or -/+ 2 q / h) is folded into the right-hand side
+ * (times beta, or -1 for Poisson), so the result u satisfies
+ * u - beta lap_h u = rhs (or lap_h u = rhs) with lap_h evaluated on u padded
+ * by bc.py's ghost fill.  dy, dx: grid spacings. */
+int mjx_r2r_solve2d_bc(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
+                       long long in_istride, void* out, long long out_pitch, long long out_istride, const void* lamx,
+                       const void* lamy, const double* inst, int mmode, const double* side_values, double dy,
+                       double dx, void* work, void* stream);
+
 /* The same solve along nlines independent 1-D lines of n points (line
  * strides in elements), one parameter row inst[0..7] for all lines.
  * Replaces moljax/core/fft_nonperiodic.py:247-355 for 1-D input. */
 int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long long in_stride, void* out,
                     long long out_stride, const void* lam, const double* inst, int mmode, void* stream);
 
+/* 1-D lines with bc.py boundary conditions at both ends, side_values =
+ * {left, right} (host doubles), as mjx_r2r_solve2d_bc. */
+int mjx_r2r_solve1d_bc(int dtype, int kind, int nlines, int n, const void* in, long long in_stride, void* out,
+                       long long out_stride, const void* lam, const double* inst, int mmode,
+                       const double* side_values, double dx, void* stream);
+
 /* Raw 1-D transforms of nlines lines, times `scale`:  kind 0: DST-I
  * X_k = sum_n x_n sin(pi (k+1)(n+1)/(n_len+1)) (its own inverse up to
  * 2/(n_len+1));  kind 1: unnormalised DCT-II (inverse = 0) or DCT-III
- * (inverse != 0).  Replaces fft_nonperiodic.py:97-133 (dst_I_fast,
+ * (inverse != 0);  kind 2: unnormalised DST-II
+ * X_k = sum_n x_n sin(pi (k+1)(2n+1)/(2N)) or DST-III
+ * x_n = sum_k X_k sin(pi (k+1)(2n+1)/(2N)).  Replaces fft_nonperiodic.py:97-133 (dst_I_fast,
  * idst_I_fast). */
 int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, const void* in, long long in_stride,
                         void* out, long long out_stride, double scale, void* stream);

@@ -52,6 +52,8 @@ PROTOTYPES = {
     "mjx_bc_fill2d": (_i, [_i, _i, _i, _i, _p, _ll, _ll, _i, _dp, _d, _d, _p]),
     "mjx_gs_reaction": (_i, [_i, _ll, _p, _p, _p, _p, _d, _d, _p]),
     "mjx_r2r_solve2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _p, _p, _p, _i, _p, _p]),
+    "mjx_r2r_solve2d_bc": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _p, _p, _p, _i, _p, _d, _d, _p, _p]),
+    "mjx_r2r_solve1d_bc": (_i, [_i, _i, _i, _i, _p, _ll, _p, _ll, _p, _p, _i, _p, _d, _p]),
     "mjx_r2r_solve1d": (_i, [_i, _i, _i, _i, _p, _ll, _p, _ll, _p, _p, _i, _p]),
     "mjx_r2r_transform1d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _p, _ll, _d, _p]),
     "mjx_etdrk4_coeff_elems": (_ll, [_i, _i]),

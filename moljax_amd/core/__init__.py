@@ -19,7 +19,9 @@ from .dt_policy import (CFLParams, PIDParams, ControllerState, create_initial_co
                         handle_rejected_step, heisenberg_cfl_dt, imex_cfl_dt, implicit_robustness_dt, propose_dt,
                         propose_dt_imex, compute_error_order)
 from .fft_nonperiodic import (BCType as NonperiodicBCType, dst_I_fast, idst_I_fast, solve_helmholtz_dirichlet,
-                              solve_helmholtz_neumann, solve_poisson_dirichlet, solve_poisson_neumann)
+                              solve_helmholtz_neumann, solve_poisson_dirichlet, solve_poisson_neumann,
+                              laplacian_symbol_dirichlet_cc, solve_helmholtz_dirichlet_cc, solve_poisson_dirichlet_cc,
+                              solve_helmholtz_bc, solve_poisson_bc)
 from .fft_operators import DiffusionOperator
 from .fft_integrators import etd1_step, etd2_step, etdrk4_step, etd_integrate
 from .newton_krylov import NKParams, NKStats, NKResult, newton_krylov_solve, create_implicit_residual

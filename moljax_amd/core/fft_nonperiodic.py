@@ -9,9 +9,17 @@ or one 1-D symbol used on both axes of a square grid.
 Sizes: Dirichlet needs N + 1 and Neumann N to be a power of two in [16, 1024]
 (a vertex grid of N = 2^k - 1 interior points for Dirichlet); other sizes
 raise ValueError.
+
+The library's own grids are cell-centred, and moljax/core/bc.py fills a
+Dirichlet ghost as 2 g - interior (bc.py:237-258).  That stencil is
+diagonalised by the DST-II, not the DST-I above; ``*_dirichlet_cc`` and
+BCType.DIRICHLET_CC serve it (N a power of two), and solve_helmholtz_bc /
+solve_poisson_bc take a FieldBCSpec with side values or fluxes so that the
+solve inverts the FD operator evaluated on bc.py's filled ghosts exactly.
 """
 from __future__ import annotations
 
+import ctypes
 from enum import Enum
 from typing import NamedTuple, Tuple, Union
 
@@ -22,7 +30,7 @@ from .. import _hip
 from ._arrays import as_device, field_dtype, inst_table, view2d
 from .grid import default_device
 
-R2R_DST, R2R_DCT = 0, 1
+R2R_DST, R2R_DCT, R2R_DST2 = 0, 1, 2
 HELM, POISSON = 0, 1
 
 
@@ -32,6 +40,7 @@ class BCType(Enum):
     NEUMANN = "neumann"
     MIXED_DN = "mixed_dn"
     MIXED_ND = "mixed_nd"
+    DIRICHLET_CC = "dirichlet_cc"  # cell-centred Dirichlet (bc.py's ghost fill)
 
 
 class NonperiodicFFTCache(NamedTuple):
@@ -70,6 +79,13 @@ def laplacian_symbol_neumann(N: int, dx: float, dtype=torch.float64) -> torch.Te
     return _dev(-4.0 / (dx * dx) * np.sin(np.pi * k / (2 * N)) ** 2, dtype)
 
 
+def laplacian_symbol_dirichlet_cc(N: int, dx: float, dtype=torch.float64) -> torch.Tensor:
+    """Cell-centred Dirichlet (ghost = -interior): -4/dx^2 sin^2(pi k/(2N)),
+    k = 1..N, the DST-II eigenvalues of the 5-point stencil."""
+    k = np.arange(1, N + 1, dtype=np.float64)
+    return _dev(-4.0 / (dx * dx) * np.sin(np.pi * k / (2 * N)) ** 2, dtype)
+
+
 def laplacian_symbol_mixed_dn(N: int, dx: float, dtype=torch.float64) -> torch.Tensor:
     """fft_nonperiodic.py:186-194 (symbol only: no mixed transform kernel)."""
     k = np.arange(1, N + 1, dtype=np.float64)
@@ -87,6 +103,8 @@ def create_nonperiodic_fft_cache(N: int, dx: float, bc_type: Union[BCType, str] 
         lam, k = laplacian_symbol_neumann(N, dx, dtype), np.arange(N)
     elif bc_type == BCType.MIXED_DN:
         lam, k = laplacian_symbol_mixed_dn(N, dx, dtype), np.arange(1, N + 1)
+    elif bc_type == BCType.DIRICHLET_CC:
+        lam, k = laplacian_symbol_dirichlet_cc(N, dx, dtype), np.arange(1, N + 1)
     else:
         raise ValueError(f"Unsupported BC type: {bc_type}")
     return NonperiodicFFTCache(N=N, dx=dx, bc_type=bc_type.value, laplacian_symbol=lam,
@@ -102,8 +120,10 @@ def create_nonperiodic_fft_cache_2d(ny: int, nx: int, dy: float, dx: float,
         ly, lx = laplacian_symbol_dirichlet(ny, dy, dtype), laplacian_symbol_dirichlet(nx, dx, dtype)
     elif bc_type == BCType.NEUMANN:
         ly, lx = laplacian_symbol_neumann(ny, dy, dtype), laplacian_symbol_neumann(nx, dx, dtype)
+    elif bc_type == BCType.DIRICHLET_CC:
+        ly, lx = laplacian_symbol_dirichlet_cc(ny, dy, dtype), laplacian_symbol_dirichlet_cc(nx, dx, dtype)
     else:
-        raise ValueError(f"2-D transforms serve Dirichlet or Neumann, not {bc_type}")
+        raise ValueError(f"2-D transforms serve Dirichlet, cell-centred Dirichlet or Neumann, not {bc_type}")
     return NonperiodicFFTCache2D(ny=ny, nx=nx, dy=dy, dx=dx, bc_type=bc_type.value, lam_y=ly, lam_x=lx)
 
 
@@ -175,19 +195,29 @@ def _symbols_2d(symbol, ny: int, nx: int, dtype) -> Tuple[torch.Tensor, torch.Te
     return ly, lx
 
 
-def _solve(kind: int, rhs, symbol, beta: float, mmode: int) -> torch.Tensor:
+def _solve(kind: int, rhs, symbol, beta: float, mmode: int, sides=None, dy: float = 0.0,
+           dx: float = 0.0) -> torch.Tensor:
+    """sides: None, or bc.py side values / fluxes (left, right, bottom, top)
+    folded into the right-hand side by the kernel (kind 1 or 2)."""
     t = as_device(rhs)
     t = t.to(field_dtype(t))
     lib = _hip.lib()
     inst = inst_table([[beta, 0.0, 0.0, 0.0]], t.device)
+    sv = None if sides is None else (ctypes.c_double * 4)(*[float(v) for v in sides])
     if t.dim() == 1:
         lam = as_device(symbol, t.dtype).contiguous()
         if lam.numel() != t.shape[0]:
             raise ValueError("symbol length does not match rhs")
         lines, ls = _lines(t)
         out = torch.empty_like(lines)
-        rc = lib.mjx_r2r_solve1d(_hip.dtype_code(t.dtype), kind, 1, t.shape[0], _hip.ptr(lines), ls,
-                                 _hip.ptr(out), t.shape[0], _hip.ptr(lam), _hip.ptr(inst), mmode, _hip.stream_ptr())
+        if sv is None:
+            rc = lib.mjx_r2r_solve1d(_hip.dtype_code(t.dtype), kind, 1, t.shape[0], _hip.ptr(lines), ls,
+                                     _hip.ptr(out), t.shape[0], _hip.ptr(lam), _hip.ptr(inst), mmode,
+                                     _hip.stream_ptr())
+        else:
+            rc = lib.mjx_r2r_solve1d_bc(_hip.dtype_code(t.dtype), kind, 1, t.shape[0], _hip.ptr(lines), ls,
+                                        _hip.ptr(out), t.shape[0], _hip.ptr(lam), _hip.ptr(inst), mmode, sv,
+                                        float(dx), _hip.stream_ptr())
         _hip.check(rc, "mjx_r2r_solve1d")
         return out.view(t.shape)
     src, pitch, istr, B = view2d(t)
@@ -196,9 +226,14 @@ def _solve(kind: int, rhs, symbol, beta: float, mmode: int) -> torch.Tensor:
     inst = inst_table([[beta, 0.0, 0.0, 0.0]] * B, t.device)
     out = torch.empty((B, ny, nx), dtype=t.dtype, device=t.device)
     work = torch.empty((B, ny, nx), dtype=t.dtype, device=t.device)
-    rc = lib.mjx_r2r_solve2d(_hip.dtype_code(t.dtype), kind, B, ny, nx, _hip.ptr(src), pitch, istr, _hip.ptr(out),
-                             nx, ny * nx, _hip.ptr(lx), _hip.ptr(ly), _hip.ptr(inst), mmode, _hip.ptr(work),
-                             _hip.stream_ptr())
+    if sv is None:
+        rc = lib.mjx_r2r_solve2d(_hip.dtype_code(t.dtype), kind, B, ny, nx, _hip.ptr(src), pitch, istr,
+                                 _hip.ptr(out), nx, ny * nx, _hip.ptr(lx), _hip.ptr(ly), _hip.ptr(inst), mmode,
+                                 _hip.ptr(work), _hip.stream_ptr())
+    else:
+        rc = lib.mjx_r2r_solve2d_bc(_hip.dtype_code(t.dtype), kind, B, ny, nx, _hip.ptr(src), pitch, istr,
+                                    _hip.ptr(out), nx, ny * nx, _hip.ptr(lx), _hip.ptr(ly), _hip.ptr(inst), mmode,
+                                    sv, float(dy), float(dx), _hip.ptr(work), _hip.stream_ptr())
     _hip.check(rc, "mjx_r2r_solve2d")
     return out.view(t.shape)
 
@@ -221,6 +256,67 @@ def solve_poisson_dirichlet(rhs, laplacian_symbol) -> torch.Tensor:
 def solve_poisson_neumann(rhs, laplacian_symbol) -> torch.Tensor:
     """lap u = f with du/dn = 0, zero-mean solution (fft_nonperiodic.py:266-305)."""
     return _solve(R2R_DCT, rhs, laplacian_symbol, 0.0, POISSON)
+
+
+def solve_helmholtz_dirichlet_cc(rhs, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
+    """(I - dt D lap) u = f on a cell-centred grid with u's ghosts at
+    -interior (homogeneous bc.py Dirichlet), by the DST-II."""
+    return _solve(R2R_DST2, rhs, laplacian_symbol, float(dt) * float(D), HELM)
+
+
+def solve_poisson_dirichlet_cc(rhs, laplacian_symbol) -> torch.Tensor:
+    """lap u = f, cell-centred homogeneous Dirichlet, by the DST-II."""
+    return _solve(R2R_DST2, rhs, laplacian_symbol, 0.0, POISSON)
+
+
+_BC_SYMBOLS: dict = {}
+
+
+def _bc_setup(rhs, grid, spec, t: float, params):
+    from .bc import BCType as FieldBC
+    from .bc_fill import _side_values
+    kind = int(spec.kind)
+    if kind == FieldBC.DIRICHLET:
+        r2r, sides = R2R_DST2, _side_values(spec.value, grid, t, params or {})
+        sym = laplacian_symbol_dirichlet_cc
+    elif kind == FieldBC.NEUMANN:
+        r2r, sides = R2R_DCT, _side_values(spec.flux, grid, t, params or {})
+        sym = laplacian_symbol_neumann
+    else:
+        raise ValueError("solve_*_bc serves Dirichlet and Neumann specs; periodic fields use fft_solvers")
+    tr = as_device(rhs)
+    dtype = field_dtype(tr)
+    if tr.dim() == 1:
+        key = (r2r, tr.shape[-1], grid.dx, dtype)
+        if key not in _BC_SYMBOLS:
+            _BC_SYMBOLS[key] = sym(tr.shape[-1], grid.dx, dtype)
+        return r2r, _BC_SYMBOLS[key], (sides[0], sides[1], 0.0, 0.0)
+    ny, nx = tr.shape[-2], tr.shape[-1]
+    if (ny, nx) != (grid.ny, grid.nx):
+        raise ValueError(f"rhs interior {(ny, nx)} does not match the grid {(grid.ny, grid.nx)}")
+    key = (r2r, ny, nx, grid.dy, grid.dx, dtype)
+    if key not in _BC_SYMBOLS:
+        _BC_SYMBOLS[key] = (sym(ny, grid.dy, dtype), sym(nx, grid.dx, dtype))
+    return r2r, _BC_SYMBOLS[key], sides
+
+
+def solve_helmholtz_bc(rhs, grid, spec, dt: float, D: float, t: float = 0.0, params=None) -> torch.Tensor:
+    """u with u - dt D lap_h u = rhs on the interior cells, lap_h the 5-point
+    Laplacian (operators.py:311-322) of u padded by bc.py's fill for ``spec``
+    (a Dirichlet or Neumann FieldBCSpec, side values / fluxes evaluated as
+    bc.py:323-349 does).  rhs: (..., ny, nx) interior arrays (or 1-D lines,
+    with the left / right values)."""
+    kind, sym, sides = _bc_setup(rhs, grid, spec, t, params)
+    return _solve(kind, rhs, sym, float(dt) * float(D), HELM, sides, grid.dy if hasattr(grid, "dy") else grid.dx,
+                  grid.dx)
+
+
+def solve_poisson_bc(rhs, grid, spec, t: float = 0.0, params=None) -> torch.Tensor:
+    """lap_h u = rhs with lap_h on u padded by bc.py's fill for ``spec``
+    (Neumann: the zero-mean solution; rhs must satisfy the compatibility
+    condition with the boundary fluxes)."""
+    kind, sym, sides = _bc_setup(rhs, grid, spec, t, params)
+    return _solve(kind, rhs, sym, 0.0, POISSON, sides, grid.dy if hasattr(grid, "dy") else grid.dx, grid.dx)
 
 
 def check_compatibility_neumann(rhs, tol: float = 1e-10) -> bool:

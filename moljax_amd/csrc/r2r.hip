@@ -1,4 +1,4 @@
-// C-ABI entry points of the non-periodic (DST-I / DCT-II) solves.
+// C-ABI entry points of the non-periodic (DST-I / DCT-II / DST-II) solves.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -11,15 +11,23 @@ namespace mjx {
 int r2r_run(int kind, int orient, int mode, int L, R2RArgs a, int B, hipStream_t st, float);
 int r2r_run(int kind, int orient, int mode, int L, R2RArgs a, int B, hipStream_t st, double);
 
-static int run(int dtype, int kind, int orient, int mode, int L, const R2RArgs& a, int B, hipStream_t st) {
+// Public kinds: 0 DST-I (vertex Dirichlet), 1 DCT-II (cell Neumann), 2 DST-II
+// (cell Dirichlet, run as the alternating-sign DCT path).
+enum { KIND_DST2 = 2 };
+
+static int run(int dtype, int kind, int orient, int mode, int L, R2RArgs a, int B, hipStream_t st) {
+  if (kind == KIND_DST2) {
+    kind = R2R_DCT;
+    a.alt = 1;
+  }
   if (dtype == MJX_F32) return r2r_run(kind, orient, mode, L, a, B, st, 0.0f);
   return r2r_run(kind, orient, mode, L, a, B, st, 0.0);
 }
 
 static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
-// Line length for a transform of n points: DST-I needs n+1, DCT n, a power of
-// two in [16, 1024].
+// Line length for a transform of n points: DST-I needs n+1, DCT / DST-II n, a
+// power of two in [16, 1024].
 static int line_len(int kind, int n) {
   const int m = kind == R2R_DST ? n + 1 : n;
   if (!pow2(m) || m < 16 || m > 1024) return -1;
@@ -28,8 +36,27 @@ static int line_len(int kind, int n) {
 
 static int check_common(int dtype, int kind) {
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
-  if (kind != R2R_DST && kind != R2R_DCT) return fail(MJX_E_ARG, "kind must be 0 (Dirichlet/DST-I) or 1 (Neumann/DCT-II)");
+  if (kind != R2R_DST && kind != R2R_DCT && kind != KIND_DST2)
+    return fail(MJX_E_ARG, "kind must be 0 (vertex Dirichlet / DST-I), 1 (Neumann / DCT-II) or 2 (cell Dirichlet / DST-II)");
   return ensure_tables(nullptr);
+}
+
+static const char* size_rule(int kind) {
+  return kind == R2R_DST ? " not supported: n+1 must be a power of two in [16, 1024]"
+                         : " not supported: n must be a power of two in [16, 1024]";
+}
+
+// Ghost-ring contributions of bc.py's fill to the 5-point Laplacian of the
+// boundary cells: Dirichlet (ghost = 2 g - interior) adds 2 g / h^2;
+// Neumann (ghost = interior -/+ 2 h q) adds -2 q / h (left, bottom) and
+// +2 q / h (right, top).
+static void set_lift(R2RArgs& a, int kind, const double* sv, int nsides, double dy, double dx) {
+  a.lift_on = 1;
+  for (int i = 0; i < 4; ++i) {
+    const double v = i < nsides ? sv[i] : 0.0;
+    const double h = i < 2 ? dx : dy;
+    a.lift[i] = kind == KIND_DST2 ? 2.0 * v / (h * h) : (i % 2 == 0 ? -2.0 : 2.0) * v / h;
+  }
 }
 
 static R2RArgs make(const void* in, long long in_es, long long in_ls, long long in_is, void* out, long long out_es,
@@ -48,6 +75,42 @@ static R2RArgs make(const void* in, long long in_es, long long in_ls, long long 
   return a;
 }
 
+static int solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
+                   long long in_istride, void* out, long long out_pitch, long long out_istride, const void* lamx,
+                   const void* lamy, const double* inst, int mmode, const double* side_values, double dy, double dx,
+                   void* work, void* stream) {
+  const int Lx = line_len(kind, nx), Ly = line_len(kind, ny);
+  int rc = MJX_OK;
+  if (B < 1 || !rhs || !out || !lamx || !lamy || !inst || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
+    return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
+  hipStream_t st = (hipStream_t)stream;
+  const long long plane = (long long)ny * nx;
+  // rows: forward transform along x into the work array
+  R2RArgs a = make(rhs, 1, in_pitch, in_istride, work, 1, nx, plane, ny);
+  if (side_values) {
+    a.inst = inst;
+    a.mmode = mmode;
+    set_lift(a, kind, side_values, 4, dy, dx);
+  }
+  rc = run(dtype, kind, R2R_ROW, R2R_FWD, Lx, a, B, st);
+  if (rc) return rc;
+  // columns: transform along y, multiply, inverse along y (in place)
+  R2RArgs c = make(work, nx, 1, plane, work, nx, 1, plane, nx);
+  c.lam_line = lamy;
+  c.lam_cross = lamx;
+  c.n_cross = kind != R2R_DST ? nx : 0;
+  c.inst = inst;
+  c.mmode = mmode;
+  rc = run(dtype, kind, R2R_COL, R2R_APPLY, Ly, c, B, st);
+  if (rc) return rc;
+  // rows: inverse transform along x into the output
+  R2RArgs r = make(work, 1, nx, plane, out, 1, out_pitch, out_istride, ny);
+  r.scale = kind == R2R_DST ? 2.0 / (nx + 1) : 1.0;
+  return run(dtype, kind, R2R_ROW, R2R_INV, Lx, r, B, st);
+}
+
 }  // namespace mjx
 
 using namespace mjx;
@@ -64,29 +127,46 @@ int mjx_r2r_solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs,
     return fail(MJX_E_SIZE, std::string("grid ") + std::to_string(ny) + "x" + std::to_string(nx) +
                                 (kind == R2R_DST ? " not supported: ny+1 and nx+1 must be powers of two in [16, 1024]"
                                                  : " not supported: ny and nx must be powers of two in [16, 1024]"));
-  if (B < 1 || !rhs || !out || !lamx || !lamy || !inst || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
-  if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
-  if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
-    return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
-  hipStream_t st = (hipStream_t)stream;
-  const long long plane = (long long)ny * nx;
-  // rows: forward transform along x into the work array
-  R2RArgs a = make(rhs, 1, in_pitch, in_istride, work, 1, nx, plane, ny);
-  rc = run(dtype, kind, R2R_ROW, R2R_FWD, Lx, a, B, st);
+  return solve2d(dtype, kind, B, ny, nx, rhs, in_pitch, in_istride, out, out_pitch, out_istride, lamx, lamy, inst,
+                 mmode, nullptr, 0.0, 0.0, work, stream);
+}
+
+int mjx_r2r_solve2d_bc(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
+                       long long in_istride, void* out, long long out_pitch, long long out_istride, const void* lamx,
+                       const void* lamy, const double* inst, int mmode, const double* side_values, double dy,
+                       double dx, void* work, void* stream) {
+  int rc = check_common(dtype, kind);
   if (rc) return rc;
-  // columns: transform along y, multiply, inverse along y (in place)
-  R2RArgs c = make(work, nx, 1, plane, work, nx, 1, plane, nx);
-  c.lam_line = lamy;
-  c.lam_cross = lamx;
-  c.n_cross = kind == R2R_DCT ? nx : 0;
-  c.inst = inst;
-  c.mmode = mmode;
-  rc = run(dtype, kind, R2R_COL, R2R_APPLY, Ly, c, B, st);
+  if (kind == R2R_DST) return fail(MJX_E_ARG, "boundary values need kind 1 (Neumann) or 2 (cell Dirichlet)");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON)
+    return fail(MJX_E_ARG, "boundary values need mmode 0 (Helmholtz, alpha = 0) or 1 (Poisson)");
+  if (!side_values || !(dx > 0.0) || !(dy > 0.0)) return fail(MJX_E_ARG, "side_values and positive dx, dy required");
+  if (line_len(kind, nx) < 0 || line_len(kind, ny) < 0)
+    return fail(MJX_E_SIZE, std::string("grid ") + std::to_string(ny) + "x" + std::to_string(nx) +
+                                " not supported: ny and nx must be powers of two in [16, 1024]");
+  return solve2d(dtype, kind, B, ny, nx, rhs, in_pitch, in_istride, out, out_pitch, out_istride, lamx, lamy, inst,
+                 mmode, side_values, dy, dx, work, stream);
+}
+
+int mjx_r2r_solve1d_bc(int dtype, int kind, int nlines, int n, const void* in, long long in_stride, void* out,
+                       long long out_stride, const void* lam, const double* inst, int mmode,
+                       const double* side_values, double dx, void* stream) {
+  int rc = check_common(dtype, kind);
   if (rc) return rc;
-  // rows: inverse transform along x into the output
-  R2RArgs r = make(work, 1, nx, plane, out, 1, out_pitch, out_istride, ny);
-  r.scale = kind == R2R_DST ? 2.0 / (nx + 1) : 1.0;
-  return run(dtype, kind, R2R_ROW, R2R_INV, Lx, r, B, st);
+  if (kind == R2R_DST) return fail(MJX_E_ARG, "boundary values need kind 1 (Neumann) or 2 (cell Dirichlet)");
+  if (mmode != R2R_HELM && mmode != R2R_POISSON)
+    return fail(MJX_E_ARG, "boundary values need mmode 0 (Helmholtz, alpha = 0) or 1 (Poisson)");
+  if (!side_values || !(dx > 0.0)) return fail(MJX_E_ARG, "side_values and positive dx required");
+  const int L = line_len(kind, n);
+  if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
+  if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
+  R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
+  a.lam_line = lam;
+  a.inst = inst;
+  a.mmode = mmode;
+  set_lift(a, kind, side_values, 2, dx, dx);
+  return run(dtype, kind, R2R_ROW, R2R_APPLY, L, a, 1, (hipStream_t)stream);
 }
 
 int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long long in_stride, void* out,
@@ -94,10 +174,7 @@ int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long
   int rc = check_common(dtype, kind);
   if (rc) return rc;
   const int L = line_len(kind, n);
-  if (L < 0)
-    return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) +
-                                (kind == R2R_DST ? " not supported: n+1 must be a power of two in [16, 1024]"
-                                                 : " not supported: n must be a power of two in [16, 1024]"));
+  if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
   if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
@@ -119,7 +196,7 @@ int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, con
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.scale = scale;
-  const int mode = (kind == R2R_DCT && inverse) ? R2R_INV : R2R_FWD;
+  const int mode = (kind != R2R_DST && inverse) ? R2R_INV : R2R_FWD;
   return run(dtype, kind, R2R_ROW, mode, L, a, 1, (hipStream_t)stream);
 }
 

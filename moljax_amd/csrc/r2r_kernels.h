@@ -20,6 +20,18 @@
 // The ortho DCT of the reference (jax.scipy.fft.dct norm='ortho',
 // fft_nonperiodic.py:352-355) is the unnormalised pair with f_k^2 = 1/N (k=0),
 // 2/N (k>0) folded into the spectral multiplier.
+//   DST-II (cell-centred Dirichlet, ghost = -interior as moljax/core/bc.py
+//     :237-258 fills it): S(x)_k = sum_n x_n sin(pi (k+1)(2n+1)/(2N)) equals
+//     C((-1)^n x_n)_{N-1-k}, since sin(pi(n+1/2) - t) = (-1)^n cos t.  With
+//     `alt` set, the DCT path alternates the sign along the line on input
+//     (forward, apply) and output (inverse, apply) and reads the line's
+//     symbol in reverse order, so the DCT-II machinery diagonalises the
+//     Dirichlet stencil; the forward transform stores its output reversed
+//     (DST-II order) and the inverse (DST-III) reads its input reversed.
+// A boundary lift (bc.py ghost values folded into the right-hand side) is
+// added while the first pass loads its lines: lift = {left, right, bottom,
+// top} contributions of the ghost ring to the 5-point Laplacian, scaled by
+// beta (Helmholtz) or -1 (Poisson) per instance.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -51,6 +63,9 @@ struct R2RArgs {
   const double* inst;             // per-instance rows (beta_a, -, alpha_a, ...)
   int mmode;                      // R2R_HELM or R2R_POISSON
   double scale;                   // extra output scale (FWD / INV)
+  int alt;                        // DCT path as DST-II / DST-III (cell-centred Dirichlet)
+  int lift_on;                    // add the boundary lift while loading
+  double lift[4];                 // left, right (line ends), bottom, top (first / last line)
 };
 
 template <int KIND, int L>
@@ -114,7 +129,21 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
       e = idx / NL;
     }
     const int g = line0 + l;
-    tile[l * TW + e] = g < a.nlines ? in[(long long)g * a.in_ls + (long long)e * a.in_es] : (T)0;
+    T x = g < a.nlines ? in[(long long)g * a.in_ls + (long long)e * a.in_es] : (T)0;
+    int es = e;
+    if constexpr (KIND == R2R_DCT) {
+      if (a.lift_on && g < a.nlines) {
+        const double w = a.mmode == R2R_POISSON ? -1.0 : a.inst[8 * inst];
+        const double add = (e == 0 ? a.lift[0] : 0.0) + (e == N - 1 ? a.lift[1] : 0.0) +
+                           (g == 0 ? a.lift[2] : 0.0) + (g == a.nlines - 1 ? a.lift[3] : 0.0);
+        x += (T)(w * add);
+      }
+      if (a.alt) {
+        if constexpr (MODE == R2R_INV) es = N - 1 - e;
+        else if (e & 1) x = -x;
+      }
+    }
+    tile[l * TW + es] = x;
   }
   __syncthreads();
 
@@ -171,13 +200,17 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
       const T* lam_line = (const T*)a.lam_line;
       const T* lam_cross = (const T*)a.lam_cross;
       const int ga = line0 + 2 * t, gb = ga + 1;
+      // The cross dimension arrives in stored order: DCT index order, or
+      // DST-II order (reversed DCT index) with alt; the symbols follow the
+      // stored order and the DCT normalisation the DCT index.
       const T ca = lam_cross ? lam_cross[ga < a.nlines ? ga : 0] : (T)0;
       const T cb = lam_cross ? lam_cross[gb < a.nlines ? gb : 0] : (T)0;
       T fa = (T)1, fb = (T)1;
       if constexpr (KIND == R2R_DCT) {
         if (a.n_cross > 0) {
-          fa = (T)((ga == 0 ? 1.0 : 2.0) / a.n_cross);
-          fb = (T)((gb == 0 ? 1.0 : 2.0) / a.n_cross);
+          const int da = a.alt ? a.n_cross - 1 - ga : ga, db = a.alt ? a.n_cross - 1 - gb : gb;
+          fa = (T)((da == 0 ? 1.0 : 2.0) / a.n_cross);
+          fb = (T)((db == 0 ? 1.0 : 2.0) / a.n_cross);
         }
       }
 #pragma unroll
@@ -200,7 +233,7 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
             const int k = e < N ? e : L - e;
             const cpx<T> q = e < N ? cmul(dct_rot<T, N, -1>(tw, k), v[m]) : cmul(dct_rot<T, N, +1>(tw, k), v[m]);
             const T f2 = (T)((k == 0 ? 1.0 : 2.0) / N);
-            const T ll = lam_line[k];
+            const T ll = lam_line[a.alt ? N - 1 - k : k];
             const T ma = r2r_mult<T>(ll + ca, alpha, beta, a.mmode, power) * f2 * fa;
             const T mb = r2r_mult<T>(ll + cb, alpha, beta, a.mmode, power) * f2 * fb;
             // Q'/1 with the 1/2 of Q = e^{-i th} Z / 2 and the doubled DC
@@ -256,7 +289,16 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
       e = idx / NL;
     }
     const int g = line0 + l;
-    if (g < a.nlines) out[(long long)g * a.out_ls + (long long)e * a.out_es] = tile[l * TW + e];
+    if (g < a.nlines) {
+      T y = tile[l * TW + e];
+      if constexpr (KIND == R2R_DCT) {
+        if (a.alt) {
+          if constexpr (MODE == R2R_FWD) y = tile[l * TW + N - 1 - e];
+          else if (e & 1) y = -y;
+        }
+      }
+      out[(long long)g * a.out_ls + (long long)e * a.out_es] = y;
+    }
   }
 }
 

@@ -217,6 +217,56 @@ def test_nonperiodic_solves_fd_residual(kind):
         assert np.max(np.abs(lap(p, dy, dx) - r0)) < 1e-9 and abs(p.mean()) < 1e-12
 
 
+def test_dst2_brute_force_and_cell_dirichlet_eigenmodes():
+    """DST-II (ortho) against its O(N^2) sum; the modes sin(pi k (j+1/2)/N)
+    are eigenvectors of the 5-point Laplacian on cells padded by bc.py's
+    Dirichlet fill (ghost = -interior) with eigenvalue -4/h^2 sin^2(pi k/(2N))."""
+    from oracle import fd
+    from oracle import nonperiodic as onp
+    N, h = 16, 0.1
+    x = np.random.default_rng(2).standard_normal(N)
+    n = np.arange(N)
+    brute = np.array([np.sum(x * np.sin(np.pi * (k + 1) * (2 * n + 1) / (2 * N))) for k in range(N)])
+    w = np.where(n == N - 1, np.sqrt(1.0 / N), np.sqrt(2.0 / N))  # ortho weights (last one 1/sqrt(N))
+    assert np.allclose(onp.dst2_ortho(x), w * brute, atol=1e-13)
+    lam = onp.laplacian_symbol_dirichlet_cc(N, h)
+    for k in (1, 5, N):
+        mode = np.sin(np.pi * k * (n + 0.5) / N)
+        f2 = np.tile(mode, (3, 1))  # constant along y: y-direction Neumann fill keeps it exact
+        p = fd.apply_bc(np.pad(f2, 1), 1, dy=h, dx=h)
+        p[0, :], p[-1, :] = p[1, :], p[-2, :]
+        lap = (p[1:-1, 2:] - 2 * f2 + p[1:-1, :-2]) / h ** 2
+        assert np.allclose(lap[1], lam[k - 1] * mode, atol=1e-10)
+
+
+@pytest.mark.parametrize("kind", ["dirichlet_cc", "neumann"])
+def test_bc_solves_invert_fd_operator_on_filled_ghosts(kind):
+    """solve_bc_2d with nonzero side values: lap_h evaluated on u padded by
+    oracle.fd.apply_bc (bc.py:218-307, pinned by bc_matrix.json) gives
+    u - beta lap_h u = rhs and, for Poisson, lap_h u = rhs."""
+    from oracle import fd
+    from oracle import nonperiodic as onp
+    rng = np.random.default_rng(9)
+    ny, nx = 16, 32
+    dy, dx = 1.0 / ny, 2.0 / nx
+    values = (0.3, -0.7, 1.1, 0.25)
+    rhs = rng.standard_normal((ny, nx))
+    fk = 1 if kind == "dirichlet_cc" else 2
+
+    def lap_h(u):
+        return fd.laplacian_2d(fd.apply_bc(np.pad(u, 1), fk, values, dy, dx), dy, dx)[1:-1, 1:-1]
+
+    beta = 0.04
+    u = onp.solve_bc_2d(kind, rhs, values, dy, dx, beta)
+    assert np.max(np.abs(u - beta * lap_h(u) - rhs)) < 1e-12
+    r = rhs
+    if kind == "neumann":  # compatibility: sum(rhs) = boundary flux total
+        b = onp.bc_lift(rhs.shape, kind, values, dy, dx)
+        r = rhs - (rhs - b).mean()
+    p = onp.solve_bc_2d(kind, r, values, dy, dx)
+    assert np.max(np.abs(lap_h(p) - r)) < 1e-9
+
+
 def test_poisson_dirichlet_sine_mode():
     """test_fft_nonperiodic.py:90-106 intent: Poisson with f = -pi^2 sin(pi x)
     on a vertex grid gives sin(pi x) to second order."""
