@@ -101,9 +101,12 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
  * scheme 0 Strang (identical to mjx_gs_imex_strang), 1 IMEX-Euler
  *   y_new = (I - dt D lap)^-1 (y + dt R(y))      (stepping.py:317-367),
  * 2 IMEX-SSPRK2  y_new = y/2 + E(E(y))/2 with E the IMEX-Euler map
- *   (stepping.py:438-495).  inst beta columns: dt/2*D for Strang, dt*D for
- * Euler and SSPRK2.  Replaces moljax/core/stepping.py:317-495 for the model
- * of moljax/core/model.py:529-584.  work as mjx_gs_imex_strang. */
+ *   (stepping.py:438-495), 3 the splitting of the Gray-Scott benchmark
+ *   (benchmarks/benchmark_gray_scott.py:268-289): y += dt/2 R(y);
+ *   y^ <- e^{dt D lam} y^; y += dt/2 R(y).  inst beta columns: dt/2*D for
+ * Strang, dt*D for Euler, SSPRK2 and the benchmark splitting.  Replaces
+ * moljax/core/stepping.py:317-495 for the model of
+ * moljax/core/model.py:529-584.  work as mjx_gs_imex_strang. */
 int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch,
                 long long istride, int fill_ghosts, const void* lamx, const void* lamy, const void* lam2d,
                 const double* inst, int n_steps, void* work, void* stream);
@@ -302,6 +305,13 @@ int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, con
 long long mjx_etdrk4_coeff_elems(int ny, int nx);
 int mjx_etdrk4_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
                       double dt, void* coef, void* stream);
+
+/* The same tables with a choice of the f2 weight: form 0 the Cox-Matthews
+ * weight 2 f2 (as mjx_etdrk4_coeffs; the b2 of moljax/core/fft_integrators
+ * .py:110-115), form 1 the benchmark's combination as written
+ * (benchmarks/benchmark_gray_scott.py:414-415, f2 without the factor 2). */
+int mjx_etdrk4_coeffs_form(int dtype, int form, int ny, int nx, const double* lamx, const double* lamy, double Du,
+                           double Dv, double dt, void* coef, void* stream);
 
 /* Batched coupled Gray-Scott ETDRK4 (Cox-Matthews), n_steps in place on
  * (u, v) of B square n x n instances (n in {32, ..., 1024}); per-instance F, k

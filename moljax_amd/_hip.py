@@ -58,6 +58,7 @@ PROTOTYPES = {
     "mjx_r2r_transform1d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _p, _ll, _d, _p]),
     "mjx_etdrk4_coeff_elems": (_ll, [_i, _i]),
     "mjx_etdrk4_coeffs": (_i, [_i, _i, _i, _p, _p, _d, _d, _d, _p, _p]),
+    "mjx_etdrk4_coeffs_form": (_i, [_i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p]),
     "mjx_etdrk4_workspace_bytes": (_ll, [_i, _i, _i, _i]),
     "mjx_gs_etdrk4": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _i, _p, _p]),
     "mjx_etd_phi_coeff_elems": (_ll, [_i, _i]),

@@ -41,13 +41,15 @@ _COEF_CACHE: Dict[tuple, torch.Tensor] = {}
 
 
 def etdrk4_coefficients(lamx: np.ndarray, lamy: np.ndarray, Du: float, Dv: float, dt: float,
-                        dtype=torch.float32) -> torch.Tensor:
+                        dtype=torch.float32, cox_matthews: bool = True) -> torch.Tensor:
     """(6, ny, nx+2) device tables E, E2, Q, f1, f2, f3 for both fields in the
     separated spectral layout, FP64 contour means rounded once to ``dtype``.
-    Cached per (symbol, Du, Dv, dt, dtype)."""
+    ``cox_matthews=False`` keeps the benchmark's f2 as written
+    (benchmark_gray_scott.py:414-415, without the factor 2).  Cached per
+    (symbol, Du, Dv, dt, dtype, form)."""
     lamx = np.ascontiguousarray(lamx, dtype=np.float64)
     lamy = np.ascontiguousarray(lamy, dtype=np.float64)
-    key = (lamx.tobytes(), lamy.tobytes(), float(Du), float(Dv), float(dt), dtype)
+    key = (lamx.tobytes(), lamy.tobytes(), float(Du), float(Dv), float(dt), dtype, bool(cox_matthews))
     hit = _COEF_CACHE.get(key)
     if hit is not None:
         return hit
@@ -56,9 +58,10 @@ def etdrk4_coefficients(lamx: np.ndarray, lamy: np.ndarray, Du: float, Dv: float
     lx = torch.from_numpy(lamx).cuda()
     ly = torch.from_numpy(lamy).cuda()
     coef = torch.empty((6, ny, nx + 2), dtype=dtype, device="cuda")
-    rc = lib.mjx_etdrk4_coeffs(_hip.dtype_code(dtype), ny, nx, _hip.ptr(lx), _hip.ptr(ly), float(Du), float(Dv),
-                               float(dt), _hip.ptr(coef), _hip.stream_ptr())
-    _hip.check(rc, "mjx_etdrk4_coeffs")
+    rc = lib.mjx_etdrk4_coeffs_form(_hip.dtype_code(dtype), 0 if cox_matthews else 1, ny, nx, _hip.ptr(lx),
+                                    _hip.ptr(ly), float(Du), float(Dv), float(dt), _hip.ptr(coef),
+                                    _hip.stream_ptr())
+    _hip.check(rc, "mjx_etdrk4_coeffs_form")
     _COEF_CACHE[key] = coef
     return coef
 

@@ -12,9 +12,9 @@ int etd_run(int B, int n, float* u, float* v, long long pitch, long long istride
 int etd_run(int B, int n, double* u, double* v, long long pitch, long long istride, int ghosts, const double* coef,
             const InstParams* ip, int n_steps, cpx<double>* work, hipStream_t st);
 int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, float* coef,
-               hipStream_t st);
+               double f2w, hipStream_t st);
 int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, double* coef,
-               hipStream_t st);
+               double f2w, hipStream_t st);
 
 int etd_exp_run(int B, int n, float* u, float* v, long long pitch, long long istride, int ghosts, const float* coef,
                 const InstParams* ip, int order, int prev, const float* nu, const float* nv, int n_steps,
@@ -41,14 +41,21 @@ long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx) {
   return es * (long long)B * (3LL * ny * nx + 3LL * ny * (nx + 2));
 }
 
-int mjx_etdrk4_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
-                      double dt, void* coef, void* stream) {
+int mjx_etdrk4_coeffs_form(int dtype, int form, int ny, int nx, const double* lamx, const double* lamy, double Du,
+                           double Dv, double dt, void* coef, void* stream) {
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (form != 0 && form != 1) return fail(MJX_E_ARG, "form must be 0 (Cox-Matthews) or 1 (benchmark as written)");
   if (ny < 2 || nx < 2 || (nx & 1)) return fail(MJX_E_ARG, "grid must be at least 2x2 with even nx");
   if (!lamx || !lamy || !coef) return fail(MJX_E_ARG, "null pointer argument");
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == MJX_F32) return etd_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (float*)coef, st);
-  return etd_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (double*)coef, st);
+  const double f2w = form == 0 ? 2.0 : 1.0;
+  if (dtype == MJX_F32) return etd_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (float*)coef, f2w, st);
+  return etd_coeffs(ny, nx, lamx, lamy, Du, Dv, dt, (double*)coef, f2w, st);
+}
+
+int mjx_etdrk4_coeffs(int dtype, int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv,
+                      double dt, void* coef, void* stream) {
+  return mjx_etdrk4_coeffs_form(dtype, 0, ny, nx, lamx, lamy, Du, Dv, dt, coef, stream);
 }
 
 int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,

@@ -56,10 +56,10 @@ int etd_phi_coeffs(int ny, int nx, const double* lamx, const double* lamy, doubl
 }
 
 int etd_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, T* coef,
-               hipStream_t st) {
+               double f2w, hipStream_t st) {
   const long long n = (long long)ny * (nx + 2);
   hipLaunchKernelGGL(k_etd_coeffs<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lamx, lamy, ny, nx, Du, Dv,
-                     dt, coef);
+                     dt, coef, f2w);
   return check_launch("etd_coeffs");
 }
 

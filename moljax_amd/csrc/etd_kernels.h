@@ -226,9 +226,10 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
 // z = dt * D_f * lam(kx, ky), LR = z + r_j,
 //   E = e^z, E2 = e^{z/2}, Q = dt <(e^{LR/2} - 1)/LR>,
 //   f1 = dt <(-4 - LR + e^{LR}(4 - 3 LR + LR^2))/LR^3>,
-//   f2 = 2 dt <(2 + LR + e^{LR}(-2 + LR))/LR^3>  (the Cox-Matthews weight
-//        of N_a + N_b, dt(2 phi2 - 4 phi3); the benchmark multiplies by the
-//        mean without Kassam-Trefethen's factor 2, an inconsistent scheme),
+//   f2 = f2w dt <(2 + LR + e^{LR}(-2 + LR))/LR^3>  (f2w = 2: the Cox-Matthews
+//        weight of N_a + N_b, dt(2 phi2 - 4 phi3); f2w = 1: the benchmark's
+//        combination as written, without Kassam-Trefethen's factor 2, an
+//        inconsistent scheme),
 //   f3 = dt <(-4 - 3 LR - LR^2 + e^{LR}(4 - LR))/LR^3>   (real parts).
 struct Cd {
   double re, im;
@@ -250,7 +251,7 @@ __device__ __forceinline__ Cd cexpd(Cd a) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_etd_coeffs(const double* __restrict__ lamx, const double* __restrict__ lamy,
                                                     int ny, int nx, double Du, double Dv, double dt,
-                                                    T* __restrict__ coef) {
+                                                    T* __restrict__ coef, double f2w) {
   const int sw = nx + 2, half = nx / 2;
   const long long n = (long long)ny * sw;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(256) void k_etd_coeffs(const double* __restrict__ l
   coef[g + n] = (T)exp(0.5 * z);
   coef[g + 2 * n] = (T)(dt * q.re / M);
   coef[g + 3 * n] = (T)(dt * a1.re / M);
-  coef[g + 4 * n] = (T)(2.0 * dt * a2.re / M);
+  coef[g + 4 * n] = (T)(f2w * dt * a2.re / M);
   coef[g + 5 * n] = (T)(dt * a3.re / M);
 }
 

@@ -12,15 +12,20 @@
 //   SSPRK2  per step: fwd_pack -> row(Euler) -> col(H) -> row(Euler) -> col(H)
 //           -> inv_unpack(out = y/2 + E(E(y))/2), in place only
 //           (stepping.py:438-495)
+//   BENCH_STRANG  the Gray-Scott benchmark's splitting
+//           (benchmarks/benchmark_gray_scott.py:268-289): reaction Euler
+//           half-step, exact diffusion e^{dt D lam}, reaction Euler half-step:
+//           fwd_pack -> row(half) -> col(E) -> [row(half, half) -> col(E)]*(n-1)
+//           -> inv_unpack(closing half-step)
 // The instance table's beta columns carry dt/2 D (STRANG) or dt D (EULER,
-// SSPRK2); its dt column is the step size.
+// SSPRK2, BENCH_STRANG); its dt column is the step size.
 #pragma once
 #include "mjx_common.h"
 #include "spectral_api.h"
 
 namespace mjx {
 
-enum { GS_STRANG = 0, GS_EULER = 1, GS_SSPRK2 = 2 };
+enum { GS_STRANG = 0, GS_EULER = 1, GS_SSPRK2 = 2, GS_BENCH_STRANG = 3 };
 
 template <typename T>
 int gs_chain_t(int scheme, int B, int ny, int nx, const T* u_in, const T* v_in, long long in_pitch,
@@ -50,6 +55,14 @@ int gs_chain_t(int scheme, int B, int ny, int nx, const T* u_in, const T* v_in, 
       MJX_TRY(spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, 1, st));
     }
     return spec_row_inv_unpack(nx, B, ny, W, u_out, v_out, out_pitch, out_istride, ghosts, st);
+  }
+  if (scheme == GS_BENCH_STRANG) {
+    MJX_TRY(spec_row_fwd_pack(nx, B, ny, u_in, v_in, in_pitch, in_istride, W, st, first_row_scale));
+    for (int s = 0; s < n_steps; ++s) {
+      MJX_TRY(spec_row_gs(nx, B, ny, W, inst, st, s == 0 ? RXN_HALF : RXN_HALF2));
+      MJX_TRY(spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, -2, st));
+    }
+    return spec_row_inv_unpack(nx, B, ny, W, u_out, v_out, out_pitch, out_istride, ghosts, st, 0.0, inst);
   }
   if (scheme == GS_SSPRK2) {
     if (u_in != u_out || v_in != v_out || in_pitch != out_pitch || in_istride != out_istride)

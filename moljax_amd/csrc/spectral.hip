@@ -210,7 +210,8 @@ int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, 
   if (scheme == GS_STRANG)
     return mjx_gs_imex_strang(dtype, B, ny, nx, u, v, pitch, istride, fill_ghosts, lamx, lamy, lam2d, inst,
                               n_steps, work, stream);
-  if (scheme != GS_EULER && scheme != GS_SSPRK2) return fail(MJX_E_ARG, "scheme must be 0 (Strang), 1 (Euler) or 2 (SSPRK2)");
+  if (scheme != GS_EULER && scheme != GS_SSPRK2 && scheme != GS_BENCH_STRANG)
+    return fail(MJX_E_ARG, "scheme must be 0 (Strang), 1 (Euler), 2 (SSPRK2) or 3 (benchmark Strang)");
   int rc = check_grid(dtype, B, ny, nx);
   if (rc) return rc;
   if (!u || !v || !inst || !work) return fail(MJX_E_ARG, "null pointer argument");

@@ -8,8 +8,9 @@
 
 namespace mjx {
 
-// Explicit reaction stage of the fused row pass (k_row_gs).
-enum { RXN_HEUN = 0, RXN_EULER = 1 };
+// Explicit reaction stage of the fused row pass (k_row_gs): Heun over dt,
+// forward Euler over dt, over dt/2, or two forward-Euler steps of dt/2.
+enum { RXN_HEUN = 0, RXN_EULER = 1, RXN_HALF = 2, RXN_HALF2 = 3 };
 // Column-pass power taken per instance from the instance row's aux column.
 constexpr int NREP_PER_INSTANCE = -3;
 
@@ -17,7 +18,8 @@ constexpr int NREP_PER_INSTANCE = -3;
   int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,   \
                         cpx<T>* W, hipStream_t st, double in_scale = 1.0);                                   \
   int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,              \
-                          long long istride, int ghosts, hipStream_t st, double keep = 0.0);                 \
+                          long long istride, int ghosts, hipStream_t st, double keep = 0.0,                  \
+                          const InstParams* react = nullptr);                                                \
   int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = 0,      \
                   int mstep = -1);                                                                           \
   int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,        \

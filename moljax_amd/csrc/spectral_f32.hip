@@ -11,8 +11,8 @@ int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long p
   return Launch<T>::row_fwd_pack(nx, B, ny, a, b, pitch, istride, W, st, (T)in_scale);
 }
 int spec_row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
-                        int ghosts, hipStream_t st, double keep) {
-  return Launch<T>::row_inv_unpack(nx, B, ny, W, a, b, pitch, istride, ghosts, st, (T)keep);
+                        int ghosts, hipStream_t st, double keep, const InstParams* react) {
+  return Launch<T>::row_inv_unpack(nx, B, ny, W, a, b, pitch, istride, ghosts, st, (T)keep, react);
 }
 int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn, int mstep) {
   return Launch<T>::row_gs(nx, B, ny, W, ip, st, rxn, mstep);

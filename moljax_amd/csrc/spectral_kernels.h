@@ -195,10 +195,14 @@ __device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, cpx
 // Packed x-spectra in W -> real fields (a, b), optional periodic ghost ring.
 // keep != 0 blends with the destination's previous content:
 // out = keep * old + (1 - keep) * new (the averaging stage of IMEX-SSPRK2).
+// react != null applies one forward-Euler Gray-Scott step of dt/2 (F, k, dt
+// from the instance row) to the physical values before they are written
+// (the closing reaction half-step of benchmark_gray_scott.py:268-289).
 template <typename T, int NX, int TR>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
     const cpx<T>* __restrict__ W, T* __restrict__ a, T* __restrict__ b, long long pitch, long long istride, int ny,
-    int rows_per_inst_tiles, int ghosts, T keep, const cpx<T>* __restrict__ tw) {
+    int rows_per_inst_tiles, int ghosts, T keep, const cpx<T>* __restrict__ tw,
+    const InstParams* __restrict__ react) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -212,6 +216,15 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
   cpx<T> v[C::P];
   load_unpack<T, NX, TR>(W + ((long long)inst * ny + y) * NX, lrow, j, v);
   line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+  if (react) {
+    const T F = (T)react[inst].F, kk = (T)react[inst].k, hdt = (T)(0.5 * react[inst].dt);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      T ru, rv;
+      gs_react(v[m].re, v[m].im, F, kk, ru, rv);
+      v[m] = mk(v[m].re + hdt * ru, v[m].im + hdt * rv);
+    }
+  }
   T* ra = a + inst * istride + (long long)y * pitch;
   T* rb = b ? b + inst * istride + (long long)y * pitch : nullptr;
   if (keep != (T)0) {
@@ -265,12 +278,15 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
 // RXN = RXN_HEUN (IMEX-Strang), as moljax/core/stepping.py:414-425:
 //   R1 = R(y*), y~ = y* + dt R1, R2 = R(y~), y** = y* + (dt/2)(R1 + R2).
 // RXN = RXN_EULER (IMEX-Euler / SSPRK2 stages, stepping.py:351-360,472-489):
-//   y + dt R(y).
+//   y + dt R(y); with rx = 1 y + (dt/2) R(y), with rx = 2 two such half
+//   steps in a row (the reaction half-steps that close one step and open
+//   the next of benchmark_gray_scott.py:268-289).
 template <typename T, int NX, int TR, int RXN>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W,
                                                                   const InstParams* __restrict__ ip, int ny,
                                                                   int rows_per_inst_tiles,
-                                                                  const cpx<T>* __restrict__ tw, int mstep) {
+                                                                  const cpx<T>* __restrict__ tw, int mstep,
+                                                                  int rx) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -294,10 +310,20 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
     const T u0 = v[m].re, v0 = v[m].im;
     T r1u, r1v, r2u, r2v;
     gs_react(u0, v0, F, kk, r1u, r1v);
-    const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
     if constexpr (RXN == RXN_EULER) {
-      v[m] = mk(ut, vt);
+      if (rx == 0) {
+        v[m] = mk(u0 + dt * r1u, v0 + dt * r1v);
+      } else {
+        T ut = u0 + hdt * r1u, vt = v0 + hdt * r1v;
+        if (rx == 2) {
+          gs_react(ut, vt, F, kk, r2u, r2v);
+          ut = ut + hdt * r2u;
+          vt = vt + hdt * r2v;
+        }
+        v[m] = mk(ut, vt);
+      }
     } else {
+      const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
       gs_react(ut, vt, F, kk, r2u, r2v);
       v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
     }
@@ -448,20 +474,20 @@ struct RowLaunch {
     return check_launch("row_fwd_pack");
   }
   static int inv_unpack(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
-                        int ghosts, T keep, hipStream_t st) {
+                        int ghosts, T keep, const InstParams* react, hipStream_t st) {
     const int tiles = ny / L;
     hipLaunchKernelGGL((k_row_inv_unpack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, a, b,
-                       pitch, istride, ny, tiles, ghosts, keep, twiddles<T>());
+                       pitch, istride, ny, tiles, ghosts, keep, twiddles<T>(), react);
     return check_launch("row_inv_unpack");
   }
   static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, int mstep, hipStream_t st) {
     const int tiles = ny / L;
-    if (rxn == RXN_EULER)
+    if (rxn != RXN_HEUN)
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
-                         ny, tiles, twiddles<T>(), mstep);
+                         ny, tiles, twiddles<T>(), mstep, rxn - RXN_EULER);
     else
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
-                         ny, tiles, twiddles<T>(), mstep);
+                         ny, tiles, twiddles<T>(), mstep, 0);
     return check_launch("row_gs");
   }
 };
@@ -526,8 +552,8 @@ struct Launch {
     template <int L>
     struct F {
       static int run(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride, int ghosts,
-                     T keep, hipStream_t st) {
-        return RowLaunch<T, N, L>::inv_unpack(B, ny, W, a, b, pitch, istride, ghosts, keep, st);
+                     T keep, const InstParams* react, hipStream_t st) {
+        return RowLaunch<T, N, L>::inv_unpack(B, ny, W, a, b, pitch, istride, ghosts, keep, react, st);
       }
     };
   };
@@ -564,12 +590,13 @@ struct Launch {
     }
   }
   static int row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,
-                            long long istride, int ghosts, hipStream_t st, T keep = (T)0) {
+                            long long istride, int ghosts, hipStream_t st, T keep = (T)0,
+                            const InstParams* react = nullptr) {
     switch (nx) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
     return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, a, b, pitch, \
-                                                   istride, ghosts, keep, st);
+                                                   istride, ghosts, keep, react, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:

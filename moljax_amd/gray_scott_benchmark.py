@@ -1,0 +1,201 @@
+"""The Gray-Scott method comparison of benchmarks/benchmark_gray_scott.py on
+the MI355X kernels, reported in that script's results-JSON schema.
+
+Conventions are the script's: N = 256 points on [0, 1)^2 (dx = 1/N), the
+pseudo-spectral symbol -(kx^2 + ky^2) (:52-57), the initial condition of
+np.random.seed(42) with a 20 x 20 perturbation (:60-70), Du = 2e-5,
+Dv = 1e-5, F = 0.035, k = 0.065, t_final = 10000 with ceil(t_final / dt)
+steps, and the median and IQR of n_reps timed runs after a warm-up.
+
+Served methods:
+  'IMEX-Strang'  the script's splitting (:268-289: reaction Euler half-step,
+                 exact diffusion, reaction Euler half-step), dt = 0.5, as
+                 mjx_gs_imex scheme 3: one row and one column pass per step;
+  'ETDRK4'       the script's ETDRK4 (:336-417, contour coefficients with its
+                 f2 as written), dt = 1, as mjx_gs_etdrk4 with
+                 mjx_etdrk4_coeffs_form(form 1).
+The script's 'RK4 (explicit)' and 'CN (Newton-Krylov)' rows are not served
+here and are left out of ``methods``.
+
+Schema (benchmark_gray_scott.py:73,141-148,312-319,440-447,458-475):
+  {"methods": [{"name", "time_s", "iqr_s", "steps", "dt", "t_final"}, ...],
+   "config": {"N", "Du", "Dv", "F", "k", "t_final", "scaling_used",
+              "n_reps", "dtype", "device", "laplacian"}}
+"""
+import json
+import math
+import time
+from pathlib import Path
+from typing import Dict, Iterable, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _hip
+from .core._arrays import inst_table, spectral_work
+
+N_DEFAULT = 256
+DU, DV, F, K = 2e-5, 1e-5, 0.035, 0.065
+T_FINAL = 10000.0
+METHOD_DT = {"IMEX-Strang": 0.5, "ETDRK4": 1.0}
+
+METHOD_KEYS = {"name": str, "time_s": float, "iqr_s": float, "steps": int, "dt": float, "t_final": float}
+CONFIG_KEYS = {"N": int, "Du": float, "Dv": float, "F": float, "k": float, "t_final": float, "scaling_used": bool,
+               "n_reps": int, "dtype": str, "device": str, "laplacian": str}
+
+
+def benchmark_ic(N: int = N_DEFAULT) -> Tuple[np.ndarray, np.ndarray]:
+    """benchmark_gray_scott.py:60-70: u0 = 1 - 0.05 U, v0 = 0.25 + 0.05 U'
+    (U, U' the first two draws of the seed-42 generator), then u0 = 0.5,
+    v0 = 0.25 on [N/2-10, N/2+10)^2."""
+    rs = np.random.RandomState(42)
+    u0 = np.ones((N, N)) - 0.5 * rs.rand(N, N) * 0.1
+    v0 = 0.25 * np.ones((N, N)) + 0.5 * rs.rand(N, N) * 0.1
+    c, r = N // 2, 10
+    u0[c - r:c + r, c - r:c + r] = 0.5
+    v0[c - r:c + r, c - r:c + r] = 0.25
+    return u0, v0
+
+
+def spectral_symbol(N: int = N_DEFAULT) -> np.ndarray:
+    """-(2 pi fftfreq(N, 1/N))^2 per axis (benchmark_gray_scott.py:52-57)."""
+    kk = np.fft.fftfreq(N, 1.0 / N) * 2 * np.pi
+    return -(kk ** 2)
+
+
+class GrayScottMethod:
+    """One method of the comparison on B = 1 resident (N, N) state."""
+
+    def __init__(self, name: str, N: int, dtype: torch.dtype):
+        if name not in METHOD_DT:
+            raise ValueError(f"unknown or unserved method {name!r}; served: {sorted(METHOD_DT)}")
+        self.name, self.N, self.dtype, self.dt = name, N, dtype, METHOD_DT[name]
+        lam = spectral_symbol(N)
+        self.u = torch.empty((N, N), dtype=dtype, device="cuda")
+        self.v = torch.empty_like(self.u)
+        lib = _hip.lib()
+        if name == "IMEX-Strang":
+            self.lam = torch.from_numpy(lam).to("cuda", dtype)
+            self.inst = inst_table([[self.dt * DU, self.dt * DV, 0.0, 0.0, F, K, self.dt]], "cuda")
+            self.work = spectral_work(dtype, 1, N, N, "cuda")
+        else:
+            from .core.fft_integrators import etdrk4_coefficients
+            self.coef = etdrk4_coefficients(lam, lam, DU, DV, self.dt, dtype, cox_matthews=False)
+            self.inst = inst_table([[0.0, 0.0, 0.0, 0.0, F, K, self.dt]], "cuda")
+            nbytes = lib.mjx_etdrk4_workspace_bytes(_hip.dtype_code(dtype), 1, N, N)
+            self.work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+
+    def load(self, u0: torch.Tensor, v0: torch.Tensor):
+        self.u.copy_(u0)
+        self.v.copy_(v0)
+
+    def advance(self, n_steps: int):
+        lib, N = _hip.lib(), self.N
+        code = _hip.dtype_code(self.dtype)
+        if self.name == "IMEX-Strang":
+            rc = lib.mjx_gs_imex(code, 3, 1, N, N, _hip.ptr(self.u), _hip.ptr(self.v), N, N * N, 0,
+                                 _hip.ptr(self.lam), _hip.ptr(self.lam), None, _hip.ptr(self.inst), int(n_steps),
+                                 _hip.ptr(self.work), _hip.stream_ptr())
+        else:
+            rc = lib.mjx_gs_etdrk4(code, 1, N, N, _hip.ptr(self.u), _hip.ptr(self.v), N, N * N, 0,
+                                   _hip.ptr(self.coef), _hip.ptr(self.inst), int(n_steps), _hip.ptr(self.work),
+                                   _hip.stream_ptr())
+        _hip.check(rc, self.name)
+
+
+def run_benchmark(t_final: float = T_FINAL, n_reps: int = 10, dtype: torch.dtype = torch.float64,
+                  methods: Iterable[str] = ("IMEX-Strang", "ETDRK4"), N: int = N_DEFAULT,
+                  warmup_steps: int = 10, progress=None) -> Dict:
+    """Time each method over ceil(t_final / dt) steps from the benchmark's
+    initial condition (already resident on the device when the clock
+    starts), n_reps times after a warm-up, and return the results dict."""
+    u0, v0 = benchmark_ic(N)
+    U0 = torch.from_numpy(u0).to("cuda", dtype)
+    V0 = torch.from_numpy(v0).to("cuda", dtype)
+    results: Dict = {"methods": []}
+    for name in methods:
+        m = GrayScottMethod(name, N, dtype)
+        n_steps = int(math.ceil(t_final / m.dt))
+        m.load(U0, V0)
+        m.advance(warmup_steps)
+        times = []
+        for _ in range(n_reps):
+            m.load(U0, V0)
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            m.advance(n_steps)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - start)
+        q25, q75 = np.percentile(times, [25, 75])
+        results["methods"].append({"name": name, "time_s": float(np.median(times)), "iqr_s": float(q75 - q25),
+                                   "steps": n_steps, "dt": float(m.dt), "t_final": float(n_steps * m.dt)})
+        if progress:
+            progress(results["methods"][-1])
+    results["config"] = {"N": N, "Du": DU, "Dv": DV, "F": F, "k": K, "t_final": float(t_final),
+                         "scaling_used": False, "n_reps": int(n_reps),
+                         "dtype": "float64" if dtype == torch.float64 else "float32",
+                         "device": f"cuda:0 ({torch.cuda.get_device_name(0)})",
+                         "laplacian": "pseudo-spectral (-k^2)"}
+    return results
+
+
+def validate_results(obj: Dict) -> None:
+    """Raise ValueError unless ``obj`` has the benchmark's results schema."""
+    if not isinstance(obj, dict) or set(obj) != {"methods", "config"}:
+        raise ValueError("results must be {'methods': [...], 'config': {...}}")
+    if not isinstance(obj["methods"], list):
+        raise ValueError("'methods' must be a list")
+    for m in obj["methods"]:
+        if not isinstance(m, dict) or set(m) != set(METHOD_KEYS):
+            raise ValueError(f"method entry keys must be {sorted(METHOD_KEYS)}")
+        for key, typ in METHOD_KEYS.items():
+            if typ is float and not isinstance(m[key], (int, float)) or typ is not float and not isinstance(m[key], typ):
+                raise ValueError(f"method field {key!r} must be {typ.__name__}")
+    cfg = obj["config"]
+    if not isinstance(cfg, dict) or set(cfg) != set(CONFIG_KEYS):
+        raise ValueError(f"config keys must be {sorted(CONFIG_KEYS)}")
+    for key, typ in CONFIG_KEYS.items():
+        val = cfg[key]
+        ok = isinstance(val, (int, float)) and not isinstance(val, bool) if typ is float else isinstance(val, typ)
+        if typ is int and isinstance(val, bool):
+            ok = False
+        if not ok:
+            raise ValueError(f"config field {key!r} must be {typ.__name__}")
+
+
+def save_results(results: Dict, path) -> Path:
+    """Write ``results`` as the benchmark does (json, indent 2)."""
+    validate_results(results)
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(results, f, indent=2)
+    return path
+
+
+def load_results(path) -> Dict:
+    with open(path) as f:
+        obj = json.load(f)
+    validate_results(obj)
+    return obj
+
+
+def compare(ours: Dict, reference: Dict) -> Dict[str, Optional[float]]:
+    """Speed-up per method name present in both (reference time / ours)."""
+    ref = {m["name"]: m for m in reference["methods"]}
+    return {m["name"]: (ref[m["name"]]["time_s"] / m["time_s"] if m["name"] in ref else None)
+            for m in ours["methods"]}
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--t-final", type=float, default=T_FINAL)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--fp32", action="store_true")
+    ap.add_argument("--out", default="results/gray_scott.json")
+    a = ap.parse_args()
+    res = run_benchmark(a.t_final, a.reps, torch.float32 if a.fp32 else torch.float64,
+                        progress=lambda m: print(f"{m['name']}: {m['time_s']:.3f} s +- {m['iqr_s']:.3f} "
+                                                 f"({m['steps']} steps, dt={m['dt']})", flush=True))
+    print(f"saved {save_results(res, a.out)}")

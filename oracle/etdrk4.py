@@ -24,6 +24,12 @@ step, so u and v couple; ETD2 without N_prev is ETD1).  Pinned by the
 series / expm1 forms of phi1, phi2, linear exactness, the reduction to
 forward Euler / Adams-Bashforth-2 on spatially constant fields (lam = 0),
 and first / second-order convergence (tests/test_oracle_golden.py).
+
+The benchmark's own splitting (benchmark_gray_scott.py:268-289, reaction
+half-step / exact diffusion / reaction half-step) is restated as
+benchmark_strang; pinned by pure diffusion (F = k = 0, v = 0: e^{n dt D lam}
+exactly) and, on spatially constant fields, by two forward-Euler half-steps
+per step.
 """
 import numpy as np
 
@@ -149,3 +155,25 @@ def gs_etd(u, v, lamx, lamy, Du, Dv, F, k, dt, n_steps, order, N_prev=None):
         else:
             u, v, N_prev = gs_etd2_step(u, v, lamx, lamy, Du, Dv, F, k, dt, N_prev)
     return u, v, N_prev
+
+
+def benchmark_strang_step(u, v, lamx, lamy, Du, Dv, F, k, dt):
+    """benchmark_gray_scott.py:268-289: forward-Euler reaction half-step,
+    exact diffusion e^{dt D lam} by FFT, forward-Euler reaction half-step
+    (lam(kx, ky) = lamx[kx] + lamy[ky], axis 0 = y)."""
+    lam = lamy[:, None] + lamx[None, :]
+
+    def half(a, b):
+        abb = a * b * b
+        return a + 0.5 * dt * (-abb + F * (1 - a)), b + 0.5 * dt * (abb - (F + k) * b)
+
+    u, v = half(u, v)
+    u = np.real(np.fft.ifft2(np.fft.fft2(u) * np.exp(dt * Du * lam)))
+    v = np.real(np.fft.ifft2(np.fft.fft2(v) * np.exp(dt * Dv * lam)))
+    return half(u, v)
+
+
+def benchmark_strang(u, v, lamx, lamy, Du, Dv, F, k, dt, n_steps):
+    for _ in range(n_steps):
+        u, v = benchmark_strang_step(u, v, lamx, lamy, Du, Dv, F, k, dt)
+    return u, v

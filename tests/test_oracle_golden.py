@@ -402,6 +402,27 @@ def test_etd1_etd2_reductions_and_order():
         assert all(lo < r < order + 0.5 for r in rates), (order, rates)
 
 
+def test_benchmark_strang_pure_diffusion_and_constant_fields():
+    """benchmark_gray_scott.py:268-289 restated: with F = k = 0 and v = 0 it
+    is the exact heat semigroup; on constant fields (only lam = 0) each step
+    is two forward-Euler half-steps of the reaction ODE."""
+    from oracle import etdrk4 as oe
+    n = 32
+    lam = oe.spectral_symbol_1d(n)
+    u0 = np.random.default_rng(3).standard_normal((n, n))
+    u, v = oe.benchmark_strang(u0, np.zeros((n, n)), lam, lam, 2e-4, 1e-4, 0.0, 0.0, 0.5, 4)
+    expect = np.fft.ifft2(np.fft.fft2(u0) * np.exp(2.0 * 2e-4 * (lam[None, :] + lam[:, None]))).real
+    assert np.max(np.abs(u - expect)) < 1e-13 and np.max(np.abs(v)) == 0.0
+    F, k, dt = 0.035, 0.065, 0.5
+    a, b = 0.6, 0.3
+    for _ in range(5):
+        for _h in range(2):
+            abb = a * b * b
+            a, b = a + 0.25 * (-abb + F * (1 - a)), b + 0.25 * (abb - (F + k) * b)
+    u, v = oe.benchmark_strang(np.full((n, n), 0.6), np.full((n, n), 0.3), lam, lam, 2e-5, 1e-5, F, k, dt, 5)
+    assert np.max(np.abs(u - a)) < 1e-14 and np.max(np.abs(v - b)) < 1e-14
+
+
 # --------------------------------------------------------- Newton-Krylov
 def test_hand_jvp_matches_golden_ad_norm():
     """benchmarks/results/jvp_vs_fd_sweep.json:3: AD-JVP norm of the BE
