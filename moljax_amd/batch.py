@@ -20,7 +20,8 @@ import torch
 from . import _hip
 from .core._arrays import inst_table
 
-SCHEME_ETDRK4, SCHEME_STRANG = 0, 1
+SCHEME_ETDRK4, SCHEME_STRANG, SCHEME_BENCH_STRANG = 0, 1, 2
+_SCHEMES = {"etdrk4": SCHEME_ETDRK4, "strang": SCHEME_STRANG, "benchmark_strang": SCHEME_BENCH_STRANG}
 _TUNED: Dict[tuple, Tuple[int, int]] = {}
 
 
@@ -56,19 +57,23 @@ class GrayScottBatch:
 
     sizes[i] is instance i's square grid size; F[i], k[i] its feed and kill
     rates; Du, Dv, dt are shared (one coefficient table per size).
-    scheme 'etdrk4' (Cox-Matthews, FP64 contour coefficients) or 'strang'
-    (IMEX-Strang with implicit diffusion half-steps).  symbol 'spectral'
-    (-k^2, the ETDRK4 benchmark's) or 'fd' (5-point eigenvalues); the domain
-    is [0, L)^2 for every instance."""
+    scheme 'etdrk4' (Cox-Matthews, FP64 contour coefficients; with
+    ``cox_matthews=False`` the benchmark's f2 as written), 'strang'
+    (IMEX-Strang with implicit diffusion half-steps) or 'benchmark_strang'
+    (the Gray-Scott benchmark's reaction / exact diffusion / reaction
+    splitting, benchmarks/benchmark_gray_scott.py:268-289).  symbol
+    'spectral' (-k^2, the benchmark's) or 'fd' (5-point eigenvalues); the
+    domain is [0, L)^2 for every instance."""
 
     def __init__(self, sizes: Sequence[int], F: Sequence[float], k: Sequence[float], Du: float, Dv: float,
                  dt: float, scheme: str = "etdrk4", dtype: torch.dtype = torch.float32, L: float = 1.0,
-                 symbol: str = "spectral", steps_per_graph: int = 16, tune: bool = True):
-        if scheme not in ("etdrk4", "strang"):
-            raise ValueError("scheme must be 'etdrk4' or 'strang'")
+                 symbol: str = "spectral", steps_per_graph: int = 16, tune: bool = True,
+                 cox_matthews: bool = True):
+        if scheme not in _SCHEMES:
+            raise ValueError(f"scheme must be one of {sorted(_SCHEMES)}")
         if len(F) != len(sizes) or len(k) != len(sizes):
             raise ValueError("one F and one k per instance")
-        self.scheme = SCHEME_ETDRK4 if scheme == "etdrk4" else SCHEME_STRANG
+        self.scheme = _SCHEMES[scheme]
         self.dtype = dtype
         self.sizes = [int(s) for s in sizes]
         self.order = sorted(set(self.sizes))
@@ -89,14 +94,15 @@ class GrayScottBatch:
             lam = sym(n, L)
             if self.scheme == SCHEME_ETDRK4:
                 from .core.fft_integrators import etdrk4_coefficients
-                coef = etdrk4_coefficients(lam, lam, Du, Dv, dt, dtype)
+                coef = etdrk4_coefficients(lam, lam, Du, Dv, dt, dtype, cox_matthews)
                 self.tables.append((coef, None))
                 rows = [[0, 0, 0, 0, F[i], k[i], dt] for i in idx]
                 nbytes = lib.mjx_etdrk4_workspace_bytes(_hip.dtype_code(dtype), B, n, n)
             else:
                 t = torch.from_numpy(lam).to("cuda", dtype)
                 self.tables.append((t, t))
-                rows = [[0.5 * dt * Du, 0.5 * dt * Dv, 0, 0, F[i], k[i], dt] for i in idx]
+                h = 0.5 * dt if self.scheme == SCHEME_STRANG else dt
+                rows = [[h * Du, h * Dv, 0, 0, F[i], k[i], dt] for i in idx]
                 nbytes = lib.mjx_spectral_workspace_bytes(_hip.dtype_code(dtype), B, n, n)
             self.rows.append(inst_table(rows))
             self.work.append(torch.empty(nbytes, dtype=torch.uint8, device="cuda"))

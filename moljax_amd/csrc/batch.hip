@@ -9,6 +9,9 @@
 //      groups' kernels are independent branches of one graph and small
 //      groups run alongside large ones instead of behind them;
 //   3. the remainder (n_steps mod K) and the unpack run as direct launches.
+// Schemes: 0 ETDRK4 (tables: coefficients), 1 IMEX-Strang (tables: lamx,
+// lamy), 2 the Gray-Scott benchmark's splitting (tables: lamx, lamy; inst
+// beta = dt D; benchmarks/benchmark_gray_scott.py:268-289).
 // Kernel tiles (lines per workgroup) come from the autotuner table, which
 // mjx_autotune fills by timing the candidate tiles on scratch data of the
 // group's shape.  Results are bitwise those of the same instances advanced
@@ -44,9 +47,10 @@ int etd_advance(int B, int n, const double* coef, const InstParams* ip, int n_st
 
 namespace {
 
-enum { SCHEME_ETDRK4 = 0, SCHEME_STRANG = 1 };
+enum { SCHEME_ETDRK4 = 0, SCHEME_STRANG = 1, SCHEME_BENCH_STRANG = 2 };
 
 struct Group {
+  int scheme;
   int n, B;
   void *u, *v;
   long long pitch, istride;
@@ -58,26 +62,42 @@ struct Group {
 template <typename T>
 struct Ops {
   static int pack(const Group& g, hipStream_t st) {
-    if (g.t1 == nullptr)
+    if (g.scheme == SCHEME_ETDRK4)
       return etd_pack(g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
+    if (g.scheme == SCHEME_BENCH_STRANG) {
+      // the first step up to its closing reaction half-step
+      int rc = spec_row_fwd_pack(g.n, g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work,
+                                 st, 1.0 / g.n);
+      if (!rc) rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, RXN_HALF);
+      if (!rc) rc = spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, -2, st);
+      return rc;
+    }
     int rc = spec_row_fwd_pack(g.n, g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
     if (rc) return rc;
     return spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, 1, st);
   }
   // Strang: `steps` middle steps (row_gs, col(H^2)); the last step's
   // trailing half-solve is applied by finish().
+  // Benchmark splitting: each middle step closes the previous step's
+  // reaction and opens its own (two half-steps in one row pass).
   static int advance(const Group& g, int steps, hipStream_t st) {
-    if (g.t1 == nullptr) return etd_advance(g.B, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
+    if (g.scheme == SCHEME_ETDRK4) return etd_advance(g.B, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
+    const bool bench = g.scheme == SCHEME_BENCH_STRANG;
     for (int s = 0; s < steps; ++s) {
-      int rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, RXN_HEUN);
-      if (!rc) rc = spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, 2, st);
+      int rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, bench ? RXN_HALF2 : RXN_HEUN);
+      if (!rc)
+        rc = spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip,
+                            bench ? -2 : 2, st);
       if (rc) return rc;
     }
     return MJX_OK;
   }
   static int finish(const Group& g, int ghosts, hipStream_t st) {
-    if (g.t1 == nullptr)
+    if (g.scheme == SCHEME_ETDRK4)
       return etd_unpack(g.B, g.n, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts, (cpx<T>*)g.work, st);
+    if (g.scheme == SCHEME_BENCH_STRANG)
+      return spec_row_inv_unpack(g.n, g.B, g.n, (const cpx<T>*)g.work, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts,
+                                 st, 0.0, g.ip);
     int rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, RXN_HEUN);
     if (!rc) rc = spec_col_apply(g.n, g.B, g.n, (cpx<T>*)g.work, (const T*)g.t0, (const T*)g.t1, nullptr, g.ip, 1, st);
     if (!rc)
@@ -160,8 +180,10 @@ struct BatchPlan {
       if (rc) return rc;
     }
     // Strang: the first step's leading half-solve is in pack(), the last
-    // step's row pass and trailing half-solve in finish().
-    const int middle = scheme == SCHEME_STRANG ? n_steps - 1 : n_steps;
+    // step's row pass and trailing half-solve in finish().  Benchmark
+    // splitting: pack() runs the first step but its closing half-step,
+    // which finish() applies.
+    const int middle = scheme == SCHEME_ETDRK4 ? n_steps : n_steps - 1;
     if (middle > 0) {
       const int reps = middle / K;
       if (reps > 0 && !exec) {
@@ -255,7 +277,7 @@ int autotune_t(int scheme, int n, int B, int reps, hipStream_t st, int* chosen) 
   const int dflt = lines_default_rt(sizeof(T) == 8 ? MJX_F64 : MJX_F32, n);
   if (!rc) {
     if (scheme == SCHEME_ETDRK4) {
-      Group g{n, B, u, u + pts, n, (long long)n * n, tab, nullptr, ip, work};
+      Group g{SCHEME_ETDRK4, n, B, u, u + pts, n, (long long)n * n, tab, nullptr, ip, work};
       auto body = [&]() { return Ops<T>::advance(g, 1, st); };
       rc = time_candidates<T>(TK_ETD_ROW, n, B, reps, candidates(dflt, n, 3), st, body, &chosen[0], nullptr);
       const int cm = etd_col_max(sizeof(T) == 8 ? MJX_F64 : MJX_F32, n);
@@ -289,7 +311,8 @@ int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const in
   if (!plan) return fail(MJX_E_ARG, "null plan pointer");
   *plan = nullptr;
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
-  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG) return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4) or 1 (Strang)");
+  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG && scheme != SCHEME_BENCH_STRANG)
+    return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4), 1 (Strang) or 2 (benchmark splitting)");
   if (n_groups < 1 || !n || !B || !u || !v || !pitch || !istride || !tables || !inst || !work)
     return fail(MJX_E_ARG, "null pointer argument or no groups");
   if (steps_per_graph < 1 || steps_per_graph > 256) return fail(MJX_E_ARG, "steps_per_graph must be in [1, 256]");
@@ -305,11 +328,11 @@ int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const in
                                             : is_pow2_len(ng);
     if (!ok) return fail(MJX_E_SIZE, "batch group " + std::to_string(g) + ": unsupported size " + std::to_string(ng));
     if (B[g] < 1 || !u[g] || !v[g] || !inst[g] || !work[g] || !tables[2 * g] ||
-        (scheme == SCHEME_STRANG && !tables[2 * g + 1]))
+        (scheme != SCHEME_ETDRK4 && !tables[2 * g + 1]))
       return fail(MJX_E_ARG, "batch group " + std::to_string(g) + ": null pointer or empty");
     if (pitch[g] < ng) return fail(MJX_E_ARG, "batch group " + std::to_string(g) + ": pitch smaller than n");
-    Group gr{ng, B[g], u[g], v[g], pitch[g], istride[g], tables[2 * g],
-             scheme == SCHEME_STRANG ? tables[2 * g + 1] : nullptr, reinterpret_cast<const InstParams*>(inst[g]),
+    Group gr{scheme, ng, B[g], u[g], v[g], pitch[g], istride[g], tables[2 * g],
+             scheme != SCHEME_ETDRK4 ? tables[2 * g + 1] : nullptr, reinterpret_cast<const InstParams*>(inst[g]),
              work[g]};
     p->groups.push_back(gr);
   }
@@ -335,7 +358,8 @@ int mjx_batch_destroy(void* plan) {
 
 int mjx_autotune(int dtype, int scheme, int n, int B, int reps, int* chosen, void* stream) {
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
-  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG) return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4) or 1 (Strang)");
+  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG && scheme != SCHEME_BENCH_STRANG)
+    return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4), 1 (Strang) or 2 (benchmark splitting)");
   if (!is_pow2_len(n) || (scheme == SCHEME_ETDRK4 && (n < 32 || n > 1024)))
     return fail(MJX_E_SIZE, "unsupported size " + std::to_string(n));
   if (B < 1 || reps < 1 || !chosen) return fail(MJX_E_ARG, "B, reps >= 1 and chosen required");

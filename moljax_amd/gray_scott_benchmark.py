@@ -9,11 +9,14 @@ steps, and the median and IQR of n_reps timed runs after a warm-up.
 
 Served methods:
   'IMEX-Strang'  the script's splitting (:268-289: reaction Euler half-step,
-                 exact diffusion, reaction Euler half-step), dt = 0.5, as
-                 mjx_gs_imex scheme 3: one row and one column pass per step;
+                 exact diffusion, reaction Euler half-step), dt = 0.5: one
+                 row and one column pass per step (mjx_gs_imex scheme 3, or
+                 batch-plan scheme 2 replayed as a hipGraph);
   'ETDRK4'       the script's ETDRK4 (:336-417, contour coefficients with its
-                 f2 as written), dt = 1, as mjx_gs_etdrk4 with
-                 mjx_etdrk4_coeffs_form(form 1).
+                 f2 as written), dt = 1 (mjx_gs_etdrk4 with
+                 mjx_etdrk4_coeffs_form(form 1), or batch-plan scheme 0).
+A single 256^2 instance is launch-bound (each kernel moves ~1-2 MB), so by
+default the steps are replayed from hipGraphs of ``steps_per_graph`` steps.
 The script's 'RK4 (explicit)' and 'CN (Newton-Krylov)' rows are not served
 here and are left out of ``methods``.
 
@@ -64,12 +67,22 @@ def spectral_symbol(N: int = N_DEFAULT) -> np.ndarray:
 
 
 class GrayScottMethod:
-    """One method of the comparison on B = 1 resident (N, N) state."""
+    """One method of the comparison on B = 1 resident (N, N) state;
+    ``steps_per_graph`` > 0 replays hipGraphs of that many steps (a batch
+    plan of one instance), 0 launches every kernel directly."""
 
-    def __init__(self, name: str, N: int, dtype: torch.dtype):
+    def __init__(self, name: str, N: int, dtype: torch.dtype, steps_per_graph: int = 100):
         if name not in METHOD_DT:
             raise ValueError(f"unknown or unserved method {name!r}; served: {sorted(METHOD_DT)}")
         self.name, self.N, self.dtype, self.dt = name, N, dtype, METHOD_DT[name]
+        self.plan = None
+        if steps_per_graph > 0:
+            from .batch import GrayScottBatch
+            scheme = "benchmark_strang" if name == "IMEX-Strang" else "etdrk4"
+            self.plan = GrayScottBatch([N], [F], [K], DU, DV, self.dt, scheme=scheme, dtype=dtype,
+                                       steps_per_graph=steps_per_graph, cox_matthews=False)
+            self.u, self.v = self.plan.state(0)
+            return
         lam = spectral_symbol(N)
         self.u = torch.empty((N, N), dtype=dtype, device="cuda")
         self.v = torch.empty_like(self.u)
@@ -90,6 +103,9 @@ class GrayScottMethod:
         self.v.copy_(v0)
 
     def advance(self, n_steps: int):
+        if self.plan is not None:
+            self.plan.run(n_steps)
+            return
         lib, N = _hip.lib(), self.N
         code = _hip.dtype_code(self.dtype)
         if self.name == "IMEX-Strang":
@@ -105,7 +121,7 @@ class GrayScottMethod:
 
 def run_benchmark(t_final: float = T_FINAL, n_reps: int = 10, dtype: torch.dtype = torch.float64,
                   methods: Iterable[str] = ("IMEX-Strang", "ETDRK4"), N: int = N_DEFAULT,
-                  warmup_steps: int = 10, progress=None) -> Dict:
+                  warmup_steps: int = 10, progress=None, steps_per_graph: int = 100) -> Dict:
     """Time each method over ceil(t_final / dt) steps from the benchmark's
     initial condition (already resident on the device when the clock
     starts), n_reps times after a warm-up, and return the results dict."""
@@ -114,10 +130,10 @@ def run_benchmark(t_final: float = T_FINAL, n_reps: int = 10, dtype: torch.dtype
     V0 = torch.from_numpy(v0).to("cuda", dtype)
     results: Dict = {"methods": []}
     for name in methods:
-        m = GrayScottMethod(name, N, dtype)
+        m = GrayScottMethod(name, N, dtype, steps_per_graph)
         n_steps = int(math.ceil(t_final / m.dt))
         m.load(U0, V0)
-        m.advance(warmup_steps)
+        m.advance(max(warmup_steps, steps_per_graph + 1))  # captures the graph
         times = []
         for _ in range(n_reps):
             m.load(U0, V0)
@@ -134,7 +150,7 @@ def run_benchmark(t_final: float = T_FINAL, n_reps: int = 10, dtype: torch.dtype
     results["config"] = {"N": N, "Du": DU, "Dv": DV, "F": F, "k": K, "t_final": float(t_final),
                          "scaling_used": False, "n_reps": int(n_reps),
                          "dtype": "float64" if dtype == torch.float64 else "float32",
-                         "device": f"cuda:0 ({torch.cuda.get_device_name(0)})",
+                         "device": f"cuda:0 ({torch.cuda.get_device_properties(0).gcnArchName})",
                          "laplacian": "pseudo-spectral (-k^2)"}
     return results
 
@@ -194,8 +210,10 @@ if __name__ == "__main__":
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--out", default="results/gray_scott.json")
+    ap.add_argument("--steps-per-graph", type=int, default=100, help="0: direct launches")
     a = ap.parse_args()
     res = run_benchmark(a.t_final, a.reps, torch.float32 if a.fp32 else torch.float64,
+                        steps_per_graph=a.steps_per_graph,
                         progress=lambda m: print(f"{m['name']}: {m['time_s']:.3f} s +- {m['iqr_s']:.3f} "
                                                  f"({m['steps']} steps, dt={m['dt']})", flush=True))
     print(f"saved {save_results(res, a.out)}")

@@ -15,13 +15,15 @@ from oracle import gray_scott as ogs
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("spg", [0, 4])
 @pytest.mark.parametrize("name", ["IMEX-Strang", "ETDRK4"])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
-def test_benchmark_methods_match_oracle(gpu, name, dtype, tol):
+def test_benchmark_methods_match_oracle(gpu, name, dtype, tol, spg):
+    """Direct launches (spg 0) and hipGraph replays of 4 steps (spg 4)."""
     from moljax_amd.gray_scott_benchmark import DU, DV, F, K, GrayScottMethod, benchmark_ic, spectral_symbol
     u0, v0 = benchmark_ic(256)
     lam = spectral_symbol(256)
-    m = GrayScottMethod(name, 256, dtype)
+    m = GrayScottMethod(name, 256, dtype, steps_per_graph=spg)
     m.load(torch.from_numpy(u0).to("cuda", dtype), torch.from_numpy(v0).to("cuda", dtype))
     steps = 20 if name == "IMEX-Strang" else 10
     m.advance(steps)
@@ -31,18 +33,19 @@ def test_benchmark_methods_match_oracle(gpu, name, dtype, tol):
         ru, rv = oe.gs_etdrk4(u0, v0, lam, lam, DU, DV, F, K, 1.0, steps, cox_matthews=False)
     eu = ogs.rel_l2(m.u.double().cpu().numpy(), ru)
     ev = ogs.rel_l2(m.v.double().cpu().numpy(), rv)
-    print(f"{name} {dtype} {steps} steps: rel_l2 u={eu:.2e} v={ev:.2e}", flush=True)
+    print(f"{name} {dtype} {steps} steps (graph {spg}): rel_l2 u={eu:.2e} v={ev:.2e}", flush=True)
     assert eu <= tol and ev <= tol
 
 
 def test_benchmark_strang_chunks_and_batch_rows(gpu):
     """Splitting the run over calls (the closing / opening half-steps then
     meet at a call boundary instead of inside one row pass) gives the same
-    trajectory to rounding."""
+    trajectory to rounding; so does a batch of two benchmark-splitting
+    instances in one graph plan, each bitwise equal to its lone plan."""
     from moljax_amd.gray_scott_benchmark import GrayScottMethod, benchmark_ic
     u0, v0 = benchmark_ic(256)
-    a = GrayScottMethod("IMEX-Strang", 256, torch.float64)
-    b = GrayScottMethod("IMEX-Strang", 256, torch.float64)
+    a = GrayScottMethod("IMEX-Strang", 256, torch.float64, steps_per_graph=0)
+    b = GrayScottMethod("IMEX-Strang", 256, torch.float64, steps_per_graph=0)
     for m in (a, b):
         m.load(torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda())
     a.advance(12)
@@ -50,11 +53,30 @@ def test_benchmark_strang_chunks_and_batch_rows(gpu):
         b.advance(n)
     assert ogs.rel_l2(b.u.cpu().numpy(), a.u.cpu().numpy()) < 1e-14
     assert ogs.rel_l2(b.v.cpu().numpy(), a.v.cpu().numpy()) < 1e-14
+    from moljax_amd.batch import GrayScottBatch
+    from moljax_amd.gray_scott_benchmark import DU, DV
+    mix = GrayScottBatch([64, 128, 64], [0.03, 0.035, 0.04], [0.06, 0.065, 0.062], DU, DV, 0.5,
+                         scheme="benchmark_strang", dtype=torch.float64, steps_per_graph=3)
+    rng = np.random.default_rng(1)
+    ics = [(1 - 0.1 * rng.random((n, n)), 0.25 + 0.1 * rng.random((n, n))) for n in (64, 128, 64)]
+    for i, (uu, vv) in enumerate(ics):
+        mix.set_state(i, torch.from_numpy(uu), torch.from_numpy(vv))
+    mix.run(8)
+    lam128 = oe.spectral_symbol_1d(128)
+    ru, _ = oe.benchmark_strang(ics[1][0], ics[1][1], lam128, lam128, DU, DV, 0.035, 0.065, 0.5, 8)
+    assert ogs.rel_l2(mix.state(1)[0].cpu().numpy(), ru) < 1e-12
+    alone = GrayScottBatch([64], [0.04], [0.062], DU, DV, 0.5, scheme="benchmark_strang", dtype=torch.float64,
+                           steps_per_graph=3)
+    alone.set_state(0, torch.from_numpy(ics[2][0]), torch.from_numpy(ics[2][1]))
+    alone.run(8)
+    assert torch.equal(alone.state(0)[0], mix.state(2)[0]) and torch.equal(alone.state(0)[1], mix.state(2)[1])
+    mix.close()
+    alone.close()
 
 
 def test_run_benchmark_schema(gpu, tmp_path):
     from moljax_amd.gray_scott_benchmark import load_results, run_benchmark, save_results
-    res = run_benchmark(t_final=5.25, n_reps=2, dtype=torch.float32, warmup_steps=2)
+    res = run_benchmark(t_final=5.25, n_reps=2, dtype=torch.float32, warmup_steps=2, steps_per_graph=3)
     p = save_results(res, tmp_path / "gray_scott.json")
     back = load_results(p)
     names = [m["name"] for m in back["methods"]]
