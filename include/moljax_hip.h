@@ -219,13 +219,22 @@ int mjx_tune_set(int kind, int dtype, int n, int lines);
  *   MJX_RED_SCALED_SQ  sum (a / (atol + rtol |b|))^2 (scaled_error_norm
  *                      before mean and sqrt)
  *   MJX_RED_NONFINITE  count of non-finite a        (is_finite)
+ *   MJX_RED_SUM        sum a                        (sum_interior, mean_interior;
+ *                                                    utils.py:236-262)
  * accumulate != 0 combines with the existing out[b] (sum, or max for
  * MAXABS), so several fields reduce into one value with successive calls.
  * work: mjx_reduce_workspace_elems(B) doubles.
  * Replaces: moljax/core/state.py:115-189 (tree_norm2, tree_norm_inf,
  * tree_vdot), state.py:312-343 (scaled_error_norm), moljax/core/utils.py:186-197
  * (is_finite). */
-enum { MJX_RED_SUMSQ = 0, MJX_RED_DOT = 1, MJX_RED_MAXABS = 2, MJX_RED_SCALED_SQ = 3, MJX_RED_NONFINITE = 4 };
+enum {
+  MJX_RED_SUMSQ = 0,
+  MJX_RED_DOT = 1,
+  MJX_RED_MAXABS = 2,
+  MJX_RED_SCALED_SQ = 3,
+  MJX_RED_NONFINITE = 4,
+  MJX_RED_SUM = 5
+};
 long long mjx_reduce_workspace_elems(int B);
 int mjx_reduce2d(int dtype, int op, int B, int ny, int nx, const void* a, long long a_pitch, long long a_istride,
                  const void* b, long long b_pitch, long long b_istride, double atol, double rtol, double* out,

@@ -48,7 +48,7 @@ def _interior(f: torch.Tensor, grid, interior_only: bool) -> torch.Tensor:
     return f[..., grid.interior_slice]
 
 
-RED_SUMSQ, RED_DOT, RED_MAXABS, RED_SCALED_SQ, RED_NONFINITE = 0, 1, 2, 3, 4
+RED_SUMSQ, RED_DOT, RED_MAXABS, RED_SCALED_SQ, RED_NONFINITE, RED_SUM = 0, 1, 2, 3, 4, 5
 
 
 def _rview(f: torch.Tensor):

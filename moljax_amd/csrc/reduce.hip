@@ -43,6 +43,7 @@ __device__ __forceinline__ double term(T a, T b, double atol, double rtol) {
   if constexpr (OP == MJX_RED_SUMSQ) return x * x;
   else if constexpr (OP == MJX_RED_DOT) return x * (double)b;
   else if constexpr (OP == MJX_RED_MAXABS) return fabs(x);
+  else if constexpr (OP == MJX_RED_SUM) return x;
   else if constexpr (OP == MJX_RED_SCALED_SQ) {
     const double q = x / (atol + rtol * fabs((double)b));
     return q * q;
@@ -109,6 +110,7 @@ int dispatch(int op, int B, int ny, int nx, const T* a, long long ap, long long 
       return launch<T, MJX_RED_SCALED_SQ>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
     case MJX_RED_NONFINITE:
       return launch<T, MJX_RED_NONFINITE>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
+    case MJX_RED_SUM: return launch<T, MJX_RED_SUM>(B, ny, nx, a, ap, ai, b, bp, bi, atol, rtol, out, work, acc, st);
     default: return fail(MJX_E_ARG, "unknown reduction op");
   }
 }

@@ -73,3 +73,32 @@ def test_batched_reduction_per_instance(gpu):
     out = _reduce(RED_SUMSQ, [inner]).cpu().numpy()
     for b in range(5):
         assert out[b] == pytest.approx(math.fsum((inner[b].cpu().numpy() ** 2).ravel()), rel=1e-14)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_interior_sum_mean_max_and_history(gpu, dtype):
+    """utils.py:63-153,222-262: interior sum / mean / max |x| (native FP64
+    reductions) against math.fsum, and the history buffers."""
+    from moljax_amd.core.utils import (allocate_scalar_history, allocate_state_history, create_padded_array,
+                                       max_abs_interior, mean_interior, save_to_history, set_interior,
+                                       sum_interior)
+    rng = np.random.default_rng(17)
+    g = _G(48, 80)
+    f = torch.from_numpy(rng.standard_normal((50, 82))).to("cuda", dtype)
+    x = _np(f)[1:-1, 1:-1]
+    assert sum_interior(f, g) == pytest.approx(math.fsum(x.ravel()), rel=1e-12, abs=1e-12)
+    assert mean_interior(f, g) == pytest.approx(math.fsum(x.ravel()) / x.size, rel=1e-12, abs=1e-14)
+    assert max_abs_interior(f, g) == float(np.abs(x).max())
+    batch = torch.stack([f, 2 * f])
+    assert max_abs_interior(batch, g) == 2 * float(np.abs(x).max())
+    assert sum_interior(batch, g) == pytest.approx(3 * math.fsum(x.ravel()), rel=1e-12, abs=1e-12)
+    g.ny_total, g.nx_total = 50, 82
+    p = create_padded_array(g, 0.5, dtype)
+    assert p.shape == (50, 82) and float(p.sum()) == 0.5 * 50 * 82
+    q = set_interior(p, torch.from_numpy(x).to("cuda", dtype), g)
+    assert torch.equal(q[1:-1, 1:-1], f[1:-1, 1:-1]) and float(q[0, 0]) == 0.5 and float(p[1, 1]) == 0.5
+    hist = allocate_state_history({"u": f, "v": f}, g, 4, dtype=dtype)
+    assert hist["u"].shape == (4, 48, 80) and allocate_scalar_history(4).shape == (4,)
+    save_to_history(hist, {"u": f, "v": 3 * f}, g, 2)
+    assert torch.equal(hist["u"][2], f[1:-1, 1:-1]) and torch.equal(hist["v"][2], 3 * f[1:-1, 1:-1])
+    assert float(hist["u"][1].abs().max()) == 0.0
