@@ -337,6 +337,31 @@ long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx);
 int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
                   int fill_ghosts, const void* coef, const double* inst, int n_steps, void* work, void* stream);
 
+/* The remaining two methods of the Gray-Scott benchmark, periodic with the
+ * pseudo-spectral symbol lam = lamx[kx] + lamy[ky] (device tables of
+ * dtype), n_steps in place on (u, v) of B instances (ny, nx powers of two in
+ * [32, 1024]); work: mjx_gs_bench_workspace_bytes(dtype, B, ny, nx).
+ *
+ * mjx_gs_rk4_spectral: classical RK4 of y' = D lap y + R(y)
+ *   (benchmarks/benchmark_gray_scott.py:89-127); inst rows carry
+ *   alpha_a = Du, alpha_b = Dv, F, k, dt.
+ * mjx_gs_cn_fixed_point: the benchmark's Crank-Nicolson step
+ *   (benchmark_gray_scott.py:162-229): from y1 = y0 iterate
+ *   r = y1 - y0 - dt/2 (D lap(y1 + y0) + R(y1) + R(y0)),
+ *   y1 += (I - dt/2 D lap)^-1 (-r) while max|r_u| + max|r_v| > tol and
+ *   fewer than max_iter iterations ran (at least one).  inst_lap rows:
+ *   alpha = D (beta 0); inst_pc rows: beta = dt/2 D (alpha 0); both carry
+ *   F, k, dt, and their aux columns are overwritten (per-instance iteration
+ *   masks).  iters_total (device int[B], may be NULL) accumulates the
+ *   iteration counts. */
+long long mjx_gs_bench_workspace_bytes(int dtype, int B, int ny, int nx);
+int mjx_gs_rk4_spectral(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                        const void* lamx, const void* lamy, const double* inst, int n_steps, void* work,
+                        void* stream);
+int mjx_gs_cn_fixed_point(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
+                          const void* lamx, const void* lamy, double* inst_lap, double* inst_pc, double tol,
+                          int max_iter, int n_steps, int* iters_total, void* work, void* stream);
+
 /* Exponential Euler / exponential Adams-Bashforth-2 tables for periodic
  * Gray-Scott: three real tables E = e^z, dt phi1(z), dt phi2(z) of ny*(nx+2)
  * elements each, z = dt * D_f * (lamx[kx] + lamy[ky]) in the separated

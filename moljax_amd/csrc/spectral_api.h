@@ -13,6 +13,8 @@ namespace mjx {
 enum { RXN_HEUN = 0, RXN_EULER = 1, RXN_HALF = 2, RXN_HALF2 = 3 };
 // Column-pass power taken per instance from the instance row's aux column.
 constexpr int NREP_PER_INSTANCE = -3;
+// Column-pass mask: skip instances whose row has aux <= 0.
+constexpr int MSTEP_ACTIVE = -2;
 
 #define MJX_SPEC_DECL(T)                                                                                     \
   int spec_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,   \

@@ -363,11 +363,14 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   // Per-instance step counts (instance row aux): NREP_PER_INSTANCE raises the
   // multiplier to that power; a masked chain step mstep >= 0 applies H^2
   // before the instance's last step, H at it, and leaves the instance
-  // untouched after it (the whole workgroup belongs to one instance).
+  // untouched after it; MSTEP_ACTIVE leaves instances with aux <= 0
+  // untouched (the whole workgroup belongs to one instance).
   int nrep = nrep_arg;
-  if (mstep >= 0 || nrep_arg == NREP_PER_INSTANCE) {
+  if (mstep >= 0 || nrep_arg == NREP_PER_INSTANCE || mstep == MSTEP_ACTIVE) {
     const int ni = (int)ip[inst].aux;
-    if (mstep >= 0) {
+    if (mstep == MSTEP_ACTIVE) {
+      if (ni <= 0) return;
+    } else if (mstep >= 0) {
       if (mstep >= ni) return;
       nrep = mstep == ni - 1 ? 1 : 2;
     } else {

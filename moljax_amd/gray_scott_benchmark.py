@@ -14,11 +14,17 @@ Served methods:
                  batch-plan scheme 2 replayed as a hipGraph);
   'ETDRK4'       the script's ETDRK4 (:336-417, contour coefficients with its
                  f2 as written), dt = 1 (mjx_gs_etdrk4 with
-                 mjx_etdrk4_coeffs_form(form 1), or batch-plan scheme 0).
+                 mjx_etdrk4_coeffs_form(form 1), or batch-plan scheme 0);
+  'RK4 (explicit)'  the script's pseudo-spectral RK4 (:89-127) at 90 % of
+                 its stability limit, dt = 0.9 * 2.8 / (max(Du, Dv) lam_max)
+                 (mjx_gs_rk4_spectral: three passes per stage);
+  'CN (Newton-Krylov)'  the script's preconditioned Crank-Nicolson iteration
+                 (:162-229, tol 1e-8 on max|r_u| + max|r_v|, at most 10
+                 iterations), dt = 0.5 (mjx_gs_cn_fixed_point: five passes per
+                 iteration, converged instances masked on the device).
 A single 256^2 instance is launch-bound (each kernel moves ~1-2 MB), so by
-default the steps are replayed from hipGraphs of ``steps_per_graph`` steps.
-The script's 'RK4 (explicit)' and 'CN (Newton-Krylov)' rows are not served
-here and are left out of ``methods``.
+default the steps are replayed from hipGraphs of ``steps_per_graph`` steps
+(batch plans for the first two, captured call sequences for RK4 and CN).
 
 Schema (benchmark_gray_scott.py:73,141-148,312-319,440-447,458-475):
   {"methods": [{"name", "time_s", "iqr_s", "steps", "dt", "t_final"}, ...],
@@ -40,7 +46,21 @@ from .core._arrays import inst_table, spectral_work
 N_DEFAULT = 256
 DU, DV, F, K = 2e-5, 1e-5, 0.035, 0.065
 T_FINAL = 10000.0
-METHOD_DT = {"IMEX-Strang": 0.5, "ETDRK4": 1.0}
+METHODS = ("RK4 (explicit)", "CN (Newton-Krylov)", "IMEX-Strang", "ETDRK4")
+CN_TOL, CN_MAX_ITER = 1e-8, 10
+
+
+def method_dt(name: str, N: int = N_DEFAULT) -> float:
+    """The step of each method as the script sets it (:86-88, :156, :264,
+    :326)."""
+    if name == "RK4 (explicit)":
+        lam = spectral_symbol(N)
+        lam_max = float(np.max(-(lam[:, None] + lam[None, :])))
+        return 0.9 * 2.8 / (max(DU, DV) * lam_max)
+    fixed = {"CN (Newton-Krylov)": 0.5, "IMEX-Strang": 0.5, "ETDRK4": 1.0}
+    if name not in fixed:
+        raise ValueError(f"unknown method {name!r}; served: {list(METHODS)}")
+    return fixed[name]
 
 METHOD_KEYS = {"name": str, "time_s": float, "iqr_s": float, "steps": int, "dt": float, "t_final": float}
 CONFIG_KEYS = {"N": int, "Du": float, "Dv": float, "F": float, "k": float, "t_final": float, "scaling_used": bool,
@@ -72,10 +92,13 @@ class GrayScottMethod:
     plan of one instance), 0 launches every kernel directly."""
 
     def __init__(self, name: str, N: int, dtype: torch.dtype, steps_per_graph: int = 100):
-        if name not in METHOD_DT:
-            raise ValueError(f"unknown or unserved method {name!r}; served: {sorted(METHOD_DT)}")
-        self.name, self.N, self.dtype, self.dt = name, N, dtype, METHOD_DT[name]
+        self.name, self.N, self.dtype, self.dt = name, N, dtype, method_dt(name, N)
         self.plan = None
+        self.graph, self.spg = None, int(steps_per_graph)
+        self.iters = None
+        if name in ("RK4 (explicit)", "CN (Newton-Krylov)"):
+            self._init_bench_kernel()
+            return
         if steps_per_graph > 0:
             from .batch import GrayScottBatch
             scheme = "benchmark_strang" if name == "IMEX-Strang" else "etdrk4"
@@ -98,13 +121,57 @@ class GrayScottMethod:
             nbytes = lib.mjx_etdrk4_workspace_bytes(_hip.dtype_code(dtype), 1, N, N)
             self.work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
 
+    def _init_bench_kernel(self):
+        N, dt = self.N, self.dt
+        self.u = torch.empty((N, N), dtype=self.dtype, device="cuda")
+        self.v = torch.empty_like(self.u)
+        self.lam = torch.from_numpy(spectral_symbol(N)).to("cuda", self.dtype)
+        nbytes = _hip.lib().mjx_gs_bench_workspace_bytes(_hip.dtype_code(self.dtype), 1, N, N)
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        if self.name == "RK4 (explicit)":
+            self.inst = inst_table([[0.0, 0.0, DU, DV, F, K, dt]], "cuda")
+        else:
+            self.inst = inst_table([[0.0, 0.0, DU, DV, F, K, dt, 1.0]], "cuda")
+            self.inst_pc = inst_table([[0.5 * dt * DU, 0.5 * dt * DV, 0.0, 0.0, F, K, dt, 1.0]], "cuda")
+            self.iters = torch.zeros(1, dtype=torch.int32, device="cuda")
+
     def load(self, u0: torch.Tensor, v0: torch.Tensor):
         self.u.copy_(u0)
         self.v.copy_(v0)
 
+    def _direct(self, n_steps: int) -> int:
+        lib, N = _hip.lib(), self.N
+        code = _hip.dtype_code(self.dtype)
+        if self.name == "RK4 (explicit)":
+            return lib.mjx_gs_rk4_spectral(code, 1, N, N, _hip.ptr(self.u), _hip.ptr(self.v), N, N * N,
+                                           _hip.ptr(self.lam), _hip.ptr(self.lam), _hip.ptr(self.inst),
+                                           int(n_steps), _hip.ptr(self.work), _hip.stream_ptr())
+        return lib.mjx_gs_cn_fixed_point(code, 1, N, N, _hip.ptr(self.u), _hip.ptr(self.v), N, N * N,
+                                         _hip.ptr(self.lam), _hip.ptr(self.lam), _hip.ptr(self.inst),
+                                         _hip.ptr(self.inst_pc), CN_TOL, CN_MAX_ITER, int(n_steps),
+                                         _hip.ptr(self.iters), _hip.ptr(self.work), _hip.stream_ptr())
+
     def advance(self, n_steps: int):
         if self.plan is not None:
             self.plan.run(n_steps)
+            return
+        if self.name in ("RK4 (explicit)", "CN (Newton-Krylov)"):
+            K = self.spg
+            if K > 0 and n_steps >= K and self.graph is None:
+                _hip.check(self._direct(1), self.name)  # first-call set-up outside the capture
+                n_steps -= 1
+                torch.cuda.synchronize()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    rc = self._direct(K)
+                _hip.check(rc, self.name)
+                self.graph = g
+            if self.graph is not None:
+                for _ in range(n_steps // K):
+                    self.graph.replay()
+                n_steps %= K
+            if n_steps:
+                _hip.check(self._direct(n_steps), self.name)
             return
         lib, N = _hip.lib(), self.N
         code = _hip.dtype_code(self.dtype)
@@ -120,7 +187,7 @@ class GrayScottMethod:
 
 
 def run_benchmark(t_final: float = T_FINAL, n_reps: int = 10, dtype: torch.dtype = torch.float64,
-                  methods: Iterable[str] = ("IMEX-Strang", "ETDRK4"), N: int = N_DEFAULT,
+                  methods: Iterable[str] = METHODS, N: int = N_DEFAULT,
                   warmup_steps: int = 10, progress=None, steps_per_graph: int = 100) -> Dict:
     """Time each method over ceil(t_final / dt) steps from the benchmark's
     initial condition (already resident on the device when the clock

@@ -29,7 +29,11 @@ The benchmark's own splitting (benchmark_gray_scott.py:268-289, reaction
 half-step / exact diffusion / reaction half-step) is restated as
 benchmark_strang; pinned by pure diffusion (F = k = 0, v = 0: e^{n dt D lam}
 exactly) and, on spatially constant fields, by two forward-Euler half-steps
-per step.
+per step.  Its RK4 (:89-127) and preconditioned Crank-Nicolson iteration
+(:162-229) are benchmark_rk4_step and benchmark_cn_step; pinned by the RK4
+and Crank-Nicolson amplification factors of each mode under pure diffusion
+(the CN iteration stops after its second update there) and by classical
+RK4 of the reaction ODE on constant fields.
 """
 import numpy as np
 
@@ -177,3 +181,48 @@ def benchmark_strang(u, v, lamx, lamy, Du, Dv, F, k, dt, n_steps):
     for _ in range(n_steps):
         u, v = benchmark_strang_step(u, v, lamx, lamy, Du, Dv, F, k, dt)
     return u, v
+
+
+def _bench_lap(f, lam):
+    return np.real(np.fft.ifft2(lam * np.fft.fft2(f)))
+
+
+def benchmark_rk4_step(u, v, lamx, lamy, Du, Dv, F, k, dt):
+    """benchmark_gray_scott.py:89-127: classical RK4 of y' = D lap y + R(y)
+    with the pseudo-spectral Laplacian."""
+    lam = lamy[:, None] + lamx[None, :]
+
+    def rhs(a, b):
+        abb = a * b * b
+        return (Du * _bench_lap(a, lam) - abb + F * (1 - a), Dv * _bench_lap(b, lam) + abb - (F + k) * b)
+
+    k1u, k1v = rhs(u, v)
+    k2u, k2v = rhs(u + 0.5 * dt * k1u, v + 0.5 * dt * k1v)
+    k3u, k3v = rhs(u + 0.5 * dt * k2u, v + 0.5 * dt * k2v)
+    k4u, k4v = rhs(u + dt * k3u, v + dt * k3v)
+    return (u + dt / 6 * (k1u + 2 * k2u + 2 * k3u + k4u), v + dt / 6 * (k1v + 2 * k2v + 2 * k3v + k4v))
+
+
+def benchmark_cn_step(u_old, v_old, lamx, lamy, Du, Dv, F, k, dt, tol=1e-8, max_iter=10):
+    """benchmark_gray_scott.py:162-229: the preconditioned Crank-Nicolson
+    iteration; returns (u, v, iterations)."""
+    lam = lamy[:, None] + lamx[None, :]
+    pu = 1.0 / (1.0 - 0.5 * dt * Du * lam)
+    pv = 1.0 / (1.0 - 0.5 * dt * Dv * lam)
+
+    def R(a, b):
+        abb = a * b * b
+        return -abb + F * (1 - a), abb - (F + k) * b
+
+    Ruo, Rvo = R(u_old, v_old)
+    u, v = u_old, v_old
+    res_norm, i = 1.0, 0
+    while res_norm > tol and i < max_iter:
+        Ru, Rv = R(u, v)
+        res_u = u - u_old - 0.5 * dt * (Du * (_bench_lap(u, lam) + _bench_lap(u_old, lam)) + Ru + Ruo)
+        res_v = v - v_old - 0.5 * dt * (Dv * (_bench_lap(v, lam) + _bench_lap(v_old, lam)) + Rv + Rvo)
+        u = u + np.real(np.fft.ifft2(pu * np.fft.fft2(-res_u)))
+        v = v + np.real(np.fft.ifft2(pv * np.fft.fft2(-res_v)))
+        res_norm = np.max(np.abs(res_u)) + np.max(np.abs(res_v))
+        i += 1
+    return u, v, i

@@ -1,8 +1,11 @@
 """The Gray-Scott benchmark's own schemes on the GPU vs the float64 oracle:
-its splitting (benchmark_gray_scott.py:268-289, mjx_gs_imex scheme 3) and its
-ETDRK4 with f2 as written (:336-417, mjx_etdrk4_coeffs_form 1), from its
-seed-42 initial condition on 256^2.  Tolerances (relative L2): float64
-1e-12, float32 1e-5.  The results dict follows the script's schema."""
+its splitting (benchmark_gray_scott.py:268-289, mjx_gs_imex scheme 3), its
+ETDRK4 with f2 as written (:336-417, mjx_etdrk4_coeffs_form 1), its
+pseudo-spectral RK4 (:89-127, mjx_gs_rk4_spectral) and its preconditioned
+Crank-Nicolson iteration (:162-229, mjx_gs_cn_fixed_point, iteration counts
+identical), from its seed-42 initial condition on 256^2.  Tolerances
+(relative L2): float64 1e-12, float32 1e-5.  The results dict follows the
+script's schema."""
 import math
 
 import numpy as np
@@ -74,16 +77,55 @@ def test_benchmark_strang_chunks_and_batch_rows(gpu):
     alone.close()
 
 
+@pytest.mark.parametrize("spg", [0, 3])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+def test_rk4_and_cn_match_oracle(gpu, dtype, tol, spg):
+    """RK4 (20 steps at the script's dt) and CN (3 steps of dt 0.5); in FP64
+    the CN iteration count of every step equals the oracle's.  spg 3 replays
+    captured 3-step sequences (twice, then one step directly)."""
+    from moljax_amd.gray_scott_benchmark import (DU, DV, F, K, GrayScottMethod, benchmark_ic, method_dt,
+                                                 spectral_symbol)
+    u0, v0 = benchmark_ic(256)
+    lam = spectral_symbol(256)
+    rk = GrayScottMethod("RK4 (explicit)", 256, dtype, steps_per_graph=spg)
+    rk.load(torch.from_numpy(u0).to("cuda", dtype), torch.from_numpy(v0).to("cuda", dtype))
+    rk.advance(7)
+    ru, rv = u0, v0
+    for _ in range(7):
+        ru, rv = oe.benchmark_rk4_step(ru, rv, lam, lam, DU, DV, F, K, method_dt("RK4 (explicit)"))
+    eu = ogs.rel_l2(rk.u.double().cpu().numpy(), ru)
+    ev = ogs.rel_l2(rk.v.double().cpu().numpy(), rv)
+    print(f"RK4 {dtype} graph {spg}: rel_l2 u={eu:.2e} v={ev:.2e}", flush=True)
+    assert eu <= tol and ev <= tol
+    cn = GrayScottMethod("CN (Newton-Krylov)", 256, dtype, steps_per_graph=spg)
+    cn.load(torch.from_numpy(u0).to("cuda", dtype), torch.from_numpy(v0).to("cuda", dtype))
+    cn.advance(7)
+    cu, cv, its = u0, v0, []
+    for _ in range(7):
+        cu, cv, it = oe.benchmark_cn_step(cu, cv, lam, lam, DU, DV, F, K, 0.5)
+        its.append(it)
+    eu = ogs.rel_l2(cn.u.double().cpu().numpy(), cu)
+    ev = ogs.rel_l2(cn.v.double().cpu().numpy(), cv)
+    got = int(cn.iters.item())
+    print(f"CN {dtype} graph {spg}: rel_l2 u={eu:.2e} v={ev:.2e} iterations {got} (oracle {sum(its)}: {its})",
+          flush=True)
+    assert eu <= tol and ev <= tol
+    if dtype == torch.float64:
+        assert got == sum(its)
+    else:
+        assert sum(its) <= got <= 10 * 7
+
+
 def test_run_benchmark_schema(gpu, tmp_path):
-    from moljax_amd.gray_scott_benchmark import load_results, run_benchmark, save_results
+    from moljax_amd.gray_scott_benchmark import METHODS, load_results, run_benchmark, save_results
     res = run_benchmark(t_final=5.25, n_reps=2, dtype=torch.float32, warmup_steps=2, steps_per_graph=3)
     p = save_results(res, tmp_path / "gray_scott.json")
     back = load_results(p)
     names = [m["name"] for m in back["methods"]]
-    assert names == ["IMEX-Strang", "ETDRK4"]
+    assert names == list(METHODS)
     for m in back["methods"]:
         assert m["steps"] == math.ceil(5.25 / m["dt"]) and m["t_final"] == m["steps"] * m["dt"] >= 5.25
         assert m["time_s"] > 0 and m["iqr_s"] >= 0
     assert back["config"]["dtype"] == "float32" and back["config"]["N"] == 256
     with pytest.raises(ValueError):
-        run_benchmark(t_final=1.0, n_reps=1, methods=("RK4 (explicit)",))
+        run_benchmark(t_final=1.0, n_reps=1, methods=("Heun",))

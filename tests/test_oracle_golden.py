@@ -423,6 +423,39 @@ def test_benchmark_strang_pure_diffusion_and_constant_fields():
     assert np.max(np.abs(u - a)) < 1e-14 and np.max(np.abs(v - b)) < 1e-14
 
 
+def test_benchmark_rk4_and_cn_closed_forms():
+    """benchmark_gray_scott.py:89-127,162-229 restated.  Pure diffusion
+    (F = k = 0, v = 0): RK4 multiplies each mode by 1 + z + z^2/2 + z^3/6 +
+    z^4/24 and the exactly preconditioned CN iteration reaches
+    (1 + z/2)/(1 - z/2) after one update (stopping after the second); on
+    constant fields RK4 is classical RK4 of the reaction ODE."""
+    from oracle import etdrk4 as oe
+    n, D, dt = 32, 2e-4, 0.3
+    lam = oe.spectral_symbol_1d(n)
+    L = lam[:, None] + lam[None, :]
+    z = dt * D * L
+    u0 = np.random.default_rng(12).standard_normal((n, n))
+    u, v = oe.benchmark_rk4_step(u0, np.zeros((n, n)), lam, lam, D, D, 0.0, 0.0, dt)
+    g = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert np.max(np.abs(u - np.real(np.fft.ifft2(g * np.fft.fft2(u0))))) < 1e-13 and np.max(np.abs(v)) == 0
+    u, v, it = oe.benchmark_cn_step(u0, np.zeros((n, n)), lam, lam, D, D, 0.0, 0.0, dt)
+    cn = np.real(np.fft.ifft2((1 + z / 2) / (1 - z / 2) * np.fft.fft2(u0)))
+    assert np.max(np.abs(u - cn)) < 1e-13 and it == 2
+    F, k, h = 0.035, 0.065, 0.4
+
+    def f(a, b):
+        return -a * b * b + F * (1 - a), a * b * b - (F + k) * b
+
+    a, b = 0.6, 0.3
+    k1 = f(a, b)
+    k2 = f(a + 0.5 * h * k1[0], b + 0.5 * h * k1[1])
+    k3 = f(a + 0.5 * h * k2[0], b + 0.5 * h * k2[1])
+    k4 = f(a + h * k3[0], b + h * k3[1])
+    ea = a + h / 6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
+    u, v = oe.benchmark_rk4_step(np.full((n, n), a), np.full((n, n), b), lam, lam, 2e-5, 1e-5, F, k, h)
+    assert np.max(np.abs(u - ea)) < 1e-14
+
+
 # --------------------------------------------------------- Newton-Krylov
 def test_hand_jvp_matches_golden_ad_norm():
     """benchmarks/results/jvp_vs_fd_sweep.json:3: AD-JVP norm of the BE
