@@ -11,21 +11,25 @@
 //   col     D lam                                                     (col)
 //   unpack  inverse x-FFT -> D lap y_s; k_s = D lap y_s + R(y_s);
 //           acc = k1, acc += 2 k2, acc += 2 k3, y += dt/6 (acc + k4)  (row)
+// Each unpack but the call's last also forms the next stage's input (the
+// next step's y after stage 4) and packs it, so a step is four row and four
+// column passes.
 //
-// CN, per step: y0 <- y1 (the first pack), then up to max_iter iterations
-//   pack    y1 + y0, x-FFT                                            (row)
+// CN, per step: pack 2 y1 and set y0 <- y1, then up to max_iter iterations
 //   col     D lam                                                     (col)
 //   res     inverse x-FFT -> D lap(y1 + y0);
 //           r = y1 - y0 - dt/2 (D lap(y1 + y0) + R(y1) + R(y0));
 //           per-tile max |r_u|, max |r_v|; x-FFT of -r               (row)
-//   col     1 / (1 - dt/2 D lam)                                      (col)
-//   update  inverse x-FFT -> d; y1 += d                               (row)
-//   norm    |r| = max |r_u| + max |r_v|; the instance stays active while
+//   norm    |r| = max |r_u| + max |r_v|; the next iteration runs while
 //           |r| > tol and fewer than max_iter iterations ran           (tiny)
-// Inactive instances are skipped by every later kernel of the step (their
-// instance rows' aux columns hold the mask), so a step is a fixed launch
-// sequence that a graph can hold, with the reference's data-dependent
-// iteration count per instance.
+//   col     1 / (1 - dt/2 D lam)                                      (col)
+//   update  inverse x-FFT -> d; y1 += d; when the next iteration runs,
+//           x-FFT of y1 + y0                                          (row)
+// Two masks per instance, in the aux columns of the instance rows: inst_lap
+// (this iteration's residual, then the next iteration's) and inst_pc (this
+// iteration's update).  Skipped instances leave every later kernel of the
+// step untouched, so a step is a fixed launch sequence that a graph can
+// hold, with the reference's data-dependent iteration count per instance.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -38,7 +42,7 @@
 namespace mjx {
 namespace {
 
-enum { BOP_RK_PACK = 0, BOP_RK_UNPACK = 1, BOP_CN_PACK = 2, BOP_CN_RES = 3, BOP_CN_UPD = 4 };
+enum { BOP_RK_PACK = 0, BOP_RK_UNPACK = 1, BOP_CN_PACK = 2, BOP_CN_RES = 3, BOP_CN_UPD = 4, BOP_RK_NEXT = 5 };
 
 template <typename T>
 struct BenchArgs {
@@ -48,6 +52,7 @@ struct BenchArgs {
   T *p0, *p1, *p2, *p3;  // RK4: ku, kv, acc_u, acc_v; CN: u0, v0 (dense ny x nx planes)
   cpx<T>* W;
   const InstParams* ip;  // F, k, dt; aux = active mask when masked
+  const InstParams* ip2;  // CN update: its aux is the update's mask
   double* partial;       // CN residual: (B, tiles, 2) max |r| per tile
   int ny, tiles, stage, masked;
 };
@@ -66,7 +71,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / a.tiles;
   const int tile = w - inst * a.tiles;
-  if (a.masked && a.ip[inst].aux <= 0.0) return;
+  if (a.masked && (OP == BOP_CN_UPD ? a.ip2 : a.ip)[inst].aux <= 0.0) return;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
   const int y = tile * C::TR + rl;
   cpx<T>* lrow = lds + rl * C::LP;
@@ -106,8 +111,9 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
   } else {
     load_unpack<T, NX, TR>(wrow, lrow, j, v);
     line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
-    if constexpr (OP == BOP_RK_UNPACK) {
+    if constexpr (OP == BOP_RK_UNPACK || OP == BOP_RK_NEXT) {
       const T w6 = (T)(a.ip[inst].dt / 6.0);
+      const T cn = a.stage == 3 ? dt : hdt;  // the next stage's offset (stage 4 -> next step's y)
 #pragma unroll
       for (int m = 0; m < C::P; ++m) {
         const int x = j + C::TN * m;
@@ -127,23 +133,31 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
           q2[x] = q2[x] + (T)2 * ku;
           q3[x] = q3[x] + (T)2 * kv;
         } else {
-          su[x] = yu + w6 * (q2[x] + ku);
-          sv[x] = yv + w6 * (q3[x] + kv);
+          const T nu = yu + w6 * (q2[x] + ku), nv = yv + w6 * (q3[x] + kv);
+          su[x] = nu;
+          sv[x] = nv;
+          if constexpr (OP == BOP_RK_NEXT) v[m] = mk(nu, nv);
         }
         if (a.stage < 4) {
           q0[x] = ku;
           q1[x] = kv;
+          if constexpr (OP == BOP_RK_NEXT) v[m] = mk(yu + cn * ku, yv + cn * kv);
         }
       }
-      return;
+      if constexpr (OP == BOP_RK_UNPACK) return;
+      line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
     } else if constexpr (OP == BOP_CN_UPD) {
+      const bool next = a.ip[inst].aux > 0.0;
 #pragma unroll
       for (int m = 0; m < C::P; ++m) {
         const int x = j + C::TN * m;
-        su[x] = su[x] + v[m].re;
-        sv[x] = sv[x] + v[m].im;
+        const T nu = su[x] + v[m].re, nv = sv[x] + v[m].im;
+        su[x] = nu;
+        sv[x] = nv;
+        if (next) v[m] = mk(nu + q0[x], nv + q1[x]);
       }
-      return;
+      if (!next) return;
+      line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
     } else {  // BOP_CN_RES
       T mu = (T)0, mv = (T)0;
 #pragma unroll
@@ -207,10 +221,16 @@ __global__ void k_cn_begin(InstParams* lap, InstParams* pc, int* iters, int B) {
 
 // |r| = max |r_u| + max |r_v| over the instance's tiles (NaN propagates, and
 // a NaN norm ends the iteration as the reference's `res_norm > tol` does).
+// An instance that ran this iteration keeps its update (pc mask) and runs
+// the next one while |r| > tol (lap mask); one that did not run neither.
 __global__ void k_cn_norm(InstParams* lap, InstParams* pc, const double* partial, int tiles, int* iters,
                           int* iters_total, double tol, int max_iter, int B) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B || lap[b].aux <= 0.0) return;
+  if (b >= B) return;
+  if (lap[b].aux <= 0.0) {
+    pc[b].aux = 0.0;
+    return;
+  }
   double mu = 0.0, mv = 0.0;
   for (int t = 0; t < tiles; ++t) {
     mu = nanmax_t(mu, partial[2 * ((long long)b * tiles + t)]);
@@ -220,10 +240,7 @@ __global__ void k_cn_norm(InstParams* lap, InstParams* pc, const double* partial
   const int it = iters[b] + 1;
   iters[b] = it;
   if (iters_total) iters_total[b] += 1;
-  if (!(norm > tol) || it >= max_iter) {
-    lap[b].aux = 0.0;
-    pc[b].aux = 0.0;
-  }
+  if (!(norm > tol) || it >= max_iter) lap[b].aux = 0.0;
 }
 
 template <typename T, int N, int OP>
@@ -293,15 +310,16 @@ template <typename T>
 int rk4_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, const T* lamx, const T* lamy,
             const InstParams* ip, int n_steps, void* work, hipStream_t st) {
   const BenchWork w = bench_planes(work, sizeof(T), B, ny, nx);
-  BenchArgs<T> a{u, v, pitch, istride, (T*)w.p0, (T*)w.p1, (T*)w.p2, (T*)w.p3, (cpx<T>*)w.W, ip, nullptr,
+  BenchArgs<T> a{u, v, pitch, istride, (T*)w.p0, (T*)w.p1, (T*)w.p2, (T*)w.p3, (cpx<T>*)w.W, ip, nullptr, nullptr,
                  ny, 0, 1, 0};
-  int rc = MJX_OK;
+  a.stage = 1;
+  int rc = bench_row<T, BOP_RK_PACK>(nx, B, a, st);
   for (int s = 0; s < n_steps && !rc; ++s)
     for (int stage = 1; stage <= 4 && !rc; ++stage) {
       a.stage = stage;
-      rc = bench_row<T, BOP_RK_PACK>(nx, B, a, st);
-      if (!rc) rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, ip, -1, st);
-      if (!rc) rc = bench_row<T, BOP_RK_UNPACK>(nx, B, a, st);
+      rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, ip, -1, st);
+      const bool last = s == n_steps - 1 && stage == 4;
+      if (!rc) rc = last ? bench_row<T, BOP_RK_UNPACK>(nx, B, a, st) : bench_row<T, BOP_RK_NEXT>(nx, B, a, st);
     }
   return rc;
 }
@@ -311,25 +329,25 @@ int cn_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride
            InstParams* lap, InstParams* pc, double tol, int max_iter, int n_steps, int* iters_total, void* work,
            hipStream_t st) {
   const BenchWork w = bench_planes(work, sizeof(T), B, ny, nx);
-  BenchArgs<T> a{u, v, pitch, istride, (T*)w.p0, (T*)w.p1, nullptr, nullptr, (cpx<T>*)w.W, lap, w.partial,
+  BenchArgs<T> a{u, v, pitch, istride, (T*)w.p0, (T*)w.p1, nullptr, nullptr, (cpx<T>*)w.W, lap, pc, w.partial,
                  ny, 0, 0, 1};
   const int tiles = bench_tiles<T>(nx, ny);
   int rc = MJX_OK;
   for (int s = 0; s < n_steps && !rc; ++s) {
     hipLaunchKernelGGL(k_cn_begin, dim3((B + 63) / 64), dim3(64), 0, st, lap, pc, w.iters, B);
     rc = check_launch("cn_begin");
+    a.stage = 1;
+    if (!rc) rc = bench_row<T, BOP_CN_PACK>(nx, B, a, st);
     for (int it = 0; it < max_iter && !rc; ++it) {
-      a.stage = it == 0;
-      rc = bench_row<T, BOP_CN_PACK>(nx, B, a, st);
-      if (!rc) rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, lap, -1, st, MSTEP_ACTIVE);
+      rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, lap, -1, st, MSTEP_ACTIVE);
       if (!rc) rc = bench_row<T, BOP_CN_RES>(nx, B, a, st);
-      if (!rc) rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, pc, 1, st, MSTEP_ACTIVE);
-      if (!rc) rc = bench_row<T, BOP_CN_UPD>(nx, B, a, st);
       if (!rc) {
         hipLaunchKernelGGL(k_cn_norm, dim3((B + 63) / 64), dim3(64), 0, st, lap, pc, w.partial, tiles, w.iters,
                            iters_total, tol, max_iter, B);
         rc = check_launch("cn_norm");
       }
+      if (!rc) rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, pc, 1, st, MSTEP_ACTIVE);
+      if (!rc) rc = bench_row<T, BOP_CN_UPD>(nx, B, a, st);
     }
   }
   return rc;

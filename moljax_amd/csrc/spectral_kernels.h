@@ -385,8 +385,13 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   cpx<T> v[C::P];
 #pragma unroll
   for (int m = 0; m < C::P; ++m) v[m] = col[(long long)(j + C::TN * m) * nx];
+  // Every column of a tile belongs to one field (half is a multiple of TC).
+  // The exponential multiplier e^{beta (lx + ly)} = e^{beta lx} e^{beta ly}
+  // stages e^{beta ly} in LDS (NY exponentials per workgroup, not per thread).
+  const bool expmode = NREP == 0 && !LAM2D && nrep == -2;
   if constexpr (!LAM2D) {
-    for (int i = threadIdx.x; i < NY; i += C::BT) lds_lamy[i] = lamy[i];
+    const T beta_w = (T)(p0 >= half ? ip[inst].beta_b : ip[inst].beta_a);
+    for (int i = threadIdx.x; i < NY; i += C::BT) lds_lamy[i] = expmode ? exp(beta_w * lamy[i]) : lamy[i];
   }
   TwCache<T, NY> twc;
   load_twiddles(twc, j, tw);
@@ -398,6 +403,12 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   const T beta = (T)(fb ? ip[inst].beta_b : ip[inst].beta_a);
   auto multiply_plain = [&]() {
     const T lx = LAM2D ? (T)0 : lamx[kx];
+    if (expmode) {
+      const T ex = exp(beta * lx) * s;
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) v[m] = scale(v[m], ex * lds_lamy[j + C::TN * m]);
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < C::P; ++m) {
       const int ky = j + C::TN * m;
@@ -420,7 +431,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
         const int ky = j + C::TN * m;
         const int kym = (NY - ky) & (NY - 1);
         const cpx<T> zm = conjg(lds[kym * TC + c]);
-        T la, lb;
+        T la, lb, ma, mb;
         if constexpr (LAM2D) {
           la = lam2d[(long long)ky * nx + 0];
           lb = lam2d[(long long)ky * nx + half];
@@ -428,8 +439,13 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
           la = lamx[0] + lds_lamy[ky];
           lb = lamx[half] + lds_lamy[ky];
         }
-        const T ma = spec_mult<T, NREP>(la, alpha, beta, nrep, s);
-        const T mb = spec_mult<T, NREP>(lb, alpha, beta, nrep, s);
+        if (expmode) {
+          ma = exp(beta * lamx[0]) * s * lds_lamy[ky];
+          mb = exp(beta * lamx[half]) * s * lds_lamy[ky];
+        } else {
+          ma = spec_mult<T, NREP>(la, alpha, beta, nrep, s);
+          mb = spec_mult<T, NREP>(lb, alpha, beta, nrep, s);
+        }
         const T cp = (T)0.5 * (ma + mb), cm = (T)0.5 * (ma - mb);
         v[m] = mk(cp * v[m].re + cm * zm.re, cp * v[m].im + cm * zm.im);
       }

@@ -122,7 +122,9 @@ class GrayScottMethod:
             self.work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
 
     def _init_bench_kernel(self):
+        from .batch import SCHEME_STRANG, autotune
         N, dt = self.N, self.dt
+        autotune(self.dtype, SCHEME_STRANG, N, 1)  # row / column tiles for one instance
         self.u = torch.empty((N, N), dtype=self.dtype, device="cuda")
         self.v = torch.empty_like(self.u)
         self.lam = torch.from_numpy(spectral_symbol(N)).to("cuda", self.dtype)
