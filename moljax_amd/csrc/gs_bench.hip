@@ -364,7 +364,7 @@ int check_common(int dtype, int B, int ny, int nx, const void* u, const void* v,
   if (B < 1 || !u || !v || !lamx || !lamy || !work) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
   if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
-  return ensure_tables(nullptr);
+  return MJX_OK;
 }
 
 }  // namespace
@@ -387,6 +387,7 @@ int mjx_gs_rk4_spectral(int dtype, int B, int ny, int nx, void* u, void* v, long
   if (rc) return rc;
   if (!inst) return fail(MJX_E_ARG, "null inst");
   if (n_steps == 0) return MJX_OK;
+  if ((rc = ensure_tables(nullptr))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
   if (dtype == MJX_F32)
@@ -404,6 +405,7 @@ int mjx_gs_cn_fixed_point(int dtype, int B, int ny, int nx, void* u, void* v, lo
   if (!inst_lap || !inst_pc) return fail(MJX_E_ARG, "null inst tables");
   if (max_iter < 1 || max_iter > 1000) return fail(MJX_E_ARG, "max_iter must be in [1, 1000]");
   if (n_steps == 0) return MJX_OK;
+  if ((rc = ensure_tables(nullptr))) return rc;
   hipStream_t st = (hipStream_t)stream;
   InstParams* lap = reinterpret_cast<InstParams*>(inst_lap);
   InstParams* pc = reinterpret_cast<InstParams*>(inst_pc);

@@ -38,7 +38,7 @@ static int check_common(int dtype, int kind) {
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
   if (kind != R2R_DST && kind != R2R_DCT && kind != KIND_DST2)
     return fail(MJX_E_ARG, "kind must be 0 (vertex Dirichlet / DST-I), 1 (Neumann / DCT-II) or 2 (cell Dirichlet / DST-II)");
-  return ensure_tables(nullptr);
+  return MJX_OK;
 }
 
 static const char* size_rule(int kind) {
@@ -85,6 +85,7 @@ static int solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs, 
   if (in_pitch < nx || out_pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
   if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
     return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
+  if ((rc = ensure_tables(nullptr))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const long long plane = (long long)ny * nx;
   // rows: forward transform along x into the work array
@@ -161,6 +162,7 @@ int mjx_r2r_solve1d_bc(int dtype, int kind, int nlines, int n, const void* in, l
   if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
   if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
+  if ((rc = ensure_tables(nullptr))) return rc;
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.lam_line = lam;
   a.inst = inst;
@@ -179,6 +181,7 @@ int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
     return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
+  if ((rc = ensure_tables(nullptr))) return rc;
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.lam_line = lam;
   a.inst = inst;
@@ -194,6 +197,7 @@ int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, con
   if (L < 0) return fail(MJX_E_SIZE, "unsupported transform length " + std::to_string(n));
   if (nlines < 1 || !in || !out) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
+  if ((rc = ensure_tables(nullptr))) return rc;
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.scale = scale;
   const int mode = (kind != R2R_DST && inverse) ? R2R_INV : R2R_FWD;
