@@ -31,11 +31,26 @@ def laplacian_symbol_2d(ny, nx, dy, dx):
     return lam_x + lam_y
 
 
+_FFT_WORKERS = [None]
+
+
+def set_fft_workers(workers=None):
+    """None: numpy.fft complex transforms, the reference's dataflow (the
+    default).  An int: scipy.fft real-to-complex transforms with that many
+    threads, the same arithmetic up to rounding (a faster CPU baseline)."""
+    _FFT_WORKERS[0] = workers
+
+
 def solve_helmholtz_2d(rhs, lam, dt, D):
     """fft_solvers.py:261-294: full complex fft2, clamped divide, ifft2.real."""
     denom = 1.0 - dt * D * lam
     denom = np.where(np.abs(denom) < 1e-14, 1e-14, denom)
-    return np.fft.ifft2(np.fft.fft2(rhs) / denom).real
+    w = _FFT_WORKERS[0]
+    if w is None:
+        return np.fft.ifft2(np.fft.fft2(rhs) / denom).real
+    import scipy.fft as sf
+    h = rhs.shape[-1] // 2 + 1
+    return sf.irfft2(sf.rfft2(rhs, workers=w) / denom[..., :h], s=rhs.shape[-2:], workers=w)
 
 
 def solve_helmholtz_1d(rhs, lam, dt, D):

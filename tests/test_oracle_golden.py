@@ -456,6 +456,21 @@ def test_benchmark_rk4_and_cn_closed_forms():
     assert np.max(np.abs(u - ea)) < 1e-14
 
 
+def test_cpu_baseline_fft_variant_matches():
+    """The scipy.fft R2C variant of the oracle's Helmholtz solve (a CPU
+    baseline option) equals the numpy.fft complex dataflow to rounding."""
+    from oracle import spectral as osp
+    rhs = np.random.default_rng(4).standard_normal((32, 64))
+    lam = osp.laplacian_symbol_2d(32, 64, 0.1, 0.05)
+    a = osp.solve_helmholtz_2d(rhs, lam, 0.5, 0.02)
+    osp.set_fft_workers(2)
+    try:
+        b = osp.solve_helmholtz_2d(rhs, lam, 0.5, 0.02)
+    finally:
+        osp.set_fft_workers(None)
+    assert np.max(np.abs(a - b)) < 1e-14
+
+
 # --------------------------------------------------------- Newton-Krylov
 def test_hand_jvp_matches_golden_ad_norm():
     """benchmarks/results/jvp_vs_fd_sweep.json:3: AD-JVP norm of the BE
