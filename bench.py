@@ -282,7 +282,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-curve", action="store_true")
-    ap.add_argument("--sweep", type=str, default="", help="comma-separated batch sizes to time as well")
+    ap.add_argument("--sweep", type=str, default="1,8,64,2048",
+                    help="comma-separated batch sizes to time as well ('' to skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (byte-counter calibration)")
     ap.add_argument("--c4-steps", type=int, default=100, help="steps of the uneven batched ETDRK4 workload (0: skip)")
