@@ -223,19 +223,28 @@ __global__ void k_cn_begin(InstParams* lap, InstParams* pc, int* iters, int B) {
 // a NaN norm ends the iteration as the reference's `res_norm > tol` does).
 // An instance that ran this iteration keeps its update (pc mask) and runs
 // the next one while |r| > tol (lap mask); one that did not run neither.
-__global__ void k_cn_norm(InstParams* lap, InstParams* pc, const double* partial, int tiles, int* iters,
-                          int* iters_total, double tol, int max_iter, int B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per instance: each lane folds a strided share of the tiles, then
+// the lanes combine by shuffles (the maximum does not depend on the order).
+__global__ __launch_bounds__(64) void k_cn_norm(InstParams* lap, InstParams* pc, const double* partial, int tiles,
+                                                int* iters, int* iters_total, double tol, int max_iter, int B) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
   if (b >= B) return;
   if (lap[b].aux <= 0.0) {
-    pc[b].aux = 0.0;
+    if (lane == 0) pc[b].aux = 0.0;
     return;
   }
   double mu = 0.0, mv = 0.0;
-  for (int t = 0; t < tiles; ++t) {
+  for (int t = lane; t < tiles; t += 64) {
     mu = nanmax_t(mu, partial[2 * ((long long)b * tiles + t)]);
     mv = nanmax_t(mv, partial[2 * ((long long)b * tiles + t) + 1]);
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mu = nanmax_t(mu, __shfl_xor(mu, off));
+    mv = nanmax_t(mv, __shfl_xor(mv, off));
+  }
+  if (lane != 0) return;
   const double norm = mu + mv;
   const int it = iters[b] + 1;
   iters[b] = it;
@@ -342,8 +351,8 @@ int cn_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride
       rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, lap, -1, st, MSTEP_ACTIVE);
       if (!rc) rc = bench_row<T, BOP_CN_RES>(nx, B, a, st);
       if (!rc) {
-        hipLaunchKernelGGL(k_cn_norm, dim3((B + 63) / 64), dim3(64), 0, st, lap, pc, w.partial, tiles, w.iters,
-                           iters_total, tol, max_iter, B);
+        hipLaunchKernelGGL(k_cn_norm, dim3(B), dim3(64), 0, st, lap, pc, w.partial, tiles, w.iters, iters_total,
+                           tol, max_iter, B);
         rc = check_launch("cn_norm");
       }
       if (!rc) rc = spec_col_apply(ny, B, nx, (cpx<T>*)w.W, lamx, lamy, nullptr, pc, 1, st, MSTEP_ACTIVE);

@@ -92,9 +92,11 @@ __device__ __forceinline__ T recip(T d) {
 // s * ((1 + alpha*lam) / (1 - beta*lam))^nrep with |1 - beta*lam| clamped
 // below at 1e-14 (moljax/core/fft_solvers.py:280-283).  NREP > 0 fixes the
 // power at compile time; NREP == 0 takes the runtime nrep, where nrep = -1
-// selects s * alpha * lam and nrep = -2 selects s * exp(beta * lam).
+// selects s * alpha * lam and nrep = -2 selects s * exp(beta * lam);
+// NREP == -1 is the s * alpha * lam form fixed at compile time.
 template <typename T, int NREP>
 __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
+  if constexpr (NREP == -1) return alpha * lam * s;  // the matvec form, without the runtime dispatch
   T den = (T)1 - beta * lam;
   if (fabs(den) < (T)1e-14) den = (T)1e-14;
   const T num = (T)1 + alpha * lam;
@@ -531,6 +533,7 @@ struct ColLaunch {
     } else {
       if (fixed == 1) go<false, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
       else if (fixed == 2) go<false, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else if (fixed == -1) go<false, -1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
       else go<false, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
     }
     return check_launch("col_apply");
