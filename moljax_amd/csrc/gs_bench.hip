@@ -109,6 +109,18 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
     }
     line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
   } else {
+    // The pointwise operands, loaded before the transform so that their
+    // latency overlaps the row's LDS exchanges.
+    T yu_[C::P], yv_[C::P], a0_[C::P], a1_[C::P];
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int x = j + C::TN * m;
+      yu_[m] = su[x];
+      yv_[m] = sv[x];
+      const bool need = OP == BOP_CN_RES || OP == BOP_CN_UPD || a.stage != 1;
+      a0_[m] = need ? q0[x] : (T)0;
+      a1_[m] = need ? q1[x] : (T)0;
+    }
     load_unpack<T, NX, TR>(wrow, lrow, j, v);
     line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
     if constexpr (OP == BOP_RK_UNPACK || OP == BOP_RK_NEXT) {
@@ -117,11 +129,11 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
 #pragma unroll
       for (int m = 0; m < C::P; ++m) {
         const int x = j + C::TN * m;
-        const T yu = su[x], yv = sv[x];
+        const T yu = yu_[m], yv = yv_[m];
         T xu = yu, xv = yv;
         if (a.stage != 1) {
-          xu = yu + c * q0[x];
-          xv = yv + c * q1[x];
+          xu = yu + c * a0_[m];
+          xv = yv + c * a1_[m];
         }
         T ru, rv;
         gs_react(xu, xv, F, kk, ru, rv);
@@ -151,10 +163,10 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
 #pragma unroll
       for (int m = 0; m < C::P; ++m) {
         const int x = j + C::TN * m;
-        const T nu = su[x] + v[m].re, nv = sv[x] + v[m].im;
+        const T nu = yu_[m] + v[m].re, nv = yv_[m] + v[m].im;
         su[x] = nu;
         sv[x] = nv;
-        if (next) v[m] = mk(nu + q0[x], nv + q1[x]);
+        if (next) v[m] = mk(nu + a0_[m], nv + a1_[m]);
       }
       if (!next) return;
       line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
@@ -162,8 +174,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
       T mu = (T)0, mv = (T)0;
 #pragma unroll
       for (int m = 0; m < C::P; ++m) {
-        const int x = j + C::TN * m;
-        const T u1 = su[x], v1 = sv[x], u0 = q0[x], v0 = q1[x];
+        const T u1 = yu_[m], v1 = yv_[m], u0 = a0_[m], v0 = a1_[m];
         T r1u, r1v, r0u, r0v;
         gs_react(u1, v1, F, kk, r1u, r1v);
         gs_react(u0, v0, F, kk, r0u, r0v);
