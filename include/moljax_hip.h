@@ -298,6 +298,17 @@ int mjx_r2r_solve1d_bc(int dtype, int kind, int nlines, int n, const void* in, l
                        long long out_stride, const void* lam, const double* inst, int mmode,
                        const double* side_values, double dx, void* stream);
 
+/* Exponential Euler for diffusion on non-periodic lines (nlines lines of n
+ * points, u and N read with in_stride, out written with out_stride):
+ *   out = T^-1[ e^{dt eig} T u + dt phi1(dt eig) T N ],
+ * phi1(z) = (e^z - 1)/z, or 1 + z/2 + z^2/6 for |z| < 1e-4; eig: the
+ * diffusion eigenvalues D lam indexed like the transform coefficients; kind
+ * 0 DST-I (vertex Dirichlet), 1 orthonormal DCT-II (Neumann), 2 DST-II (cell
+ * Dirichlet).  Replaces moljax/core/fft_nonperiodic.py:363-437
+ * (etd1_dirichlet, etd1_neumann); out must not alias u or N. */
+int mjx_r2r_etd1(int dtype, int kind, int nlines, int n, const void* u, const void* nonlin, long long in_stride,
+                 void* out, long long out_stride, const void* eig, double dt, void* stream);
+
 /* Raw 1-D transforms of nlines lines, times `scale`:  kind 0: DST-I
  * X_k = sum_n x_n sin(pi (k+1)(n+1)/(n_len+1)) (its own inverse up to
  * 2/(n_len+1));  kind 1: unnormalised DCT-II (inverse = 0) or DCT-III

@@ -319,6 +319,46 @@ def solve_poisson_bc(rhs, grid, spec, t: float = 0.0, params=None) -> torch.Tens
     return _solve(kind, rhs, sym, 0.0, POISSON, sides, grid.dy if hasattr(grid, "dy") else grid.dx, grid.dx)
 
 
+def _etd1(kind: int, u, N_u, eigenvalues, dt: float) -> torch.Tensor:
+    tu = as_device(u)
+    tu = tu.to(field_dtype(tu))
+    tn = as_device(N_u, tu.dtype)
+    if tn.shape != tu.shape:
+        raise ValueError("u and N_u must have the same shape")
+    n = tu.shape[-1]
+    eig = as_device(eigenvalues, tu.dtype).reshape(-1).contiguous()
+    if eig.numel() != n:
+        raise ValueError("eigenvalues length does not match the line length")
+    lu, ls = _lines(tu)
+    ln, ns = _lines(tn)
+    if ns != ls:
+        ln, ns = ln.contiguous(), n
+        lu, ls = lu.contiguous(), n
+    out = torch.empty(lu.shape, dtype=tu.dtype, device=tu.device)
+    rc = _hip.lib().mjx_r2r_etd1(_hip.dtype_code(tu.dtype), kind, lu.shape[0], n, _hip.ptr(lu), _hip.ptr(ln), ls,
+                                 _hip.ptr(out), out.stride(0), _hip.ptr(eig), float(dt), _hip.stream_ptr())
+    _hip.check(rc, "mjx_r2r_etd1")
+    return out.view(tu.shape)
+
+
+def etd1_dirichlet(u, N_u, eigenvalues, dt: float) -> torch.Tensor:
+    """fft_nonperiodic.py:363-398: u <- DST^-1[e^z DST u + dt phi1(z) DST N],
+    z = dt * eigenvalues (the diffusion eigenvalues D lam of the vertex
+    Dirichlet grid), along the last axis."""
+    return _etd1(R2R_DST, u, N_u, eigenvalues, dt)
+
+
+def etd1_neumann(u, N_u, eigenvalues, dt: float) -> torch.Tensor:
+    """fft_nonperiodic.py:401-437, with the orthonormal DCT-II."""
+    return _etd1(R2R_DCT, u, N_u, eigenvalues, dt)
+
+
+def etd1_dirichlet_cc(u, N_u, eigenvalues, dt: float) -> torch.Tensor:
+    """The same step on a cell-centred Dirichlet grid (bc.py's ghost fill),
+    with the DST-II."""
+    return _etd1(R2R_DST2, u, N_u, eigenvalues, dt)
+
+
 def check_compatibility_neumann(rhs, tol: float = 1e-10) -> bool:
     """fft_nonperiodic.py:440-450."""
     return abs(float(as_device(rhs).double().sum())) < tol

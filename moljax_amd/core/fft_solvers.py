@@ -196,13 +196,46 @@ def solve_helmholtz_2d(rhs_interior, laplacian_symbol, dt: float, D: float) -> t
     return out.view(rhs.shape)
 
 
+_LINE_SYMBOLS: Dict[tuple, torch.Tensor] = {}
+
+
+def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
+    """fft_solvers.py:222-258: (I - dt D lap) u = rhs on periodic lines of nx
+    points (a batch along the leading axes).  The lines are stacked as the
+    rows of a 2-D array whose y symbol is zero, so the 2-D kernels apply the
+    1-D multiplier row by row; stacks are padded with zero rows to a power of
+    two of at least 16 rows."""
+    t = as_device(rhs_interior)
+    t = t.to(field_dtype(t))
+    nx = t.shape[-1]
+    lam = as_device(laplacian_symbol, torch.float64).reshape(-1)
+    if lam.numel() != nx:
+        raise ValueError(f"symbol length {lam.numel()} does not match the line length {nx}")
+    lines = t.reshape(-1, nx)
+    lam_np = lam.cpu().numpy()
+    out = torch.empty_like(lines)
+    for i0 in range(0, lines.shape[0], 4096):
+        chunk = lines[i0:i0 + 4096]
+        L = chunk.shape[0]
+        rows = max(16, 1 << (L - 1).bit_length())
+        key = (lam_np.tobytes(), rows)
+        lam2d = _LINE_SYMBOLS.get(key)
+        if lam2d is None:
+            lam2d = _tag_separable(torch.from_numpy(np.broadcast_to(lam_np, (rows, nx)).copy()).to(default_device()),
+                                   lam_np, np.zeros(rows))
+            _LINE_SYMBOLS[key] = lam2d
+        buf = torch.zeros((rows, nx), dtype=t.dtype, device=t.device)
+        buf[:L] = chunk
+        out[i0:i0 + L] = solve_helmholtz_2d(buf, lam2d, dt, D)[:L]
+    return out.reshape(t.shape)
+
+
 def solve_helmholtz(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
-    """fft_solvers.py:297-320 (2-D arrays; 1-D periodic solves are not on the
-    accelerated path)."""
-    rhs = as_device(rhs_interior)
-    if rhs.dim() == 1:
-        raise NotImplementedError("1-D periodic Helmholtz is not on the MI355X path; pass a 2-D field")
-    return solve_helmholtz_2d(rhs, laplacian_symbol, dt, D)
+    """fft_solvers.py:297-320: 1-D or 2-D by the symbol's rank."""
+    lam = laplacian_symbol
+    if getattr(lam, "dim", lambda: np.ndim(lam))() == 1:
+        return solve_helmholtz_1d(rhs_interior, lam, dt, D)
+    return solve_helmholtz_2d(rhs_interior, lam, dt, D)
 
 
 def extract_interior(f_padded: torch.Tensor, grid: GridType) -> torch.Tensor:

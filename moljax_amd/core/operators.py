@@ -84,6 +84,62 @@ def _fd(f, grid: Grid2D, op: int, axis: int = 1, vel=None) -> torch.Tensor:
     return out.view(t.shape)
 
 
+def _fd1(f, grid, op: int, vel=None) -> torch.Tensor:
+    """1-D padded fields (..., nx + 2) through the 2-D kernel as single rows:
+    x-direction stencils read only their own row, and the kernel's zeroed
+    ghost rows land in a scratch row above and below."""
+    if grid.n_ghost != 1:
+        raise ValueError("the FD kernels serve n_ghost = 1 grids")
+    t = as_device(f)
+    t = t.to(field_dtype(t))
+    W = grid.nx + 2
+    if t.shape[-1] != W:
+        raise ValueError(f"expected padded shape (..., {W}), got {tuple(t.shape)}")
+    lead = t.shape[:-1]
+    src = t.reshape(-1, W).contiguous()
+    B = src.shape[0]
+    out = torch.empty((B, 3, W), dtype=t.dtype, device=t.device)
+    vel_t, vp, vs, vsc = None, 0, 0, 0.0
+    if vel is not None:
+        if isinstance(vel, (int, float)) or (isinstance(vel, torch.Tensor) and vel.dim() == 0) \
+                or (isinstance(vel, np.ndarray) and vel.ndim == 0):
+            vsc = float(vel)
+        else:
+            vt = as_device(vel, t.dtype)
+            if vt.shape[-1] != W:
+                raise ValueError("velocity array must have the padded field shape")
+            vel_t = vt.expand(t.shape).reshape(-1, W).contiguous()
+            vp, vs = W, W
+    rc = _hip.lib().mjx_fd_op2d(_hip.dtype_code(t.dtype), op, B, 1, grid.nx, _hip.ptr(src[:, 1:]), W, W,
+                                _hip.ptr(out[:, 1, 1:]), W, 3 * W, 1, float(grid.dx), 1.0,
+                                _hip.ptr(vel_t[:, 1:]) if vel_t is not None else None, vp, vs, vsc,
+                                _hip.stream_ptr())
+    _hip.check(rc, "mjx_fd_op2d")
+    return out[:, 1, :].reshape(lead + (W,))
+
+
+def d1_central_1d(f, grid) -> torch.Tensor:
+    """operators.py:34-59: (f[i+1] - f[i-1]) / (2 dx) on the interior, zero
+    ghosts."""
+    return _fd1(f, grid, FD_D1C)
+
+
+def d1_upwind_1d(f, v, grid) -> torch.Tensor:
+    """operators.py:62-111: backward difference where v >= 0, forward where
+    v < 0 (v scalar or a padded array)."""
+    return _fd1(f, grid, FD_D1UP, v)
+
+
+def d2_central_1d(f, grid) -> torch.Tensor:
+    """operators.py:114-138."""
+    return _fd1(f, grid, FD_D2)
+
+
+def laplacian_1d(f, grid) -> torch.Tensor:
+    """operators.py:141-152 (the same as d2_central_1d)."""
+    return _fd1(f, grid, FD_D2)
+
+
 def laplacian_2d(f, grid: Grid2D) -> torch.Tensor:
     """operators.py:311-322: d2/dy2 + d2/dx2 on the interior, zero ghosts."""
     return _fd(f, grid, FD_LAP)

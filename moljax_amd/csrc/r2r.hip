@@ -189,6 +189,31 @@ int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long
   return run(dtype, kind, R2R_ROW, R2R_APPLY, L, a, 1, (hipStream_t)stream);
 }
 
+int mjx_r2r_etd1(int dtype, int kind, int nlines, int n, const void* u, const void* nonlin, long long in_stride,
+                 void* out, long long out_stride, const void* eig, double dt, void* stream) {
+  int rc = check_common(dtype, kind);
+  if (rc) return rc;
+  const int L = line_len(kind, n);
+  if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
+  if (nlines < 1 || !u || !nonlin || !out || !eig) return fail(MJX_E_ARG, "null pointer or empty batch");
+  if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
+  if ((rc = ensure_tables(nullptr))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  // out = T^-1[e^{dt eig} T u], then out += T^-1[dt phi1(dt eig) T N]
+  R2RArgs a = make(u, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
+  a.lam_line = eig;
+  a.mmode = R2R_EXP;
+  a.beta0 = dt;
+  rc = run(dtype, kind, R2R_ROW, R2R_APPLY, L, a, 1, st);
+  if (rc) return rc;
+  R2RArgs b = make(nonlin, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
+  b.lam_line = eig;
+  b.mmode = R2R_PHI1;
+  b.beta0 = dt;
+  b.accumulate = 1;
+  return run(dtype, kind, R2R_ROW, R2R_APPLY, L, b, 1, st);
+}
+
 int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, const void* in, long long in_stride,
                         void* out, long long out_stride, double scale, void* stream) {
   int rc = check_common(dtype, kind);

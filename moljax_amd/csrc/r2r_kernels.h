@@ -45,7 +45,7 @@ namespace mjx {
 enum { R2R_DST = 0, R2R_DCT = 1 };
 enum { R2R_FWD = 0, R2R_INV = 1, R2R_APPLY = 2 };
 enum { R2R_ROW = 0, R2R_COL = 1 };
-enum { R2R_HELM = 0, R2R_POISSON = 1, R2R_HELM_POW = 2 };
+enum { R2R_HELM = 0, R2R_POISSON = 1, R2R_HELM_POW = 2, R2R_EXP = 3, R2R_PHI1 = 4 };
 
 const cpx<float>* twiddles_f32();
 const cpx<double>* twiddles_f64();
@@ -64,6 +64,8 @@ struct R2RArgs {
   int mmode;                      // R2R_HELM or R2R_POISSON
   double scale;                   // extra output scale (FWD / INV)
   int alt;                        // DCT path as DST-II / DST-III (cell-centred Dirichlet)
+  int accumulate;                 // add the result to the output instead of storing it
+  double beta0;                   // beta when inst is null (alpha 0, power 1)
   int lift_on;                    // add the boundary lift while loading
   double lift[4];                 // left, right (line ends), bottom, top (first / last line)
 };
@@ -88,8 +90,15 @@ __device__ __forceinline__ cpx<T> dct_rot(const cpx<T>* __restrict__ tw, int k) 
 // Helmholtz ((1 + alpha lam) / (1 - beta lam)), raised to `power` in
 // R2R_HELM_POW mode (n backward-Euler / Crank-Nicolson steps in one solve),
 // or Poisson 1 / lam.
+// R2R_EXP: e^{beta lam}; R2R_PHI1: beta phi1(beta lam) with the quadratic
+// series below |z| = 1e-4 (moljax/core/fft_nonperiodic.py:381-389).
 template <typename T>
 __device__ __forceinline__ T r2r_mult(T lam, T alpha, T beta, int mmode, int power) {
+  if (mmode == R2R_EXP) return exp(beta * lam);
+  if (mmode == R2R_PHI1) {
+    const T z = beta * lam;
+    return beta * (fabs(z) < (T)1e-4 ? (T)1 + z / (T)2 + z * z / (T)6 : (exp(z) - (T)1) / z);
+  }
   if (mmode == R2R_POISSON) {
     if (fabs(lam) < (T)1e-14) return (T)0;
     if constexpr (std::is_same<T, double>::value) return 1.0 / lam;
@@ -194,9 +203,9 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
     line_fft<T, L, -1, 1, 1>(v, j, lline, twc);
 
     if constexpr (MODE == R2R_APPLY) {
-      const double* ip = a.inst + 8 * inst;
-      const T beta = (T)ip[0], alpha = (T)ip[2];
-      const int power = a.mmode == R2R_HELM_POW ? (int)ip[7] : 1;
+      const double* ip = a.inst ? a.inst + 8 * inst : nullptr;
+      const T beta = ip ? (T)ip[0] : (T)a.beta0, alpha = ip ? (T)ip[2] : (T)0;
+      const int power = (ip && a.mmode == R2R_HELM_POW) ? (int)ip[7] : 1;
       const T* lam_line = (const T*)a.lam_line;
       const T* lam_cross = (const T*)a.lam_cross;
       const int ga = line0 + 2 * t, gb = ga + 1;
@@ -297,7 +306,8 @@ __global__ __launch_bounds__((TL * R2RCfg<KIND, L>::TN)) void k_r2r(R2RArgs a, c
           else if (e & 1) y = -y;
         }
       }
-      out[(long long)g * a.out_ls + (long long)e * a.out_es] = y;
+      T* o = out + (long long)g * a.out_ls + (long long)e * a.out_es;
+      *o = a.accumulate ? *o + y : y;
     }
   }
 }

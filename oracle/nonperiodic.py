@@ -174,3 +174,23 @@ def neumann_laplacian_2d(u, dy, dx):
     """5-point Laplacian on cells with mirrored (zero-flux) ghosts."""
     p = np.pad(u, 1, mode="edge")
     return ((p[2:, 1:-1] - 2 * u + p[:-2, 1:-1]) / dy ** 2 + (p[1:-1, 2:] - 2 * u + p[1:-1, :-2]) / dx ** 2)
+
+
+def _phi1_quadratic(z):
+    """fft_nonperiodic.py:384-389 (the quadratic series below |z| = 1e-4)."""
+    z = np.asarray(z, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        direct = (np.exp(z) - 1.0) / z
+    return np.where(np.abs(z) < 1e-4, 1.0 + z / 2.0 + z ** 2 / 6.0, direct)
+
+
+def etd1_dirichlet(u, N_u, eig, dt):
+    """fft_nonperiodic.py:363-398."""
+    z = dt * eig
+    return idst_I_fast(np.exp(z) * dst_I_fast(u) + dt * _phi1_quadratic(z) * dst_I_fast(N_u))
+
+
+def etd1_neumann(u, N_u, eig, dt):
+    """fft_nonperiodic.py:401-437."""
+    z = dt * eig
+    return idct2_ortho(np.exp(z) * dct2_ortho(u) + dt * _phi1_quadratic(z) * dct2_ortho(N_u))

@@ -157,3 +157,68 @@ def test_bc_values_1d_and_batch(gpu):
                                            _hip.ptr(lamx), _hip.ptr(lamy), _hip.ptr(inst3), mmode, sv4, 1 / 16, 1 / 32,
                                            _hip.ptr(torch.empty_like(X)), _hip.stream_ptr())
         assert rc != 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_etd1_nonperiodic(gpu, dtype):
+    """etd1_dirichlet / etd1_neumann (fft_nonperiodic.py:363-437) and the
+    cell-centred variant vs the oracle, batched lines."""
+    from moljax_amd.core import fft_nonperiodic as fn
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    rng = np.random.default_rng(11)
+    dt, D = 0.2, 0.05
+    for kind, N in (("dirichlet", 63), ("neumann", 64), ("dirichlet_cc", 64)):
+        x, tu = _in(rng.standard_normal((3, N)), dtype)
+        y, tn = _in(rng.standard_normal((3, N)), dtype)
+        if kind == "dirichlet":
+            eig = D * onp.laplacian_symbol_dirichlet(N, 1.0 / (N + 1))
+            got = fn.etd1_dirichlet(tu, tn, torch.from_numpy(eig), dt)
+            want = np.stack([onp.etd1_dirichlet(x[i], y[i], eig, dt) for i in range(3)])
+        elif kind == "neumann":
+            eig = D * onp.laplacian_symbol_neumann(N, 1.0 / N)
+            got = fn.etd1_neumann(tu, tn, torch.from_numpy(eig), dt)
+            want = np.stack([onp.etd1_neumann(x[i], y[i], eig, dt) for i in range(3)])
+        else:
+            eig = D * onp.laplacian_symbol_dirichlet_cc(N, 1.0 / N)
+            z = dt * eig
+            ph = onp._phi1_quadratic(z)
+            want = np.stack([onp.idst2_ortho(np.exp(z) * onp.dst2_ortho(x[i]) + dt * ph * onp.dst2_ortho(y[i]))
+                             for i in range(3)])
+            got = fn.etd1_dirichlet_cc(tu, tn, torch.from_numpy(eig), dt)
+        err = ogs.rel_l2(_np(got), want)
+        print(f"etd1 {kind} {dtype}: rel_l2 {err:.2e}", flush=True)
+        assert err <= tol, kind
+
+
+def test_periodic_helmholtz_1d_and_fd_1d(gpu):
+    """solve_helmholtz_1d (fft_solvers.py:222-258) vs the oracle; the 1-D FD
+    operators (operators.py:34-152) on a sine mode against their exact
+    discrete values, ghosts zero."""
+    from moljax_amd.core import Grid1D
+    from moljax_amd.core.fft_solvers import laplacian_symbol_1d, solve_helmholtz, solve_helmholtz_1d
+    from moljax_amd.core.operators import d1_central_1d, d1_upwind_1d, d2_central_1d, laplacian_1d
+    from oracle import spectral as osp
+    n, dx = 64, 0.05
+    lam = laplacian_symbol_1d(n, dx)
+    x = np.random.default_rng(2).standard_normal((5, n))
+    got = solve_helmholtz_1d(torch.from_numpy(x).cuda(), lam, 0.3, 0.02)
+    want = np.stack([osp.solve_helmholtz_1d(x[i], osp.laplacian_symbol_1d(n, dx), 0.3, 0.02) for i in range(5)])
+    assert ogs.rel_l2(_np(got), want) < 1e-13
+    one = solve_helmholtz(torch.from_numpy(x[2]).cuda(), lam, 0.3, 0.02)
+    assert ogs.rel_l2(_np(one), want[2]) < 1e-13
+    g = Grid1D.uniform(n, 0.0, n * dx)
+    k = 2 * np.pi * 3 / (n * dx)
+    xc = (np.arange(n + 2) - 0.5) * dx
+    f = torch.from_numpy(np.sin(k * xc)).cuda()
+    interior = np.sin(k * xc[1:-1])
+    d2 = _np(d2_central_1d(f, g))
+    assert np.allclose(d2[1:-1], (2 * np.cos(k * dx) - 2) / dx ** 2 * interior, atol=1e-9) and d2[0] == d2[-1] == 0
+    assert torch.equal(laplacian_1d(f, g), d2_central_1d(f, g))
+    d1 = _np(d1_central_1d(f, g))
+    assert np.allclose(d1[1:-1], np.sin(k * dx) / dx * np.cos(k * xc[1:-1]), atol=1e-10)
+    lin = torch.from_numpy(3.0 * xc).cuda()
+    vel = torch.from_numpy(np.where(np.arange(n + 2) % 2 == 0, 1.0, -1.0)).cuda()
+    up = _np(d1_upwind_1d(lin, vel, g))
+    assert np.allclose(up[1:-1], 3.0, atol=1e-12) and up[0] == 0
+    upb = _np(d1_upwind_1d(f, -2.0, g))
+    assert np.allclose(upb[1:-1], (np.sin(k * xc[2:]) - interior) / dx, atol=1e-12)

@@ -471,6 +471,31 @@ def test_cpu_baseline_fft_variant_matches():
     assert np.max(np.abs(a - b)) < 1e-14
 
 
+def test_etd1_nonperiodic_modes():
+    """fft_nonperiodic.py:363-437 restated: on an eigenmode of the stencil
+    (DST-I sine on the vertex grid, DCT-II cosine on cells) the step is
+    e^{z_k} u_k + dt phi1(z_k) N_k, with phi1 from expm1 (z_k = dt eig_k)."""
+    from oracle import nonperiodic as onp
+    N, dt, D = 31, 0.4, 0.03
+    dx = 1.0 / (N + 1)
+    j = np.arange(N)
+    eig = D * onp.laplacian_symbol_dirichlet(N, dx)
+    for k in (1, 7, N):
+        mode = np.sin(np.pi * k * (j + 1) / (N + 1))
+        z = dt * eig[k - 1]
+        out = onp.etd1_dirichlet(2.0 * mode, -0.5 * mode, eig, dt)
+        assert np.allclose(out, (2.0 * np.exp(z) - 0.5 * dt * np.expm1(z) / z) * mode, atol=1e-13)
+    M = 32
+    eig = D * onp.laplacian_symbol_neumann(M, 1.0 / M)
+    jj = np.arange(M)
+    for k in (0, 3, M - 1):
+        mode = np.cos(np.pi * k * (jj + 0.5) / M)
+        z = dt * eig[k]
+        phi = 1.0 if k == 0 else np.expm1(z) / z
+        out = onp.etd1_neumann(mode, mode, eig, dt)
+        assert np.allclose(out, (np.exp(z) + dt * phi) * mode, atol=1e-13)
+
+
 # --------------------------------------------------------- Newton-Krylov
 def test_hand_jvp_matches_golden_ad_norm():
     """benchmarks/results/jvp_vs_fd_sweep.json:3: AD-JVP norm of the BE
