@@ -14,7 +14,8 @@ from .fft_solvers import (FFTCache1D, FFTCache2D, build_wavenumbers_1d, build_wa
                           laplacian_symbol_1d, laplacian_symbol_2d, create_fft_cache, create_fft_cache_1d,
                           create_fft_cache_2d, solve_helmholtz_1d, solve_helmholtz_2d, solve_helmholtz,
                           extract_interior,
-                          embed_interior, solve_diffusion_fft_field, apply_diffusion_inverse_fft, fft_cn_step)
+                          embed_interior, solve_diffusion_fft_field, apply_diffusion_inverse_fft, fft_cn_step,
+                          apply_diffusion_fft, diffusion_rhs_fft)
 from .model import (MOLModel, create_gray_scott_model, create_gray_scott_periodic_fft, get_fft_cache,
                     get_diffusivities)
 from .stepping import (IntegratorType, AdaptiveResult, imex_strang_step, imex_euler_step, imex_ssprk2_step,
@@ -30,8 +31,10 @@ from .fft_nonperiodic import (BCType as NonperiodicBCType, dst_I_fast, idst_I_fa
                               laplacian_symbol_dirichlet, laplacian_symbol_neumann, laplacian_symbol_mixed_dn,
                               create_nonperiodic_fft_cache, create_nonperiodic_fft_cache_2d, etd1_dirichlet,
                               etd1_neumann, etd1_dirichlet_cc)
-from .fft_operators import DiffusionOperator
-from .fft_integrators import etd1_step, etd2_step, etdrk4_step, etd_integrate
+from .fft_operators import DiffusionOperator, FFTLinearOperator, SpectralBounds, exact_cfl_dt
+from .fft_integrators import (etd1_step, etd2_step, etdrk4_step, etd_integrate, batched_fft_matvec,
+                              batched_fft_solve, batched_fft_exp_matvec, stacked_fft_solve_shared_op,
+                              diffusion_only_etd1)
 from .newton_krylov import NKParams, NKStats, NKResult, newton_krylov_solve, create_implicit_residual
 from .preconditioners import (PrecondContext, IdentityPreconditioner, DiffusionPreconditioner,
                               FFTDiffusionPreconditioner, DSTDiffusionPreconditioner,

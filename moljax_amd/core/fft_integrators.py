@@ -285,3 +285,46 @@ def etd_integrate(u0: StateDict, t_span: Tuple[float, float], dt: float, linear_
             t_hist.append(t0 + done * dt)
             hist.append({"u": u.clone(), "v": v.clone()})
     return torch.tensor(t_hist, dtype=torch.float64), hist
+
+
+def batched_fft_matvec(state: StateDict, linear_ops: Dict[str, DiffusionOperator]) -> StateDict:
+    """fft_integrators.py:400-420: L u per field (zeros without an operator)."""
+    out = {}
+    for name, f in state.items():
+        op = linear_ops.get(name)
+        out[name] = op.matvec(f) if op is not None else torch.zeros_like(as_device(f))
+    return out
+
+
+def batched_fft_solve(rhs: StateDict, linear_ops: Dict[str, DiffusionOperator], dt: float) -> StateDict:
+    """fft_integrators.py:423-445: (I - dt L)^-1 per field (identity without
+    an operator)."""
+    return {n: (linear_ops[n].solve(f, dt) if n in linear_ops else f) for n, f in rhs.items()}
+
+
+def batched_fft_exp_matvec(state: StateDict, linear_ops: Dict[str, DiffusionOperator], dt: float) -> StateDict:
+    """fft_integrators.py:448-470: exp(dt L) per field."""
+    return {n: (linear_ops[n].exp_matvec(f, dt) if n in linear_ops else f) for n, f in state.items()}
+
+
+def stacked_fft_solve_shared_op(rhs: StateDict, op: DiffusionOperator, dt: float) -> StateDict:
+    """fft_integrators.py:473-510: every field solved with one operator in one
+    batched launch (the fields stacked along a leading axis)."""
+    names = list(rhs.keys())
+    if not names:
+        return {}
+    first = as_device(rhs[names[0]])
+    stacked = torch.stack([as_device(rhs[n], field_dtype(first)) for n in names])
+    out = op.solve(stacked, dt)
+    return {n: out[i] for i, n in enumerate(names)}
+
+
+def diffusion_only_etd1(u0, t_end: float, dt: float, op: DiffusionOperator) -> torch.Tensor:
+    """fft_integrators.py:513-542: max(1, int(t_end/dt)) steps of exp(dt' L)
+    with dt' = t_end / n_steps."""
+    n = max(1, int(t_end / dt))
+    h = t_end / n
+    u = as_device(u0)
+    for _ in range(n):
+        u = op.exp_matvec(u, h)
+    return u

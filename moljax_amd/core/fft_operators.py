@@ -6,7 +6,7 @@ symbol; matvec (D lap u), solve ((I - dt D lap)^-1) and exp_matvec
 (exp(dt D lap) u) each run as one spectral round trip (mjx_spectral_pair).
 """
 from dataclasses import dataclass, field
-from typing import NamedTuple
+from typing import NamedTuple, Protocol, runtime_checkable
 
 import numpy as np
 import torch
@@ -22,6 +22,49 @@ class SpectralBounds(NamedTuple):
     re_max: float
     im_max: float
     methods_used: dict
+
+
+@runtime_checkable
+class FFTLinearOperator(Protocol):
+    """fft_operators.py:47-78: an FFT-diagonalised operator (eigenvalues,
+    matvec, solve, exp_matvec, spectral_bounds); the ETD and implicit
+    drivers accept any object with these members."""
+    name: str
+    grid: GridType
+
+    @property
+    def eigenvalues(self) -> torch.Tensor: ...
+
+    def matvec(self, u) -> torch.Tensor: ...
+
+    def solve(self, rhs, dt: float) -> torch.Tensor: ...
+
+    def exp_matvec(self, u, dt: float) -> torch.Tensor: ...
+
+    def spectral_bounds(self) -> SpectralBounds: ...
+
+
+def _host_eigenvalues(op) -> np.ndarray:
+    """The operator's eigenvalues on the host (from the symbol's 1-D tables
+    when it has them, else one copy of the device array)."""
+    lam = op.eigenvalues
+    sep = getattr(getattr(op, "_cache", None), "laplacian_symbol", None)
+    sep = getattr(sep, "_mjx_separable", None)
+    if sep is not None and hasattr(op, "D"):
+        lx, ly = np.asarray(sep[0]), np.asarray(sep[1])
+        return op.D * (ly[:, None] + lx[None, :])
+    return np.asarray(lam.detach().cpu().numpy() if isinstance(lam, torch.Tensor) else lam)
+
+
+def exact_cfl_dt(op, method: str = "explicit", safety: float = 0.9) -> float:
+    """fft_operators.py:263-300: safety * 2 / max|lambda| for explicit Euler;
+    safety for 'imex' and 'etd' (no limit from L)."""
+    if method == "explicit":
+        rho = float(np.max(np.abs(_host_eigenvalues(op))))
+        return float("inf") if rho < 1e-14 else safety * 2.0 / rho
+    if method in ("imex", "etd"):
+        return safety * 1.0
+    raise ValueError(f"Unknown method: {method}")
 
 
 @dataclass(frozen=True)
@@ -68,6 +111,6 @@ class DiffusionOperator:
         return self._apply(u, 0.0, float(dt) * float(self.D), -2)
 
     def spectral_bounds(self) -> SpectralBounds:
-        lam = self.eigenvalues
-        return SpectralBounds(rho=float(lam.abs().max()), re_max=float(lam.max()), im_max=0.0,
+        lam = _host_eigenvalues(self)
+        return SpectralBounds(rho=float(np.max(np.abs(lam))), re_max=float(np.max(lam)), im_max=0.0,
                               methods_used={"exact_fft": f"DiffusionOperator(D={self.D})"})

@@ -301,6 +301,38 @@ def apply_diffusion_inverse_fft(state_rhs: Dict[str, torch.Tensor], grid: Grid2D
 
 
 # --------------------------------------------------------------- FFT-CN
+def apply_diffusion_fft(state: Dict[str, torch.Tensor], grid: Grid2D, dt: float, diffusivities: Dict[str, float],
+                        fft_cache) -> Dict[str, torch.Tensor]:
+    """fft_solvers.py:448-474: the implicit diffusion step (the same as
+    apply_diffusion_inverse_fft)."""
+    return apply_diffusion_inverse_fft(state, grid, dt, diffusivities, fft_cache)
+
+
+def diffusion_rhs_fft(state: Dict[str, torch.Tensor], grid: Grid2D, diffusivities: Dict[str, float],
+                      fft_cache) -> Dict[str, torch.Tensor]:
+    """fft_solvers.py:619-662: D lap u by FFT on each field's interior,
+    embedded in zero ghosts; fields with D <= 1e-14 (or none) give zeros.
+    Two fields with the same layout share one spectral round trip."""
+    out = {}
+    active = [n for n in state if float(diffusivities.get(n, 0.0)) > 1e-14]
+    for n in state:
+        if n not in active:
+            out[n] = torch.zeros_like(as_device(state[n]))
+    for i in range(0, len(active), 2):
+        names = active[i:i + 2]
+        fs = [as_device(state[n]) for n in names]
+        dt_ = field_dtype(fs[0])
+        ints = [extract_interior(f.to(dt_), grid) for f in fs]
+        tabs = _tables(fft_cache.laplacian_symbol, dt_)
+        Ds = [float(diffusivities[n]) for n in names] + [0.0]
+        a, b = spectral_pair(ints[0], ints[1] if len(ints) > 1 else None, tabs, [[0.0, 0.0, Ds[0], Ds[1]]], nrep=-1)
+        for n, f, r in zip(names, fs, (a, b)):
+            pad = torch.zeros_like(f.to(dt_))
+            extract_interior(pad, grid).copy_(r.view(extract_interior(pad, grid).shape))
+            out[n] = pad
+    return out
+
+
 def _spectral_symbol_np(n: int, dx: float) -> np.ndarray:
     k = np.fft.fftfreq(n, dx) * 2 * np.pi
     return -(k ** 2)

@@ -106,3 +106,47 @@ def test_diffusion_operator(gpu):
     assert ogs.rel_l2(_np(op.exp_matvec(t, 0.3)), ex) < 1e-13
     with pytest.raises(ValueError):
         DiffusionOperator(grid, 0.0)
+
+
+def test_operator_wrappers(gpu):
+    """fft_integrators.py:400-542 and fft_solvers.py:448-474,619-662 wrappers
+    over the operator kernels, and exact_cfl_dt (fft_operators.py:263-300)
+    against the closed form rho = D (4/dx^2 + 4/dy^2) of even grids."""
+    from moljax_amd.core import Grid2D, create_fft_cache
+    from moljax_amd.core.fft_integrators import (batched_fft_exp_matvec, batched_fft_matvec, batched_fft_solve,
+                                                 diffusion_only_etd1, stacked_fft_solve_shared_op)
+    from moljax_amd.core.fft_operators import DiffusionOperator, FFTLinearOperator, exact_cfl_dt
+    from moljax_amd.core.fft_solvers import apply_diffusion_fft, diffusion_rhs_fft
+    grid = Grid2D.uniform(64, 32, 0.0, 1.0, 0.0, 2.0)
+    opu, opv = DiffusionOperator(grid, 0.01), DiffusionOperator(grid, 0.002)
+    assert isinstance(opu, FFTLinearOperator)
+    lam = osp.laplacian_symbol_2d(32, 64, grid.dy, grid.dx)
+    rng = np.random.default_rng(9)
+    a, b = rng.standard_normal((32, 64)), rng.standard_normal((32, 64))
+    st = {"u": torch.from_numpy(a).cuda(), "v": torch.from_numpy(b).cuda(), "w": torch.from_numpy(a).cuda()}
+    ops = {"u": opu, "v": opv}
+    mv = batched_fft_matvec(st, ops)
+    assert ogs.rel_l2(_np(mv["u"]), 0.01 * osp.periodic_fd_laplacian(a, grid.dy, grid.dx)) < 1e-12
+    assert float(mv["w"].abs().max()) == 0.0
+    sv = batched_fft_solve(st, ops, 0.3)
+    assert ogs.rel_l2(_np(sv["v"]), osp.solve_helmholtz_2d(b, lam, 0.3, 0.002)) < 1e-13
+    assert torch.equal(sv["w"], st["w"])
+    ex = batched_fft_exp_matvec(st, ops, 0.3)
+    assert ogs.rel_l2(_np(ex["u"]), np.fft.ifft2(np.fft.fft2(a) * np.exp(0.3 * 0.01 * lam)).real) < 1e-13
+    sh = stacked_fft_solve_shared_op({"u": st["u"], "v": st["v"]}, opu, 0.3)
+    assert ogs.rel_l2(_np(sh["v"]), osp.solve_helmholtz_2d(b, lam, 0.3, 0.01)) < 1e-13
+    d1 = diffusion_only_etd1(st["u"], 1.0, 0.3, opu)
+    assert ogs.rel_l2(_np(d1), np.fft.ifft2(np.fft.fft2(a) * np.exp(1.0 * 0.01 * lam)).real) < 1e-12
+    assert exact_cfl_dt(opu) == pytest.approx(0.9 * 2.0 / (0.01 * (4 / grid.dx ** 2 + 4 / grid.dy ** 2)), rel=1e-12)
+    assert exact_cfl_dt(opu, "etd") == 0.9
+    with pytest.raises(ValueError):
+        exact_cfl_dt(opu, "rk9")
+    cache = create_fft_cache(grid)
+    pad = {"u": torch.from_numpy(np.pad(a, 1)).cuda(), "v": torch.from_numpy(np.pad(b, 1)).cuda(),
+           "w": torch.from_numpy(np.pad(a, 1)).cuda()}
+    rhs = diffusion_rhs_fft(pad, grid, {"u": 0.01, "v": 0.002}, cache)
+    assert ogs.rel_l2(_np(rhs["u"])[1:-1, 1:-1], 0.01 * osp.periodic_fd_laplacian(a, grid.dy, grid.dx)) < 1e-12
+    assert ogs.rel_l2(_np(rhs["v"])[1:-1, 1:-1], 0.002 * osp.periodic_fd_laplacian(b, grid.dy, grid.dx)) < 1e-12
+    assert float(rhs["u"][0].abs().max()) == 0.0 and float(rhs["w"].abs().max()) == 0.0
+    imp = apply_diffusion_fft(pad, grid, 0.3, {"u": 0.01, "v": 0.002}, cache)
+    assert ogs.rel_l2(_np(imp["u"])[1:-1, 1:-1], osp.solve_helmholtz_2d(a, lam, 0.3, 0.01)) < 1e-13
