@@ -473,6 +473,17 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
                 double dx, double dy, const void* vel, long long vel_pitch, long long vel_istride,
                 double vel_scalar, void* stream);
 
+/* n_iter weighted-Jacobi sweeps for (I - beta lap) x = r on whole padded
+ * arrays (moljax/core/preconditioners.py:121-198, DiffusionPreconditioner):
+ * x0 = r, x <- omega (r + beta lap x) / diag + (1 - omega) x with
+ * diag = 1 + beta (2/dx^2 + 2/dy^2) and lap the 5-point Laplacian on the
+ * interior, 0 on the ghost ring.  r, x point at the first interior element
+ * (pitches >= nx + 2); work: B (ny+2)(nx+2) elements (n_iter > 1); x must not
+ * alias r.  n_iter = 0 copies r. */
+int mjx_jacobi2d(int dtype, int B, int ny, int nx, const void* r, long long r_pitch, long long r_istride, void* x,
+                 long long x_pitch, long long x_istride, void* work, double beta, double dy, double dx, double omega,
+                 int n_iter, void* stream);
+
 /* Ghost-ring fill of a padded field in place.  kind 0 periodic, 1 Dirichlet
  * (ghost = 2 g - interior), 2 Neumann (ghost = interior -/+ 2 h flux);
  * side_values = {left, right, bottom, top} (NULL = zeros).  Replaces

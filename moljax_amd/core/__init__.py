@@ -38,6 +38,7 @@ from .fft_integrators import (etd1_step, etd2_step, etdrk4_step, etd_integrate, 
 from .newton_krylov import NKParams, NKStats, NKResult, newton_krylov_solve, create_implicit_residual
 from .preconditioners import (PrecondContext, IdentityPreconditioner, DiffusionPreconditioner,
                               FFTDiffusionPreconditioner, DSTDiffusionPreconditioner,
-                              create_gray_scott_preconditioner, create_gray_scott_fft_preconditioner)
+                              create_gray_scott_preconditioner, create_gray_scott_fft_preconditioner,
+                              create_diffusion_preconditioner, create_fft_preconditioner)
 from .reaction_diffusion import (ReactionDiffusionModel, create_reaction_diffusion_model, be_step, cn_step,
                                  bdf2_step)

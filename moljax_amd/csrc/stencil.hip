@@ -140,11 +140,91 @@ __global__ __launch_bounds__(256) void k_gs_reaction(const T* __restrict__ u, co
 
 static inline unsigned nblocks(long long total) { return (unsigned)((total + 255) / 256); }
 
+// One weighted-Jacobi sweep for (I - beta lap) x = r on the whole padded
+// array (moljax/core/preconditioners.py:166-198): x' = omega (r + beta
+// lap(x)) / diag + (1 - omega) x with lap the 5-point Laplacian on interior
+// points and 0 on the ghost ring (as laplacian_2d writes it).
+struct JacobiArgs {
+  long long rp, rs, xp, xs, op, os;  // pitches / instance strides of r, x, out
+  int ny, nx;
+  double beta, dx2, dy2, omega, diag;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi_padded(const T* __restrict__ r, const T* __restrict__ x,
+                                                       T* __restrict__ out, JacobiArgs a, long long total) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int W = a.nx + 2, H = a.ny + 2;
+  const long long per = (long long)W * H;
+  const long long b = g / per;
+  const long long rr = g - b * per;
+  const int y = (int)(rr / W) - 1, xi = (int)(rr % W) - 1;
+  const T* xc = x + b * a.xs + (long long)y * a.xp + xi;
+  const T c = *xc;
+  T lap = (T)0;
+  if (y >= 0 && y < a.ny && xi >= 0 && xi < a.nx)
+    lap = (xc[a.xp] - (T)2 * c + xc[-a.xp]) / (T)a.dy2 + (xc[1] - (T)2 * c + xc[-1]) / (T)a.dx2;
+  const T rv = r[b * a.rs + (long long)y * a.rp + xi];
+  out[b * a.os + (long long)y * a.op + xi] =
+      (T)a.omega * (rv + (T)a.beta * lap) / (T)a.diag + ((T)1 - (T)a.omega) * c;
+}
+
 }  // namespace mjx
 
 using namespace mjx;
 
 extern "C" {
+
+int mjx_jacobi2d(int dtype, int B, int ny, int nx, const void* r, long long r_pitch, long long r_istride, void* x,
+                 long long x_pitch, long long x_istride, void* work, double beta, double dy, double dx, double omega,
+                 int n_iter, void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (B < 1 || ny < 1 || nx < 1) return fail(MJX_E_ARG, "empty grid");
+  if (!r || !x || (n_iter > 1 && !work)) return fail(MJX_E_ARG, "null pointer argument");
+  if (n_iter < 0) return fail(MJX_E_ARG, "n_iter must be >= 0");
+  if (r_pitch < nx + 2 || x_pitch < nx + 2) return fail(MJX_E_ARG, "pitch must cover the ghost ring");
+  if (!(dx > 0.0) || !(dy > 0.0)) return fail(MJX_E_ARG, "dx, dy must be positive");
+  const long long W = nx + 2, per = W * (ny + 2), total = (long long)B * per;
+  const size_t es = dtype == MJX_F64 ? 8 : 4;
+  JacobiArgs a;
+  a.ny = ny;
+  a.nx = nx;
+  a.beta = beta;
+  a.dx2 = dx * dx;
+  a.dy2 = dy * dy;
+  a.diag = 1.0 + beta * (2.0 / (dx * dx) + 2.0 / (dy * dy));
+  hipStream_t st = (hipStream_t)stream;
+  // x0 = r; sweeps alternate between x and work so that the last lands in x
+  // (n_iter = 0: one sweep with omega 0, a copy)
+  const int sweeps = n_iter > 0 ? n_iter : 1;
+  const char* prev = (const char*)r;
+  long long pp = r_pitch, ps = r_istride;
+  char* wk = work ? (char*)work + es * (W + 1) : nullptr;  // origin of the dense padded work array
+  for (int i = 0; i < sweeps; ++i) {
+    const bool to_x = (sweeps - 1 - i) % 2 == 0;
+    char* dst = to_x ? (char*)x : wk;
+    a.rp = r_pitch;
+    a.rs = r_istride;
+    a.xp = pp;
+    a.xs = ps;
+    a.op = to_x ? x_pitch : W;
+    a.os = to_x ? x_istride : per;
+    a.omega = n_iter > 0 ? omega : 0.0;
+    if (dtype == MJX_F32)
+      hipLaunchKernelGGL(k_jacobi_padded<float>, dim3(nblocks(total)), dim3(256), 0, st, (const float*)r,
+                         (const float*)prev, (float*)dst, a, total);
+    else
+      hipLaunchKernelGGL(k_jacobi_padded<double>, dim3(nblocks(total)), dim3(256), 0, st, (const double*)r,
+                         (const double*)prev, (double*)dst, a, total);
+    int rc = check_launch("jacobi_padded");
+    if (rc) return rc;
+    prev = dst;
+    pp = a.op;
+    ps = a.os;
+  }
+  return MJX_OK;
+}
 
 int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long long in_pitch, long long in_istride,
                 void* out, long long out_pitch, long long out_istride, int axis, double dx, double dy,
