@@ -473,6 +473,12 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
                 double dx, double dy, const void* vel, long long vel_pitch, long long vel_istride,
                 double vel_scalar, void* stream);
 
+/* out = a x + b y + c over n contiguous elements (y may be NULL); out may
+ * alias x or y.  The element-wise StateDict algebra of
+ * moljax/core/state.py:27-81 (tree_add, tree_sub, tree_scale, tree_axpy). */
+int mjx_axpby(int dtype, long long n, double a, const void* x, double b, const void* y, double c, void* out,
+              void* stream);
+
 /* n_iter weighted-Jacobi sweeps for (I - beta lap) x = r on whole padded
  * arrays (moljax/core/preconditioners.py:121-198, DiffusionPreconditioner):
  * x0 = r, x <- omega (r + beta lap x) / diag + (1 - omega) x with

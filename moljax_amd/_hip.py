@@ -55,6 +55,7 @@ PROTOTYPES = {
     "mjx_r2r_solve2d_bc": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _p, _p, _p, _i, _p, _d, _d, _p, _p]),
     "mjx_r2r_solve1d_bc": (_i, [_i, _i, _i, _i, _p, _ll, _p, _ll, _p, _p, _i, _p, _d, _p]),
     "mjx_r2r_solve1d": (_i, [_i, _i, _i, _i, _p, _ll, _p, _ll, _p, _p, _i, _p]),
+    "mjx_axpby": (_i, [_i, _ll, _d, _p, _d, _p, _d, _p, _p]),
     "mjx_jacobi2d": (_i, [_i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _p, _d, _d, _d, _d, _i, _p]),
     "mjx_r2r_etd1": (_i, [_i, _i, _i, _i, _p, _p, _ll, _p, _ll, _p, _d, _p]),
     "mjx_r2r_transform1d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _p, _ll, _d, _p]),

@@ -361,10 +361,14 @@ def etd1_dirichlet_cc(u, N_u, eigenvalues, dt: float) -> torch.Tensor:
 
 def check_compatibility_neumann(rhs, tol: float = 1e-10) -> bool:
     """fft_nonperiodic.py:440-450."""
-    return abs(float(as_device(rhs).double().sum())) < tol
+    from .state import RED_SUM, _reduce
+    t = as_device(rhs)
+    return abs(float(_reduce(RED_SUM, [t.to(field_dtype(t))]).sum())) < tol
 
 
 def project_to_compatible(rhs) -> torch.Tensor:
     """fft_nonperiodic.py:453-461."""
+    from .state import RED_SUM, _reduce, axpby
     t = as_device(rhs)
-    return t - t.mean()
+    t = t.to(field_dtype(t))
+    return axpby(1.0, t, c=-float(_reduce(RED_SUM, [t]).sum()) / t.numel())

@@ -1,8 +1,9 @@
 """StateDict algebra (moljax/core/state.py).
 
 A state is a dict of field name -> padded array.  The element-wise tree ops
-are thin and allocate new arrays (functional semantics of the reference).
-Reductions used for accept/reject decisions accumulate in float64.
+run the native a x + b y + c kernel and allocate new arrays (the reference's
+functional semantics).  Reductions used for accept/reject decisions
+accumulate in float64.
 """
 from typing import Callable, Dict, Tuple
 
@@ -11,24 +12,42 @@ import torch
 StateDict = Dict[str, torch.Tensor]
 
 
+def axpby(a: float, x, b: float = 0.0, y=None, c: float = 0.0) -> torch.Tensor:
+    """A new array a x + b y + c (mjx_axpby), y optional, same shape/dtype."""
+    from .. import _hip
+    from ._arrays import as_device, field_dtype
+    tx = as_device(x)
+    tx = tx.to(field_dtype(tx)).contiguous()
+    ty = None
+    if y is not None:
+        ty = as_device(y, tx.dtype).contiguous()
+        if ty.shape != tx.shape:
+            raise ValueError(f"shapes differ: {tuple(tx.shape)} vs {tuple(ty.shape)}")
+    out = torch.empty_like(tx)
+    rc = _hip.lib().mjx_axpby(_hip.dtype_code(tx.dtype), tx.numel(), float(a), _hip.ptr(tx), float(b),
+                              _hip.ptr(ty) if ty is not None else None, float(c), _hip.ptr(out), _hip.stream_ptr())
+    _hip.check(rc, "mjx_axpby")
+    return out
+
+
 def tree_add(a: StateDict, b: StateDict) -> StateDict:
     """state.py:27-38."""
-    return {k: a[k] + b[k] for k in a}
+    return {k: axpby(1.0, a[k], 1.0, b[k]) for k in a}
 
 
 def tree_sub(a: StateDict, b: StateDict) -> StateDict:
     """state.py:41-52."""
-    return {k: a[k] - b[k] for k in a}
+    return {k: axpby(1.0, a[k], -1.0, b[k]) for k in a}
 
 
 def tree_scale(a: StateDict, s: float) -> StateDict:
     """state.py:55-66."""
-    return {k: v * s for k, v in a.items()}
+    return {k: axpby(float(s), v) for k, v in a.items()}
 
 
 def tree_axpy(y: StateDict, a: float, x: StateDict) -> StateDict:
     """state.py:69-81: y + a*x."""
-    return {k: y[k] + a * x[k] for k in y}
+    return {k: axpby(1.0, y[k], float(a), x[k]) for k in y}
 
 
 def tree_zeros_like(state: StateDict) -> StateDict:

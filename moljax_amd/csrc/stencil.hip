@@ -140,6 +140,19 @@ __global__ __launch_bounds__(256) void k_gs_reaction(const T* __restrict__ u, co
 
 static inline unsigned nblocks(long long total) { return (unsigned)((total + 255) / 256); }
 
+// out = a x + b y + c over n contiguous elements (y may be null): the
+// element-wise StateDict algebra (moljax/core/state.py:27-81).
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpby(long long n, double a, const T* x, double b, const T* y, double c,
+                                               T* out) {  // out may alias x or y: no restrict
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T v = (T)a * x[i];
+  if (y) v = v + (T)b * y[i];
+  if (c != 0.0) v = v + (T)c;
+  out[i] = v;
+}
+
 // One weighted-Jacobi sweep for (I - beta lap) x = r on the whole padded
 // array (moljax/core/preconditioners.py:166-198): x' = omega (r + beta
 // lap(x)) / diag + (1 - omega) x with lap the 5-point Laplacian on interior
@@ -175,6 +188,21 @@ __global__ __launch_bounds__(256) void k_jacobi_padded(const T* __restrict__ r, 
 using namespace mjx;
 
 extern "C" {
+
+int mjx_axpby(int dtype, long long n, double a, const void* x, double b, const void* y, double c, void* out,
+              void* stream) {
+  if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
+  if (n < 0 || (n > 0 && (!x || !out))) return fail(MJX_E_ARG, "null pointer or negative length");
+  if (n == 0) return MJX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == MJX_F32)
+    hipLaunchKernelGGL(k_axpby<float>, dim3(nblocks(n)), dim3(256), 0, st, n, a, (const float*)x, b,
+                       (const float*)y, c, (float*)out);
+  else
+    hipLaunchKernelGGL(k_axpby<double>, dim3(nblocks(n)), dim3(256), 0, st, n, a, (const double*)x, b,
+                       (const double*)y, c, (double*)out);
+  return check_launch("axpby");
+}
 
 int mjx_jacobi2d(int dtype, int B, int ny, int nx, const void* r, long long r_pitch, long long r_istride, void* x,
                  long long x_pitch, long long x_istride, void* work, double beta, double dy, double dx, double omega,

@@ -102,3 +102,33 @@ def test_interior_sum_mean_max_and_history(gpu, dtype):
     save_to_history(hist, {"u": f, "v": 3 * f}, g, 2)
     assert torch.equal(hist["u"][2], f[1:-1, 1:-1]) and torch.equal(hist["v"][2], 3 * f[1:-1, 1:-1])
     assert float(hist["u"][1].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_tree_algebra_and_neumann_compatibility(gpu, dtype):
+    """state.py:27-81 (tree_add / sub / scale / axpy on the native a x + b y + c
+    kernel) and fft_nonperiodic.py:440-461 (compatibility check and projection
+    with the native sum)."""
+    from moljax_amd.core.fft_nonperiodic import check_compatibility_neumann, project_to_compatible
+    from moljax_amd.core.state import tree_add, tree_axpy, tree_scale, tree_sub
+    rng = np.random.default_rng(5)
+    a = {"u": rng.standard_normal((10, 12)), "v": rng.standard_normal((3, 10, 12))}
+    b = {"u": rng.standard_normal((10, 12)), "v": rng.standard_normal((3, 10, 12))}
+    ta = {k: torch.from_numpy(v).to("cuda", dtype) for k, v in a.items()}
+    tb = {k: torch.from_numpy(v).to("cuda", dtype) for k, v in b.items()}
+    ra = {k: _np(v) for k, v in ta.items()}
+    rb = {k: _np(v) for k, v in tb.items()}
+    tol = 4e-16 if dtype == torch.float64 else 1e-6  # fused multiply-adds round once
+    for got, want in ((tree_add(ta, tb), {k: ra[k] + rb[k] for k in ra}),
+                      (tree_sub(ta, tb), {k: ra[k] - rb[k] for k in ra}),
+                      (tree_scale(ta, 0.3), {k: 0.3 * ra[k] for k in ra}),
+                      (tree_axpy(ta, -1.7, tb), {k: ra[k] + (-1.7) * rb[k] for k in ra})):
+        for k in want:
+            assert got[k].shape == ta[k].shape and got[k].dtype == dtype
+            assert np.max(np.abs(_np(got[k]) - want[k])) <= tol * (1 + np.max(np.abs(want[k]))), k
+    assert torch.equal(ta["u"], torch.from_numpy(a["u"]).to("cuda", dtype))  # inputs untouched
+    r = torch.from_numpy(rng.standard_normal(64)).to("cuda", dtype)
+    p = project_to_compatible(r)
+    assert check_compatibility_neumann(p, tol=1e-10 if dtype == torch.float64 else 1e-4)
+    assert not check_compatibility_neumann(r + 1.0)
+    assert np.allclose(_np(p), _np(r) - math.fsum(_np(r)) / 64, atol=1e-14 if dtype == torch.float64 else 1e-6)
