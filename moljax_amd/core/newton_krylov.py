@@ -21,7 +21,7 @@ import torch
 
 from .. import _hip
 from .preconditioners import PRECOND_EXACT, PRECOND_JACOBI, PRECOND_NONE, IdentityPreconditioner, PrecondContext
-from .state import StateDict
+from .state import StateDict, axpby
 
 
 class NKParams(NamedTuple):
@@ -70,7 +70,7 @@ class RDResidual:
         self.u_old = model.as_field(y_old["u"]) if g is None else model.as_field(g)
         self.fold = None
         if g is None and theta < 1.0:
-            self.fold = (1.0 - theta) * self.dt * model.rhs_field(self.u_old)
+            self.fold = axpby((1.0 - theta) * self.dt, model.rhs_field(self.u_old))
 
     def field(self, u: torch.Tensor) -> torch.Tensor:
         m = self.model
@@ -134,7 +134,7 @@ def create_bdf2_residual(model, y_n: StateDict, y_nm1: StateDict, t_new: float, 
         raise NotImplementedError("implicit residuals on the MI355X path serve reaction-diffusion models "
                                   "(create_reaction_diffusion_model)")
     a0, a1, a2, beta = bdf2_coefficients(dt, dt_prev, consistent)
-    g = -(a1 * model.as_field(y_n["u"]) + a2 * model.as_field(y_nm1["u"]))
+    g = axpby(-a1, model.as_field(y_n["u"]), -a2, model.as_field(y_nm1["u"]))
     return RDResidual(model, y_n, beta, 1.0, lead=a0, g=g)
 
 

@@ -18,7 +18,7 @@ import torch
 
 from .. import _hip
 from .newton_krylov import NKParams, NKStats, create_implicit_residual, newton_krylov_solve
-from .state import StateDict
+from .state import StateDict, axpby
 
 
 def _fd_symbol(n: int, d: float) -> np.ndarray:
@@ -66,7 +66,7 @@ class ReactionDiffusionModel:
         rc = _hip.lib().mjx_rd_residual(self.ny, self.nx, self.bc_code, self.dy, self.dx, self.D, 1.0, 1.0, 1.0, reac,
                                         _hip.ptr(u), _hip.ptr(u), 0, _hip.ptr(r), _hip.stream_ptr())
         _hip.check(rc, "mjx_rd_residual")
-        return -r
+        return axpby(-1.0, r)
 
     def rhs(self, y: StateDict, t: float = 0.0) -> StateDict:
         return {"u": self.rhs_field(y["u"])}
@@ -101,7 +101,7 @@ def create_reaction_diffusion_model(n: int, L: float = 1.0, D: float = 1e-3,
 def euler_step(model: ReactionDiffusionModel, y: StateDict, t: float, dt: float) -> StateDict:
     """stepping.py:64-85: y + dt F(y)."""
     u = model.as_field(y["u"])
-    return {"u": u + dt * model.rhs_field(u)}
+    return {"u": axpby(1.0, u, float(dt), model.rhs_field(u))}
 
 
 def be_step(model: ReactionDiffusionModel, y: StateDict, t: float, dt: float, preconditioner=None,
@@ -127,7 +127,7 @@ def bdf2_step(model: ReactionDiffusionModel, y: StateDict, y_prev: StateDict, t:
     2 y - y_prev (consistent: see bdf2_coefficients)."""
     from .newton_krylov import create_bdf2_residual
     res = create_bdf2_residual(model, y, y_prev, t + dt, dt, dt_prev, consistent)
-    guess = {"u": 2.0 * model.as_field(y["u"]) - model.as_field(y_prev["u"])}
+    guess = {"u": axpby(2.0, model.as_field(y["u"]), -1.0, model.as_field(y_prev["u"]))}
     out = newton_krylov_solve(res, guess, model, model.params, preconditioner, nk_params, dt)
     return out.solution, out.stats
 
@@ -197,7 +197,7 @@ def adaptive_integrate_implicit(model: ReactionDiffusionModel, y0: StateDict, t0
             else:
                 y_new, st = bdf2_step(model, y, {"u": u_prev}, t, dtc, dt_prev, preconditioner, nk_params)
             y_be, _ = be_step(model, y, t, dtc, preconditioner, nk_params)
-            err = y_new["u"] - y_be["u"]
+            err = axpby(1.0, y_new["u"], -1.0, y_be["u"])
         un = y_new["u"]
         e = float(torch.sqrt(_reduce(RED_SCALED_SQ, [err], [un], pid.atol, pid.rtol)[0] / un.numel()))
         e = max(e, 1e-10) if e == e else e

@@ -152,7 +152,8 @@ def estimate_error_imex_doubling(model, y: StateDict, t: float, dt: float, fft_c
     y_full = step(model, y, t, dt, fft_cache, diffusivities)
     y_half = step(model, step(model, y, t, dt / 2.0, fft_cache, diffusivities), t + dt / 2.0, dt / 2.0, fft_cache,
                   diffusivities)
-    return y_half, {n: y_half[n] - y_full[n] for n in y_half}
+    from .state import axpby
+    return y_half, {n: axpby(1.0, y_half[n], -1.0, y_full[n]) for n in y_half}
 
 
 class AdaptiveResult(NamedTuple):
