@@ -190,7 +190,7 @@ def adaptive_integrate_implicit(model: ReactionDiffusionModel, y0: StateDict, t0
         y = {"u": u}
         if m == 3 or (m == 5 and step_count < 1):
             y_new, st = be_step(model, y, t, dtc, preconditioner, nk_params)
-            err = dtc * model.rhs_field(y_new["u"])
+            err = axpby(dtc, model.rhs_field(y_new["u"]))
         else:
             if m == 4:
                 y_new, st = cn_step(model, y, t, dtc, preconditioner, nk_params)
