@@ -67,4 +67,10 @@ def inst_table(rows: Sequence[Sequence[float]], device=None) -> torch.Tensor:
 
 
 def spectral_work(dtype: torch.dtype, B: int, ny: int, nx: int, device=None) -> torch.Tensor:
-    return torch.empty((B, ny, nx, 2), dtype=dtype, device=device or "cuda")
+    """Complex work array of the periodic spectral entry points, sized by
+    mjx_spectral_workspace_bytes (nx columns per row for power-of-two grids,
+    2 (nx/2 + 1) for the any-length path)."""
+    from .. import _hip
+    nbytes = int(_hip.lib().mjx_spectral_workspace_bytes(_hip.dtype_code(dtype), B, ny, nx))
+    esize = torch.empty((), dtype=dtype).element_size()
+    return torch.empty((nbytes // (2 * esize), 2), dtype=dtype, device=device or "cuda")

@@ -6,14 +6,15 @@ and applies the transform along both axes (axis 0 = y).  For 2-D input the
 symbol argument is a pair (lam_y, lam_x) of 1-D symbols, a NonperiodicFFTCache2D,
 or one 1-D symbol used on both axes of a square grid.
 
-Sizes: Dirichlet needs N + 1 and Neumann N to be a power of two in [16, 1024]
-(a vertex grid of N = 2^k - 1 interior points for Dirichlet); other sizes
+Sizes: any N in [2, 2048].  Dirichlet with N + 1, and Neumann / cell
+Dirichlet with N, a power of two in [16, 1024] run the Stockham kernels; the
+other lengths run the chirp-z (Bluestein) kernels; sizes outside [2, 2048]
 raise ValueError.
 
 The library's own grids are cell-centred, and moljax/core/bc.py fills a
 Dirichlet ghost as 2 g - interior (bc.py:237-258).  That stencil is
 diagonalised by the DST-II, not the DST-I above; ``*_dirichlet_cc`` and
-BCType.DIRICHLET_CC serve it (N a power of two), and solve_helmholtz_bc /
+BCType.DIRICHLET_CC serve it, and solve_helmholtz_bc /
 solve_poisson_bc take a FieldBCSpec with side values or fluxes so that the
 solve inverts the FD operator evaluated on bc.py's filled ghosts exactly.
 """

@@ -197,14 +197,17 @@ def solve_helmholtz_2d(rhs_interior, laplacian_symbol, dt: float, D: float) -> t
 
 
 _LINE_SYMBOLS: Dict[tuple, torch.Tensor] = {}
+_POW2_SIDES = (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
 
 
 def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
     """fft_solvers.py:222-258: (I - dt D lap) u = rhs on periodic lines of nx
     points (a batch along the leading axes).  The lines are stacked as the
     rows of a 2-D array whose y symbol is zero, so the 2-D kernels apply the
-    1-D multiplier row by row; stacks are padded with zero rows to a power of
-    two of at least 16 rows."""
+    1-D multiplier row by row; for power-of-two nx the stacks are padded with
+    zero rows to a power of two of at least 16 rows (the Stockham kernels),
+    other lengths (2..2048) run the any-length kernels on the stack as it
+    is."""
     t = as_device(rhs_interior)
     t = t.to(field_dtype(t))
     nx = t.shape[-1]
@@ -214,10 +217,12 @@ def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> t
     lines = t.reshape(-1, nx)
     lam_np = lam.cpu().numpy()
     out = torch.empty_like(lines)
-    for i0 in range(0, lines.shape[0], 4096):
-        chunk = lines[i0:i0 + 4096]
+    pow2 = nx in _POW2_SIDES
+    step = 4096 if pow2 else 2048
+    for i0 in range(0, lines.shape[0], step):
+        chunk = lines[i0:i0 + step]
         L = chunk.shape[0]
-        rows = max(16, 1 << (L - 1).bit_length())
+        rows = max(16, 1 << (L - 1).bit_length()) if pow2 else max(2, L)
         key = (lam_np.tobytes(), rows)
         lam2d = _LINE_SYMBOLS.get(key)
         if lam2d is None:

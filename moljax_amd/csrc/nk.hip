@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "czt_api.h"
 #include "mjx_common.h"
 #include "r2r_kernels.h"
 
@@ -206,6 +207,7 @@ class NKSolver {
            const void* const* lam, hipStream_t st)
       : a_(a), c_(c), u_(u), u_old_(u_old), fold_(fold), st_(st) {
     n_ = (long long)a.ny * a.nx;
+    const long long s2 = precond_work_reals(a.ny, a.nx);
     const int m = c.restart;
     double* p = (double*)work;
     V_ = p; p += (long long)(m + 1) * n_;
@@ -215,7 +217,7 @@ class NKSolver {
     xt_ = p; p += n_;
     t64_ = p; p += n_;
     t64b_ = p; p += n_;
-    r2r64_ = p; p += 2 * n_;
+    r2r64_ = p; p += s2;
     hdev_ = p; p += 3 * (m + 2);
     partial_ = p; p += (long long)((m + 1 + DOT_CHUNK - 1) / DOT_CHUNK) * DOT_BLOCKS * DOT_CHUNK;
     inst_ = p; p += 8;
@@ -224,18 +226,24 @@ class NKSolver {
     t32_ = f; f += n_;
     w32_ = f; f += n_;
     t32b_ = f; f += n_;
-    r2r32_ = f; f += 2 * n_;
+    r2r32_ = f; f += s2;
     lamx64_ = lam[0];
     lamy64_ = lam[1];
     lamx32_ = lam[2];
     lamy32_ = lam[3];
   }
 
+  // Reals of the exact preconditioner's work array: the periodic spectral
+  // work (complex, spec_work_cols per row), which also covers the n reals of
+  // the DST solve.
+  static long long precond_work_reals(int ny, int nx) { return 2LL * ny * spec_work_cols(nx, ny); }
+
   static long long workspace_bytes(int ny, int nx, int restart) {
     const long long n = (long long)ny * nx;
-    long long d = (long long)(restart + 1) * n + 8 * n + 3 * (restart + 2) +
+    const long long s2 = precond_work_reals(ny, nx);
+    long long d = (long long)(restart + 1) * n + 6 * n + s2 + 3 * (restart + 2) +
                   (long long)((restart + 1 + DOT_CHUNK - 1) / DOT_CHUNK) * DOT_BLOCKS * DOT_CHUNK + 8;
-    return d * 8 + 6 * n * 4 + 256;
+    return d * 8 + (4 * n + s2) * 4 + 256;
   }
 
   int solve(NKStatsOut* out) {

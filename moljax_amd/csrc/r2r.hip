@@ -3,6 +3,7 @@
 
 #include <string>
 
+#include "czt_api.h"
 #include "mjx_common.h"
 #include "r2r_kernels.h"
 
@@ -41,9 +42,17 @@ static int check_common(int dtype, int kind) {
   return MJX_OK;
 }
 
-static const char* size_rule(int kind) {
-  return kind == R2R_DST ? " not supported: n+1 must be a power of two in [16, 1024]"
-                         : " not supported: n must be a power of two in [16, 1024]";
+// Lengths outside the power-of-two rule run the chirp-z kernels, n in [2, 2048].
+static bool len_ok(int kind, int n) { return line_len(kind, n) > 0 || czt_len_ok(n); }
+
+static std::string size_msg(int n) {
+  return "length " + std::to_string(n) + " not supported: n must be in [2, 2048]";
+}
+
+// One batch of nlines contiguous lines (line strides in elements).
+static CztR2RCall line_call(int dtype, int kind, int mode, int nlines, int n, const void* in, long long in_stride,
+                            void* out, long long out_stride, double scale) {
+  return CztR2RCall{dtype, kind, mode, 1, nlines, n, in, 1, in_stride, 0, out, 1, out_stride, 0, scale};
 }
 
 // Ghost-ring contributions of bc.py's fill to the 5-point Laplacian of the
@@ -75,6 +84,33 @@ static R2RArgs make(const void* in, long long in_es, long long in_ls, long long 
   return a;
 }
 
+// The chirp-z form of solve2d: rows forward into work, columns apply in
+// place (the multiplier carries both dimensions' inverse normalisation), rows
+// inverse into the output.
+static int solve2d_czt(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
+                       long long in_istride, void* out, long long out_pitch, long long out_istride, const void* lamx,
+                       const void* lamy, const double* inst, int mmode, const double* side_values, double dy,
+                       double dx, void* work, hipStream_t st) {
+  const long long plane = (long long)ny * nx;
+  R2RArgs lifted{};
+  if (side_values) set_lift(lifted, kind, side_values, 4, dy, dx);
+  CztR2RCall r{dtype, kind, R2R_FWD, B, ny, nx, rhs, 1, in_pitch, in_istride, work, 1, nx, plane, 1.0};
+  r.inst = inst;
+  r.mmode = mmode;
+  r.lift = side_values ? lifted.lift : nullptr;
+  int rc = czt_r2r(r, st);
+  if (rc) return rc;
+  CztR2RCall c{dtype, kind, R2R_APPLY, B, nx, ny, work, nx, 1, plane, work, nx, 1, plane, 1.0};
+  c.lam_line = lamy;
+  c.lam_cross = lamx;
+  c.n_cross = nx;
+  c.inst = inst;
+  c.mmode = mmode;
+  if ((rc = czt_r2r(c, st))) return rc;
+  CztR2RCall v{dtype, kind, R2R_INV, B, ny, nx, work, 1, nx, plane, out, 1, out_pitch, out_istride, 1.0};
+  return czt_r2r(v, st);
+}
+
 static int solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs, long long in_pitch,
                    long long in_istride, void* out, long long out_pitch, long long out_istride, const void* lamx,
                    const void* lamy, const double* inst, int mmode, const double* side_values, double dy, double dx,
@@ -88,6 +124,8 @@ static int solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs, 
   if ((rc = ensure_tables(nullptr))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const long long plane = (long long)ny * nx;
+  if (Lx < 0 || Ly < 0) return solve2d_czt(dtype, kind, B, ny, nx, rhs, in_pitch, in_istride, out, out_pitch,
+                                           out_istride, lamx, lamy, inst, mmode, side_values, dy, dx, work, st);
   // rows: forward transform along x into the work array
   R2RArgs a = make(rhs, 1, in_pitch, in_istride, work, 1, nx, plane, ny);
   if (side_values) {
@@ -123,11 +161,9 @@ int mjx_r2r_solve2d(int dtype, int kind, int B, int ny, int nx, const void* rhs,
                     const void* lamy, const double* inst, int mmode, void* work, void* stream) {
   int rc = check_common(dtype, kind);
   if (rc) return rc;
-  const int Lx = line_len(kind, nx), Ly = line_len(kind, ny);
-  if (Lx < 0 || Ly < 0)
+  if (!len_ok(kind, nx) || !len_ok(kind, ny))
     return fail(MJX_E_SIZE, std::string("grid ") + std::to_string(ny) + "x" + std::to_string(nx) +
-                                (kind == R2R_DST ? " not supported: ny+1 and nx+1 must be powers of two in [16, 1024]"
-                                                 : " not supported: ny and nx must be powers of two in [16, 1024]"));
+                                " not supported: ny and nx must be in [2, 2048]");
   return solve2d(dtype, kind, B, ny, nx, rhs, in_pitch, in_istride, out, out_pitch, out_istride, lamx, lamy, inst,
                  mmode, nullptr, 0.0, 0.0, work, stream);
 }
@@ -142,9 +178,9 @@ int mjx_r2r_solve2d_bc(int dtype, int kind, int B, int ny, int nx, const void* r
   if (mmode != R2R_HELM && mmode != R2R_POISSON)
     return fail(MJX_E_ARG, "boundary values need mmode 0 (Helmholtz, alpha = 0) or 1 (Poisson)");
   if (!side_values || !(dx > 0.0) || !(dy > 0.0)) return fail(MJX_E_ARG, "side_values and positive dx, dy required");
-  if (line_len(kind, nx) < 0 || line_len(kind, ny) < 0)
+  if (!len_ok(kind, nx) || !len_ok(kind, ny))
     return fail(MJX_E_SIZE, std::string("grid ") + std::to_string(ny) + "x" + std::to_string(nx) +
-                                " not supported: ny and nx must be powers of two in [16, 1024]");
+                                " not supported: ny and nx must be in [2, 2048]");
   return solve2d(dtype, kind, B, ny, nx, rhs, in_pitch, in_istride, out, out_pitch, out_istride, lamx, lamy, inst,
                  mmode, side_values, dy, dx, work, stream);
 }
@@ -159,7 +195,7 @@ int mjx_r2r_solve1d_bc(int dtype, int kind, int nlines, int n, const void* in, l
     return fail(MJX_E_ARG, "boundary values need mmode 0 (Helmholtz, alpha = 0) or 1 (Poisson)");
   if (!side_values || !(dx > 0.0)) return fail(MJX_E_ARG, "side_values and positive dx required");
   const int L = line_len(kind, n);
-  if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
+  if (!len_ok(kind, n)) return fail(MJX_E_SIZE, size_msg(n));
   if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   if ((rc = ensure_tables(nullptr))) return rc;
@@ -168,6 +204,14 @@ int mjx_r2r_solve1d_bc(int dtype, int kind, int nlines, int n, const void* in, l
   a.inst = inst;
   a.mmode = mmode;
   set_lift(a, kind, side_values, 2, dx, dx);
+  if (L < 0) {
+    CztR2RCall c = line_call(dtype, kind, R2R_APPLY, nlines, n, in, in_stride, out, out_stride, 1.0);
+    c.lam_line = lam;
+    c.inst = inst;
+    c.mmode = mmode;
+    c.lift = a.lift;
+    return czt_r2r(c, (hipStream_t)stream);
+  }
   return run(dtype, kind, R2R_ROW, R2R_APPLY, L, a, 1, (hipStream_t)stream);
 }
 
@@ -176,12 +220,19 @@ int mjx_r2r_solve1d(int dtype, int kind, int nlines, int n, const void* in, long
   int rc = check_common(dtype, kind);
   if (rc) return rc;
   const int L = line_len(kind, n);
-  if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
+  if (!len_ok(kind, n)) return fail(MJX_E_SIZE, size_msg(n));
   if (nlines < 1 || !in || !out || !lam || !inst) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   if (mmode != R2R_HELM && mmode != R2R_POISSON && mmode != R2R_HELM_POW)
     return fail(MJX_E_ARG, "mmode must be 0 (Helmholtz), 1 (Poisson) or 2 (Helmholtz power)");
   if ((rc = ensure_tables(nullptr))) return rc;
+  if (L < 0) {
+    CztR2RCall c = line_call(dtype, kind, R2R_APPLY, nlines, n, in, in_stride, out, out_stride, 1.0);
+    c.lam_line = lam;
+    c.inst = inst;
+    c.mmode = mmode;
+    return czt_r2r(c, (hipStream_t)stream);
+  }
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.lam_line = lam;
   a.inst = inst;
@@ -194,11 +245,22 @@ int mjx_r2r_etd1(int dtype, int kind, int nlines, int n, const void* u, const vo
   int rc = check_common(dtype, kind);
   if (rc) return rc;
   const int L = line_len(kind, n);
-  if (L < 0) return fail(MJX_E_SIZE, std::string("length ") + std::to_string(n) + size_rule(kind));
+  if (!len_ok(kind, n)) return fail(MJX_E_SIZE, size_msg(n));
   if (nlines < 1 || !u || !nonlin || !out || !eig) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   if ((rc = ensure_tables(nullptr))) return rc;
   hipStream_t st = (hipStream_t)stream;
+  if (L < 0) {
+    CztR2RCall c = line_call(dtype, kind, R2R_APPLY, nlines, n, u, in_stride, out, out_stride, 1.0);
+    c.lam_line = eig;
+    c.mmode = R2R_EXP;
+    c.beta0 = dt;
+    if ((rc = czt_r2r(c, st))) return rc;
+    c.in = nonlin;
+    c.mmode = R2R_PHI1;
+    c.accumulate = 1;
+    return czt_r2r(c, st);
+  }
   // out = T^-1[e^{dt eig} T u], then out += T^-1[dt phi1(dt eig) T N]
   R2RArgs a = make(u, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.lam_line = eig;
@@ -219,13 +281,16 @@ int mjx_r2r_transform1d(int dtype, int kind, int inverse, int nlines, int n, con
   int rc = check_common(dtype, kind);
   if (rc) return rc;
   const int L = line_len(kind, n);
-  if (L < 0) return fail(MJX_E_SIZE, "unsupported transform length " + std::to_string(n));
+  if (!len_ok(kind, n)) return fail(MJX_E_SIZE, size_msg(n));
   if (nlines < 1 || !in || !out) return fail(MJX_E_ARG, "null pointer or empty batch");
   if (in_stride < n || out_stride < n) return fail(MJX_E_ARG, "line stride smaller than n");
   if ((rc = ensure_tables(nullptr))) return rc;
+  const int mode = (kind != R2R_DST && inverse) ? R2R_INV : R2R_FWD;
+  if (L < 0)
+    return czt_r2r(line_call(dtype, kind, mode, nlines, n, in, in_stride, out, out_stride, scale),
+                   (hipStream_t)stream);
   R2RArgs a = make(in, 1, in_stride, 0, out, 1, out_stride, 0, nlines);
   a.scale = scale;
-  const int mode = (kind != R2R_DST && inverse) ? R2R_INV : R2R_FWD;
   return run(dtype, kind, R2R_ROW, mode, L, a, 1, (hipStream_t)stream);
 }
 

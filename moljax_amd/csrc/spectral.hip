@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "mjx_common.h"
+#include "czt_api.h"
 #include "gs_chain.h"
 #include "spectral_api.h"
 
@@ -66,9 +67,10 @@ static int timed_launch(int kind, hipStream_t st, Fn&& fn) {
 static int check_grid(int dtype, int B, int ny, int nx) {
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
   if (B < 1) return fail(MJX_E_ARG, "batch must be >= 1");
-  if (!is_pow2_len(nx) || !is_pow2_len(ny))
+  if (spec_generic(nx, ny) && (!czt_len_ok(nx) || !czt_len_ok(ny)))
     return fail(MJX_E_SIZE, "grid " + std::to_string(ny) + "x" + std::to_string(nx) +
-                                " not supported: nx and ny must be powers of two in [16, 4096]");
+                                " not supported: nx and ny must be in [2, 2048], or both powers of two in "
+                                "[16, 4096]");
   if ((long long)B * ny * (nx / 16) > 0x7fffffffLL) return fail(MJX_E_LIMIT, "batch too large for one launch");
   return MJX_OK;
 }
@@ -130,7 +132,7 @@ extern "C" {
 
 long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx) {
   const long long es = dtype == MJX_F64 ? 16 : 8;
-  return (long long)B * ny * nx * es;
+  return (long long)B * ny * spec_work_cols(nx, ny) * es;
 }
 
 int mjx_spectral_pair(int dtype, int B, int ny, int nx, const void* a_in, const void* b_in, long long in_pitch,

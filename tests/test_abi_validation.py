@@ -52,12 +52,14 @@ def test_r2r_checks(lib):
     assert bc2(kind=0) == E_ARG and "kind 1" in _msg(lib)
     assert bc2(kind=3) == E_ARG
     assert bc2(mmode=2) == E_ARG
-    assert bc2(n=24) == E_SIZE
+    assert bc2(n=3000) == E_SIZE and "[2, 2048]" in _msg(lib)
+    assert bc2(n=1) == E_SIZE
     assert bc2(side=None) == E_ARG
     assert bc2(dx=0.0) == E_ARG
-    assert lib.mjx_r2r_solve1d_bc(1, 2, 1, 48, P, 48, P, 48, P, P, 0, sv, 0.1, None) == E_SIZE
+    assert lib.mjx_r2r_solve1d_bc(1, 2, 1, 4097, P, 4097, P, 4097, P, P, 0, sv, 0.1, None) == E_SIZE
     assert lib.mjx_r2r_transform1d(1, 3, 0, 1, 64, P, 64, P, 64, 1.0, None) == E_ARG
-    assert lib.mjx_r2r_solve2d(1, 2, 1, 65, 64, P, 64, 64 * 65, P, 64, 64 * 65, P, P, P, 0, P, None) == E_SIZE
+    assert lib.mjx_r2r_solve2d(1, 2, 1, 2049, 64, P, 64, 64 * 2049, P, 64, 64 * 2049, P, P, P, 0, P, None) == E_SIZE
+    assert lib.mjx_r2r_transform1d(1, 0, 0, 1, 2049, P, 2049, P, 2049, 1.0, None) == E_SIZE
 
 
 def test_benchmark_method_checks(lib):
@@ -73,3 +75,15 @@ def test_benchmark_method_checks(lib):
     assert cn(lap=None) == E_ARG
     assert lib.mjx_gs_bench_workspace_bytes(1, 2, 64, 32) == 6 * 8 * 2 * 64 * 32 + 16 * 2 * 64 + 4 * 2
     assert lib.mjx_reduce2d(1, 9, 1, 4, 4, P, 4, 16, None, 4, 16, 0.0, 0.0, P, 0, P, None) == E_ARG
+
+
+def test_spectral_any_length_checks(lib):
+    # power-of-two grids keep the packed layout, other sizes the half-spectra
+    assert lib.mjx_spectral_workspace_bytes(0, 2, 64, 32) == 2 * 64 * 32 * 8
+    assert lib.mjx_spectral_workspace_bytes(0, 2, 50, 30) == 2 * 50 * 32 * 8
+    assert lib.mjx_spectral_workspace_bytes(1, 1, 64, 33) == 64 * 34 * 16
+    pair = lambda ny, nx: lib.mjx_spectral_pair(1, 1, ny, nx, P, None, nx, nx * ny, P, None, nx, nx * ny, 0, P, P,
+                                                None, P, 1, P, None)
+    assert pair(3000, 32) == E_SIZE and "[2, 2048]" in _msg(lib)
+    assert pair(32, 1) == E_SIZE
+    assert lib.mjx_gs_imex_strang(1, 1, 2049, 2049, P, P, 2049, 2049 * 2049, 0, P, P, None, P, 1, P, None) == E_SIZE

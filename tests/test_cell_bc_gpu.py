@@ -78,7 +78,8 @@ def test_cell_dirichlet_1d(gpu, N):
 
 @pytest.mark.parametrize("kind", ["dirichlet", "neumann"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_bc_values_invert_fd_operator(gpu, kind, dtype):
+@pytest.mark.parametrize("ny,nx", [(32, 64), (30, 45)])
+def test_bc_values_invert_fd_operator(gpu, kind, dtype, ny, nx):
     """solve_helmholtz_bc / solve_poisson_bc with per-side values: equal to
     the oracle's lift + solve, and u - beta lap_h(fill(u)) = rhs with the
     fill of oracle.fd.apply_bc (bc.py) and the native fill + FD Laplacian."""
@@ -86,7 +87,6 @@ def test_bc_values_invert_fd_operator(gpu, kind, dtype):
     from moljax_amd.core.bc_fill import fill_ghosts
     from moljax_amd.core.fft_nonperiodic import solve_helmholtz_bc, solve_poisson_bc
     from moljax_amd.core.operators import laplacian_2d
-    ny, nx = 32, 64
     grid = Grid2D.uniform(nx, ny, 0.0, 2.0, 0.0, 1.0)
     vals = {"left": 0.3, "right": -0.7, "bottom": 1.1, "top": 0.25}
     sides = (vals["left"], vals["right"], vals["bottom"], vals["top"])

@@ -55,7 +55,7 @@ def test_residual_kernel(gpu, bc, n):
 
 
 @pytest.mark.parametrize("precond", ["dst", "jacobi", "none"])
-@pytest.mark.parametrize("bc,n", [("dirichlet", 63), ("periodic", 64)])
+@pytest.mark.parametrize("bc,n", [("dirichlet", 63), ("periodic", 64), ("dirichlet", 50), ("periodic", 40)])
 def test_be_step_counts_match_oracle(gpu, precond, bc, n):
     from moljax_amd.core.newton_krylov import NKParams
     from moljax_amd.core.preconditioners import (DSTDiffusionPreconditioner, DiffusionPreconditioner,

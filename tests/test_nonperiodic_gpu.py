@@ -111,5 +111,5 @@ def test_helmholtz_2d_batched_and_poisson(gpu):
 def test_unsupported_nonperiodic_size(gpu):
     from moljax_amd.core import fft_nonperiodic as fn
     with pytest.raises(ValueError):
-        fn.solve_helmholtz_dirichlet(torch.zeros(64, dtype=torch.float64, device="cuda"),
-                                     fn.laplacian_symbol_dirichlet(64, 0.1), 0.1, 0.1)
+        fn.solve_helmholtz_dirichlet(torch.zeros(4097, dtype=torch.float64, device="cuda"),
+                                     fn.laplacian_symbol_dirichlet(4097, 0.1), 0.1, 0.1)

@@ -144,6 +144,6 @@ def test_fft_cn_scipy_comparison_golden_gpu(gpu):
 
 def test_unsupported_size_raises(gpu):
     from moljax_amd.core import fft_solvers as fs
-    lam = fs.laplacian_symbol_2d(24, 32, 0.1, 0.1)
+    lam = fs.laplacian_symbol_2d(2100, 32, 0.1, 0.1)
     with pytest.raises(ValueError):
-        fs.solve_helmholtz_2d(torch.zeros((24, 32), device="cuda", dtype=torch.float64), lam, 0.1, 0.1)
+        fs.solve_helmholtz_2d(torch.zeros((2100, 32), device="cuda", dtype=torch.float64), lam, 0.1, 0.1)
