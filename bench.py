@@ -9,7 +9,7 @@ the FP64 single-instance GPU form, the FP32-vs-FP64 relative L2 error curve,
 per-step latency percentiles, the roofline of the dominant kernel (HIP events
 on the launch stream) and the CPU oracle timed on this host.  Secondary
 workloads in the same line: the uneven-size batched ETDRK4 plan (1024
-instances of {64,128,256,512}^2), backward-Euler Newton-Krylov on a 511^2
+instances of {64,128,256,512}^2), backward-Euler Newton-Krylov on a 512^2
 Dirichlet grid (iteration counts per step), and the serving loop (Poisson
 arrivals at 200/s, p50/p99 request latency, throughput vs sequential FP64).
 
@@ -348,10 +348,13 @@ def main():
         log(f"c4 batched uneven ETDRK4 ({args.c4_instances} instances, {args.c4_steps} steps)")
         c4 = c4_batched_etdrk4(args.c4_instances, args.c4_steps)
     if args.c3_steps > 0:
-        log(f"c3 Newton-Krylov 511^2 Dirichlet, {args.c3_steps} BE steps")
-        c3 = {"grid": [511, 511], "h": 1.0 / 512, "dt": 1e-2,
-              "fp32_operator": c3_newton_krylov(511, args.c3_steps, torch.float32),
-              "fp64_operator": c3_newton_krylov(511, args.c3_steps, torch.float64)}
+        log(f"c3 Newton-Krylov 512^2 Dirichlet, {args.c3_steps} BE steps")
+        # 512 interior points (DST-I length 513: the chirp-z kernels), and
+        # 511 beside it (length 512: the Stockham kernels)
+        c3 = {"grid": [512, 512], "h": 1.0 / 513, "dt": 1e-2,
+              "fp32_operator": c3_newton_krylov(512, args.c3_steps, torch.float32),
+              "fp64_operator": c3_newton_krylov(512, args.c3_steps, torch.float64),
+              "grid_511_fp32_operator": c3_newton_krylov(511, args.c3_steps, torch.float32)}
     if args.serve_requests > 0:
         log(f"c5 serving {args.serve_requests} requests at {args.serve_rate}/s")
         c5 = c5_serving(args.serve_requests, args.serve_rate)
@@ -395,7 +398,7 @@ def main():
                      "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_s * 1e3,
                      "step_frac": 16.0 * gps32 / HBM_PEAK},
         "batched_uneven_etdrk4": c4,
-        "newton_krylov_511": c3,
+        "newton_krylov_512": c3,
         "serving": c5,
         "cpu_baseline": cpu,
     }

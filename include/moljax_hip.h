@@ -57,7 +57,10 @@ const char* mjx_last_error(void);
 int mjx_init(void);
 
 /* Number of bytes of complex work buffer needed by the periodic spectral
- * entry points for B instances of ny x nx in the given dtype. */
+ * entry points for B instances of ny x nx in the given dtype:
+ * B*ny*nx complex values when nx and ny are both powers of two in
+ * [16, 4096] (packed x-spectra), else B*ny*2*(nx/2+1) (the x-half-spectra
+ * of the two fields, chirp-z path). */
 long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx);
 
 /* Periodic 2-D spectral multiply of up to two real fields per instance:
@@ -73,7 +76,10 @@ long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx);
  *           apply_diffusion_inverse_fft) on periodic 2-D grids, and the
  *           FFT-CN step of benchmarks/benchmark_vs_scipy.py:74-91
  *           (alpha = beta = dt*D/2, lam = -(kx^2+ky^2)).
- * Sizes: nx, ny powers of two in [16, 4096]. */
+ * Sizes: nx, ny both powers of two in [16, 4096] (Stockham line FFTs), or
+ * any nx, ny in [2, 2048] (chirp-z line transforms, one workgroup per
+ * line, the same results to rounding).  The same rule holds for
+ * mjx_gs_imex_strang, mjx_gs_imex_strang_var and mjx_gs_imex. */
 int mjx_spectral_pair(int dtype, int B, int ny, int nx,
                       const void* a_in, const void* b_in, long long in_pitch, long long in_istride,
                       void* a_out, void* b_out, long long out_pitch, long long out_istride,
@@ -253,11 +259,13 @@ int mjx_gs_imex_strang_var(int dtype, int B, int ny, int nx, void* u, void* v, l
 
 /* Non-periodic 2-D Helmholtz / Poisson solve on B instances of ny x nx
  * interior points:  kind 0 homogeneous Dirichlet (DST-I in x and y, vertex
- * grid; ny+1, nx+1 powers of two in [16, 1024]), kind 1 homogeneous Neumann
- * (orthonormal DCT-II; ny, nx powers of two in [16, 1024]), kind 2
+ * grid), kind 1 homogeneous Neumann (orthonormal DCT-II), kind 2
  * homogeneous Dirichlet on a cell-centred grid (DST-II; ghost = -interior as
- * moljax/core/bc.py:237-258 fills it; ny, nx powers of two in [16, 1024];
- * symbol -4/h^2 sin^2(pi k/(2N)), k = 1..N at [k-1]).
+ * moljax/core/bc.py:237-258 fills it; symbol -4/h^2 sin^2(pi k/(2N)),
+ * k = 1..N at [k-1]).  Sizes: any ny, nx in [2, 2048]; DST-I with ny+1,
+ * nx+1 and the others with ny, nx powers of two in [16, 1024] run the
+ * Stockham kernels, other sizes the chirp-z kernels (also for every other
+ * mjx_r2r_* entry point).
  *   u = T^-1[ m(kx, ky) T[rhs] ],  mmode 0: m = (1 + alpha lam)/(1 - beta lam),
  *   mmode 1 (Poisson): m = 1/lam (0 where |lam| < 1e-14),  lam = lamx + lamy,
  *   mmode 2: the mmode-0 multiplier raised to the power inst[8b + 7] per

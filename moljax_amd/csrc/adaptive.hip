@@ -29,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "czt_api.h"
 #include "explicit_api.h"
 #include "gs_chain.h"
 #include "mjx_common.h"
@@ -290,7 +291,7 @@ Layout layout(int dtype, int scheme, int B, int ny, int nx) {
   Layout L;
   L.nblk = (int)std::min<long long>(256, std::max<long long>(1, (2 * n + 4095) / 4096));
   L.W = 0;
-  L.yf = L.W + al((long long)B * n * 2 * es);
+  L.yf = L.W + al((long long)B * ny * spec_work_cols(nx, ny) * 2 * es);
   L.yh = L.yf + al((long long)B * 2 * n * es);
   L.full = L.yh + al((long long)B * 2 * n * es);
   L.half = L.full + al((long long)B * sizeof(InstParams));
@@ -404,9 +405,10 @@ int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, 
     return fail(MJX_E_ARG, "adaptive scheme must be 0/1 (IMEX Strang/Euler) or 3/4/5 (explicit Euler/SSPRK3/RK4)");
   if (B > 65535) return fail(MJX_E_LIMIT, "at most 65535 instances per call");
   if (B < 1) return fail(MJX_E_ARG, "batch must be >= 1");
-  if (!is_explicit(scheme) && (!is_pow2_len(nx) || !is_pow2_len(ny)))
+  if (!is_explicit(scheme) && spec_generic(nx, ny) && (!czt_len_ok(nx) || !czt_len_ok(ny)))
     return fail(MJX_E_SIZE, "grid " + std::to_string(ny) + "x" + std::to_string(nx) +
-                                " not supported: nx and ny must be powers of two in [16, 4096]");
+                                " not supported: nx and ny must be in [2, 2048], or both powers of two in "
+                                "[16, 4096]");
   if (ny < 1 || nx < 1) return fail(MJX_E_ARG, "empty grid");
   if (!u || !v || !model || !ctl || !state || !t_hist || !dt_hist || !work)
     return fail(MJX_E_ARG, "null pointer argument");
