@@ -10,8 +10,9 @@ per-step latency percentiles, the roofline of the dominant kernel (HIP events
 on the launch stream) and the CPU oracle timed on this host.  Secondary
 workloads in the same line: the uneven-size batched ETDRK4 plan (1024
 instances of {64,128,256,512}^2), backward-Euler Newton-Krylov on a 512^2
-Dirichlet grid (iteration counts per step), and the serving loop (Poisson
-arrivals at 200/s, p50/p99 request latency, throughput vs sequential FP64).
+Dirichlet grid (iteration counts per step), the serving loop (Poisson
+arrivals at 200/s, p50/p99 request latency, throughput vs sequential FP64),
+and the same IMEX-Strang batch on a 250^2 grid (the chirp-z kernels).
 
 Prints ONE JSON line.  Only the cpu_baseline leg touches the oracle.
 """
@@ -40,17 +41,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def setup(dtype, B, seeds):
+def setup(dtype, B, seeds, n=N):
     import torch
     from moljax_amd.core import Grid2D, create_gray_scott_periodic_fft
     from moljax_amd.core._arrays import inst_table, spectral_work
     from moljax_amd.core.stepping import _gs_rows
     from moljax_amd.workloads import gray_scott_batch
-    grid = Grid2D.uniform(N, N, 0.0, 2.5, 0.0, 2.5)
+    grid = Grid2D.uniform(n, n, 0.0, 2.5, 0.0, 2.5)
     model, cache, diff = create_gray_scott_periodic_fft(grid, dtype=dtype, **P)
-    U, V = gray_scott_batch(N, N, seeds, dtype=dtype)
+    U, V = gray_scott_batch(n, n, seeds, dtype=dtype)
     rows = inst_table(_gs_rows(model, DT, diff, B))
-    work = spectral_work(dtype, B, N, N)
+    work = spectral_work(dtype, B, n, n)
     return cache.laplacian_symbol, U, V, rows, work
 
 
@@ -59,10 +60,10 @@ def run(U, V, lam, rows, work, steps):
     gs_imex_strang_inplace(U, V, lam, rows, steps, work=work)
 
 
-def time_form(dtype, B, steps, warmup, kernel_timing=False):
+def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N):
     import torch
     from moljax_amd import _hip
-    lam, U, V, rows, work = setup(dtype, B, list(range(B)))
+    lam, U, V, rows, work = setup(dtype, B, list(range(B)), n)
     run(U, V, lam, rows, work, warmup)
     torch.cuda.synchronize()
     if kernel_timing:
@@ -287,6 +288,8 @@ def main():
                     help="comma-separated batch sizes to time as well ('' to skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (byte-counter calibration)")
+    ap.add_argument("--anylen-n", type=int, default=250,
+                    help="side of the any-length (chirp-z) IMEX-Strang leg at the headline batch (0: skip)")
     ap.add_argument("--c4-steps", type=int, default=100, help="steps of the uneven batched ETDRK4 workload (0: skip)")
     ap.add_argument("--c4-instances", type=int, default=1024)
     ap.add_argument("--c3-steps", type=int, default=10, help="Newton-Krylov backward-Euler steps (0: skip)")
@@ -337,6 +340,15 @@ def main():
         log(f"sweep B={b}")
         t, _, _, _ = time_form(torch.float32, b, K, W)
         sweep[str(b)] = b * N * N * K / t
+
+    anylen = None
+    if args.anylen_n > 0:
+        n2 = args.anylen_n
+        log(f"any-length IMEX-Strang {n2}^2 B={B}")
+        ta, _, fina, _ = time_form(torch.float32, B, K, W, n=n2)
+        anylen = {"grid": [n2, n2], "batch": B, "steps": K, "value": B * n2 * n2 * K / ta,
+                  "unit": "grid-point-steps/s", "ms_per_step": 1e3 * ta / K, "finite": fina,
+                  "vs_power_of_two_headline": (B * n2 * n2 * K / ta) / gps32}
 
     curve = None
     if not args.no_curve:
@@ -397,6 +409,7 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dom,
                      "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_s * 1e3,
                      "step_frac": 16.0 * gps32 / HBM_PEAK},
+        "any_length_imex_strang": anylen,
         "batched_uneven_etdrk4": c4,
         "newton_krylov_512": c3,
         "serving": c5,

@@ -20,6 +20,16 @@ inline bool spec_generic(int nx, int ny) { return !(is_pow2_len(nx) && is_pow2_l
 // chirp-z layout.
 inline long long spec_work_cols(int nx, int ny) { return spec_generic(nx, ny) ? 2LL * (nx / 2 + 1) : nx; }
 
+// Chirp-z tables of one plan: n points, period P, doubled offsets K2 = 2 ko
+// (output) and N2 = 2 no (input), convolution length L (a power of two,
+// >= max(16, 2n - 1)); each table is [c (n) | d (n) | FFT_L(e)/L (L)].
+struct CztTables {
+  int n = 0, L = 0;
+  const cpx<float>* t32 = nullptr;
+  const cpx<double>* t64 = nullptr;
+};
+int czt_tables(int n, int P, int K2, int N2, CztTables& out);
+
 #define MJX_CZT_DECL(T)                                                                                       \
   int czt_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,     \
                        cpx<T>* W, hipStream_t st, double in_scale);                                           \
@@ -52,5 +62,12 @@ struct CztR2RCall {
   const double* lift;       // left, right, bottom, top contributions, or null
 };
 int czt_r2r(const CztR2RCall& c, hipStream_t st);
+
+// Per-precision launch of the real line kernel (czt_f32.hip / czt_f64.hip).
+struct CztR2RArgs;
+int czt_r2r_launch(const CztR2RArgs& a, int orient, bool apply, const CztTables& h1, const CztTables& h2, int B,
+                   hipStream_t st, float);
+int czt_r2r_launch(const CztR2RArgs& a, int orient, bool apply, const CztTables& h1, const CztTables& h2, int B,
+                   hipStream_t st, double);
 
 }  // namespace mjx
