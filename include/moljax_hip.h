@@ -345,13 +345,16 @@ int mjx_etdrk4_coeffs_form(int dtype, int form, int ny, int nx, const double* la
                            double Dv, double dt, void* coef, void* stream);
 
 /* Batched coupled Gray-Scott ETDRK4 (Cox-Matthews), n_steps in place on
- * (u, v) of B square n x n instances (n in {32, ..., 1024}); per-instance F, k
+ * (u, v) of B ny x nx instances: square n x n with n in {32, ..., 1024} on
+ * the Stockham kernels, any other sides in [2, 2048] on the chirp-z kernels
+ * (the workspace is then six planes of ny * (nx+2) complex values per
+ * instance; the coefficient tables are the same); per-instance F, k
  * from inst rows; coef from mjx_etdrk4_coeffs.  Four row passes (inverse
  * x-FFT, N(u,v), forward x-FFT) and four column passes (y-FFT, stage
  * combination, inverse y-FFT) per step.  Replaces the coupled ETDRK4 of
  * benchmarks/benchmark_gray_scott.py:369-423 and serves
  * moljax/core/fft_integrators.py:252-341 (etdrk4_step) for Gray-Scott.
- * work: mjx_etdrk4_workspace_bytes(dtype, B, n, n). */
+ * work: mjx_etdrk4_workspace_bytes(dtype, B, ny, nx). */
 long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx);
 int mjx_gs_etdrk4(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
                   int fill_ghosts, const void* coef, const double* inst, int n_steps, void* work, void* stream);
@@ -392,8 +395,9 @@ int mjx_etd_phi_coeffs(int dtype, int ny, int nx, const double* lamx, const doub
                        double dt, void* coef, void* stream);
 
 /* Batched coupled Gray-Scott ETD1 (order 1) or ETD2 (order 2), n_steps in
- * place on (u, v) of B square n x n instances, one row pass and one column
- * pass per step; coef from mjx_etd_phi_coeffs; work as for mjx_gs_etdrk4.
+ * place on (u, v) of B ny x nx instances (sizes as mjx_gs_etdrk4), one row
+ * pass and one column pass per step; coef from mjx_etd_phi_coeffs; work as
+ * for mjx_gs_etdrk4.
  *   ETD1: u^ <- E u^ + dt phi1 N^            (fft_integrators.py:119-174)
  *   ETD2: ... + dt phi2 (N^ - N^_{n-1})      (fft_integrators.py:177-249)
  * prev (ETD2 only): 0 = no N_{n-1} (the first step is ETD1, as the

@@ -37,11 +37,20 @@ int czt_tables(int n, int P, int K2, int N2, CztTables& out);
                          long long istride, int ghosts, hipStream_t st, double keep, const InstParams* react); \
   int czt_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn, int mstep); \
   int czt_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,          \
-                    const InstParams* ip, int nrep, hipStream_t st, int mstep);
+                    const InstParams* ip, int nrep, hipStream_t st, int mstep);                               \
+  int czt_etdrk4_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,       \
+                     const T* coef, const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st);          \
+  int czt_etd_exp_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,      \
+                      const T* coef, const InstParams* ip, int order, int prev, const T* nu, const T* nv,      \
+                      int n_steps, cpx<T>* work, hipStream_t st);
 
 MJX_CZT_DECL(float)
 MJX_CZT_DECL(double)
 #undef MJX_CZT_DECL
+
+// Complex elements of the any-length ETD workspace (six planes of
+// B * ny * (nx + 2)).
+inline long long czt_etd_work_elems(int B, int ny, int nx) { return 6LL * B * ny * (nx + 2); }
 
 // Non-periodic line work (kind 0 DST-I, 1 DCT-II, 2 DST-II), mode R2R_FWD,
 // R2R_INV or R2R_APPLY, over B instances of nlines lines of n points.

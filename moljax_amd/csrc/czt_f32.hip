@@ -20,6 +20,16 @@ int czt_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy
                   const InstParams* ip, int nrep, hipStream_t st, int mstep) {
   return CztLaunch<T>::col_apply(ny, B, nx, W, lamx, lamy, lam2d, ip, nrep, st, mstep);
 }
+int czt_etdrk4_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts, const T* coef,
+                   const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st) {
+  return CztLaunch<T>::etdrk4_run(B, ny, nx, u, v, pitch, istride, ghosts, coef, ip, n_steps, work, st);
+}
+int czt_etd_exp_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                    const T* coef, const InstParams* ip, int order, int prev, const T* nu, const T* nv, int n_steps,
+                    cpx<T>* work, hipStream_t st) {
+  return CztLaunch<T>::etd_exp_run(B, ny, nx, u, v, pitch, istride, ghosts, coef, ip, order, prev, nu, nv, n_steps,
+                                   work, st);
+}
 int czt_r2r_launch(const CztR2RArgs& a, int orient, bool apply, const CztTables& h1, const CztTables& h2, int B,
                    hipStream_t st, T) {
   return CztLaunch<T>::r2r(a, orient, apply, h1, h2, B, st);

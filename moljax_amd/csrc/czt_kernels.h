@@ -33,6 +33,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "etd_kernels.h"
 #include "mjx_common.h"
 #include "mjx_fft.h"
 #include "r2r_kernels.h"
@@ -339,11 +340,13 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_row_inv_unpack(
   }
 }
 
-// Fused Gray-Scott row pass on the half-spectra, in place: inverse x-DFT,
-// explicit reaction (RXN_HEUN, or forward Euler over dt, dt/2 or two dt/2
-// steps), forward x-DFT.
-template <typename T, int L>
-__global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_row_gs(cpx<T>* __restrict__ W,
+// Fused Gray-Scott row pass on the half-spectra: inverse x-DFT, a
+// pointwise step, forward x-DFT, Win -> Wout (may alias).  NLOP false: the
+// explicit reaction in place (RXN_HEUN, or forward Euler over dt, dt/2 or
+// two dt/2 steps); NLOP true: the nonlinear term N(u, v) / nx of the ETD
+// chains (their x-spectra carry the x-DFT divided by nx).
+template <typename T, int L, bool NLOP>
+__global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_row_gs(const cpx<T>* Win, cpx<T>* Wout,
                                                                  const InstParams* __restrict__ ip, int ny, int nh,
                                                                  int tiles, int rxn, int mstep, CztPlan<T> px,
                                                                  const cpx<T>* __restrict__ tw) {
@@ -357,8 +360,9 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_row_gs(cpx<T>* __res
   const int y = (w - inst * tiles) * C::NL + rl;
   const bool valid = y < ny;
   const int nx = px.n;
-  cpx<T>* wrow = W + ((long long)inst * ny + y) * 2 * nh;
+  const cpx<T>* wrow = Win + ((long long)inst * ny + y) * 2 * nh;
   const T F = (T)ip[inst].F, kk = (T)ip[inst].k, dt = (T)ip[inst].dt, hdt = (T)(0.5 * ip[inst].dt);
+  const T inv_nx = (T)(1.0 / nx);
   TwCache<T, L> twc;
   load_twiddles(twc, j, tw);
   cpx<T> v[C::P];
@@ -378,7 +382,9 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_row_gs(cpx<T>* __res
       const T u0 = z.re, v0 = z.im;
       T r1u, r1v, r2u, r2v;
       gs_react(u0, v0, F, kk, r1u, r1v);
-      if (rxn == RXN_HEUN) {
+      if constexpr (NLOP) {
+        o = mk(r1u * inv_nx, r1v * inv_nx);
+      } else if (rxn == RXN_HEUN) {
         const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
         gs_react(ut, vt, F, kk, r2u, r2v);
         o = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
@@ -398,7 +404,7 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_row_gs(cpx<T>* __res
     v[m] = o;
   }
   czt_conv<T, L, CZT_ROW, C::NL>(v, j, lrow, twc, px.e);
-  czt_split_store<T, L>(v, px, j, lrow, wrow, nh, valid);
+  czt_split_store<T, L>(v, px, j, lrow, Wout + ((long long)inst * ny + y) * 2 * nh, nh, valid);
 }
 
 // Column pass over the 2*nh half-spectrum columns: y-DFT, multiplier
@@ -463,6 +469,64 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_col_apply(
   for (int m = 0; m < C::P; ++m) {
     const int ky = j + C::TN * m;
     if (ky < ny) col[(long long)ky * cols] = conjg(cmul(py.c[ky], v[m]));
+  }
+}
+
+// ETD column stage over the half-spectrum columns: y-DFT of the stage
+// input(s), the stage arithmetic of etd_stage on the separated planes (row
+// stride sw; column c of the half-spectra is separated column c), inverse
+// y-DFT into Wout.  Every column is one field at one kx.
+template <typename T, int L, int STAGE>
+__global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_col_etd(
+    const cpx<T>* __restrict__ Win, const cpx<T>* __restrict__ Win2, cpx<T>* __restrict__ Uh, cpx<T>* __restrict__ Nn,
+    cpx<T>* __restrict__ Fh, const T* __restrict__ coef, cpx<T>* __restrict__ Wout, int nh, int sw, int tiles,
+    CztPlan<T> py, const cpx<T>* __restrict__ tw) {
+  using C = CztCfg<T, L>;
+  __shared__ cpx<T> lds[C::LDS];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int inst = w / tiles;
+  int cl, j;
+  czt_place<CZT_COL, C::TN, C::NL>(cl, j);
+  const int cols = 2 * nh;
+  const int c = (w - inst * tiles) * C::NL + cl;
+  const bool valid = c < cols;
+  const int ny = py.n;
+  const long long tstride = (long long)ny * sw;
+  const T s = (T)(1.0 / (double)ny);
+  const long long woff = (long long)inst * ny * cols + c;
+  TwCache<T, L> twc;
+  load_twiddles(twc, j, tw);
+  cpx<T> v[C::P], v2[C::P];
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int ky = j + C::TN * m;
+    const bool in = valid && ky < ny;
+    v[m] = in ? cmul(py.d[ky], Win[woff + (long long)ky * cols]) : mk((T)0, (T)0);
+    v2[m] = (two_inputs<STAGE>() && in) ? cmul(py.d[ky], Win2[woff + (long long)ky * cols]) : mk((T)0, (T)0);
+  }
+  cpx<T>* lc = czt_lline<T, L, CZT_COL>(lds, cl);
+  czt_conv<T, L, CZT_COL, C::NL>(v, j, lc, twc, py.e);
+  if constexpr (two_inputs<STAGE>()) czt_conv<T, L, CZT_COL, C::NL>(v2, j, lc, twc, py.e);
+  const long long sbase = (long long)inst * tstride;
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int ky = j + C::TN * m;
+    cpx<T> z = mk((T)0, (T)0);
+    if (valid && ky < ny) {
+      const cpx<T> x = cmul(py.c[ky], v[m]);
+      const cpx<T> x2 = two_inputs<STAGE>() ? cmul(py.c[ky], v2[m]) : mk((T)0, (T)0);
+      const long long crow = (long long)ky * sw + c;
+      z = cmul(py.d[ky], conjg(etd_stage<T, STAGE>(sbase + crow, crow, tstride, coef, x, x2, Uh, Nn, Fh, s)));
+    }
+    v[m] = z;
+  }
+  if constexpr (STAGE == STAGE_STORE_N) return;
+  czt_conv<T, L, CZT_COL, C::NL>(v, j, lc, twc, py.e);
+  if (!valid) return;
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int ky = j + C::TN * m;
+    if (ky < ny) Wout[woff + (long long)ky * cols] = conjg(cmul(py.c[ky], v[m]));
   }
 }
 

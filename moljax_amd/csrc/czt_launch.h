@@ -74,10 +74,39 @@ struct CztLaunch {
       const int tiles = (ny + C::NL - 1) / C::NL;
       int rc = czt_grid_check((long long)tiles * B);
       if (rc) return rc;
-      hipLaunchKernelGGL((k_czt_row_gs<T, L>), dim3(tiles * B), dim3(C::BT), 0, st, W, ip, ny, nh, tiles, rxn, mstep,
-                         p, twiddles<T>());
+      hipLaunchKernelGGL((k_czt_row_gs<T, L, false>), dim3(tiles * B), dim3(C::BT), 0, st, W, W, ip, ny, nh, tiles,
+                         rxn, mstep, p, twiddles<T>());
       return check_launch("czt_row_gs");
     }
+  };
+  template <int L>
+  struct RowNl {
+    static int run(int B, int ny, int nh, const cpx<T>* Win, cpx<T>* Wout, const InstParams* ip, CztPlan<T> p,
+                   hipStream_t st) {
+      using C = CztCfg<T, L>;
+      const int tiles = (ny + C::NL - 1) / C::NL;
+      int rc = czt_grid_check((long long)tiles * B);
+      if (rc) return rc;
+      hipLaunchKernelGGL((k_czt_row_gs<T, L, true>), dim3(tiles * B), dim3(C::BT), 0, st, Win, Wout, ip, ny, nh,
+                         tiles, 0, -1, p, twiddles<T>());
+      return check_launch("czt_row_nl");
+    }
+  };
+  template <int STAGE>
+  struct ColEtd {
+    template <int L>
+    struct F {
+      static int run(int B, int nh, int sw, const cpx<T>* Win, const cpx<T>* Win2, cpx<T>* Uh, cpx<T>* Nn,
+                     cpx<T>* Fh, const T* coef, cpx<T>* Wout, CztPlan<T> p, hipStream_t st) {
+        using C = CztCfg<T, L>;
+        const int tiles = (2 * nh + C::NL - 1) / C::NL;
+        int rc = czt_grid_check((long long)tiles * B);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_czt_col_etd<T, L, STAGE>), dim3(tiles * B), dim3(C::BT), 0, st, Win, Win2, Uh, Nn, Fh,
+                           coef, Wout, nh, sw, tiles, p, twiddles<T>());
+        return check_launch("czt_col_etd");
+      }
+    };
   };
   template <int L>
   struct ColApply {
@@ -145,6 +174,100 @@ struct CztLaunch {
   static int r2r(const CztR2RArgs& a, int orient, bool apply, const CztTables& h1, const CztTables& h2, int B,
                  hipStream_t st) {
     return czt_with_len<R2R>(h1.L, a, orient, apply, czt_dev<T>(h1), czt_dev<T>(h2), B, st);
+  }
+
+  // ---------------------------------------------------------------- ETD
+  // Workspace planes of the any-length ETD chains: the x-half-spectra of the
+  // state (Ws), of a stage value (Wg) and of N (Wn), then the separated
+  // spectra Uh, Nn, Fh, each B * ny * (nx + 2) complex values.
+  struct Planes {
+    cpx<T>*Ws, *Wg, *Wn, *Uh, *Nn, *Fh;
+  };
+  static Planes planes(cpx<T>* work, int B, int ny, int nx) {
+    const long long plane = (long long)B * ny * (nx + 2);
+    Planes p;
+    p.Ws = work;
+    p.Wg = p.Ws + plane;
+    p.Wn = p.Wg + plane;
+    p.Uh = p.Wn + plane;
+    p.Nn = p.Uh + plane;
+    p.Fh = p.Nn + plane;
+    return p;
+  }
+  struct Ctx {
+    CztTables hx, hy;
+    int B, ny, nx, nh, sw;
+    const InstParams* ip;
+    const T* coef;
+    hipStream_t st;
+    int row_nl(const cpx<T>* in, cpx<T>* out) const {
+      return czt_with_len<RowNl>(hx.L, B, ny, nh, in, out, ip, czt_dev<T>(hx), st);
+    }
+    template <int STAGE>
+    int col(const Planes& p, const cpx<T>* in, const cpx<T>* in2, cpx<T>* out) const {
+      return czt_with_len<ColEtd<STAGE>::template F>(hy.L, B, nh, sw, in, in2, p.Uh, p.Nn, p.Fh, coef, out,
+                                                      czt_dev<T>(hy), st);
+    }
+  };
+  static int ctx(Ctx& c, int B, int ny, int nx, const T* coef, const InstParams* ip, hipStream_t st) {
+    int rc = czt_tables(nx, nx, 0, 0, c.hx);
+    if (!rc) rc = czt_tables(ny, ny, 0, 0, c.hy);
+    c.B = B;
+    c.ny = ny;
+    c.nx = nx;
+    c.nh = nx / 2 + 1;
+    c.sw = nx + 2;
+    c.ip = ip;
+    c.coef = coef;
+    c.st = st;
+    return rc;
+  }
+  // ETDRK4 (Cox-Matthews, the four-stage chain of etd_kernels.h), n_steps
+  // steps on the packed state; the pack scales by 1/nx like row_nl.
+  static int etdrk4_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                        const T* coef, const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st) {
+    Ctx c;
+    int rc = ctx(c, B, ny, nx, coef, ip, st);
+    const Planes p = planes(work, B, ny, nx);
+    if (!rc) rc = row_fwd_pack(nx, B, ny, u, v, pitch, istride, p.Ws, st, 1.0 / nx);
+    for (int s = 0; s < n_steps && !rc; ++s) {
+      rc = c.row_nl(p.Ws, p.Wn);
+      if (!rc) rc = c.template col<1>(p, p.Ws, p.Wn, p.Wg);
+      if (!rc) rc = c.row_nl(p.Wg, p.Wn);
+      if (!rc) rc = c.template col<2>(p, p.Wn, nullptr, p.Wg);
+      if (!rc) rc = c.row_nl(p.Wg, p.Wn);
+      if (!rc) rc = c.template col<3>(p, p.Wn, nullptr, p.Wg);
+      if (!rc) rc = c.row_nl(p.Wg, p.Wn);
+      if (!rc) rc = c.template col<4>(p, p.Wn, nullptr, p.Ws);
+    }
+    if (rc) return rc;
+    return row_inv_unpack(nx, B, ny, p.Ws, u, v, pitch, istride, ghosts, st, 0.0, nullptr);
+  }
+  // Exponential Euler (order 1) / AB2 (order 2); prev as etd_exp_run of the
+  // power-of-two path (0 none, 1 physical N_{n-1} in nu/nv, 2 the Nn plane).
+  static int etd_exp_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                         const T* coef, const InstParams* ip, int order, int prev, const T* nu, const T* nv,
+                         int n_steps, cpx<T>* work, hipStream_t st) {
+    Ctx c;
+    int rc = ctx(c, B, ny, nx, coef, ip, st);
+    const Planes p = planes(work, B, ny, nx);
+    if (!rc && order == 2 && prev == 1) {
+      rc = row_fwd_pack(nx, B, ny, nu, nv, pitch, istride, p.Wg, st, 1.0 / nx);
+      if (!rc) rc = c.template col<STAGE_STORE_N>(p, p.Wg, nullptr, nullptr);
+    }
+    if (!rc) rc = row_fwd_pack(nx, B, ny, u, v, pitch, istride, p.Ws, st, 1.0 / nx);
+    cpx<T>* cur = p.Ws;
+    for (int s = 0; s < n_steps && !rc; ++s) {
+      cpx<T>* next = cur == p.Ws ? p.Wg : p.Ws;
+      rc = c.row_nl(cur, p.Wn);
+      if (rc) break;
+      if (order == 1) rc = c.template col<STAGE_ETD1>(p, cur, p.Wn, next);
+      else if (s == 0 && prev == 0) rc = c.template col<STAGE_ETD2_FIRST>(p, cur, p.Wn, next);
+      else rc = c.template col<STAGE_ETD2>(p, cur, p.Wn, next);
+      cur = next;
+    }
+    if (rc) return rc;
+    return row_inv_unpack(nx, B, ny, cur, u, v, pitch, istride, ghosts, st, 0.0, nullptr);
   }
 };
 

@@ -27,13 +27,16 @@ def test_etd_exp_checks(lib):
     assert call(order=3) == E_ARG and "order" in _msg(lib)
     assert call(order=1, prev=1) == E_ARG
     assert call(order=2, prev=3) == E_ARG
-    assert call(n=48) == E_SIZE
+    assert call(n=3000) == E_SIZE and "[2, 2048]" in _msg(lib)
     assert call(order=2, prev=1, nu=None) == E_ARG and "nu_prev" in _msg(lib)
     assert call(steps=-1) == E_ARG
     assert call(order=2, prev=1, steps=0) == 0  # nothing to do: no device work
-    assert lib.mjx_etd_phi_coeffs(1, 64, 63, P, P, 1e-5, 1e-5, 1.0, P, None) == E_ARG
+    assert lib.mjx_etd_phi_coeffs(1, 1, 63, P, P, 1e-5, 1e-5, 1.0, P, None) == E_ARG
     assert lib.mjx_etdrk4_coeffs_form(1, 2, 64, 64, P, P, 1e-5, 1e-5, 1.0, P, None) == E_ARG
     assert lib.mjx_etd_phi_coeff_elems(64, 64) == 3 * 64 * 66
+    # any-length ETD workspace: six planes of ny * (nx + 2)
+    assert lib.mjx_etdrk4_workspace_bytes(0, 2, 48, 50) == 8 * 6 * 2 * 48 * 52
+    assert lib.mjx_etdrk4_workspace_bytes(0, 2, 64, 64) == 8 * 2 * (3 * 64 * 64 + 3 * 64 * 66)
 
 
 def test_imex_and_batch_scheme_checks(lib):

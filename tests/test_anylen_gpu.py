@@ -12,6 +12,7 @@ import pytest
 import scipy.fft as sfft
 import torch
 
+from oracle import etdrk4 as oe
 from oracle import gray_scott as ogs
 from oracle import nonperiodic as onp
 from oracle import spectral as osp
@@ -218,3 +219,41 @@ def test_gray_scott_euler_ssprk2_any_length(gpu, scheme):
         ref = ogs.imex_ssprk2_step(y0, 1.0, P["F"], P["k"], P["Du"], P["Dv"], lam)
     for k in ("u", "v"):
         assert ogs.rel_l2(_np(out[k]), ref[k]) < 1e-12, k
+
+
+def _rect_ic(ny, nx, seed):
+    y = ogs.initial_condition(ny, nx, seed, square=10)
+    return y["u"][1:-1, 1:-1].copy(), y["v"][1:-1, 1:-1].copy()
+
+
+@pytest.mark.parametrize("ny,nx", [(48, 48), (40, 50), (37, 33)])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 3e-5)])
+def test_etdrk4_any_length(gpu, ny, nx, dtype, tol):
+    """10 coupled ETDRK4 steps with the FP64 contour coefficients on grids the
+    power-of-two chain does not take (rectangular, odd) vs the oracle."""
+    from moljax_amd.core.fft_integrators import etdrk4_coefficients, gs_etdrk4_inplace, spectral_symbol_1d
+    lx, ly = spectral_symbol_1d(nx), spectral_symbol_1d(ny)
+    F, k, Du, Dv, dt, steps = 0.035, 0.065, 2e-5, 1e-5, 1.0, 10
+    u0, v0 = _rect_ic(ny, nx, 0)
+    U = torch.from_numpy(u0).to("cuda", dtype)
+    V = torch.from_numpy(v0).to("cuda", dtype)
+    coef = etdrk4_coefficients(lx, ly, Du, Dv, dt, dtype)
+    gs_etdrk4_inplace(U, V, coef, [[0, 0, 0, 0, F, k, dt]], steps)
+    ru, rv = oe.gs_etdrk4(u0, v0, lx, ly, Du, Dv, F, k, dt, steps)
+    eu, ev = ogs.rel_l2(_np(U), ru), ogs.rel_l2(_np(V), rv)
+    print(f"ETDRK4 {ny}x{nx} {dtype}: rel_l2 u={eu:.2e} v={ev:.2e}", flush=True)
+    assert eu <= tol and ev <= tol
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("ny,nx", [(40, 50), (37, 33)])
+def test_etd_exp_any_length(gpu, order, ny, nx):
+    from moljax_amd.core.fft_integrators import etd_phi_coefficients, gs_etd_exp_inplace, spectral_symbol_1d
+    lx, ly = spectral_symbol_1d(nx), spectral_symbol_1d(ny)
+    F, k, dt, steps = 0.035, 0.065, 1.0, 9
+    u0, v0 = _rect_ic(ny, nx, 1)
+    U, V = torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()
+    coef = etd_phi_coefficients(lx, ly, 2e-5, 1e-5, dt, torch.float64)
+    gs_etd_exp_inplace(U, V, coef, [[0, 0, 0, 0, F, k, dt]], order, steps)
+    ru, rv, _ = oe.gs_etd(u0, v0, lx, ly, 2e-5, 1e-5, F, k, dt, steps, order)
+    assert ogs.rel_l2(_np(U), ru) < 1e-12 and ogs.rel_l2(_np(V), rv) < 1e-12

@@ -80,13 +80,14 @@ def test_batched_per_instance_and_alone(gpu, order):
         assert ogs.rel_l2(_np(U[i]), ru) < 1e-12 and ogs.rel_l2(_np(V[i]), rv) < 1e-12
 
 
-def test_etd2_history_modes_agree(gpu):
+@pytest.mark.parametrize("n", [64, 48])
+def test_etd2_history_modes_agree(gpu, n):
     """ETD2's N_{n-1} three ways: one call of 8 steps; 3 + 5 steps with the
     workspace's N^ (prev 2) — to rounding; 2 + 1 steps (prev 2), then 5 steps
     from the physical N(u_2) (prev 1) in a fresh workspace — vs the oracle."""
     from moljax_amd.core.fft_integrators import etd_phi_coefficients, gs_etd_exp_inplace, spectral_symbol_1d
     from moljax_amd.core.operators import gray_scott_reaction
-    n, F, k = 64, 0.04, 0.06
+    F, k = 0.04, 0.06
     lam = spectral_symbol_1d(n)
     u0, v0 = _ic(n, 2)
     coef = etd_phi_coefficients(lam, lam, DU, DV, 1.0, torch.float64)
@@ -96,8 +97,11 @@ def test_etd2_history_modes_agree(gpu):
     Bs = [torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()]
     w = gs_etd_exp_inplace(Bs[0], Bs[1], coef, rows, 2, 3)
     gs_etd_exp_inplace(Bs[0], Bs[1], coef, rows, 2, 5, prev=2, work=w)
-    # a call boundary round-trips the state through physical space: rounding only
-    assert ogs.rel_l2(_np(Bs[0]), _np(A[0])) < 1e-14 and ogs.rel_l2(_np(Bs[1]), _np(A[1])) < 1e-14
+    # a call boundary round-trips the state through physical space: rounding
+    # only (the chirp-z transforms of n = 48 run four FFTs of 128 per
+    # round trip against one FFT of 64, hence the wider bound)
+    rb = 1e-14 if n & (n - 1) == 0 else 1e-13
+    assert ogs.rel_l2(_np(Bs[0]), _np(A[0])) < rb and ogs.rel_l2(_np(Bs[1]), _np(A[1])) < rb
     C = [torch.from_numpy(u0).cuda(), torch.from_numpy(v0).cuda()]
     w = gs_etd_exp_inplace(C[0], C[1], coef, rows, 2, 2)
     n2 = gray_scott_reaction(C[0], C[1], F, k)  # N(u_2)
