@@ -187,9 +187,10 @@ int mjx_gs_imex_adaptive(int dtype, int scheme, int B, int ny, int nx, void* u, 
  * tables tables[2g], tables[2g+1] = {ETDRK4 coefficients (from
  * mjx_etdrk4_coeffs), NULL} or {lamx, lamy} for Strang, and workspace
  * work[g] (mjx_etdrk4_workspace_bytes / mjx_spectral_workspace_bytes).
- * scheme 0 ETDRK4 (n in 32..1024), 1 IMEX-Strang (n in 16..4096), 2 the
- * Gray-Scott benchmark's splitting (benchmarks/benchmark_gray_scott.py
- * :268-289; tables {lamx, lamy}, inst beta = dt*D; n in 16..4096).
+ * scheme 0 ETDRK4, 1 IMEX-Strang, 2 the Gray-Scott benchmark's splitting
+ * (benchmarks/benchmark_gray_scott.py:268-289; tables {lamx, lamy}, inst
+ * beta = dt*D).  Sizes: any n in [2, 2048] (chirp-z kernels), and the
+ * Stockham sides (ETDRK4 32..1024, the others 16..4096).
  * mjx_batch_run advances every instance by n_steps: the steps run as
  * replays of one hipGraph of steps_per_graph steps whose branches are the
  * groups (one forked stream per group), the remainder as direct launches.
@@ -208,7 +209,9 @@ int mjx_batch_destroy(void* plan);
  * instances of n x n (reps timed repetitions each, HIP events), keeps the
  * fastest in the process-wide tile table, and returns them in chosen[0]
  * (row) and chosen[1] (column).  scheme 0 ETDRK4, 1 IMEX-Strang, 2 the
- * benchmark splitting (the Strang kernels' tiles).
+ * benchmark splitting (the Strang kernels' tiles).  Sides the Stockham
+ * kernels do not take (chirp-z kernels, fixed tiles) return chosen = {0, 0}
+ * without timing anything.
  * mjx_tune_get / mjx_tune_set read and write that table (kind 0 spectral
  * row, 1 spectral column, 2 ETDRK4 row, 3 ETDRK4 column; lines 0 = default),
  * so callers can persist tuned tiles. */

@@ -1,6 +1,7 @@
 // Batched launch of many Gray-Scott instances of uneven grid sizes.
 //
-// A plan holds G groups; group g is B_g instances of one square size n_g,
+// A plan holds G groups; group g is B_g instances of one square size n_g
+// (any side in [2, 2048]; power-of-two sides run the Stockham kernels),
 // with its own state, parameter rows, coefficient/symbol tables and
 // workspace (all caller-owned device memory).  run(n_steps):
 //   1. pack every group into its spectral workspace (direct launches);
@@ -25,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "czt_api.h"
 #include "mjx_common.h"
 #include "mjx_fft.h"
 #include "spectral_api.h"
@@ -49,6 +51,9 @@ namespace {
 
 enum { SCHEME_ETDRK4 = 0, SCHEME_STRANG = 1, SCHEME_BENCH_STRANG = 2 };
 
+// Sizes of the Stockham ETDRK4 chain; other sides run the chirp-z one.
+bool etd_stockham(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024; }
+
 struct Group {
   int scheme;
   int n, B;
@@ -62,8 +67,11 @@ struct Group {
 template <typename T>
 struct Ops {
   static int pack(const Group& g, hipStream_t st) {
-    if (g.scheme == SCHEME_ETDRK4)
+    if (g.scheme == SCHEME_ETDRK4) {
+      if (!etd_stockham(g.n))
+        return czt_etdrk4_pack(g.B, g.n, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
       return etd_pack(g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
+    }
     if (g.scheme == SCHEME_BENCH_STRANG) {
       // the first step up to its closing reaction half-step
       int rc = spec_row_fwd_pack(g.n, g.B, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work,
@@ -81,7 +89,11 @@ struct Ops {
   // Benchmark splitting: each middle step closes the previous step's
   // reaction and opens its own (two half-steps in one row pass).
   static int advance(const Group& g, int steps, hipStream_t st) {
-    if (g.scheme == SCHEME_ETDRK4) return etd_advance(g.B, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
+    if (g.scheme == SCHEME_ETDRK4) {
+      if (!etd_stockham(g.n))
+        return czt_etdrk4_advance(g.B, g.n, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
+      return etd_advance(g.B, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
+    }
     const bool bench = g.scheme == SCHEME_BENCH_STRANG;
     for (int s = 0; s < steps; ++s) {
       int rc = spec_row_gs(g.n, g.B, g.n, (cpx<T>*)g.work, g.ip, st, bench ? RXN_HALF2 : RXN_HEUN);
@@ -93,8 +105,11 @@ struct Ops {
     return MJX_OK;
   }
   static int finish(const Group& g, int ghosts, hipStream_t st) {
-    if (g.scheme == SCHEME_ETDRK4)
+    if (g.scheme == SCHEME_ETDRK4) {
+      if (!etd_stockham(g.n))
+        return czt_etdrk4_unpack(g.B, g.n, g.n, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts, (cpx<T>*)g.work, st);
       return etd_unpack(g.B, g.n, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts, (cpx<T>*)g.work, st);
+    }
     if (g.scheme == SCHEME_BENCH_STRANG)
       return spec_row_inv_unpack(g.n, g.B, g.n, (const cpx<T>*)g.work, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts,
                                  st, 0.0, g.ip);
@@ -324,8 +339,9 @@ int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const in
   p->K = steps_per_graph;
   for (int g = 0; g < n_groups; ++g) {
     const int ng = n[g];
-    const bool ok = scheme == SCHEME_ETDRK4 ? (ng == 32 || ng == 64 || ng == 128 || ng == 256 || ng == 512 || ng == 1024)
-                                            : is_pow2_len(ng);
+    // power-of-two sides run the Stockham kernels, other sides in [2, 2048]
+    // the chirp-z ones (tables are created by the pack, before any capture)
+    const bool ok = (scheme == SCHEME_ETDRK4 ? etd_stockham(ng) : is_pow2_len(ng)) || czt_len_ok(ng);
     if (!ok) return fail(MJX_E_SIZE, "batch group " + std::to_string(g) + ": unsupported size " + std::to_string(ng));
     if (B[g] < 1 || !u[g] || !v[g] || !inst[g] || !work[g] || !tables[2 * g] ||
         (scheme != SCHEME_ETDRK4 && !tables[2 * g + 1]))
@@ -360,9 +376,14 @@ int mjx_autotune(int dtype, int scheme, int n, int B, int reps, int* chosen, voi
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
   if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG && scheme != SCHEME_BENCH_STRANG)
     return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4), 1 (Strang) or 2 (benchmark splitting)");
-  if (!is_pow2_len(n) || (scheme == SCHEME_ETDRK4 && (n < 32 || n > 1024)))
-    return fail(MJX_E_SIZE, "unsupported size " + std::to_string(n));
   if (B < 1 || reps < 1 || !chosen) return fail(MJX_E_ARG, "B, reps >= 1 and chosen required");
+  const bool stockham = scheme == SCHEME_ETDRK4 ? etd_stockham(n) : is_pow2_len(n);
+  if (!stockham) {
+    // the chirp-z kernels of other sides have fixed tiles: nothing to tune
+    if (!czt_len_ok(n)) return fail(MJX_E_SIZE, "unsupported size " + std::to_string(n));
+    chosen[0] = chosen[1] = 0;
+    return MJX_OK;
+  }
   int rc = ensure_tables(nullptr);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;

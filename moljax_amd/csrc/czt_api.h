@@ -42,7 +42,13 @@ int czt_tables(int n, int P, int K2, int N2, CztTables& out);
                      const T* coef, const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st);          \
   int czt_etd_exp_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,      \
                       const T* coef, const InstParams* ip, int order, int prev, const T* nu, const T* nv,      \
-                      int n_steps, cpx<T>* work, hipStream_t st);
+                      int n_steps, cpx<T>* work, hipStream_t st);                                              \
+  int czt_etdrk4_pack(int B, int ny, int nx, const T* u, const T* v, long long pitch, long long istride,      \
+                      cpx<T>* work, hipStream_t st);                                                           \
+  int czt_etdrk4_advance(int B, int ny, int nx, const T* coef, const InstParams* ip, int n_steps,             \
+                         cpx<T>* work, hipStream_t st);                                                        \
+  int czt_etdrk4_unpack(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,    \
+                        cpx<T>* work, hipStream_t st);
 
 MJX_CZT_DECL(float)
 MJX_CZT_DECL(double)

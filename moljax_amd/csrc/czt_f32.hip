@@ -30,6 +30,18 @@ int czt_etd_exp_run(int B, int ny, int nx, T* u, T* v, long long pitch, long lon
   return CztLaunch<T>::etd_exp_run(B, ny, nx, u, v, pitch, istride, ghosts, coef, ip, order, prev, nu, nv, n_steps,
                                    work, st);
 }
+int czt_etdrk4_pack(int B, int ny, int nx, const T* u, const T* v, long long pitch, long long istride, cpx<T>* work,
+                    hipStream_t st) {
+  return CztLaunch<T>::etdrk4_pack(B, ny, nx, u, v, pitch, istride, work, st);
+}
+int czt_etdrk4_advance(int B, int ny, int nx, const T* coef, const InstParams* ip, int n_steps, cpx<T>* work,
+                       hipStream_t st) {
+  return CztLaunch<T>::etdrk4_advance(B, ny, nx, coef, ip, n_steps, work, st);
+}
+int czt_etdrk4_unpack(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                      cpx<T>* work, hipStream_t st) {
+  return CztLaunch<T>::etdrk4_unpack(B, ny, nx, u, v, pitch, istride, ghosts, work, st);
+}
 int czt_r2r_launch(const CztR2RArgs& a, int orient, bool apply, const CztTables& h1, const CztTables& h2, int B,
                    hipStream_t st, T) {
   return CztLaunch<T>::r2r(a, orient, apply, h1, h2, B, st);

@@ -222,14 +222,29 @@ struct CztLaunch {
     c.st = st;
     return rc;
   }
-  // ETDRK4 (Cox-Matthews, the four-stage chain of etd_kernels.h), n_steps
-  // steps on the packed state; the pack scales by 1/nx like row_nl.
+  // ETDRK4 (Cox-Matthews, the four-stage chain of etd_kernels.h): the pack
+  // (scaled by 1/nx like row_nl), n_steps steps on the packed state, and
+  // the unpack.
+  static int etdrk4_pack(int B, int ny, int nx, const T* u, const T* v, long long pitch, long long istride,
+                         cpx<T>* work, hipStream_t st) {
+    return row_fwd_pack(nx, B, ny, u, v, pitch, istride, planes(work, B, ny, nx).Ws, st, 1.0 / nx);
+  }
+  static int etdrk4_unpack(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                           cpx<T>* work, hipStream_t st) {
+    return row_inv_unpack(nx, B, ny, planes(work, B, ny, nx).Ws, u, v, pitch, istride, ghosts, st, 0.0, nullptr);
+  }
   static int etdrk4_run(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
                         const T* coef, const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st) {
+    int rc = etdrk4_pack(B, ny, nx, u, v, pitch, istride, work, st);
+    if (!rc) rc = etdrk4_advance(B, ny, nx, coef, ip, n_steps, work, st);
+    if (rc) return rc;
+    return etdrk4_unpack(B, ny, nx, u, v, pitch, istride, ghosts, work, st);
+  }
+  static int etdrk4_advance(int B, int ny, int nx, const T* coef, const InstParams* ip, int n_steps, cpx<T>* work,
+                            hipStream_t st) {
     Ctx c;
     int rc = ctx(c, B, ny, nx, coef, ip, st);
     const Planes p = planes(work, B, ny, nx);
-    if (!rc) rc = row_fwd_pack(nx, B, ny, u, v, pitch, istride, p.Ws, st, 1.0 / nx);
     for (int s = 0; s < n_steps && !rc; ++s) {
       rc = c.row_nl(p.Ws, p.Wn);
       if (!rc) rc = c.template col<1>(p, p.Ws, p.Wn, p.Wg);
@@ -240,8 +255,7 @@ struct CztLaunch {
       if (!rc) rc = c.row_nl(p.Wg, p.Wn);
       if (!rc) rc = c.template col<4>(p, p.Wn, nullptr, p.Ws);
     }
-    if (rc) return rc;
-    return row_inv_unpack(nx, B, ny, p.Ws, u, v, pitch, istride, ghosts, st, 0.0, nullptr);
+    return rc;
   }
   // Exponential Euler (order 1) / AB2 (order 2); prev as etd_exp_run of the
   // power-of-two path (0 none, 1 physical N_{n-1} in nu/nv, 2 the Nn plane).

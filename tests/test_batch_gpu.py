@@ -1,7 +1,8 @@
 """Batched uneven-size plans (hipGraph replay, autotuned tiles) on the GPU.
 
-Each instance of a mixed-size batch is bitwise equal to the same instance run
-alone (a plan of one) and matches the float64 oracle (ETDRK4 <= 1e-12,
+Each instance of a mixed-size batch (power-of-two and other sides) is
+bitwise equal to the same instance run alone (a plan of one) and matches the
+float64 oracle (ETDRK4 <= 1e-12,
 IMEX-Strang <= 1e-12 relative L2 after several steps)."""
 import numpy as np
 import pytest
@@ -32,9 +33,11 @@ def _batch(sizes, F, k, scheme, dtype, steps_per_graph=2):
 
 @pytest.mark.parametrize("scheme", ["etdrk4", "strang"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_mixed_batch_equals_alone_and_oracle(gpu, scheme, dtype):
+@pytest.mark.parametrize("sizes", [[64, 32, 64, 128, 32, 64], [48, 64, 100, 48, 33]])
+def test_mixed_batch_equals_alone_and_oracle(gpu, scheme, dtype, sizes):
+    """Power-of-two groups (Stockham kernels) and, in the second size set,
+    groups of other sides (chirp-z kernels) in one plan."""
     from moljax_amd.batch import GrayScottBatch, spectral_symbol_1d
-    sizes = [64, 32, 64, 128, 32, 64]
     F = [0.02 + 0.005 * i for i in range(len(sizes))]
     k = [0.055 + 0.002 * i for i in range(len(sizes))]
     steps = 5
