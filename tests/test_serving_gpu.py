@@ -133,3 +133,29 @@ def test_masked_chain_equals_fixed_chains(gpu):
         dv = float((v[b] - vb[0]).abs().max())
         print(f"instance {b} steps {steps[b]}: max |du| {du:.3e} |dv| {dv:.3e}", flush=True)
         assert torch.equal(u[b], ub[0]) and torch.equal(v[b], vb[0]), b
+
+
+def test_any_length_requests(gpu):
+    """Sides outside the power-of-two rule (the chirp-z kernels) served in
+    one mix with power-of-two sides: alone == mix bitwise (float32), and each
+    kind against the oracle (float64, <= 1e-11)."""
+    from moljax_amd.serving import Request, Server, initial_state
+    specs = []
+    for kind in ("fft_cn", "dst", "dct", "imex_gs"):
+        for n, steps in ((48, 13), (40, 9), (64, 11)):
+            specs.append((kind, n, steps))
+    mk = lambda: [Request(rid=i, kind=k, n=n, D=1e-2 * (1 + i % 3), dt=2e-2, steps=s, seed=300 + i, arrival=0.0)
+                  for i, (k, n, s) in enumerate(specs)]
+    mix = mk()
+    Server(dtype=torch.float32, chunk=8).run(mix, realtime=False)
+    for r in mix:
+        alone = _fresh(r)
+        Server(dtype=torch.float32, chunk=8).run([alone], realtime=False)
+        for a, b in zip(r.state, alone.state):
+            assert torch.equal(a, b), f"request {r.rid} ({r.kind}, n={r.n}) differs alone vs in the mix"
+    reqs = [r for r in mk() if r.n != 64]
+    Server(dtype=torch.float64, chunk=8).run(reqs, realtime=False)
+    for r in reqs:
+        ic = [t.cpu().numpy() for t in initial_state(_fresh(r), torch.float64)]
+        for got, want in zip(r.state, _oracle(r, ic)):
+            assert ogs.rel_l2(got.cpu().numpy(), want) < 1e-11, (r.kind, r.n)
