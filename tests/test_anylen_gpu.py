@@ -257,3 +257,15 @@ def test_etd_exp_any_length(gpu, order, ny, nx):
     gs_etd_exp_inplace(U, V, coef, [[0, 0, 0, 0, F, k, dt]], order, steps)
     ru, rv, _ = oe.gs_etd(u0, v0, lx, ly, 2e-5, 1e-5, F, k, dt, steps, order)
     assert ogs.rel_l2(_np(U), ru) < 1e-12 and ogs.rel_l2(_np(V), rv) < 1e-12
+
+
+@pytest.mark.parametrize("nx,lines", [(50, 3), (100, 1), (30, 2100)])
+def test_periodic_helmholtz_1d_any_length(gpu, nx, lines):
+    """fft_solvers.solve_helmholtz_1d on line lengths outside the
+    power-of-two rule (a stack of lines, chunked at 2048 rows)."""
+    from moljax_amd.core import fft_solvers as fs
+    rhs = np.random.default_rng(nx + lines).standard_normal((lines, nx))
+    dx = 1.0 / nx
+    out = _np(fs.solve_helmholtz_1d(torch.from_numpy(rhs).cuda(), fs.laplacian_symbol_1d(nx, dx), 0.1, 0.3))
+    ref = osp.solve_helmholtz_1d(rhs, osp.laplacian_symbol_1d(nx, dx), 0.1, 0.3)
+    assert ogs.rel_l2(out, ref) < 1e-12
