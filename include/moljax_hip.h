@@ -77,9 +77,12 @@ long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx);
  *           FFT-CN step of benchmarks/benchmark_vs_scipy.py:74-91
  *           (alpha = beta = dt*D/2, lam = -(kx^2+ky^2)).
  * Sizes: nx, ny both powers of two in [16, 4096] (Stockham line FFTs), or
- * any nx, ny in [2, 2048] (chirp-z line transforms, one workgroup per
- * line, the same results to rounding).  The same rule holds for
- * mjx_gs_imex_strang, mjx_gs_imex_strang_var and mjx_gs_imex. */
+ * any nx, ny in [2, 2048] (chirp-z line transforms, the same results to
+ * rounding).  The same rule holds for mjx_gs_imex_strang,
+ * mjx_gs_imex_strang_var and mjx_gs_imex.  The first call for a side the
+ * chirp-z kernels serve evaluates and uploads that side's plan tables
+ * (blocking copies, kept for the process): make it before any stream
+ * capture. */
 int mjx_spectral_pair(int dtype, int B, int ny, int nx,
                       const void* a_in, const void* b_in, long long in_pitch, long long in_istride,
                       void* a_out, void* b_out, long long out_pitch, long long out_istride,
