@@ -269,3 +269,29 @@ def test_periodic_helmholtz_1d_any_length(gpu, nx, lines):
     out = _np(fs.solve_helmholtz_1d(torch.from_numpy(rhs).cuda(), fs.laplacian_symbol_1d(nx, dx), 0.1, 0.3))
     ref = osp.solve_helmholtz_1d(rhs, osp.laplacian_symbol_1d(nx, dx), 0.1, 0.3)
     assert ogs.rel_l2(out, ref) < 1e-12
+
+
+def test_any_length_api_inputs(gpu):
+    """Non-contiguous views, extra batch dimensions and NumPy inputs on grids
+    the chirp-z kernels serve, against the oracle."""
+    from moljax_amd.core import fft_nonperiodic as fn
+    from moljax_amd.core import fft_solvers as fs
+    rng = np.random.default_rng(21)
+    base = rng.standard_normal((50, 30))
+    lam = fs.laplacian_symbol_2d(30, 50, 1.0 / 30, 1.0 / 50)
+    lam_np = osp.laplacian_symbol_2d(30, 50, 1.0 / 30, 1.0 / 50)
+    t = torch.from_numpy(base).cuda().t()  # (30, 50), non-contiguous
+    assert not t.is_contiguous()
+    out = _np(fs.solve_helmholtz_2d(t, lam, 0.1, 0.2))
+    assert ogs.rel_l2(out, osp.solve_helmholtz_2d(base.T, lam_np, 0.1, 0.2)) < 1e-12
+    stack = rng.standard_normal((2, 3, 30, 50))
+    out = _np(fs.solve_helmholtz_2d(stack, lam, 0.1, 0.2))  # NumPy input, batch dims
+    assert out.shape == stack.shape
+    for i in range(2):
+        for j in range(3):
+            assert ogs.rel_l2(out[i, j], osp.solve_helmholtz_2d(stack[i, j], lam_np, 0.1, 0.2)) < 1e-12
+    x = rng.standard_normal((7, 45)).astype(np.float32)
+    d = fn.solve_helmholtz_neumann(x[2], fn.laplacian_symbol_neumann(45, 1.0 / 45), 0.1, 0.5)
+    assert d.dtype == torch.float32
+    ref = onp.solve_helmholtz_neumann(x[2].astype(np.float64), onp.laplacian_symbol_neumann(45, 1.0 / 45), 0.1, 0.5)
+    assert ogs.rel_l2(_np(d), ref) < 5e-6
