@@ -15,7 +15,8 @@ from .fft_solvers import (FFTCache1D, FFTCache2D, build_wavenumbers_1d, build_wa
                           create_fft_cache_2d, solve_helmholtz_1d, solve_helmholtz_2d, solve_helmholtz,
                           extract_interior,
                           embed_interior, solve_diffusion_fft_field, apply_diffusion_inverse_fft, fft_cn_step,
-                          apply_diffusion_fft, diffusion_rhs_fft)
+                          apply_diffusion_fft, diffusion_rhs_fft, exponential_filter_1d, exponential_filter_2d,
+                          apply_spectral_filter_interior, apply_spectral_filter_field, apply_spectral_filter_state)
 from .model import (MOLModel, create_gray_scott_model, create_gray_scott_periodic_fft, get_fft_cache,
                     get_diffusivities)
 from .stepping import (IntegratorType, AdaptiveResult, imex_strang_step, imex_euler_step, imex_ssprk2_step,

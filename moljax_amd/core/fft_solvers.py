@@ -200,22 +200,16 @@ _LINE_SYMBOLS: Dict[tuple, torch.Tensor] = {}
 _POW2_SIDES = (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
 
 
-def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
-    """fft_solvers.py:222-258: (I - dt D lap) u = rhs on periodic lines of nx
-    points (a batch along the leading axes).  The lines are stacked as the
-    rows of a 2-D array whose y symbol is zero, so the 2-D kernels apply the
-    1-D multiplier row by row; for power-of-two nx the stacks are padded with
-    zero rows to a power of two of at least 16 rows (the Stockham kernels),
+def _lines_spectral(t: torch.Tensor, sym: np.ndarray, row, nrep: int) -> torch.Tensor:
+    """Apply the 1-D spectral multiplier of (sym, row, nrep) along the last
+    axis of t (a batch of periodic lines): the lines are stacked as the rows
+    of a 2-D array whose y symbol is zero, so the 2-D kernels apply the 1-D
+    multiplier row by row.  For power-of-two nx the stacks are padded with
+    zero rows to a power of two of at least 16 rows (the Stockham kernels);
     other lengths (2..2048) run the any-length kernels on the stack as it
     is."""
-    t = as_device(rhs_interior)
-    t = t.to(field_dtype(t))
     nx = t.shape[-1]
-    lam = as_device(laplacian_symbol, torch.float64).reshape(-1)
-    if lam.numel() != nx:
-        raise ValueError(f"symbol length {lam.numel()} does not match the line length {nx}")
     lines = t.reshape(-1, nx)
-    lam_np = lam.cpu().numpy()
     out = torch.empty_like(lines)
     pow2 = nx in _POW2_SIDES
     step = 4096 if pow2 else 2048
@@ -223,16 +217,29 @@ def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> t
         chunk = lines[i0:i0 + step]
         L = chunk.shape[0]
         rows = max(16, 1 << (L - 1).bit_length()) if pow2 else max(2, L)
-        key = (lam_np.tobytes(), rows)
+        key = (sym.tobytes(), rows)
         lam2d = _LINE_SYMBOLS.get(key)
         if lam2d is None:
-            lam2d = _tag_separable(torch.from_numpy(np.broadcast_to(lam_np, (rows, nx)).copy()).to(default_device()),
-                                   lam_np, np.zeros(rows))
+            lam2d = _tag_separable(torch.from_numpy(np.broadcast_to(sym, (rows, nx)).copy()).to(default_device()),
+                                   sym, np.zeros(rows))
             _LINE_SYMBOLS[key] = lam2d
         buf = torch.zeros((rows, nx), dtype=t.dtype, device=t.device)
         buf[:L] = chunk
-        out[i0:i0 + L] = solve_helmholtz_2d(buf, lam2d, dt, D)[:L]
+        res, _ = spectral_pair(buf, None, _tables(lam2d, t.dtype), [row], nrep=nrep)
+        out[i0:i0 + L] = res.view(rows, nx)[:L]
     return out.reshape(t.shape)
+
+
+def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
+    """fft_solvers.py:222-258: (I - dt D lap) u = rhs on periodic lines of nx
+    points (a batch along the leading axes)."""
+    t = as_device(rhs_interior)
+    t = t.to(field_dtype(t))
+    nx = t.shape[-1]
+    lam = as_device(laplacian_symbol, torch.float64).reshape(-1)
+    if lam.numel() != nx:
+        raise ValueError(f"symbol length {lam.numel()} does not match the line length {nx}")
+    return _lines_spectral(t, lam.cpu().numpy(), [float(dt) * float(D), 0.0, 0.0, 0.0], 1)
 
 
 def solve_helmholtz(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
@@ -303,6 +310,92 @@ def apply_diffusion_inverse_fft(state_rhs: Dict[str, torch.Tensor], grid: Grid2D
         for (name, _), o in zip(pair, outs):
             result[name] = o
     return {name: result[name] for name in state_rhs}
+
+
+# ------------------------------------------------------- spectral filters
+def exponential_filter_1d(nx: int, strength: float = 36.0, order: int = 36, dtype=torch.float64) -> torch.Tensor:
+    """fft_solvers.py:477-503: sigma(k) = exp(-strength (|k|/k_max)^order)
+    with |k| folded as min(k, nx - k) and normalised by nx // 2 (clamped to
+    1), in FFT order."""
+    k = np.arange(nx, dtype=np.float64)
+    kn = np.minimum(np.minimum(k, nx - k) / float(nx // 2), 1.0)
+    sig = np.exp(-float(strength) * np.power(kn, order))
+    return torch.from_numpy(sig).to(default_device(), _np_dtype(dtype))
+
+
+def exponential_filter_2d(ny: int, nx: int, strength: float = 36.0, order: int = 36,
+                          dtype=torch.float64) -> torch.Tensor:
+    """fft_solvers.py:506-531: the outer product sigma_y sigma_x, (ny, nx)."""
+    sx = exponential_filter_1d(nx, strength, order, torch.float64)
+    sy = exponential_filter_1d(ny, strength, order, torch.float64)
+    return torch.outer(sy, sx).to(_np_dtype(dtype))
+
+
+def _symmetrised(sig: np.ndarray) -> np.ndarray:
+    """(sigma(k) + sigma(-k)) / 2 (every axis reversed modulo its length):
+    the real part of ifft(sigma * fft(u)) for real u and real sigma is the
+    transform with this even multiplier, which the half-spectrum kernels
+    apply."""
+    rev = sig
+    for ax in range(sig.ndim):
+        rev = np.roll(np.flip(rev, axis=ax), 1, axis=ax)
+    return 0.5 * (sig + rev)
+
+
+def _filter_tables(filter_kernel, dtype: torch.dtype):
+    """Kernel tables of a filter kernel (cached on the tensor): the even part
+    as a full (ny, nx) multiplier table (2-D) or a line symbol (1-D)."""
+    t = filter_kernel if isinstance(filter_kernel, torch.Tensor) else as_device(filter_kernel, torch.float64)
+    if t.is_complex():
+        raise NotImplementedError("spectral filters take real kernels")
+    cache = getattr(t, "_mjx_filter", None)
+    if cache is None:
+        sym = _symmetrised(t.detach().double().cpu().numpy())
+        cache = {"sym": sym, "full": torch.from_numpy(sym).to(default_device())}
+        try:
+            t._mjx_filter = cache
+        except AttributeError:
+            pass
+    return cache
+
+
+def apply_spectral_filter_interior(u_interior, filter_kernel) -> torch.Tensor:
+    """fft_solvers.py:534-555: ifft(fft(u) * sigma).real for a 1-D u (or a
+    batch of lines) with an (nx,) kernel, ifft2(fft2(u) * sigma).real for
+    (..., ny, nx) with an (ny, nx) kernel; one spectral round trip on the
+    kernels (multiplier sigma, the matvec form of mjx_spectral_pair)."""
+    u = as_device(u_interior)
+    u = u.to(field_dtype(u))
+    k = filter_kernel if isinstance(filter_kernel, torch.Tensor) else as_device(filter_kernel, torch.float64)
+    c = _filter_tables(k, u.dtype)
+    if k.dim() == 1:
+        if k.numel() != u.shape[-1]:
+            raise ValueError(f"filter length {k.numel()} does not match the line length {u.shape[-1]}")
+        return _lines_spectral(u, c["sym"], [0.0, 0.0, 1.0, 0.0], -1)
+    if tuple(k.shape) != tuple(u.shape[-2:]):
+        raise ValueError(f"filter shape {tuple(k.shape)} does not match the grid {tuple(u.shape[-2:])}")
+    out, _ = spectral_pair(u, None, _tables(c["full"], u.dtype), [[0.0, 0.0, 1.0, 0.0]], nrep=-1)
+    return out.view(u.shape)
+
+
+def apply_spectral_filter_field(f_padded, grid: GridType, filter_kernel) -> torch.Tensor:
+    """fft_solvers.py:558-581: filter the interior, keep the ghosts."""
+    f = as_device(f_padded)
+    f = f.to(field_dtype(f))
+    return embed_interior(apply_spectral_filter_interior(extract_interior(f, grid), filter_kernel), grid, f)
+
+
+def apply_spectral_filter_state(state: Dict[str, torch.Tensor], grid: GridType, filter_kernel,
+                                filter_fields: Optional[Dict[str, bool]] = None) -> Dict[str, torch.Tensor]:
+    """fft_solvers.py:584-616: every field (or those filter_fields marks
+    True, default True) filtered; the others returned as they are."""
+    out = {}
+    for name, f in state.items():
+        if filter_fields is None or filter_fields.get(name, True):
+            out[name] = apply_spectral_filter_field(f, grid, filter_kernel)
+        else:
+            out[name] = f
+    return out
 
 
 # --------------------------------------------------------------- FFT-CN

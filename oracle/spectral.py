@@ -113,3 +113,24 @@ def periodic_fd_laplacian(u, dy, dx):
     """Periodic 5-point Laplacian on an interior-only array (axis 0 = y)."""
     return ((np.roll(u, -1, 0) - 2 * u + np.roll(u, 1, 0)) / dy ** 2
             + (np.roll(u, -1, 1) - 2 * u + np.roll(u, 1, 1)) / dx ** 2)
+
+
+def exponential_filter_1d(nx, strength=36.0, order=36):
+    """fft_solvers.py:477-503: exp(-strength (min(k, nx-k) / (nx//2))^order),
+    the normalised index clamped to 1."""
+    k = np.arange(nx, dtype=np.float64)
+    kn = np.minimum(np.minimum(k, nx - k) / (nx // 2), 1.0)
+    return np.exp(-strength * kn ** order)
+
+
+def exponential_filter_2d(ny, nx, strength=36.0, order=36):
+    """fft_solvers.py:506-531: outer(sigma_y, sigma_x)."""
+    return np.outer(exponential_filter_1d(ny, strength, order), exponential_filter_1d(nx, strength, order))
+
+
+def apply_spectral_filter_interior(u, sigma):
+    """fft_solvers.py:534-555: ifft(fft(u) sigma).real (1-D) or
+    ifft2(fft2(u) sigma).real (2-D, the last two axes)."""
+    if sigma.ndim == 1:
+        return np.fft.ifft(np.fft.fft(u, axis=-1) * sigma, axis=-1).real
+    return np.fft.ifft2(np.fft.fft2(u) * sigma).real

@@ -582,3 +582,31 @@ def test_bdf2_coefficients_and_order():
     e1, e2 = abs(r1 - np.exp(2.0 / 3.0 * lam * T)), abs(r2 - np.exp(2.0 / 3.0 * lam * T))
     assert 1.7 < e1 / e2 < 2.3 and e2 < 5e-3, (e1, e2)
     assert abs(r2 - np.exp(lam * T)) > 0.1
+
+
+def test_spectral_filter_closed_forms():
+    """Oracle spectral filters (fft_solvers.py:477-555) against closed forms:
+    sigma(0) = 1, sigma(nx/2) = e^-strength, even in k, order 2 in closed
+    form; the 2-D kernel is the outer product; a real Fourier mode comes back
+    times the even part (sigma(k) + sigma(-k))/2 of any real kernel."""
+    for nx in (16, 17, 50):
+        s = spectral.exponential_filter_1d(nx, strength=3.0, order=2)
+        k = np.arange(nx)
+        kf = np.minimum(k, nx - k)
+        assert s[0] == 1.0
+        assert np.allclose(s, np.exp(-3.0 * np.minimum(kf / (nx // 2), 1.0) ** 2), rtol=0, atol=1e-15)
+        assert np.array_equal(s[1:], s[1:][::-1])
+        if nx % 2 == 0:
+            assert s[nx // 2] == math.exp(-3.0)
+    s2 = spectral.exponential_filter_2d(12, 20, 5.0, 4)
+    assert np.allclose(s2, np.outer(spectral.exponential_filter_1d(12, 5.0, 4), spectral.exponential_filter_1d(20, 5.0, 4)))
+    rng = np.random.default_rng(0)
+    ny, nx, p, q = 12, 20, 3, 5
+    y, x = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    mode = np.cos(2 * np.pi * (p * y / ny + q * x / nx))
+    sig = rng.uniform(0.2, 1.0, size=(ny, nx))  # not even
+    even = 0.5 * (sig[p, q] + sig[-p, -q])
+    assert np.allclose(spectral.apply_spectral_filter_interior(mode, sig), even * mode, atol=1e-14)
+    line = np.sin(2 * np.pi * 3 * np.arange(nx) / nx)
+    s1 = rng.uniform(0.2, 1.0, size=nx)
+    assert np.allclose(spectral.apply_spectral_filter_interior(line, s1), 0.5 * (s1[3] + s1[-3]) * line, atol=1e-14)
