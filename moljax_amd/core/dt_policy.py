@@ -179,3 +179,21 @@ def propose_dt_imex(grid, params: Dict[str, Any], state, t: float, dt_old: float
         dt_ad, new_state = handle_rejected_step(dt_old, err_ratio, controller_state, pid_params, order)
     dt_final = _min(dt_ad, imex_cfl_dt(grid, params, cfl_params))
     return _clip(dt_final, pid_params.dt_min, pid_params.dt_max), new_state
+
+
+def estimate_reaction_stiffness(state, params: Dict[str, Any]) -> float:
+    """dt_policy.py:505-541: a bound on |df/du| of the reaction: with F, k
+    (Gray-Scott) max(F + k, max|v|^2 + F, 2 max|u| max|v| + F + k), with r
+    (Fisher-KPP) at least r; max|.| by the native FP64 reduction."""
+    from .state import RED_MAXABS, _device_fields, _reduce
+    stiffness = 0.0
+    if "F" in params and "k" in params:
+        F, k = float(params["F"]), float(params["k"])
+        stiffness = max(stiffness, F + k)
+        if "u" in state and "v" in state:
+            um = float(_reduce(RED_MAXABS, _device_fields([state["u"]])).max())
+            vm = float(_reduce(RED_MAXABS, _device_fields([state["v"]])).max())
+            stiffness = max(stiffness, vm * vm + F, 2.0 * um * vm + F + k)
+    if "r" in params:
+        stiffness = max(stiffness, float(params["r"]))
+    return stiffness

@@ -14,7 +14,7 @@ row pass and one column pass per step (mjx_gs_etd_exp) with FP64 tables
 e^z, dt phi1(z), dt phi2(z) (mjx_etd_phi_coeffs); ETD2 keeps the previous
 step's spectral nonlinear term on the device between steps and calls.
 """
-from typing import Callable, Dict, Optional, Tuple
+from typing import Callable, Dict, NamedTuple, Optional, Tuple
 
 import numpy as np
 import torch
@@ -328,3 +328,25 @@ def diffusion_only_etd1(u0, t_end: float, dt: float, op: DiffusionOperator) -> t
     for _ in range(n):
         u = op.exp_matvec(u, h)
     return u
+
+
+class ETDStepResult(NamedTuple):
+    """fft_integrators.py:32-36."""
+    state: StateDict
+    t: float
+    diagnostics: dict
+
+
+def imex_euler_step(state: StateDict, t: float, dt: float, linear_ops: Dict[str, DiffusionOperator],
+                    nonlinear_rhs: Callable[[StateDict, float], StateDict]) -> StateDict:
+    """fft_integrators.py:545-585: per field, (I - dt L) u_new = u + dt N(u)
+    with L's FFT solve, or u + dt N(u) for fields without an operator (the
+    a x + b y kernel and the operators' spectral solve)."""
+    from .state import axpby
+    N = nonlinear_rhs(state, t)
+    out = {}
+    for name, u in state.items():
+        rhs = axpby(1.0, as_device(u), float(dt), as_device(N[name]))
+        op = linear_ops.get(name)
+        out[name] = op.solve(rhs, dt) if op is not None else rhs
+    return out

@@ -161,6 +161,45 @@ def d1_upwind_2d(f, v, grid: Grid2D, axis: int) -> torch.Tensor:
     return _fd(f, grid, FD_D1UP, axis, vel=v)
 
 
+def _diffusion_op(field_name: str, D_param: str, lap, cfl: float) -> LinearOp:
+    from .state import axpby
+
+    def apply_diffusion(state: StateDict, grid, t: float, params: Dict[str, Any]) -> StateDict:
+        D = float(params.get(D_param, 1.0))
+        result = tree_zeros_like(state)
+        result[field_name] = axpby(D, lap(state[field_name], grid))
+        return result
+
+    def dt_bound(grid, params: Dict[str, Any]) -> float:
+        return cfl * grid.min_dx2 / (float(params.get(D_param, 1.0)) + 1e-14)
+
+    return LinearOp(name=f"diffusion_{field_name}", apply=apply_diffusion, dt_bound=dt_bound)
+
+
+def make_diffusion_op_1d(field_name: str, D_param: str = "D") -> LinearOp:
+    """operators.py:392-417: D lap on one field (other fields zero); dt
+    bound 0.5 dx^2 / D."""
+    return _diffusion_op(field_name, D_param, laplacian_1d, 0.5)
+
+
+def make_diffusion_op_2d(field_name: str, D_param: str = "D") -> LinearOp:
+    """operators.py:420-444: D lap_h on one field (other fields zero); dt
+    bound 0.25 min(dx, dy)^2 / D."""
+    return _diffusion_op(field_name, D_param, laplacian_2d, 0.25)
+
+
+def make_reaction_op(reactions: Dict[str, Callable[[StateDict, Dict[str, Any]], torch.Tensor]]) -> NonlinearOp:
+    """operators.py:623-641: the given per-field reaction callables (other
+    fields zero)."""
+    def apply_reaction(state: StateDict, grid, t: float, params: Dict[str, Any]) -> StateDict:
+        result = tree_zeros_like(state)
+        for name, fn in reactions.items():
+            result[name] = fn(state, params)
+        return result
+
+    return NonlinearOp(name="reactions", apply=apply_reaction, dt_bound=None)
+
+
 def gray_scott_reaction(u, v, F: float, k: float):
     """(-u v^2 + F(1-u), u v^2 - (F+k) v) point-wise (mjx_gs_reaction)."""
     tu = as_device(u)

@@ -5,7 +5,7 @@ run the native a x + b y + c kernel and allocate new arrays (the reference's
 functional semantics).  Reductions used for accept/reject decisions
 accumulate in float64.
 """
-from typing import Callable, Dict, Tuple
+from typing import Any, Callable, Dict, Tuple
 
 import torch
 
@@ -176,3 +176,32 @@ def flatten_to_vec(state: StateDict) -> Tuple[torch.Tensor, Callable[[torch.Tens
         return out
 
     return vec, unravel
+
+
+def unflatten_from_vec(flat_vec: torch.Tensor, template_state: StateDict) -> StateDict:
+    """state.py:220-238: the state of template_state's structure and shapes
+    holding flat_vec (views into it)."""
+    return flatten_to_vec(template_state)[1](flat_vec)
+
+
+def state_total_size(state: StateDict) -> int:
+    """state.py:241-251."""
+    return int(sum(f.numel() for f in state.values()))
+
+
+def create_state_from_interior(interior_arrays: Dict[str, Any], grid, dtype=torch.float64) -> StateDict:
+    """state.py:268-298: padded arrays with zero ghosts around the given
+    interiors (fill the ghosts with apply_bc)."""
+    from ._arrays import as_device
+    from .grid import Grid1D, default_device
+    out = {}
+    for name, interior in interior_arrays.items():
+        if isinstance(grid, Grid1D):
+            pad = torch.zeros(grid.nx_total, dtype=dtype, device=default_device())
+            pad[grid.interior_slice] = as_device(interior, dtype)
+        else:
+            pad = torch.zeros((grid.ny_total, grid.nx_total), dtype=dtype, device=default_device())
+            sy, sx = grid.interior_slice
+            pad[sy, sx] = as_device(interior, dtype)
+        out[name] = pad
+    return out

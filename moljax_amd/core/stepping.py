@@ -13,7 +13,7 @@ generic per-op fallback.
 """
 import ctypes
 from enum import IntEnum
-from typing import Dict, NamedTuple, Optional, Tuple
+from typing import Any, Dict, NamedTuple, Optional, Tuple
 
 import numpy as np
 import torch
@@ -21,7 +21,7 @@ import torch
 from .. import _hip
 from ._arrays import as_device, field_dtype, inst_table, spectral_work, view2d
 from .bc import BCType
-from .state import StateDict
+from .state import StateDict, tree_sub
 
 
 class IntegratorType(IntEnum):
@@ -321,6 +321,75 @@ def ssprk3_step(model, y: StateDict, t: float, dt: float) -> StateDict:
 def rk4_step(model, y: StateDict, t: float, dt: float) -> StateDict:
     """stepping.py:127-167 (classical RK4)."""
     return _explicit_step(model, y, dt, 2)
+
+
+def step_explicit(model, y: StateDict, t: float, dt: float, method: int) -> StateDict:
+    """stepping.py:615-643: one explicit step, method 0 Euler, 1 SSPRK3,
+    2 RK4."""
+    if int(method) not in (0, 1, 2):
+        raise ValueError("method must be 0 (Euler), 1 (SSPRK3) or 2 (RK4)")
+    return (euler_step, ssprk3_step, rk4_step)[int(method)](model, y, t, dt)
+
+
+def estimate_error_doubling(model, y: StateDict, t: float, dt: float, step_fn) -> Tuple[StateDict, StateDict]:
+    """stepping.py:502-536: one step of dt and two of dt/2; returns the
+    two-half-step state and (y_half2 - y_full), unscaled by the order."""
+    y_full = step_fn(model, y, t, dt)
+    y_half1 = step_fn(model, y, t, dt / 2.0)
+    y_half2 = step_fn(model, y_half1, t + dt / 2.0, dt / 2.0)
+    return y_half2, tree_sub(y_half2, y_full)
+
+
+def estimate_error_implicit(model, y: StateDict, t: float, dt: float, y_be: StateDict,
+                            y_cn: StateDict) -> StateDict:
+    """stepping.py:539-563: y_cn - y_be."""
+    return tree_sub(y_cn, y_be)
+
+
+def step_explicit_with_error(model, y: StateDict, t: float, dt: float, method: int) -> Tuple[StateDict, StateDict]:
+    """stepping.py:646-680: step doubling with the method's step."""
+    if int(method) not in (0, 1, 2):
+        raise ValueError("method must be 0 (Euler), 1 (SSPRK3) or 2 (RK4)")
+    return estimate_error_doubling(model, y, t, dt, (euler_step, ssprk3_step, rk4_step)[int(method)])
+
+
+def be_step(model, y: StateDict, t: float, dt: float, preconditioner=None, nk_params=None):
+    """stepping.py:170-214 (reaction-diffusion models, native Newton-Krylov)."""
+    from .reaction_diffusion import be_step as _be
+    return _be(model, y, t, dt, preconditioner, nk_params)
+
+
+def cn_step(model, y: StateDict, t: float, dt: float, preconditioner=None, nk_params=None):
+    """stepping.py:217-260."""
+    from .reaction_diffusion import cn_step as _cn
+    return _cn(model, y, t, dt, preconditioner, nk_params)
+
+
+def bdf2_step(model, y: StateDict, y_prev: StateDict, t: float, dt: float, dt_prev: float, preconditioner=None,
+              nk_params=None):
+    """stepping.py:263-310."""
+    from .reaction_diffusion import bdf2_step as _bdf2
+    return _bdf2(model, y, y_prev, t, dt, dt_prev, preconditioner, nk_params)
+
+
+class AdaptiveState(NamedTuple):
+    """stepping.py:683-708: the adaptive loop's carried state (the drivers
+    here keep it on the device and in Python locals; the type is kept for
+    callers that build or inspect it)."""
+    t: Any
+    y: StateDict
+    dt: Any
+    y_prev: StateDict
+    dt_prev: Any
+    step_count: Any
+    controller: Any
+    t_history: Any
+    y_history: Dict[str, Any]
+    dt_history: Any
+    write_idx: Any
+    n_accepted: Any
+    n_rejected: Any
+    status: Any
 
 
 def integrate_fixed_dt(model, y0: StateDict, t0: float, t_end: float, dt: float,
