@@ -47,3 +47,7 @@ from .preconditioners import (PrecondContext, Preconditioner, IdentityPreconditi
                               create_diffusion_preconditioner, create_fft_preconditioner)
 from .reaction_diffusion import (ReactionDiffusionModel, create_reaction_diffusion_model, be_step, cn_step,
                                  bdf2_step)
+from .jit_kernels import (phi1, phi2, phi3, etd1_kernel_1d, etd1_kernel_2d, etd2_kernel_1d, etdrk4_kernel_1d,
+                          helmholtz_solve_1d, helmholtz_solve_2d, batched_fft_solve_1d, batched_exp_matvec_1d,
+                          make_etd1_integrator, batched_etd1_kernel_1d, batched_etd2_kernel_1d, multi_operator_etd1_1d,
+                          batched_etd1_kernel_2d, batched_helmholtz_solve_2d)

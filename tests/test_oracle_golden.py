@@ -610,3 +610,29 @@ def test_spectral_filter_closed_forms():
     line = np.sin(2 * np.pi * 3 * np.arange(nx) / nx)
     s1 = rng.uniform(0.2, 1.0, size=nx)
     assert np.allclose(spectral.apply_spectral_filter_interior(line, s1), 0.5 * (s1[3] + s1[-3]) * line, atol=1e-14)
+
+
+def test_jit_kernels_oracle_closed_forms():
+    """oracle/jit_kernels.py pinned by closed forms: phi_k(0) = 1/k!, the
+    recurrences phi_{k+1} = (phi_k - 1/k!)/z away from 0; ETD1 with N = 0
+    multiplies a Fourier mode by e^{dt lam_k}, with lam = 0 gives u + dt N;
+    ETD2 with N_prev = N_n is ETD1; ETDRK4 with four equal N is ETD1 (the
+    weights sum to phi1); the Helmholtz solve inverts 1 - dt D lam on a
+    mode."""
+    from oracle import jit_kernels as oj
+    z0 = np.array([0.0])
+    assert oj.phi1(z0)[0] == 1.0 and oj.phi2(z0)[0] == 0.5 and abs(oj.phi3(z0)[0] - 1 / 6) < 1e-16
+    z = np.array([-3.0, -0.5, 0.2, 1.3])
+    assert np.allclose(oj.phi2(z), (oj.phi1(z) - 1.0) / z, rtol=1e-13)
+    assert np.allclose(oj.phi3(z), (oj.phi2(z) - 0.5) / z, rtol=1e-12)
+    nx, dt, q = 32, 0.3, 5
+    x = np.arange(nx) / nx
+    lam = -(2 * np.pi * np.fft.fftfreq(nx, 1.0 / nx)) ** 2 * 1e-3
+    mode = np.cos(2 * np.pi * q * x)
+    assert np.allclose(oj.etd1_kernel(mode, 0 * mode, lam, dt), np.exp(dt * lam[q]) * mode, atol=1e-14)
+    N = np.sin(2 * np.pi * 3 * x) + 0.3
+    assert np.allclose(oj.etd1_kernel(mode, N, 0 * lam, dt), mode + dt * N, atol=1e-14)
+    assert np.allclose(oj.etd2_kernel_1d(mode, N, N, lam, dt), oj.etd1_kernel(mode, N, lam, dt), atol=1e-14)
+    assert np.allclose(oj.etdrk4_kernel_1d(mode, lam, dt, (N, N, N, N)), oj.etd1_kernel(mode, N, lam, dt),
+                       atol=1e-13)
+    assert np.allclose(oj.helmholtz_solve(mode, lam, dt, 2.0), mode / (1 - 2.0 * dt * lam[q]), atol=1e-14)
