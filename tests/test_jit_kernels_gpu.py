@@ -103,3 +103,30 @@ def test_multi_operator_and_integrator(gpu):
     assert _rel(_np(uf), ref) < 1e-12 and tf == pytest.approx(0.25 + 20 * dt, rel=1e-15)
     with pytest.raises(NotImplementedError):
         jk.etd1_kernel_1d(torch.from_numpy(u).cuda(), torch.from_numpy(N).cuda(), lam[0] + 1j, dt)
+
+
+def test_reference_known_answers(gpu):
+    """The reference's own checks (tests/test_jit_kernels.py:149-196,467-490):
+    ETD1 with N = 0 decays sin(2 pi x) by e^{-4 pi^2 D dt} (1-D, rel 1e-6) and
+    sin(2 pi x) sin(2 pi y) by e^{-8 pi^2 D dt} (2-D, rel 1e-5); a stacked
+    call equals the single-line calls."""
+    from moljax_amd.core import jit_kernels as jk
+    nx, D, dt = 128, 0.01, 0.01
+    x = np.arange(nx) / nx
+    u0 = np.sin(2 * np.pi * x)
+    lam = D * osp.laplacian_symbol_1d(nx, 1.0 / nx)
+    u1 = _np(jk.etd1_kernel_1d(torch.from_numpy(u0).cuda(), torch.zeros(nx, dtype=torch.float64, device="cuda"),
+                               lam, dt))
+    assert _rel(u1, np.exp(-4 * np.pi ** 2 * D * dt) * u0) < 1e-6
+    n2 = 64
+    X, Y = np.meshgrid(np.arange(n2) / n2, np.arange(n2) / n2, indexing="ij")
+    v0 = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+    lam2 = D * osp.laplacian_symbol_2d(n2, n2, 1.0 / n2, 1.0 / n2)
+    v1 = _np(jk.etd1_kernel_2d(torch.from_numpy(v0).cuda(), torch.zeros_like(torch.from_numpy(v0)).cuda(), lam2, dt))
+    assert _rel(v1, np.exp(-8 * np.pi ** 2 * D * dt) * v0) < 1e-5
+    rng = np.random.default_rng(11)
+    U, N = rng.standard_normal((4, nx)), rng.standard_normal((4, nx))
+    stacked = _np(jk.batched_etd1_kernel_1d(torch.from_numpy(U).cuda(), torch.from_numpy(N).cuda(), lam, dt))
+    for i in range(4):
+        one = _np(jk.etd1_kernel_1d(torch.from_numpy(U[i]).cuda(), torch.from_numpy(N[i]).cuda(), lam, dt))
+        assert np.max(np.abs(one - stacked[i])) < 1e-14
