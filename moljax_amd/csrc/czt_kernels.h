@@ -417,7 +417,7 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_col_apply(
     const cpx<T>* __restrict__ tw) {
   using C = CztCfg<T, L>;
   __shared__ cpx<T> lds[C::LDS];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = xcd_remap_rev(blockIdx.x, gridDim.x);
   const int inst = w / tiles;
   int nrep = nrep_arg;
   if (mstep >= 0 || nrep_arg == NREP_PER_INSTANCE || mstep == MSTEP_ACTIVE) {
@@ -483,7 +483,7 @@ __global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_col_etd(
     CztPlan<T> py, const cpx<T>* __restrict__ tw) {
   using C = CztCfg<T, L>;
   __shared__ cpx<T> lds[C::LDS];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = xcd_remap_rev(blockIdx.x, gridDim.x);
   const int inst = w / tiles;
   int cl, j;
   czt_place<CZT_COL, C::TN, C::NL>(cl, j);

@@ -140,7 +140,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
   using C = ColCfg<NY, TC_>;
   constexpr int TC = C::TC;
   __shared__ cpx<T> lds[NY * TC];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = xcd_remap_rev(blockIdx.x, gridDim.x);
   const int inst = w / col_tiles;
   const int tile = w - inst * col_tiles;
   const int c = threadIdx.x % TC, j = threadIdx.x / TC;

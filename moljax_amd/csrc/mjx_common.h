@@ -45,4 +45,9 @@ __device__ __forceinline__ int xcd_remap(int w, int G) {
   return (w & 7) * (G >> 3) + (w >> 3);
 }
 
+// The same tiles in the opposite order, for column passes: the row pass
+// before them wrote the last instances last, and those are the ones still
+// held by the Infinity Cache when the column pass starts.
+__device__ __forceinline__ int xcd_remap_rev(int w, int G) { return G - 1 - xcd_remap(w, G); }
+
 }  // namespace mjx

@@ -359,7 +359,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   constexpr int TC = C::TC;
   __shared__ cpx<T> lds[NY * TC];
   __shared__ T lds_lamy[LAM2D ? 1 : NY];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = xcd_remap_rev(blockIdx.x, gridDim.x);
   const int inst = w / col_tiles;
   const int tile = w - inst * col_tiles;
   // Per-instance step counts (instance row aux): NREP_PER_INSTANCE raises the
