@@ -476,6 +476,14 @@ constexpr int lines_default() {
 
 inline int pick_lines(int def, int limit) { return def < limit ? def : limit; }
 
+// Small batches: halve the lines per workgroup until a launch has about 512
+// workgroups (two per CU), so that one instance still spreads over the chip.
+// Each line's arithmetic does not depend on the tile, so results are the same.
+inline int fill_lines(int l, long long lines_total) {
+  while (l > 1 && lines_total / l < 512) l >>= 1;
+  return l;
+}
+
 // Tuned lines per workgroup for a kernel family, else the default; always a
 // power of two no larger than the default (what with_lines instantiates).
 template <typename T, int N>
@@ -604,7 +612,7 @@ struct Launch {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, a, b, pitch, istride, W, in_scale, st);
+    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, a, b, pitch, istride, W, in_scale, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -617,7 +625,7 @@ struct Launch {
     switch (nx) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
-    return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, a, b, pitch, \
+    return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, W, a, b, pitch, \
                                                    istride, ghosts, keep, react, st);
       MJX_FOR_SIZES(X)
 #undef X
@@ -630,7 +638,7 @@ struct Launch {
     switch (nx) {
 #define X(N) \
   case N:    \
-    return with_lines<T, N, Gs<N>::template F>(pick_lines(lines_for<T, N>(TK_ROW), ny), B, ny, W, ip, rxn, mstep, st);
+    return with_lines<T, N, Gs<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, W, ip, rxn, mstep, st);
       MJX_FOR_SIZES(X)
 #undef X
       default:
@@ -643,7 +651,7 @@ struct Launch {
     switch (ny) {
 #define X(N)                                                                                                 \
   case N:                                                                                                    \
-    return with_lines<T, N, Col<N>::template F>(pick_lines(lines_for<T, N>(TK_COL), nx / 2), B, nx, W, lamx, lamy, lam2d, \
+    return with_lines<T, N, Col<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_COL), (long long)B * nx), nx / 2), B, nx, W, lamx, lamy, lam2d, \
                                              ip, nrep, s, mstep, st);
       MJX_FOR_SIZES(X)
 #undef X
