@@ -2,8 +2,8 @@
 
 This package is a NumPy/SciPy float64 restatement of the reference moljax
 algorithms (each function cites the reference file:line it follows).  It is
-the checker the HIP path is compared against and the CPU baseline timed by
-bench.py.  Only ``tests/``, ``__graft_entry__.smoke()`` and bench.py's
+what the HIP path is compared against in the tests, and the CPU baseline
+timed by bench.py.  Only ``tests/``, ``__graft_entry__.smoke()`` and bench.py's
 ``cpu_baseline`` leg may import it; the product package ``moljax_amd`` never
 does, and has no CPU fallback.
 
@@ -18,7 +18,24 @@ Pinning (what fixes each function independently of this restatement):
   gray_scott.*                 fixed point (u=1, v=0), pure-diffusion closed
                                form (1 - dt/2 D lam)^-2 per step, reaction
                                formula on scalars.
-  jvp.*                        golden AD-JVP norm
-                               (benchmarks/results/jvp_vs_fd_sweep.json:3) and
-                               central finite differences.
+  newton_krylov.*              golden AD-JVP norm
+                               (benchmarks/results/jvp_vs_fd_sweep.json:3),
+                               central finite differences, GMRES against a
+                               dense solve, the DST preconditioner's exact
+                               inverse, BDF2 coefficients and order.
+  fd.*                         sine modes (exact discrete eigenvalues) and the
+                               periodic wrap of the ghost fill.
+  nonperiodic.*                scipy.fft.dst / dct of the same definitions,
+                               brute-force DST-II, FD residual on filled ghosts.
+  etdrk4.*                     contour limits (Q -> dt/2, f1..f3 -> dt/6), the
+                               linear problem (N = 0) exactly E u, fourth order
+                               (ETDRK4), first and second order (ETD1, ETD2).
+  adaptive.*                   controller closed forms, fixed-point dt
+                               sequence, first-attempt error in closed form,
+                               status codes.
+  jit_kernels.*                phi_k(0) = 1/k! and the phi recurrences, ETD1 on
+                               a Fourier mode, ETD2 / ETDRK4 with equal N
+                               reduce to ETD1, the Helmholtz inverse on a mode.
+  spectral.exponential_filter_* and apply_spectral_filter_*: closed forms of
+                               the filter values and their action on modes.
 """
