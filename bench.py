@@ -275,6 +275,48 @@ def cpu_model():
     return platform.processor()
 
 
+def kernel_legs(B=512, n=N, reps=50):
+    """Per-kernel HBM rates of the row-independent operators at the headline
+    shape (B padded FP32 fields of (n+2)^2): the FD Laplacian, central and
+    upwind first derivatives (each reads the field once and writes the
+    interior, 8 B per interior point), the periodic ghost fill (reads the
+    two interior rings it copies and writes the ghost ring) and the interior
+    sum of squares (reads 4 B per interior point, FP64 accumulation).  Each
+    leg: torch events on the launch stream around `reps` back-to-back calls."""
+    import torch
+    from moljax_amd.core import Grid2D, laplacian_2d, d1_central_2d, d1_upwind_2d, tree_norm2
+    from moljax_amd.core.bc import FieldBCSpec, BCType
+    from moljax_amd.core.bc_fill import fill_ghosts
+    grid = Grid2D.uniform(n, n, 0.0, 2.5, 0.0, 2.5)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    f = torch.rand((B, n + 2, n + 2), generator=gen, device="cuda", dtype=torch.float32)
+    pts = B * n * n
+    ring = B * 4 * (n + 1) * 4  # ghost values per field ring, FP32
+    spec = FieldBCSpec(kind=BCType.PERIODIC)
+    legs = {
+        "fd_laplacian_2d": (lambda: laplacian_2d(f, grid), 8 * pts),
+        "fd_d1_central_2d_x": (lambda: d1_central_2d(f, grid, 1), 8 * pts),
+        "fd_d1_upwind_2d_y": (lambda: d1_upwind_2d(f, 0.7, grid, 0), 8 * pts),
+        "bc_fill_periodic": (lambda: fill_ghosts(f, grid, spec, 0.0, None), 2 * ring),
+        "tree_norm2_interior": (lambda: tree_norm2({"u": f}, grid), 4 * pts),
+    }
+    out = {}
+    for name, (fn, nbytes) in legs.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        s = e0.elapsed_time(e1) / 1e3 / reps
+        out[name] = {"us_per_call": s * 1e6, "algorithmic_bytes": nbytes, "GB_per_s": nbytes / s / 1e9,
+                     "roofline_frac": nbytes / s / HBM_PEAK}
+    return {"shape": [B, n + 2, n + 2], "dtype": "f32", "reps": reps, "legs": out}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +337,8 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=10, help="Newton-Krylov backward-Euler steps (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="serving-loop requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--kernel-reps", type=int, default=50,
+                    help="calls per leg of the FD / ghost-fill / reduction kernel rates (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         if int(os.environ.get("RANK", "0")) == 0:
@@ -371,6 +415,11 @@ def main():
         log(f"c5 serving {args.serve_requests} requests at {args.serve_rate}/s")
         c5 = c5_serving(args.serve_requests, args.serve_rate)
 
+    kl = None
+    if args.kernel_reps > 0:
+        log("FD operators, ghost fill, reduction at the headline shape")
+        kl = kernel_legs(B, N, args.kernel_reps)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -413,6 +462,7 @@ def main():
         "batched_uneven_etdrk4": c4,
         "newton_krylov_512": c3,
         "serving": c5,
+        "kernel_rates": kl,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

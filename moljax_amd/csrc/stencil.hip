@@ -12,6 +12,8 @@
 // already in L2/L1.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <string>
 
 #include "mjx_common.h"
@@ -32,14 +34,14 @@ struct FdArgs {
 
 template <typename T, int OP>
 __global__ __launch_bounds__(256) void k_fd2d(const T* __restrict__ in, T* __restrict__ out,
-                                              const T* __restrict__ vel, FdArgs a, long long total) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const int W = a.nx + 2, H = a.ny + 2;
-  const long long per = (long long)W * H;
-  const long long b = g / per;
-  const long long r = g - b * per;
-  const int y = (int)(r / W) - 1, x = (int)(r % W) - 1;
+                                              const T* __restrict__ vel, FdArgs a, int per, long long b0) {
+  // instance on grid y (32-bit index arithmetic within an instance)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= per) return;
+  const long long b = b0 + blockIdx.y;
+  const int W = a.nx + 2;
+  const int row = r / W;
+  const int y = row - 1, x = r - row * W - 1;
   T* o = out + b * a.os + (long long)y * a.op + x;
   if (y < 0 || y >= a.ny || x < 0 || x >= a.nx) {
     *o = (T)0;
@@ -91,11 +93,10 @@ __device__ __forceinline__ T xghost(const T* row, int side, const BcArgs& a) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bc_fill(T* __restrict__ f, BcArgs a, int per, long long total) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const long long b = g / per;
-  int r = (int)(g - b * per);
+__global__ __launch_bounds__(256) void k_bc_fill(T* __restrict__ f, BcArgs a, int per, long long b0) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;  // instance on grid y
+  if (r >= per) return;
+  const long long b = b0 + blockIdx.y;
   T* base = f + b * a.is;
   const int W = a.nx + 2;
   if (r < 2 * a.ny) {
@@ -139,6 +140,9 @@ __global__ __launch_bounds__(256) void k_gs_reaction(const T* __restrict__ u, co
 }
 
 static inline unsigned nblocks(long long total) { return (unsigned)((total + 255) / 256); }
+// Batched stencil launches put the instance on grid y, at most this many per
+// launch (larger batches loop on the host).
+constexpr long long MAX_GRID_Y = 65535;
 
 // out = a x + b y + c over n contiguous elements (y may be null): the
 // element-wise StateDict algebra (moljax/core/state.py:27-81).
@@ -165,14 +169,13 @@ struct JacobiArgs {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_jacobi_padded(const T* __restrict__ r, const T* __restrict__ x,
-                                                       T* __restrict__ out, JacobiArgs a, long long total) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const int W = a.nx + 2, H = a.ny + 2;
-  const long long per = (long long)W * H;
-  const long long b = g / per;
-  const long long rr = g - b * per;
-  const int y = (int)(rr / W) - 1, xi = (int)(rr % W) - 1;
+                                                       T* __restrict__ out, JacobiArgs a, int per, long long b0) {
+  const int rr = blockIdx.x * blockDim.x + threadIdx.x;  // instance on grid y
+  if (rr >= per) return;
+  const long long b = b0 + blockIdx.y;
+  const int W = a.nx + 2;
+  const int row = rr / W;
+  const int y = row - 1, xi = rr - row * W - 1;
   const T* xc = x + b * a.xs + (long long)y * a.xp + xi;
   const T c = *xc;
   T lap = (T)0;
@@ -213,7 +216,9 @@ int mjx_jacobi2d(int dtype, int B, int ny, int nx, const void* r, long long r_pi
   if (n_iter < 0) return fail(MJX_E_ARG, "n_iter must be >= 0");
   if (r_pitch < nx + 2 || x_pitch < nx + 2) return fail(MJX_E_ARG, "pitch must cover the ghost ring");
   if (!(dx > 0.0) || !(dy > 0.0)) return fail(MJX_E_ARG, "dx, dy must be positive");
-  const long long W = nx + 2, per = W * (ny + 2), total = (long long)B * per;
+  if ((long long)(ny + 2) * (nx + 2) > INT_MAX) return fail(MJX_E_LIMIT, "grid too large for one instance");
+  const long long W = nx + 2;
+  const int per = (ny + 2) * (nx + 2);
   const size_t es = dtype == MJX_F64 ? 8 : 4;
   JacobiArgs a;
   a.ny = ny;
@@ -239,12 +244,15 @@ int mjx_jacobi2d(int dtype, int B, int ny, int nx, const void* r, long long r_pi
     a.op = to_x ? x_pitch : W;
     a.os = to_x ? x_istride : per;
     a.omega = n_iter > 0 ? omega : 0.0;
-    if (dtype == MJX_F32)
-      hipLaunchKernelGGL(k_jacobi_padded<float>, dim3(nblocks(total)), dim3(256), 0, st, (const float*)r,
-                         (const float*)prev, (float*)dst, a, total);
-    else
-      hipLaunchKernelGGL(k_jacobi_padded<double>, dim3(nblocks(total)), dim3(256), 0, st, (const double*)r,
-                         (const double*)prev, (double*)dst, a, total);
+    for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y) {
+      const dim3 grid((per + 255) / 256, (unsigned)std::min<long long>(MAX_GRID_Y, B - b0));
+      if (dtype == MJX_F32)
+        hipLaunchKernelGGL(k_jacobi_padded<float>, grid, dim3(256), 0, st, (const float*)r, (const float*)prev,
+                           (float*)dst, a, per, b0);
+      else
+        hipLaunchKernelGGL(k_jacobi_padded<double>, grid, dim3(256), 0, st, (const double*)r,
+                           (const double*)prev, (double*)dst, a, per, b0);
+    }
     int rc = check_launch("jacobi_padded");
     if (rc) return rc;
     prev = dst;
@@ -278,11 +286,13 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
   a.dx2 = dx * dx;
   a.dy2 = dy * dy;
   a.vscalar = vel_scalar;
-  const long long total = (long long)B * (ny + 2) * (nx + 2);
+  if ((long long)(ny + 2) * (nx + 2) > INT_MAX) return fail(MJX_E_LIMIT, "grid too large for one instance");
+  const int per = (ny + 2) * (nx + 2);
   hipStream_t st = (hipStream_t)stream;
-#define MJX_FD_LAUNCH(T, OPV)                                                                       \
-  hipLaunchKernelGGL((k_fd2d<T, OPV>), dim3(nblocks(total)), dim3(256), 0, st, (const T*)in, (T*)out, \
-                     (const T*)vel, a, total)
+#define MJX_FD_LAUNCH(T, OPV)                                                                                  \
+  for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y)                                                             \
+  hipLaunchKernelGGL((k_fd2d<T, OPV>), dim3((per + 255) / 256, (unsigned)std::min<long long>(MAX_GRID_Y, B - b0)), \
+                     dim3(256), 0, st, (const T*)in, (T*)out, (const T*)vel, a, per, b0)
   if (dtype == MJX_F32) {
     switch (op) {
       case FD_LAP: MJX_FD_LAUNCH(float, FD_LAP); break;
@@ -321,16 +331,19 @@ int mjx_bc_fill2d(int dtype, int B, int ny, int nx, void* f, long long pitch, lo
   a.vt = side_values ? side_values[3] : 0.0;
   a.dx = dx;
   a.dy = dy;
+  if ((long long)ny + nx > INT_MAX / 4) return fail(MJX_E_LIMIT, "grid too large for one instance");
   const int per = 2 * ny + 2 * (nx + 2);
-  const long long total = (long long)B * per;
   hipStream_t st = (hipStream_t)stream;
   // x ghosts of the interior rows must be final before the ghost rows copy
   // them; the ghost rows compute them again from interior values instead, so
   // one launch suffices and no ordering between the two groups is needed.
-  if (dtype == MJX_F32)
-    hipLaunchKernelGGL(k_bc_fill<float>, dim3(nblocks(total)), dim3(256), 0, st, (float*)f, a, per, total);
-  else
-    hipLaunchKernelGGL(k_bc_fill<double>, dim3(nblocks(total)), dim3(256), 0, st, (double*)f, a, per, total);
+  for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y) {
+    const dim3 grid((per + 255) / 256, (unsigned)std::min<long long>(MAX_GRID_Y, B - b0));
+    if (dtype == MJX_F32)
+      hipLaunchKernelGGL(k_bc_fill<float>, grid, dim3(256), 0, st, (float*)f, a, per, b0);
+    else
+      hipLaunchKernelGGL(k_bc_fill<double>, grid, dim3(256), 0, st, (double*)f, a, per, b0);
+  }
   return check_launch("bc_fill2d");
 }
 
