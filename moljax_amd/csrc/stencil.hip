@@ -22,6 +22,11 @@ namespace mjx {
 
 enum FdOp { FD_LAP = 0, FD_D2 = 1, FD_D1C = 2, FD_D1UP = 3 };
 
+// Batched stencil launches put the instance on grid y (grid z for the FD
+// kernel), at most this many per launch; larger batches loop on the host.
+constexpr long long MAX_GRID_Y = 65535;
+constexpr int FD_RY = 8;  // padded rows per thread in the FD kernel
+
 struct FdArgs {
   long long ip, is;  // input pitch / instance stride
   long long op, os;  // output pitch / instance stride
@@ -34,39 +39,64 @@ struct FdArgs {
 
 template <typename T, int OP>
 __global__ __launch_bounds__(256) void k_fd2d(const T* __restrict__ in, T* __restrict__ out,
-                                              const T* __restrict__ vel, FdArgs a, int per, long long b0) {
-  // instance on grid y (32-bit index arithmetic within an instance)
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= per) return;
-  const long long b = b0 + blockIdx.y;
-  const int W = a.nx + 2;
-  const int row = r / W;
-  const int y = row - 1, x = r - row * W - 1;
-  T* o = out + b * a.os + (long long)y * a.op + x;
-  if (y < 0 || y >= a.ny || x < 0 || x >= a.nx) {
-    *o = (T)0;
-    return;
-  }
-  const T* f = in + b * a.is + (long long)y * a.ip + x;
+                                              const T* __restrict__ vel, FdArgs a, long long b0) {
+  // 64 x 4 threads; each thread marches down FD_RY padded rows of one padded
+  // column, keeping the y-neighbours in registers (the Laplacian loads 3
+  // values per output instead of 5).  Instance on grid z.
+  const int W = a.nx + 2, H = a.ny + 2;
+  const int xp = blockIdx.x * 64 + threadIdx.x;
+  const int yp0 = (blockIdx.y * 4 + threadIdx.y) * FD_RY;
+  if (xp >= W || yp0 >= H) return;
+  const long long b = b0 + blockIdx.z;
+  const int x = xp - 1;
+  const bool xin = x >= 0 && x < a.nx;
+  const int yend = yp0 + FD_RY < H ? yp0 + FD_RY : H;
   const long long sy = a.ip;
-  if constexpr (OP == FD_LAP) {
-    // d2_central_2d(axis=0) + d2_central_2d(axis=1), operators.py:311-322
-    const T c = f[0];
-    const T yy = (f[sy] - (T)2 * c + f[-sy]) / (T)a.dy2;
-    const T xx = (f[1] - (T)2 * c + f[-1]) / (T)a.dx2;
-    *o = yy + xx;
-  } else {
-    const long long s = a.axis == 0 ? sy : 1;
-    if constexpr (OP == FD_D2) {
-      *o = (f[s] - (T)2 * f[0] + f[-s]) / (T)a.h2;  // operators.py:271-308
-    } else if constexpr (OP == FD_D1C) {
-      *o = (f[s] - f[-s]) / ((T)2 * (T)a.h);  // operators.py:159-197
+  const bool ymarch = OP == FD_LAP || a.axis == 0;
+  T up = (T)0, c = (T)0;
+  bool have = false;
+  for (int yp = yp0; yp < yend; ++yp) {
+    const int y = yp - 1;
+    T* o = out + b * a.os + (long long)y * a.op + x;
+    if (!xin || y < 0 || y >= a.ny) {
+      *o = (T)0;
+      continue;
+    }
+    const T* f = in + b * a.is + (long long)y * a.ip + x;
+    T dn = (T)0;
+    if (ymarch) {
+      if (!have) {
+        up = f[-sy];
+        c = f[0];
+        have = true;
+      }
+      dn = f[sy];
     } else {
-      // operators.py:200-268: backward where v >= 0, forward otherwise
-      const T v = vel ? vel[b * a.vs + (long long)y * a.vp + x] : (T)a.vscalar;
-      const T bwd = (f[0] - f[-s]) / (T)a.h;
-      const T fwd = (f[s] - f[0]) / (T)a.h;
-      *o = v >= (T)0 ? bwd : fwd;
+      c = f[0];
+    }
+    if constexpr (OP == FD_LAP) {
+      // d2_central_2d(axis=0) + d2_central_2d(axis=1), operators.py:311-322
+      const T yy = (dn - (T)2 * c + up) / (T)a.dy2;
+      const T xx = (f[1] - (T)2 * c + f[-1]) / (T)a.dx2;
+      *o = yy + xx;
+    } else {
+      const T lo = a.axis == 0 ? up : f[-1];
+      const T hi = a.axis == 0 ? dn : f[1];
+      if constexpr (OP == FD_D2) {
+        *o = (hi - (T)2 * c + lo) / (T)a.h2;  // operators.py:271-308
+      } else if constexpr (OP == FD_D1C) {
+        *o = (hi - lo) / ((T)2 * (T)a.h);  // operators.py:159-197
+      } else {
+        // operators.py:200-268: backward where v >= 0, forward otherwise
+        const T v = vel ? vel[b * a.vs + (long long)y * a.vp + x] : (T)a.vscalar;
+        const T bwd = (c - lo) / (T)a.h;
+        const T fwd = (hi - c) / (T)a.h;
+        *o = v >= (T)0 ? bwd : fwd;
+      }
+    }
+    if (ymarch) {
+      up = c;
+      c = dn;
     }
   }
 }
@@ -140,9 +170,7 @@ __global__ __launch_bounds__(256) void k_gs_reaction(const T* __restrict__ u, co
 }
 
 static inline unsigned nblocks(long long total) { return (unsigned)((total + 255) / 256); }
-// Batched stencil launches put the instance on grid y, at most this many per
-// launch (larger batches loop on the host).
-constexpr long long MAX_GRID_Y = 65535;
+
 
 // out = a x + b y + c over n contiguous elements (y may be null): the
 // element-wise StateDict algebra (moljax/core/state.py:27-81).
@@ -286,13 +314,14 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
   a.dx2 = dx * dx;
   a.dy2 = dy * dy;
   a.vscalar = vel_scalar;
-  if ((long long)(ny + 2) * (nx + 2) > INT_MAX) return fail(MJX_E_LIMIT, "grid too large for one instance");
-  const int per = (ny + 2) * (nx + 2);
+  if (nx > INT_MAX - 64 || ny > 65535 * 4 * FD_RY - 2) return fail(MJX_E_LIMIT, "grid too large for one launch");
   hipStream_t st = (hipStream_t)stream;
-#define MJX_FD_LAUNCH(T, OPV)                                                                                  \
-  for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y)                                                             \
-  hipLaunchKernelGGL((k_fd2d<T, OPV>), dim3((per + 255) / 256, (unsigned)std::min<long long>(MAX_GRID_Y, B - b0)), \
-                     dim3(256), 0, st, (const T*)in, (T*)out, (const T*)vel, a, per, b0)
+#define MJX_FD_LAUNCH(T, OPV)                                                                         \
+  for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y)                                                    \
+  hipLaunchKernelGGL((k_fd2d<T, OPV>),                                                                \
+                     dim3((nx + 2 + 63) / 64, (ny + 2 + 4 * FD_RY - 1) / (4 * FD_RY),                 \
+                          (unsigned)std::min<long long>(MAX_GRID_Y, B - b0)),                         \
+                     dim3(64, 4), 0, st, (const T*)in, (T*)out, (const T*)vel, a, b0)
   if (dtype == MJX_F32) {
     switch (op) {
       case FD_LAP: MJX_FD_LAUNCH(float, FD_LAP); break;
