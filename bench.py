@@ -60,7 +60,11 @@ def run(U, V, lam, rows, work, steps):
     gs_imex_strang_inplace(U, V, lam, rows, steps, work=work)
 
 
-def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N):
+def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N, keep_first=False):
+    """Seconds for ``steps`` steps of a B-instance batch after ``warmup``
+    untimed ones.  With kernel_timing, every launch of the chain is also
+    bracketed by HIP events on its stream (a separate pass: the events cost
+    time of their own).  keep_first returns instance 0's final (u, v)."""
     import torch
     from moljax_amd import _hip
     lam, U, V, rows, work = setup(dtype, B, list(range(B)), n)
@@ -79,8 +83,27 @@ def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N):
         kts = {k: _hip.kernel_timing_samples(k) for k in (KT_ROW_GS, KT_COL)}
         _hip.lib().mjx_kernel_timing_enable(0)
     finite = bool(torch.isfinite(U).all() and torch.isfinite(V).all())
+    first = (U[0].double().clone(), V[0].double().clone()) if keep_first else None
     del U, V, work
+    if keep_first:
+        return t1 - t0, kt, finite, kts, first
     return t1 - t0, kt, finite, kts
+
+
+def rel_l2_pair(a, b):
+    """Relative L2 of the (u, v) pair a against the pair b."""
+    import torch
+    num = torch.sqrt(((a[0] - b[0]) ** 2).sum() + ((a[1] - b[1]) ** 2).sum())
+    return float(num / torch.sqrt((b[0] ** 2).sum() + (b[1] ** 2).sum()))
+
+
+def fp64_reference_state(n, steps, seed=0):
+    """Instance ``seed`` advanced ``steps`` steps alone in FP64 on the GPU
+    (the FP64 chain is held to the CPU oracle at 1e-12 by the test suite)."""
+    import torch
+    lam, U, V, rows, work = setup(torch.float64, 1, [seed], n)
+    run(U, V, lam, rows, work, steps)
+    return U[0].clone(), V[0].clone()
 
 
 def error_curve(checkpoints, seed=0):
@@ -193,12 +216,29 @@ def c4_batched_etdrk4(n_inst, steps, seed=0):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     finite = all(bool(torch.isfinite(t).all()) for t in b.u + b.v)
+    # FP32 against FP64: the first instance of every size group, advanced
+    # the same steps in an FP64 plan of its own
+    pick = [b.members[n][0] for n in b.order]
+    ref = GrayScottBatch([sizes[i] for i in pick], [F[i] for i in pick], [k[i] for i in pick], 2e-5, 1e-5, 1.0,
+                         scheme="etdrk4", dtype=torch.float64, steps_per_graph=16, tune=False)
+    for j, i in enumerate(pick):
+        n = sizes[i]
+        u, v = gray_scott_ic(n, n, i, square=max(2, round(20 * n / 256)))
+        ref.set_state(j, torch.from_numpy(u[1:-1, 1:-1]), torch.from_numpy(v[1:-1, 1:-1]))
+    ref.run(2 + steps)
+    torch.cuda.synchronize()
+    errs = {}
+    for j, i in enumerate(pick):
+        a = tuple(t.double() for t in b.state(i))
+        errs[str(sizes[i])] = rel_l2_pair(a, ref.state(j))
+    ref.close()
     from moljax_amd import _hip
     tiles = {str(n): [_hip.lib().mjx_tune_get(2, 0, n), _hip.lib().mjx_tune_get(3, 0, n)] for n in b.order}
     out = {"value": b.n_points * steps / el, "unit": "grid-point-steps/s", "instances": n_inst,
            "points": b.n_points, "state_bytes": b.state_bytes(), "steps": steps, "ms_per_step": 1e3 * el / steps,
            "sizes": {str(n): len(b.members[n]) for n in b.order}, "tiles_row_col": tiles,
            "setup_s": setup_s, "finite": finite,
+           "rel_l2_fp32_vs_fp64": errs, "rel_l2_fp32_vs_fp64_max": max(errs.values()),
            "note": "2000-step workload timed on its first steps (per-step cost is constant)"}
     b.close()
     return out
@@ -227,7 +267,8 @@ def c3_newton_krylov(n, steps, op_dtype):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     return {"ms_per_step": 1e3 * el / steps, "newton_iters": newton, "gmres_iters": gmres,
-            "max_final_residual": max(res), "operator_dtype": str(op_dtype).replace("torch.", "")}
+            "max_final_residual": max(res), "operator_dtype": str(op_dtype).replace("torch.", ""),
+            "_u": y["u"].double()}
 
 
 def c5_serving(n_req, rate, seed=0):
@@ -252,16 +293,28 @@ def c5_serving(n_req, rate, seed=0):
     t0 = time.perf_counter()
     srv2.run(sat, realtime=False)
     sat_s = time.perf_counter() - t0
-    seq = sequential(attach_inputs(poisson_stream(n_req, rate=rate, seed=seed), torch.float64), torch.float64)
+    seq_reqs = attach_inputs(poisson_stream(n_req, rate=rate, seed=seed), torch.float64)
+    seq = sequential(seq_reqs, torch.float64, keep=True)
+    # every served FP32 result against the same request run alone in FP64
+    errs = []
+    for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(seq_reqs, key=lambda r: r.rid)):
+        num = sum(float(((x.double() - y) ** 2).sum()) for x, y in zip(a.state, b.state))
+        den = sum(float((y ** 2).sum()) for y in b.state)
+        errs.append((num / den) ** 0.5 if den > 0 else 0.0)
     return {"requests": n_req, "rate_per_s": rate, **latency_stats(reqs), "makespan_s": makespan,
             "offered_gps": gps / makespan, "saturated_s": sat_s, "saturated_gps": gps / sat_s,
             "saturated_iterations": srv2.iterations, "saturated_max_group": srv2.max_group,
             "sequential_fp64_s": seq, "sequential_fp64_gps": gps / seq,
             "saturated_vs_sequential_fp64": seq / sat_s, "iterations": srv.iterations,
+            "rel_l2_fp32_served_vs_fp64_alone_max": max(errs),
+            "rel_l2_fp32_served_vs_fp64_alone_p50": float(np.median(errs)),
+            "requests_per_s_saturated": n_req / sat_s,
             "peak_active": srv.peak_active, "max_group": srv.max_group,
             "mix": {k: sum(r.kind == k for r in reqs) for k in ("fft_cn", "dst", "dct", "imex_gs")},
             "note": "p50/p99: open-loop Poisson arrivals, arrival to result on the device; "
-                    "saturated: all requests queued at t=0"}
+                    "saturated: all requests queued at t=0; gps counts request points x steps, and the "
+                    "linear kinds (fft_cn, dst, dct) take each chunk of steps as one spectral power "
+                    "multiply, in the served and the sequential runs alike"}
 
 
 def cpu_model():
@@ -324,6 +377,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--fp64-steps", type=int, default=2000)
+    ap.add_argument("--fp64-batch", type=int, default=512,
+                    help="batch of the FP64-batched leg (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-curve", action="store_true")
     ap.add_argument("--sweep", type=str, default="1,8,64,2048",
@@ -360,13 +415,25 @@ def main():
         del src, dst
 
     log(f"fp32 batched B={B} steps={K}")
-    t32, kt, fin32, kts = time_form(torch.float32, B, K, W, kernel_timing=True)
+    t32, _, fin32, _ = time_form(torch.float32, B, K, W)
     gps32 = B * N * N * K / t32
     ms_step = 1e3 * t32 / K
+    # per-launch HIP events in a pass of their own (the events cost time)
+    ke = max(10, min(K, 100))
+    log(f"fp32 batched B={B}: kernel events over {ke} steps")
+    _, kt, _, kts = time_form(torch.float32, B, ke, W, kernel_timing=True)
 
     log(f"fp64 single steps={args.fp64_steps}")
     t64, _, fin64, _ = time_form(torch.float64, 1, args.fp64_steps, W)
     gps64 = N * N * args.fp64_steps / t64
+
+    fp64b = None
+    if args.fp64_batch > 0:
+        b64, k64 = args.fp64_batch, max(10, K // 2)
+        log(f"fp64 batched B={b64} steps={k64}")
+        tb, _, finb, _ = time_form(torch.float64, b64, k64, W)
+        fp64b = {"value": b64 * N * N * k64 / tb, "unit": "grid-point-steps/s", "batch": b64, "steps": k64,
+                 "ms_per_step": 1e3 * tb / k64, "finite": finb}
 
     # roofline of the dominant kernel: the step's compulsory 16 B/GPS (FP32
     # u, v read + written once) is split evenly over its two kernels.
@@ -389,10 +456,12 @@ def main():
     if args.anylen_n > 0:
         n2 = args.anylen_n
         log(f"any-length IMEX-Strang {n2}^2 B={B}")
-        ta, _, fina, _ = time_form(torch.float32, B, K, W, n=n2)
+        ta, _, fina, _, first = time_form(torch.float32, B, K, W, n=n2, keep_first=True)
+        ref = fp64_reference_state(n2, W + K, seed=0)
         anylen = {"grid": [n2, n2], "batch": B, "steps": K, "value": B * n2 * n2 * K / ta,
                   "unit": "grid-point-steps/s", "ms_per_step": 1e3 * ta / K, "finite": fina,
-                  "vs_power_of_two_headline": (B * n2 * n2 * K / ta) / gps32}
+                  "vs_power_of_two_headline": (B * n2 * n2 * K / ta) / gps32,
+                  "rel_l2_fp32_vs_fp64_instance0": rel_l2_pair(first, ref), "rel_l2_after_steps": W + K}
 
     curve = None
     if not args.no_curve:
@@ -411,6 +480,9 @@ def main():
               "fp32_operator": c3_newton_krylov(512, args.c3_steps, torch.float32),
               "fp64_operator": c3_newton_krylov(512, args.c3_steps, torch.float64),
               "grid_511_fp32_operator": c3_newton_krylov(511, args.c3_steps, torch.float32)}
+        u32, u64 = c3["fp32_operator"].pop("_u"), c3["fp64_operator"].pop("_u")
+        c3["grid_511_fp32_operator"].pop("_u")
+        c3["rel_l2_fp32_operator_vs_fp64_operator"] = float(torch.linalg.norm(u32 - u64) / torch.linalg.norm(u64))
     if args.serve_requests > 0:
         log(f"c5 serving {args.serve_requests} requests at {args.serve_rate}/s")
         c5 = c5_serving(args.serve_requests, args.serve_rate)
@@ -429,6 +501,18 @@ def main():
     if curve:
         ok = [int(c) for c, e in curve.items() if e <= 1e-5]
         horizon = max(ok) if ok else 0
+    # which form meets the metric's 1e-5 at config 2's 10 000-step horizon
+    target = None
+    if curve and fp64b:
+        e10k = curve.get("10000")
+        fp32_ok = e10k is not None and e10k <= 1e-5
+        form = ("fp32_batched", gps32) if fp32_ok else ("fp64_batched", fp64b["value"])
+        target = {"horizon_steps": 10000, "tolerance": 1e-5, "form": form[0], "value": form[1],
+                  "unit": "grid-point-steps/s", "ratio_to_fp64_single": form[1] / gps64,
+                  "fp32_batched_rel_l2_at_10000": e10k,
+                  "fp64_rel_l2_vs_oracle": "<= 1e-12 per 20 steps (test suite, step for step)",
+                  "note": "FP32 state drifts above 1e-5 after ~2000 steps (the dynamics amplify rounding); the "
+                          "FP64 batch carries the 1e-5 target to 10 000 steps"}
     line = {
         "metric": METRIC,
         "value": gps32,
@@ -448,6 +532,9 @@ def main():
         "fp64_single_instance": {"value": gps64, "unit": "grid-point-steps/s", "steps": args.fp64_steps,
                                  "ms_per_step": 1e3 * t64 / args.fp64_steps},
         "speedup_fp32_batched_vs_fp64_single": gps32 / gps64,
+        "fp64_batched": fp64b,
+        "speedup_fp64_batched_vs_fp64_single": fp64b["value"] / gps64 if fp64b else None,
+        "form_within_1e-5_at_10000_steps": target,
         "rel_l2_fp32_vs_fp64": curve,
         "steps_within_1e-5": horizon,
         "finite": fin32 and fin64,

@@ -71,9 +71,15 @@ def _fd(f, grid: Grid2D, op: int, axis: int = 1, vel=None) -> torch.Tensor:
             vt = as_device(vel, t.dtype)
             if vt.shape[-2:] != t.shape[-2:]:
                 raise ValueError("velocity array must have the padded field shape")
-            if vt.dim() < t.dim():
-                vt = vt.expand(t.shape).contiguous()
-            vel_t, vp, vs, _ = view2d(vt)
+            if vt.shape != t.shape:
+                try:  # broadcast to the field's leading dims (torch raises if it cannot)
+                    vt = vt.expand(t.shape).contiguous()
+                except RuntimeError as e:
+                    raise ValueError(f"velocity shape {tuple(vel.shape)} does not broadcast to the field "
+                                     f"shape {tuple(t.shape)}") from e
+            vel_t, vp, vs, vB = view2d(vt)
+            if vB != B:
+                raise ValueError(f"velocity holds {vB} instances, the field {B}")
             vel_t = vel_t[:, 1:, 1:]
     lib = _hip.lib()
     rc = lib.mjx_fd_op2d(_hip.dtype_code(t.dtype), op, B, grid.ny, grid.nx,

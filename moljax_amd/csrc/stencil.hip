@@ -299,6 +299,8 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
   if (!in || !out) return fail(MJX_E_ARG, "null pointer argument");
   if (axis != 0 && axis != 1) return fail(MJX_E_ARG, "axis must be 0 (y) or 1 (x)");
   if (in_pitch < nx + 2 || out_pitch < nx + 2) return fail(MJX_E_ARG, "pitch must cover the ghost ring");
+  if (vel && op == FD_D1UP && (vel_pitch < nx + 2 || (B > 1 && vel_istride < vel_pitch * (ny + 2))))
+    return fail(MJX_E_ARG, "velocity pitch must cover the ghost ring and instances must not overlap");
   FdArgs a;
   a.ip = in_pitch;
   a.is = in_istride;
@@ -314,7 +316,7 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
   a.dx2 = dx * dx;
   a.dy2 = dy * dy;
   a.vscalar = vel_scalar;
-  if (nx > INT_MAX - 64 || ny > 65535 * 4 * FD_RY - 2) return fail(MJX_E_LIMIT, "grid too large for one launch");
+  if (nx > INT_MAX - 66 || ny > 65535 * 4 * FD_RY - 2) return fail(MJX_E_LIMIT, "grid too large for one launch");
   hipStream_t st = (hipStream_t)stream;
 #define MJX_FD_LAUNCH(T, OPV)                                                                         \
   for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y)                                                    \

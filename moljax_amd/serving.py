@@ -350,9 +350,10 @@ def latency_stats(reqs: List[Request]) -> Dict[str, float]:
             "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3)}
 
 
-def sequential(reqs: List[Request], dtype: torch.dtype = torch.float64) -> float:
+def sequential(reqs: List[Request], dtype: torch.dtype = torch.float64, keep: bool = False) -> float:
     """Seconds to run the same requests one at a time, each in one launch
-    sequence of all its steps (no batching), in ``dtype``."""
+    sequence of all its steps (no batching), in ``dtype``; with ``keep`` each
+    request's final state is stored in ``state``."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in reqs:
@@ -360,6 +361,8 @@ def sequential(reqs: List[Request], dtype: torch.dtype = torch.float64) -> float
         p.admit(r, initial_state(r, dtype))
         done = r.done_steps
         p.launch(r.steps, torch.cuda.current_stream().cuda_stream)
+        if keep:
+            r.state = [f[0] for f in p.fields]
         p.slots[0] = None
         r.done_steps = done
     torch.cuda.synchronize()

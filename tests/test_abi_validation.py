@@ -90,3 +90,14 @@ def test_spectral_any_length_checks(lib):
     assert pair(3000, 32) == E_SIZE and "[2, 2048]" in _msg(lib)
     assert pair(32, 1) == E_SIZE
     assert lib.mjx_gs_imex_strang(1, 1, 2049, 2049, P, P, 2049, 2049 * 2049, 0, P, P, None, P, 1, P, None) == E_SIZE
+
+
+def test_fd_op_checks(lib):
+    # an upwind velocity array must cover the padded row and must not overlap between instances
+    fd = lambda B=2, vel=P, vp=66, vs=66 * 66, nx=64: lib.mjx_fd_op2d(
+        1, 3, B, 64, nx, P, 66, 66 * 66, P, 66, 66 * 66, 1, 0.1, 0.1, vel, vp, vs, 0.0, None)
+    assert fd(vp=65) == E_ARG and "velocity pitch" in _msg(lib)
+    assert fd(vs=66) == E_ARG
+    assert lib.mjx_fd_op2d(1, 3, 1, 4, 2 ** 31 - 66, P, 2 ** 31 - 1, 0, P, 2 ** 31 - 1, 0, 1, 0.1, 0.1,
+                           None, 0, 0, 1.0, None) == -4 and "too large" in _msg(lib)
+

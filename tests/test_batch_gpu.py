@@ -160,3 +160,41 @@ def test_strang_every_tile_matches_oracle(gpu):
     finally:
         lib.mjx_tune_set(0, 1, n, 0)
         lib.mjx_tune_set(1, 1, n, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_c4_four_size_plan(gpu, dtype):
+    """The batched ETDRK4 workload's own shape: one plan holding all four
+    sizes {64, 128, 256, 512}^2 (two instances each, c4's F, k ranges and IC
+    convention), 3 steps.  Each instance is bitwise equal to its lone run;
+    in FP64 each is within 1e-12 of the oracle, in FP32 within 1e-5."""
+    from moljax_amd.batch import GrayScottBatch, spectral_symbol_1d
+    from moljax_amd.workloads import gray_scott_ic
+    sizes = [512, 64, 256, 128, 64, 512, 128, 256]
+    rng = np.random.default_rng(11)
+    F = rng.uniform(0.02, 0.06, len(sizes)).tolist()
+    k = rng.uniform(0.055, 0.068, len(sizes)).tolist()
+    ics = []
+    for i, n in enumerate(sizes):
+        u, v = gray_scott_ic(n, n, i, square=max(2, round(20 * n / 256)))
+        ics.append((u[1:-1, 1:-1].copy(), v[1:-1, 1:-1].copy()))
+    steps = 3
+    mix = GrayScottBatch(sizes, F, k, DU, DV, DT, scheme="etdrk4", dtype=dtype, steps_per_graph=2)
+    for i, (u, v) in enumerate(ics):
+        mix.set_state(i, torch.from_numpy(u), torch.from_numpy(v))
+    mix.run(steps)
+    for i, n in enumerate(sizes):
+        print(f"c4 plan instance {i} n={n} {dtype}", flush=True)
+        alone = GrayScottBatch([n], [F[i]], [k[i]], DU, DV, DT, scheme="etdrk4", dtype=dtype, steps_per_graph=2)
+        alone.set_state(0, torch.from_numpy(ics[i][0]), torch.from_numpy(ics[i][1]))
+        alone.run(steps)
+        mu, mv = mix.state(i)
+        au, av = alone.state(0)
+        assert torch.equal(mu, au) and torch.equal(mv, av), f"instance {i} (n={n}) differs from its lone run"
+        alone.close()
+        lam = spectral_symbol_1d(n)
+        ru, rv = oetd.gs_etdrk4(ics[i][0], ics[i][1], lam, lam, DU, DV, F[i], k[i], DT, steps)
+        tol = 1e-12 if dtype == torch.float64 else 1e-5
+        assert ogs.rel_l2(mu.double().cpu().numpy(), ru) < tol
+        assert ogs.rel_l2(mv.double().cpu().numpy(), rv) < tol
+    mix.close()

@@ -1,0 +1,89 @@
+"""Closed-form pins of the native Newton-Krylov driver's counting and of the
+implicit adaptive loop (the reference reports no true GMRES count,
+moljax/core/newton_krylov.py:83-132,128,322).
+
+* u_t = D lap u + a1 u with the exact DST preconditioner: M J has one
+  eigenvalue per distinct Laplacian eigenvalue, so a first Newton residual
+  made of s sine modes takes exactly s GMRES iterations, and the update
+  divides each mode by (1 - dt a1) - dt D lambda.
+* a1 = 0: M J = I, one GMRES iteration per Newton iteration, two Newton
+  iterations (the convergence test comes before the update).
+* krylov_tol = 0 is never met: every restart cycle runs all m Arnoldi steps.
+* y' = lam y with a tiny lam: saturated controller, dt_k = min(dt0 6^k, 1).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _mode(n, p, q):
+    h = 1.0 / (n + 1)
+    x = h * np.arange(1, n + 1)
+    X, Y = np.meshgrid(x, x)
+    return np.sin(np.pi * p * X) * np.sin(np.pi * q * Y)
+
+
+def _eig(n, p, q):
+    h = 1.0 / (n + 1)
+    return -4.0 / h ** 2 * (np.sin(np.pi * p / (2 * (n + 1))) ** 2 + np.sin(np.pi * q / (2 * (n + 1))) ** 2)
+
+
+def _be(n, a1, u0, op_dtype=torch.float64, **kw):
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import DSTDiffusionPreconditioner
+    from moljax_amd.core.reaction_diffusion import be_step, create_reaction_diffusion_model
+    model = create_reaction_diffusion_model(n, D=0.05, reaction=(0.0, 0.0, a1, 0.0), bc="dirichlet")
+    P = DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"})
+    nkp = NKParams(operator_dtype=op_dtype, **kw)
+    y, st = be_step(model, {"u": torch.from_numpy(u0)}, 0.0, 0.1, P, nkp)
+    return y["u"].double().cpu().numpy(), st
+
+
+@pytest.mark.parametrize("n", [31, 64, 512])
+@pytest.mark.parametrize("modes", [[(1, 1)], [(1, 2), (3, 1)], [(1, 1), (2, 3), (4, 4), (5, 1)]])
+def test_linear_problem_gmres_counts_modes(gpu, n, modes):
+    amps = [1.0, -0.7, 0.4, 0.25][:len(modes)]
+    u0 = sum(a * _mode(n, p, q) for a, (p, q) in zip(amps, modes))
+    a1, dt, D = -3.0, 0.1, 0.05
+    u, st = _be(n, a1, u0, max_newton_iters=1, krylov_tol=1e-10)
+    print(f"n={n} modes={len(modes)}: newton {st.newton_iters} gmres {st.lin_iters}", flush=True)
+    assert (st.newton_iters, st.lin_iters) == (1, len(modes))
+    exact = sum(a / ((1 - dt * a1) - dt * D * _eig(n, p, q)) * _mode(n, p, q) for a, (p, q) in zip(amps, modes))
+    assert np.linalg.norm(u - exact) <= 1e-10 * np.linalg.norm(exact)
+
+
+@pytest.mark.parametrize("op_dtype", [torch.float64, torch.float32])
+def test_pure_diffusion_one_gmres_per_newton(gpu, op_dtype):
+    from oracle import newton_krylov as onk
+    n = 63
+    u, st = _be(n, 0.0, onk.sine_mode_ic(n, seed=2), op_dtype=op_dtype, krylov_tol=1e-6)
+    assert (st.converged, st.newton_iters, st.lin_iters) == (True, 2, 2)
+
+
+@pytest.mark.parametrize("m,cycles", [(3, 4), (2, 5), (1, 7)])
+def test_restart_cycles_count_every_arnoldi_step(gpu, m, cycles):
+    from oracle import newton_krylov as onk
+    n = 31
+    _, st = _be(n, -3.0, onk.sine_mode_ic(n, seed=4), max_newton_iters=1, krylov_tol=0.0, gmres_restart=m,
+                max_krylov_iters=cycles)
+    assert (st.newton_iters, st.lin_iters) == (1, m * cycles)
+
+
+@pytest.mark.parametrize("method", ["be", "cn", "bdf2"])
+def test_adaptive_implicit_saturated_growth(gpu, method):
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import DSTDiffusionPreconditioner
+    from moljax_amd.core.reaction_diffusion import create_reaction_diffusion_model
+    from moljax_amd.core.stepping import IntegratorType, adaptive_integrate
+    n, dt0, t_end = 8, 1e-3, 3.0
+    model = create_reaction_diffusion_model(n, D=0.0, reaction=(0.0, 0.0, -1e-12, 0.0), bc="dirichlet")
+    m = {"be": IntegratorType.BE, "cn": IntegratorType.CN, "bdf2": IntegratorType.BDF2}[method]
+    res = adaptive_integrate(model, {"u": torch.ones((n, n), dtype=torch.float64)}, 0.0, t_end, dt0, method=m,
+                             preconditioner=DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"}),
+                             nk_params=NKParams(), max_steps=100)
+    want = [1e-3, 6e-3, 0.036, 0.216, 1.0, 1.0, 3.0 - 2.259]
+    assert (res.n_accepted, res.n_rejected) == (7, 0)
+    assert np.allclose(res.dt_history.numpy()[:7], want, rtol=1e-12)
+    assert abs(float(res.t_final) - t_end) <= 1e-15 * t_end

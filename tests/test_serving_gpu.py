@@ -159,3 +159,33 @@ def test_any_length_requests(gpu):
         ic = [t.cpu().numpy() for t in initial_state(_fresh(r), torch.float64)]
         for got, want in zip(r.state, _oracle(r, ic)):
             assert ogs.rel_l2(got.cpu().numpy(), want) < 1e-11, (r.kind, r.n)
+
+
+def test_c5_sizes_up_to_1024(gpu):
+    """The serving workload's own size range: requests of every kind at
+    1024^2 beside 64^2 and 512^2 ones, served together.  Alone == mix
+    bitwise (float32); each 1024^2 kind against the oracle (float64,
+    <= 1e-11 relative L2)."""
+    from moljax_amd.serving import Request, Server, initial_state
+    specs = []
+    for kind in ("fft_cn", "dst", "dct", "imex_gs"):
+        for n, steps in ((1024, 9), (64, 13), (512, 6)):
+            specs.append((kind, n, steps))
+    mk = lambda: [Request(rid=i, kind=k, n=n, D=10 ** (-1 - (i % 3)), dt=10 ** (-1 - (i % 2)), steps=s,
+                          seed=700 + i, arrival=0.0) for i, (k, n, s) in enumerate(specs)]
+    mix = mk()
+    Server(dtype=torch.float32, chunk=4).run(mix, realtime=False)
+    for r in mix:
+        print(f"c5 alone vs mix: {r.kind} n={r.n}", flush=True)
+        alone = _fresh(r)
+        Server(dtype=torch.float32, chunk=4).run([alone], realtime=False)
+        for a, b in zip(r.state, alone.state):
+            assert torch.equal(a, b), f"request {r.rid} ({r.kind}, n={r.n}) differs alone vs in the mix"
+    reqs = [r for r in mk() if r.n == 1024]
+    Server(dtype=torch.float64, chunk=4).run(reqs, realtime=False)
+    for r in reqs:
+        ic = [t.cpu().numpy() for t in initial_state(_fresh(r), torch.float64)]
+        for got, want in zip(r.state, _oracle(r, ic)):
+            err = ogs.rel_l2(got.cpu().numpy(), want)
+            print(f"c5 {r.kind} n={r.n} steps={r.steps}: rel_l2={err:.2e}", flush=True)
+            assert err < 1e-11, (r.kind, r.n)

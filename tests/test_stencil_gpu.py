@@ -77,3 +77,21 @@ def test_gs_reaction(gpu):
     uvv = u * v * v
     assert np.allclose(_np(ru), -uvv + 0.035 * (1 - u), rtol=1e-14, atol=1e-16)
     assert np.allclose(_np(rv), uvv - (0.035 + 0.065) * v, rtol=1e-14, atol=1e-16)
+
+
+def test_fd_upwind_velocity_broadcast(gpu):
+    """A (1, H, W) velocity with a (3, H, W) field applies to every instance
+    (checked per instance against the oracle); a (2, H, W) velocity cannot be
+    broadcast and raises before any launch."""
+    from moljax_amd.core import Grid2D
+    from moljax_amd.core import operators as op
+    grid = Grid2D.uniform(40, 24, 0.0, 2.0, 0.0, 1.0)
+    f = np.stack([_field(24, 40, s) for s in range(3)])
+    vel = _field(24, 40, 77)[None]
+    for axis in (0, 1):
+        out = _np(op.d1_upwind_2d(torch.from_numpy(f).cuda(), torch.from_numpy(vel).cuda(), grid, axis))
+        for b in range(3):
+            ref = ofd.d1_upwind_2d(f[b], vel[0], grid.dy, grid.dx, axis)
+            assert np.allclose(out[b], ref, rtol=1e-13, atol=1e-11)
+    with pytest.raises(ValueError, match="broadcast"):
+        op.d1_upwind_2d(torch.from_numpy(f).cuda(), torch.from_numpy(np.zeros((2, 26, 42))).cuda(), grid, 0)
