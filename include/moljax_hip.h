@@ -106,6 +106,23 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                        long long istride, int fill_ghosts, const void* lamx, const void* lamy,
                        const void* lam2d, const double* inst, int n_steps, void* work, void* stream);
 
+/* mjx_gs_imex_strang without any allocation: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The spectral workspace
+ * is a fixed, library-owned device array of MJX_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends); requests whose
+ * mjx_spectral_workspace_bytes(dtype, B, ny, nx) exceed it return
+ * MJX_E_LIMIT.  Power-of-two sides in [16, 4096] only (MJX_E_SIZE
+ * otherwise).  mjx_init must have run on the device (MJX_E_ARG otherwise).
+ * Calls no hipMalloc/hipFree and creates no events.  Every call on a GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (different
+ * streams or threads) race; calls on one stream are ordered.  Results are
+ * bitwise those of mjx_gs_imex_strang. */
+#define MJX_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int mjx_gs_imex_strang_noalloc(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch,
+                               long long istride, int fill_ghosts, const void* lamx, const void* lamy,
+                               const void* lam2d, const double* inst, int n_steps, void* stream);
+
 /* Gray-Scott IMEX with a choice of scheme, n_steps fixed steps in place:
  * scheme 0 Strang (identical to mjx_gs_imex_strang), 1 IMEX-Euler
  *   y_new = (I - dt D lap)^-1 (y + dt R(y))      (stepping.py:317-367),

@@ -33,6 +33,7 @@ PROTOTYPES = {
     "mjx_spectral_workspace_bytes": (_ll, [_i, _i, _i, _i]),
     "mjx_spectral_pair": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_imex_strang": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
+    "mjx_gs_imex_strang_noalloc": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p]),
     "mjx_gs_imex_strang_var": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_imex": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_adaptive_workspace_bytes": (_ll, [_i, _i, _i, _i, _i]),

@@ -24,6 +24,11 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_launch(const char* what);
 int ensure_tables(hipStream_t stream);
+// Resolves the current device's address of the allocation-free entry point's
+// scratch array (spectral.hip); called by mjx_init.
+int scratch_setup();
+// True when the current device's twiddle tables exist (mjx_init has run).
+bool tables_ready();
 
 // Tile shapes chosen by the autotuner: lines (rows or columns) per
 // workgroup for a kernel family, precision and transform length; 0 means the

@@ -71,6 +71,13 @@ int ensure_tables(hipStream_t) {
   return MJX_OK;
 }
 
+bool tables_ready() {
+  const int dev = current_device();
+  if (dev < 0) return false;
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  return g_tw64[dev] != nullptr && g_tw32[dev] != nullptr;
+}
+
 const cpx<float>* twiddles_f32() {
   const int dev = current_device();
   return dev < 0 ? nullptr : g_tw32[dev];
@@ -124,6 +131,9 @@ int mjx_version(void) { return 100; }
 
 const char* mjx_last_error(void) { return mjx::g_err.c_str(); }
 
-int mjx_init(void) { return mjx::ensure_tables(nullptr); }
+int mjx_init(void) {
+  const int rc = mjx::ensure_tables(nullptr);
+  return rc ? rc : mjx::scratch_setup();
+}
 
 }  // extern "C"

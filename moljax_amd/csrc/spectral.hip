@@ -87,20 +87,23 @@ static int spectral_pair_t(int B, int ny, int nx, const T* a_in, const T* b_in, 
   return spec_row_inv_unpack(nx, B, ny, W, a_out, b_in ? b_out : nullptr, op_, os_, ghosts, st);
 }
 
+// timed: the opt-in per-kernel events may bracket the hot launches (the
+// allocation-free entry point passes false: it creates no events).
 template <typename T>
 static int gs_imex_t(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
                      const T* lamx, const T* lamy, const T* lam2d, const InstParams* inst, int n_steps, cpx<T>* W,
-                     hipStream_t st) {
+                     hipStream_t st, bool timed = true) {
   int rc = spec_row_fwd_pack(nx, B, ny, u, v, pitch, istride, W, st);
   if (rc) return rc;
   rc = spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, 1, st);
   if (rc) return rc;
   for (int s = 0; s < n_steps; ++s) {
-    rc = timed_launch(KT_ROW_GS, st, [&] { return spec_row_gs(nx, B, ny, W, inst, st); });
+    auto row = [&] { return spec_row_gs(nx, B, ny, W, inst, st); };
+    rc = timed ? timed_launch(KT_ROW_GS, st, row) : row();
     if (rc) return rc;
     const int nrep = (s == n_steps - 1) ? 1 : 2;
-    rc = timed_launch(KT_COL_APPLY, st,
-                      [&] { return spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, nrep, st); });
+    auto col = [&] { return spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, nrep, st); };
+    rc = timed ? timed_launch(KT_COL_APPLY, st, col) : col();
     if (rc) return rc;
   }
   return spec_row_inv_unpack(nx, B, ny, W, u, v, pitch, istride, ghosts, st);
@@ -122,6 +125,38 @@ static int gs_imex_var_t(int B, int ny, int nx, T* u, T* v, long long pitch, lon
   }
   if (rc) return rc;
   return spec_row_inv_unpack(nx, B, ny, W, u, v, pitch, istride, ghosts, st);
+}
+
+// ---------------------------------------------- allocation-free entry point
+// The spectral workspace of mjx_gs_imex_strang_noalloc lives in this
+// file-scope device array: one copy per GPU, held until the process ends.
+// In host code `scratch` only names the symbol; its device address on the
+// current GPU comes from hipGetSymbolAddress (scratch_setup) and that
+// address, never the array's name, is what the kernels receive.
+static __device__ __align__(256) unsigned char scratch[MJX_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {nullptr};
+
+// Resolves the current device's address of `scratch` once per device (called
+// by mjx_init, and by the entry point's first call on a device).
+int scratch_setup() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail(MJX_E_HIP, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return MJX_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return fail(MJX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+  return MJX_OK;
+}
+
+static void* scratch_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
 }
 
 }  // namespace mjx
@@ -182,6 +217,39 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                             (const float*)lamy, (const float*)lam2d, ip, n_steps, (cpx<float>*)work, st);
   return gs_imex_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)lamx,
                            (const double*)lamy, (const double*)lam2d, ip, n_steps, (cpx<double>*)work, st);
+}
+
+int mjx_gs_imex_strang_noalloc(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch,
+                               long long istride, int fill_ghosts, const void* lamx, const void* lamy,
+                               const void* lam2d, const double* inst, int n_steps, void* stream) {
+  int rc = check_grid(dtype, B, ny, nx);
+  if (rc) return rc;
+  if (spec_generic(nx, ny))
+    return fail(MJX_E_SIZE, "mjx_gs_imex_strang_noalloc serves power-of-two sides in [16, 4096] only");
+  if (!u || !v || !inst) return fail(MJX_E_ARG, "null pointer argument");
+  if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
+  if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
+  if (pitch < nx) return fail(MJX_E_ARG, "pitch smaller than nx");
+  if (fill_ghosts && pitch < nx + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= nx + 2");
+  const long long need = mjx_spectral_workspace_bytes(dtype, B, ny, nx);
+  if (need > MJX_NOALLOC_SCRATCH_BYTES)
+    return fail(MJX_E_LIMIT, "mjx_gs_imex_strang_noalloc: the workspace needs " + std::to_string(need) +
+                                 " bytes, above the fixed scratch of " + std::to_string(MJX_NOALLOC_SCRATCH_BYTES));
+  if (n_steps == 0) return MJX_OK;
+  if (!tables_ready())
+    return fail(MJX_E_ARG, "mjx_gs_imex_strang_noalloc: call mjx_init on this device first");
+  if ((rc = scratch_setup())) return rc;
+  void* W = scratch_ptr();
+  if (!W) return fail(MJX_E_HIP, "scratch address unavailable");
+  hipStream_t st = (hipStream_t)stream;
+  const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  // every workspace element is written by the first pass before any read,
+  // so the scratch needs no clearing between calls
+  if (dtype == MJX_F32)
+    return gs_imex_t<float>(B, ny, nx, (float*)u, (float*)v, pitch, istride, fill_ghosts, (const float*)lamx,
+                            (const float*)lamy, (const float*)lam2d, ip, n_steps, (cpx<float>*)W, st, false);
+  return gs_imex_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts, (const double*)lamx,
+                           (const double*)lamy, (const double*)lam2d, ip, n_steps, (cpx<double>*)W, st, false);
 }
 
 int mjx_gs_imex_strang_var(int dtype, int B, int ny, int nx, void* u, void* v, long long pitch, long long istride,
