@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -182,8 +184,378 @@ __global__ __launch_bounds__(256) void k_to_f64(const float* __restrict__ x, dou
   if (g < n) y[g] = (double)x[g];
 }
 
-static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+// ------------------------------------------------ device-resident GMRES(m)
+// The Krylov loop keeps every scalar on the device: the Hessenberg column,
+// the Givens rotations, the residual estimate and the convergence test are
+// computed by one-workgroup kernels that finish the fixed-order reductions
+// of the orthogonalisation kernels, and later launches of a finished cycle
+// exit at their first instruction.  The host enqueues Arnoldi steps in
+// chunks and reads the control block back once per chunk (a few times per
+// solve) instead of after every step.  Reductions end at kernel boundaries
+// (partials by RB workgroups, then one workgroup), so no cross-workgroup
+// fence is needed and every sum has the same order on every run.
+constexpr int RB = 256;  // workgroups of the partial reductions (fixed partition -> deterministic sums)
+constexpr int RT = 256;  // threads per workgroup
+constexpr int VC = 8;    // basis vectors per register chunk
 
+struct GmresCtl {
+  double beta, target, res, inv_hn;  // cycle norm, rtol * ||M b||, |g_{k}|, 1 / ||w|| of the last step
+  double rtol, newton_tol, damping, bt_factor, min_dec;
+  double rn, r_bt, alpha;  // Newton: residual norm at the iterate, backtracking state
+  int k;                   // Arnoldi steps taken in the current cycle
+  int iters;               // Arnoldi steps in this GMRES solve
+  int done;                // cycle finished (converged or breakdown): later steps exit
+  int conv;                // GMRES converged
+  int newton_conv;         // rn < newton_tol at the current Newton iterate
+  int accepted;            // a backtracking trial was accepted
+  int moved;               // the accepted trial differs from the iterate (not allclose)
+  int pad;
+};
+
+// Sum over the workgroup in a fixed tree (wave64 shuffles, then the waves in
+// order); the result is valid in thread 0.
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  return threadIdx.x == 0 ? ((red[0] + red[1]) + red[2]) + red[3] : 0.0;
+}
+
+// Per-workgroup dot products of w with basis vectors [0, k) over this
+// workgroup's grid-stride slice, written as part[(c*RB + b)*VC + i].
+__device__ void slice_dots(const double* __restrict__ V, const double* w, long long n, int k, double* part,
+                           double* red) {
+  const int nchunk = (k + VC - 1) / VC;
+  for (int c = 0; c < nchunk; ++c) {
+    double a[VC];
+#pragma unroll
+    for (int i = 0; i < VC; ++i) a[i] = 0.0;
+    for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
+      const double wv = w[g];
+#pragma unroll
+      for (int i = 0; i < VC; ++i)
+        if (c * VC + i < k) a[i] += wv * V[(long long)(c * VC + i) * n + g];
+    }
+#pragma unroll
+    for (int i = 0; i < VC; ++i) {
+      const double s = wg_sum(a[i], red);
+      if (threadIdx.x == 0) part[((long long)c * RB + blockIdx.x) * VC + i] = s;
+    }
+  }
+}
+
+// One workgroup: out[idx] = fixed-order sum over workgroups of part for idx < k.
+__global__ __launch_bounds__(256) void k_gm_final_h(const double* part, int k, const GmresCtl* ctl,
+                                                    double* __restrict__ out) {
+  if (ctl->done) return;
+  __shared__ double red[4];
+  for (int idx = 0; idx < k; ++idx) {
+    const int c = idx / VC, i = idx % VC;
+    const double s = wg_sum(part[((long long)c * RB + threadIdx.x) * VC + i], red);
+    if (threadIdx.x == 0) out[idx] = s;
+  }
+}
+
+// Step j, part 1: v_j = w * inv_hn (written to V_j), t = J v_j in T.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gm_jvp(const double* __restrict__ u, const double* __restrict__ w,
+                                                const GmresCtl* ctl, double* __restrict__ Vj, T* __restrict__ t,
+                                                RDArgs a) {
+  if (ctl->done) return;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)a.ny * a.nx;
+  if (g >= n) return;
+  const double s = ctl->inv_hn;
+  const int y = (int)(g / a.nx), x = (int)(g % a.nx);
+  const double vc = w[g] * s;
+  Vj[g] = vc;
+  auto vat = [&](int yy, int xx) -> T {
+    if (a.bc == 1) {
+      yy = yy < 0 ? yy + a.ny : (yy >= a.ny ? yy - a.ny : yy);
+      xx = xx < 0 ? xx + a.nx : (xx >= a.nx ? xx - a.nx : xx);
+    } else if (yy < 0 || yy >= a.ny || xx < 0 || xx >= a.nx) {
+      return (T)0;
+    }
+    return (T)(w[(long long)yy * a.nx + xx] * s);
+  };
+  const T c = (T)vc;
+  const T lap = (vat(y + 1, x) - (T)2 * c + vat(y - 1, x)) / (T)a.dy2 +
+                (vat(y, x + 1) - (T)2 * c + vat(y, x - 1)) / (T)a.dx2;
+  const double uu = u[g];
+  const T fp = (T)((3.0 * a.a3 * uu + 2.0 * a.a2) * uu + a.a1);
+  t[g] = (T)a.lead * c - (T)a.theta_dt * ((T)a.D * lap + fp * c);
+}
+
+// Step j, part 2: partials of h1 = V[0..k)^T w (w given in TIN; for float it
+// is widened into w64 on the way).
+template <typename TIN>
+__global__ __launch_bounds__(256) void k_gm_dot1(const double* __restrict__ V, const TIN* win, double* w64,
+                                                 long long n, int k, double* part, const GmresCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double red[4];
+  if constexpr (!std::is_same<TIN, double>::value) {
+    for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT)
+      w64[g] = (double)win[g];
+  }
+  slice_dots(V, w64, n, k, part, red);
+}
+
+// Step j, part 3: w -= V h1, then partials of h2 = V^T w.
+__global__ __launch_bounds__(256) void k_gm_upd_dot2(const double* __restrict__ V, double* w, long long n, int k,
+                                                     const double* __restrict__ h1, double* part,
+                                                     const GmresCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double red[4];
+  __shared__ double hs[256];
+  for (int i = threadIdx.x; i < k; i += RT) hs[i] = h1[i];
+  __syncthreads();
+  for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += hs[i] * V[(long long)i * n + g];
+    w[g] -= s;
+  }
+  slice_dots(V, w, n, k, part, red);
+}
+
+// Step j, part 4: w -= V h2 and partials of ||w||^2.
+__global__ __launch_bounds__(256) void k_gm_upd_norm(const double* __restrict__ V, double* w, long long n, int k,
+                                                     const double* __restrict__ h2, double* part,
+                                                     const GmresCtl* ctl) {
+  if (ctl->done) return;
+  __shared__ double red[4];
+  __shared__ double hs[256];
+  for (int i = threadIdx.x; i < k; i += RT) hs[i] = h2[i];
+  __syncthreads();
+  double acc = 0.0;
+  for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += hs[i] * V[(long long)i * n + g];
+    const double wn = w[g] - s;
+    w[g] = wn;
+    acc += wn * wn;
+  }
+  const double ps = wg_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = ps;
+}
+
+// Step j, part 5 (one workgroup): hn = ||w||, Hessenberg column j (h1 + h2,
+// hn), the previous rotations, rotation j, g and the residual estimate; ends
+// the cycle on convergence (|g_{j+1}| <= target) or breakdown (hn == 0).
+__global__ __launch_bounds__(256) void k_gm_givens(const double* part, int j, int m, const double* __restrict__ h1,
+                                                   const double* __restrict__ h2, GmresCtl* ctl, double* H,
+                                                   double* cs, double* sn, double* gv) {
+  if (ctl->done) return;
+  __shared__ double red[4];
+  const double hn2 = wg_sum(part[threadIdx.x], red);
+  if (threadIdx.x != 0) return;
+  const int k = j + 1;
+  const double hn = sqrt(hn2);
+  for (int i = 0; i < k; ++i) H[(long long)i * m + j] = h1[i] + h2[i];
+  H[(long long)(j + 1) * m + j] = hn;
+  for (int i = 0; i < j; ++i) {
+    const double a0 = H[(long long)i * m + j], a1 = H[(long long)(i + 1) * m + j];
+    H[(long long)i * m + j] = cs[i] * a0 + sn[i] * a1;
+    H[(long long)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+  }
+  const double hjj = H[(long long)j * m + j], hj1 = H[(long long)(j + 1) * m + j];
+  const double den = hypot(hjj, hj1);
+  cs[j] = hjj / den;
+  sn[j] = hj1 / den;
+  H[(long long)j * m + j] = den;
+  H[(long long)(j + 1) * m + j] = 0.0;
+  const double gj = gv[j];
+  gv[j + 1] = -sn[j] * gj;
+  gv[j] = cs[j] * gj;
+  const double res = fabs(gv[j + 1]);
+  ctl->res = res;
+  ctl->k = k;
+  ctl->iters += 1;
+  if (res <= ctl->target || hn == 0.0) {
+    ctl->done = 1;
+    ctl->conv = res <= ctl->target ? 1 : 0;
+  } else {
+    ctl->inv_hn = 1.0 / hn;
+  }
+}
+
+// Partials of ||w||^2 (no control check: cycle starts always run).
+__global__ __launch_bounds__(256) void k_norm2_partial(const double* w, long long n, double* part) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) acc += w[g] * w[g];
+  const double ps = wg_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = ps;
+}
+
+// Start of a restart cycle (one workgroup): beta = ||w|| (w = M (b - A x), or
+// M b in cycle 0, where the target rtol * ||M b|| is set).
+__global__ __launch_bounds__(256) void k_gm_cycle_start(const double* part, int first, GmresCtl* ctl, double* gv) {
+  __shared__ double red[4];
+  const double b2 = wg_sum(part[threadIdx.x], red);
+  if (threadIdx.x != 0) return;
+  const double beta = sqrt(b2);
+  ctl->k = 0;
+  if (first) {
+    ctl->iters = 0;
+    ctl->target = ctl->rtol * beta;
+  }
+  ctl->beta = beta;
+  if (beta == 0.0 || beta <= ctl->target) {
+    ctl->done = 1;
+    ctl->conv = 1;
+    return;
+  }
+  ctl->done = 0;
+  ctl->conv = 0;
+  gv[0] = beta;
+  ctl->inv_hn = 1.0 / beta;
+}
+
+// End of a cycle: y = H[:k,:k]^-1 g[:k] (back substitution, repeated in every
+// workgroup) and x += V[:k]^T y.
+__global__ __launch_bounds__(256) void k_gm_update_x(const double* __restrict__ V, long long n, int m,
+                                                     const double* __restrict__ H, const double* __restrict__ gv,
+                                                     const GmresCtl* ctl, double* __restrict__ x) {
+  const int k = ctl->k;
+  if (k == 0) return;
+  __shared__ double y[256];
+  if (threadIdx.x == 0) {
+    for (int i = k - 1; i >= 0; --i) {
+      double s = gv[i];
+      for (int l = i + 1; l < k; ++l) s -= H[(long long)i * m + l] * y[l];
+      y[i] = s / H[(long long)i * m + i];
+    }
+  }
+  __syncthreads();
+  for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)gridDim.x * RT) {
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += y[i] * V[(long long)i * n + g];
+    x[g] += s;
+  }
+}
+
+// ------------------------------------------------ device-resident Newton
+// Residual at the iterate: b = -R(u) and partials of ||R(u)||^2.
+__global__ __launch_bounds__(256) void k_nk_residual(const double* __restrict__ u, const double* __restrict__ u_old,
+                                                     const double* __restrict__ fold, double* __restrict__ b,
+                                                     RDArgs a, double* part) {
+  __shared__ double red[4];
+  const long long n = (long long)a.ny * a.nx;
+  double acc = 0.0;
+  for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
+    const int y = (int)(g / a.nx), x = (int)(g % a.nx);
+    const double uu = u[g];
+    const double f = ((a.a3 * uu + a.a2) * uu + a.a1) * uu + a.a0;
+    double r = a.lead * uu - u_old[g] - a.theta_dt * (a.D * lap5(u, y, x, a) + f);
+    if (fold) r -= fold[g];
+    b[g] = -r;
+    acc += r * r;
+  }
+  const double ps = wg_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = ps;
+}
+
+// One workgroup: rn = ||R(u)||, the convergence flag, and the backtracking
+// state of the iteration reset.
+__global__ __launch_bounds__(256) void k_nk_residual_final(const double* part, GmresCtl* ctl) {
+  __shared__ double red[4];
+  const double r2 = wg_sum(part[threadIdx.x], red);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(r2);
+  ctl->rn = rn;
+  ctl->newton_conv = rn < ctl->newton_tol ? 1 : 0;
+  ctl->r_bt = rn;
+  ctl->alpha = ctl->damping;
+  ctl->accepted = 0;
+  ctl->moved = 0;
+}
+
+// One backtracking trial (newton_krylov.py:272-316): x_t = u + alpha dx and
+// partials of ||R(x_t)||^2, its neighbours formed the same way.  Trials after
+// an acceptance leave the state unchanged in the reference, so they are
+// skipped.
+__global__ __launch_bounds__(256) void k_nk_trial(const double* __restrict__ u, const double* __restrict__ dx,
+                                                  double* __restrict__ xt, const double* __restrict__ u_old,
+                                                  const double* __restrict__ fold, RDArgs a, double* part,
+                                                  const GmresCtl* ctl) {
+  if (ctl->accepted) return;
+  __shared__ double red[4];
+  const double alpha = ctl->alpha;
+  const long long n = (long long)a.ny * a.nx;
+  auto xa = [&](int yy, int xx) -> double {
+    if (a.bc == 1) {
+      yy = yy < 0 ? yy + a.ny : (yy >= a.ny ? yy - a.ny : yy);
+      xx = xx < 0 ? xx + a.nx : (xx >= a.nx ? xx - a.nx : xx);
+    } else if (yy < 0 || yy >= a.ny || xx < 0 || xx >= a.nx) {
+      return 0.0;
+    }
+    const long long q = (long long)yy * a.nx + xx;
+    const double ad = alpha * dx[q];
+    return u[q] + ad;
+  };
+  double acc = 0.0;
+  for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
+    const int y = (int)(g / a.nx), x = (int)(g % a.nx);
+    const double c = xa(y, x);
+    xt[g] = c;
+    const double lap = (xa(y + 1, x) - 2.0 * c + xa(y - 1, x)) / a.dy2 + (xa(y, x + 1) - 2.0 * c + xa(y, x - 1)) / a.dx2;
+    const double f = ((a.a3 * c + a.a2) * c + a.a1) * c + a.a0;
+    double r = a.lead * c - u_old[g] - a.theta_dt * (a.D * lap + f);
+    if (fold) r -= fold[g];
+    acc += r * r;
+  }
+  const double ps = wg_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = ps;
+}
+
+// One workgroup: accept when ||R(x_t)|| < (1 - min_dec alpha) r_bt, halve
+// alpha otherwise.
+__global__ __launch_bounds__(256) void k_nk_trial_final(const double* part, GmresCtl* ctl) {
+  if (ctl->accepted) return;
+  __shared__ double red[4];
+  const double r2 = wg_sum(part[threadIdx.x], red);
+  if (threadIdx.x != 0) return;
+  const double rnew = sqrt(r2);
+  const double alpha = ctl->alpha;
+  if (rnew < (1.0 - ctl->min_dec * alpha) * ctl->r_bt) {
+    ctl->accepted = 1;
+    ctl->r_bt = rnew;
+  } else {
+    ctl->alpha = alpha * ctl->bt_factor;
+  }
+}
+
+// moved = 1 if any |x_t - u| > 1e-8 + 1e-5 |u| (jnp.allclose defaults), for
+// an accepted trial.
+__global__ __launch_bounds__(256) void k_nk_allclose(const double* __restrict__ xt, const double* __restrict__ u,
+                                                     long long n, GmresCtl* ctl) {
+  if (!ctl->accepted) return;
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n && !(fabs(xt[g] - u[g]) <= 1e-8 + 1e-5 * fabs(u[g]))) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && any) atomicOr(&ctl->moved, 1);  // one atomic per workgroup
+}
+
+// The Newton update: the accepted trial, or u + damping dx when no trial was
+// accepted or the accepted one did not move the iterate.
+__global__ __launch_bounds__(256) void k_nk_update(double* __restrict__ u, const double* __restrict__ xt,
+                                                   const double* __restrict__ dx, long long n, const GmresCtl* ctl) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  if (ctl->accepted && ctl->moved) {
+    u[g] = xt[g];
+  } else {
+    const double ad = ctl->damping * dx[g];
+    u[g] = u[g] + ad;
+  }
+}
+
+static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
 // ------------------------------------------------------------- driver
 struct NKConfig {
@@ -201,25 +573,54 @@ struct NKStatsOut {
   double res_norm;
 };
 
+// Pinned host copy of the control block, one per device (read back once per
+// chunk of Arnoldi steps).
+static std::mutex g_ctl_mu;
+static GmresCtl* g_ctl_host[64] = {nullptr};
+
+static GmresCtl* ctl_host() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_ctl_mu);
+  if (!g_ctl_host[dev]) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sizeof(GmresCtl), hipHostMallocDefault) != hipSuccess) return nullptr;
+    g_ctl_host[dev] = (GmresCtl*)p;
+  }
+  return g_ctl_host[dev];
+}
+
+// Arnoldi steps enqueued before the first read-back of a solve: the previous
+// solve's count (a restart cycle at most).
+static thread_local int g_gmres_hint = 4;
+
 class NKSolver {
  public:
   NKSolver(const RDArgs& a, const NKConfig& c, double* u, const double* u_old, const double* fold, char* work,
            const void* const* lam, hipStream_t st)
       : a_(a), c_(c), u_(u), u_old_(u_old), fold_(fold), st_(st) {
     n_ = (long long)a.ny * a.nx;
-    const long long s2 = precond_work_reals(a.ny, a.nx);
     const int m = c.restart;
-    double* p = (double*)work;
+    char* base = work;
+    ctl_ = (GmresCtl*)base;
+    double* p = (double*)(base + 256);
+    H_ = p; p += (long long)(m + 1) * m;
+    cs_ = p; p += m;
+    sn_ = p; p += m;
+    g_ = p; p += m + 1;
+    h1_ = p; p += m + 1;
+    h2_ = p; p += m + 1;
+    part_ = p; p += part_elems(m);
+    p = (double*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
     V_ = p; p += (long long)(m + 1) * n_;
     w_ = p; p += n_;
-    r_ = p; p += n_;
+    b_ = p; p += n_;
     dx_ = p; p += n_;
     xt_ = p; p += n_;
     t64_ = p; p += n_;
     t64b_ = p; p += n_;
+    const long long s2 = precond_work_reals(a.ny, a.nx);
     r2r64_ = p; p += s2;
-    hdev_ = p; p += 3 * (m + 2);
-    partial_ = p; p += (long long)((m + 1 + DOT_CHUNK - 1) / DOT_CHUNK) * DOT_BLOCKS * DOT_CHUNK;
     inst_ = p; p += 8;
     float* f = (float*)p;
     v32_ = f; f += n_;
@@ -233,6 +634,8 @@ class NKSolver {
     lamy32_ = lam[3];
   }
 
+  static long long part_elems(int m) { return (long long)((m + 1 + VC - 1) / VC) * RB * VC; }
+
   // Reals of the exact preconditioner's work array: the periodic spectral
   // work (complex, spec_work_cols per row), which also covers the n reals of
   // the DST solve.
@@ -240,208 +643,158 @@ class NKSolver {
 
   static long long workspace_bytes(int ny, int nx, int restart) {
     const long long n = (long long)ny * nx;
+    const int m = restart;
     const long long s2 = precond_work_reals(ny, nx);
-    long long d = (long long)(restart + 1) * n + 6 * n + s2 + 3 * (restart + 2) +
-                  (long long)((restart + 1 + DOT_CHUNK - 1) / DOT_CHUNK) * DOT_BLOCKS * DOT_CHUNK + 8;
-    return d * 8 + (4 * n + s2) * 4 + 256;
+    const long long small = (long long)(m + 1) * m + 2LL * m + 3LL * (m + 1) + part_elems(m);
+    long long d = (long long)(m + 1) * n + 6 * n + s2 + 8;
+    return 256 + small * 8 + 256 + d * 8 + (4 * n + s2) * 4 + 256;
   }
 
   int solve(NKStatsOut* out) {
-    int rc = setup_inst();
+    int rc = setup();
     if (rc) return rc;
-    double r0 = 0;
-    if ((rc = residual(u_, r_, &r0))) return rc;
-    bool converged = r0 < c_.newton_tol;
-    double res_norm = r0;
+    GmresCtl* hc = ctl_host();
+    if (!hc) return fail(MJX_E_HIP, "nk: pinned control block unavailable");
+    // residual at the initial guess; its flags serve the first iteration
+    residual();
+    if ((rc = read_ctl(hc))) return rc;
+    bool converged = hc->newton_conv != 0;
     int it = 0, lin_total = 0;
     while (it < c_.max_newton && !converged) {
-      double rn = 0;
-      if ((rc = residual(u_, r_, &rn))) return rc;
-      converged = rn < c_.newton_tol;
-      // b = -r
-      hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, r_, -1.0, r_, n_);
+      if (it > 0) residual();
       int lin = 0;
-      if ((rc = gmres(r_, dx_, &lin))) return rc;
+      if ((rc = gmres(b_, dx_, hc, &lin))) return rc;
       lin_total += lin;
-      // backtracking (newton_krylov.py:272-316): always max_backtrack trials
-      double alpha = c_.damping, r_bt = rn;
-      bool accepted = false;
-      const double* x_bt = u_;
-      for (int b = 0; b < c_.max_backtrack; ++b) {
-        hipLaunchKernelGGL(k_axpby, dim3(nblk(n_)), dim3(256), 0, st_, 1.0, x_bt, alpha, (const double*)dx_, t64b_,
-                           n_);
-        double rnew = 0;
-        if ((rc = residual(t64b_, t64_, &rnew))) return rc;
-        const bool acc = rnew < (1.0 - c_.min_decrease * alpha) * r_bt;
-        if (!(acc || accepted)) alpha *= c_.backtrack_factor;
-        if (acc && !accepted) {
-          copy(t64b_, xt_);
-          x_bt = xt_;
-          r_bt = rnew;
-        }
-        accepted = accepted || acc;
+      converged = hc->newton_conv != 0;  // this iteration's test (made before its update)
+      for (int t = 0; t < c_.max_backtrack; ++t) {
+        hipLaunchKernelGGL(k_nk_trial, dim3(RB), dim3(RT), 0, st_, (const double*)u_, (const double*)dx_, xt_,
+                           u_old_, fold_, a_, part_, (const GmresCtl*)ctl_);
+        hipLaunchKernelGGL(k_nk_trial_final, dim3(1), dim3(RT), 0, st_, (const double*)part_, ctl_);
       }
-      // fallback when the step is (numerically) not taken: x + damping dx,
-      // as jnp.allclose(x_new, x) (rtol 1e-5, atol 1e-8) decides
-      bool same = !accepted;
-      if (accepted) {
-        int sr = allclose(xt_, u_, &same);
-        if (sr) return sr;
-      }
-      if (same) {
-        hipLaunchKernelGGL(k_axpby, dim3(nblk(n_)), dim3(256), 0, st_, 1.0, (const double*)u_, c_.damping,
-                           (const double*)dx_, u_, n_);
-      } else {
-        copy(xt_, u_);
-      }
-      res_norm = r_bt;
+      hipLaunchKernelGGL(k_nk_allclose, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)xt_, (const double*)u_,
+                         n_, ctl_);
+      hipLaunchKernelGGL(k_nk_update, dim3(nblk(n_)), dim3(256), 0, st_, u_, (const double*)xt_,
+                         (const double*)dx_, n_, (const GmresCtl*)ctl_);
       ++it;
     }
+    if ((rc = read_ctl(hc))) return rc;
     out->converged = converged ? 1 : 0;
     out->newton_iters = it;
     out->gmres_iters = lin_total;
-    out->res_norm = res_norm;
+    out->res_norm = it > 0 ? hc->r_bt : hc->rn;
     out->status = 0;
     return check_launch("newton");
   }
 
-  // Left-preconditioned restarted GMRES(m) for J dx = b (b in bvec),
-  // x0 = 0.  One iteration = one Arnoldi step; stops when the Givens
-  // residual <= rtol * ||M b|| (checked every step); max_krylov restart cycles.
-  int gmres(const double* bvec, double* x, int* iters_out) {
+  // Left-preconditioned restarted GMRES(m) for J dx = b, x0 = 0.  One
+  // iteration = one Arnoldi step; the Givens residual is compared with rtol *
+  // ||M b|| after every step; at most max_krylov restart cycles.
+  int gmres(const double* bvec, double* x, GmresCtl* hc, int* iters_out) {
     const int m = c_.restart;
     int rc;
     hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, bvec, 0.0, x, n_);
     if ((rc = precond(bvec, w_))) return rc;  // w = M b
-    double bnorm = 0;
-    if ((rc = norm(w_, &bnorm))) return rc;
-    *iters_out = 0;
-    if (bnorm == 0.0) return MJX_OK;
-    const double target = c_.krylov_tol * bnorm;
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), hh(3 * (m + 2));
     int iters = 0;
-    double res = bnorm;
     for (int cyc = 0; cyc < c_.max_krylov; ++cyc) {
-      if (iters > 0) {  // r = M (b - A x); xt_ is free during the Krylov solve
+      if (cyc > 0) {  // w = M (b - A x)
         if ((rc = apply_A(x, t64_))) return rc;
         hipLaunchKernelGGL(k_axpby, dim3(nblk(n_)), dim3(256), 0, st_, 1.0, bvec, -1.0, (const double*)t64_, xt_,
                            n_);
         if ((rc = precond(xt_, w_))) return rc;
       }
-      double beta = 0;
-      if ((rc = norm(w_, &beta))) return rc;
-      if (beta <= target) break;
-      hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)w_, 1.0 / beta, V_, n_);
-      std::fill(g.begin(), g.end(), 0.0);
-      std::fill(H.begin(), H.end(), 0.0);
-      g[0] = beta;
-      int k = 0;
-      for (int j = 0; j < m; ++j) {
-        if ((rc = apply_op(V_ + (long long)j * n_, w_))) return rc;
-        ++iters;
-        const int kk = j + 1;
-        // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2; hn = ||w||
-        multidot(V_, w_, kk, hdev_);
-        hipLaunchKernelGGL(k_multiaxpy, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)V_,
-                           (const double*)hdev_, -1.0, kk, w_, n_);
-        multidot(V_, w_, kk, hdev_ + (m + 2));
-        hipLaunchKernelGGL(k_multiaxpy, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)V_,
-                           (const double*)(hdev_ + (m + 2)), -1.0, kk, w_, n_);
-        multidot(w_, w_, 1, hdev_ + 2 * (m + 2));
-        if (hipMemcpyAsync(hh.data(), hdev_, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, st_) != hipSuccess ||
-            hipStreamSynchronize(st_) != hipSuccess)
-          return fail(MJX_E_HIP, "gmres: device to host copy failed");
-        const double hn = std::sqrt(hh[2 * (m + 2)]);
-        for (int i = 0; i < kk; ++i) H[(size_t)i * m + j] = hh[i] + hh[(m + 2) + i];
-        H[(size_t)(j + 1) * m + j] = hn;
-        for (int i = 0; i < j; ++i) {
-          const double t = cs[i] * H[(size_t)i * m + j] + sn[i] * H[(size_t)(i + 1) * m + j];
-          H[(size_t)(i + 1) * m + j] = -sn[i] * H[(size_t)i * m + j] + cs[i] * H[(size_t)(i + 1) * m + j];
-          H[(size_t)i * m + j] = t;
-        }
-        const double den = std::hypot(H[(size_t)j * m + j], H[(size_t)(j + 1) * m + j]);
-        cs[j] = H[(size_t)j * m + j] / den;
-        sn[j] = H[(size_t)(j + 1) * m + j] / den;
-        H[(size_t)j * m + j] = den;
-        H[(size_t)(j + 1) * m + j] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        res = std::fabs(g[j + 1]);
-        k = kk;
-        if (res <= target || hn == 0.0) break;
-        hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)w_, 1.0 / hn,
-                           V_ + (long long)kk * n_, n_);
+      hipLaunchKernelGGL(k_norm2_partial, dim3(RB), dim3(RT), 0, st_, (const double*)w_, n_, part_);
+      hipLaunchKernelGGL(k_gm_cycle_start, dim3(1), dim3(RT), 0, st_, (const double*)part_, cyc == 0 ? 1 : 0, ctl_,
+                         g_);
+      if (cyc > 0) {  // a restart may find the target met before any step
+        if ((rc = read_ctl(hc))) return rc;
+        if (hc->done) break;
       }
-      for (int i = k - 1; i >= 0; --i) {
-        double s = g[i];
-        for (int l = i + 1; l < k; ++l) s -= H[(size_t)i * m + l] * y[l];
-        y[i] = s / H[(size_t)i * m + i];
+      int j = 0;
+      int chunk = std::max(1, std::min(m, g_gmres_hint));
+      while (j < m) {
+        const int c = std::min(chunk, m - j);
+        for (int jj = j; jj < j + c; ++jj)
+          if ((rc = arnoldi_step(jj))) return rc;
+        j += c;
+        if ((rc = read_ctl(hc))) return rc;
+        if (hc->done) break;
+        chunk = 2;
       }
-      if (hipMemcpyAsync(hdev_, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, st_) != hipSuccess)
-        return fail(MJX_E_HIP, "gmres: host to device copy failed");
-      hipLaunchKernelGGL(k_multiaxpy, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)V_, (const double*)hdev_, 1.0,
-                         k, x, n_);
-      if (hipStreamSynchronize(st_) != hipSuccess) return fail(MJX_E_HIP, "gmres: sync failed");
-      if (res <= target) break;
+      iters = hc->iters;
+      hipLaunchKernelGGL(k_gm_update_x, dim3(RB), dim3(RT), 0, st_, (const double*)V_, n_, m, (const double*)H_,
+                         (const double*)g_, (const GmresCtl*)ctl_, x);
+      if (hc->conv) break;
     }
+    iters = hc->iters;
+    if (iters > 0) g_gmres_hint = iters;
     *iters_out = iters;
     return check_launch("gmres");
   }
 
  private:
-  int setup_inst() {
+  int setup() {
+    GmresCtl c0{};
+    c0.rtol = c_.krylov_tol;
+    c0.newton_tol = c_.newton_tol;
+    c0.damping = c_.damping;
+    c0.bt_factor = c_.backtrack_factor;
+    c0.min_dec = c_.min_decrease;
+    c0.done = 1;
     const double row[8] = {c_.precond == 1 ? c_.precond_dt * a_.D : 0.0, 0, 0, 0, 0, 0, 0, 0};
-    if (hipMemcpyAsync(inst_, row, sizeof(row), hipMemcpyHostToDevice, st_) != hipSuccess)
+    if (hipMemcpyAsync(ctl_, &c0, sizeof(c0), hipMemcpyHostToDevice, st_) != hipSuccess ||
+        hipMemcpyAsync(inst_, row, sizeof(row), hipMemcpyHostToDevice, st_) != hipSuccess)
       return fail(MJX_E_HIP, "nk: parameter upload failed");
     return hipStreamSynchronize(st_) == hipSuccess ? MJX_OK : fail(MJX_E_HIP, "nk: sync failed");
   }
 
-  void copy(const double* x, double* y) {
-    hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, x, 1.0, y, n_);
+  void residual() {
+    hipLaunchKernelGGL(k_nk_residual, dim3(RB), dim3(RT), 0, st_, (const double*)u_, u_old_, fold_, b_, a_, part_);
+    hipLaunchKernelGGL(k_nk_residual_final, dim3(1), dim3(RT), 0, st_, (const double*)part_, ctl_);
   }
 
-  void multidot(const double* V, const double* w, int k, double* out) {
-    const int chunks = (k + DOT_CHUNK - 1) / DOT_CHUNK;
-    hipLaunchKernelGGL(k_multidot_partial, dim3(DOT_BLOCKS, chunks), dim3(256), 0, st_, V, w, n_, k, partial_);
-    hipLaunchKernelGGL(k_multidot_final, dim3(k), dim3(256), 0, st_, (const double*)partial_, k, out);
-  }
-
-  int norm(const double* x, double* out) {
-    multidot(x, x, 1, hdev_ + 2 * (c_.restart + 2));
-    double v = 0;
-    if (hipMemcpyAsync(&v, hdev_ + 2 * (c_.restart + 2), sizeof(double), hipMemcpyDeviceToHost, st_) != hipSuccess ||
+  int read_ctl(GmresCtl* hc) {
+    if (hipMemcpyAsync(hc, ctl_, sizeof(GmresCtl), hipMemcpyDeviceToHost, st_) != hipSuccess ||
         hipStreamSynchronize(st_) != hipSuccess)
-      return fail(MJX_E_HIP, "norm: copy failed");
-    *out = std::sqrt(v);
+      return fail(MJX_E_HIP, "nk: control block read-back failed");
     return MJX_OK;
   }
 
-  int residual(const double* u, double* r, double* rnorm) {
-    hipLaunchKernelGGL(k_rd_residual, dim3(nblk(n_)), dim3(256), 0, st_, u, u_old_, fold_, r, a_);
-    return norm(r, rnorm);
+  // One Arnoldi step j of the current cycle (every kernel exits at once when
+  // the cycle has already finished).
+  int arnoldi_step(int j) {
+    double* Vj = V_ + (long long)j * n_;
+    const int k = j + 1;
+    if (c_.mv_dtype == MJX_F64) {
+      hipLaunchKernelGGL(k_gm_jvp<double>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, (const double*)w_,
+                         (const GmresCtl*)ctl_, Vj, t64_, a_);
+      int rc = precond_t<double>(t64_, w_, t64b_, r2r64_, lamx64_, lamy64_);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_gm_dot1<double>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const double*)w_, w_,
+                         n_, k, part_, (const GmresCtl*)ctl_);
+    } else {
+      hipLaunchKernelGGL(k_gm_jvp<float>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, (const double*)w_,
+                         (const GmresCtl*)ctl_, Vj, t32_, a_);
+      int rc = precond32(t32_, w32_);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_gm_dot1<float>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const float*)w32_, w_,
+                         n_, k, part_, (const GmresCtl*)ctl_);
+    }
+    const GmresCtl* cc = ctl_;
+    hipLaunchKernelGGL(k_gm_final_h, dim3(1), dim3(RT), 0, st_, (const double*)part_, k, cc, h1_);
+    hipLaunchKernelGGL(k_gm_upd_dot2, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)h1_,
+                       part_, cc);
+    hipLaunchKernelGGL(k_gm_final_h, dim3(1), dim3(RT), 0, st_, (const double*)part_, k, cc, h2_);
+    hipLaunchKernelGGL(k_gm_upd_norm, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)h2_,
+                       part_, cc);
+    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(RT), 0, st_, (const double*)part_, j, c_.restart,
+                       (const double*)h1_, (const double*)h2_, ctl_, H_, cs_, sn_, g_);
+    return check_launch("arnoldi");
   }
-
-  int allclose(const double* a, const double* b, bool* same);
 
   // x (FP64) -> J x (FP64), always in FP64 (used for restart residuals).
   int apply_A(const double* x, double* y) {
     hipLaunchKernelGGL(k_rd_jvp<double>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, x, y, a_);
     return check_launch("jvp");
-  }
-
-  // w = M A v in the configured operator precision.
-  int apply_op(const double* v, double* w) {
-    if (c_.mv_dtype == MJX_F64) {
-      hipLaunchKernelGGL(k_rd_jvp<double>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, v, t64_, a_);
-      return precond(t64_, w);
-    }
-    hipLaunchKernelGGL(k_to_f32, dim3(nblk(n_)), dim3(256), 0, st_, v, v32_, n_);
-    hipLaunchKernelGGL(k_rd_jvp<float>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, (const float*)v32_,
-                       t32_, a_);
-    int rc = precond32(t32_, w32_);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_to_f64, dim3(nblk(n_)), dim3(256), 0, st_, (const float*)w32_, w, n_);
-    return check_launch("apply_op");
   }
 
   int precond(const double* in, double* out) {
@@ -497,32 +850,15 @@ class NKSolver {
   const double* fold_;
   hipStream_t st_;
   long long n_;
-  double *V_, *w_, *r_, *dx_, *xt_, *t64_, *t64b_, *r2r64_, *hdev_, *partial_, *inst_;
+  GmresCtl* ctl_;
+  double *H_, *cs_, *sn_, *g_, *h1_, *h2_, *part_;
+  double *V_, *w_, *b_, *dx_, *xt_, *t64_, *t64b_, *r2r64_, *inst_;
   float *v32_, *t32_, *w32_, *t32b_, *r2r32_;
   const void *lamx64_, *lamy64_, *lamx32_, *lamy32_;
 };
 
-// flag = 1 if any |a - b| > atol + rtol |b| (jnp.allclose defaults).
-__global__ __launch_bounds__(256) void k_allclose(const double* __restrict__ a, const double* __restrict__ b,
-                                                  long long n, int* __restrict__ flag) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n && !(fabs(a[g] - b[g]) <= 1e-8 + 1e-5 * fabs(b[g]))) atomicOr(flag, 1);
-}
-
-int NKSolver::allclose(const double* a, const double* b, bool* same) {
-  int* flag = (int*)(partial_);  // scratch: the dot partials are not live here
-  int zero = 0, host = 0;
-  if (hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, st_) != hipSuccess)
-    return fail(MJX_E_HIP, "allclose: upload failed");
-  hipLaunchKernelGGL(k_allclose, dim3(nblk(n_)), dim3(256), 0, st_, a, b, n_, flag);
-  if (hipMemcpyAsync(&host, flag, sizeof(int), hipMemcpyDeviceToHost, st_) != hipSuccess ||
-      hipStreamSynchronize(st_) != hipSuccess)
-    return fail(MJX_E_HIP, "allclose: copy failed");
-  *same = host == 0;
-  return MJX_OK;
-}
-
 }  // namespace mjx
+
 
 using namespace mjx;
 
