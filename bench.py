@@ -354,12 +354,23 @@ def kernel_legs(B=512, n=N, reps=50):
     pts = B * n * n
     ring = B * 4 * (n + 1) * 4  # ghost values per field ring, FP32
     spec = FieldBCSpec(kind=BCType.PERIODIC)
+    from moljax_amd import _hip
+    red_out = torch.empty(B, dtype=torch.float64, device="cuda")
+    red_work = torch.empty(_hip.lib().mjx_reduce_workspace_elems(B), dtype=torch.float64, device="cuda")
+    fi = f[:, 1:, 1:]
+
+    def reduce_kernel():  # the C entry point alone: per-instance sums of squares of the interiors
+        _hip.check(_hip.lib().mjx_reduce2d(0, 0, B, n, n, _hip.ptr(fi), n + 2, (n + 2) ** 2, None, 0, 0, 0.0, 0.0,
+                                           _hip.ptr(red_out), 0, _hip.ptr(red_work), _hip.stream_ptr()), "reduce2d")
     legs = {
         "fd_laplacian_2d": (lambda: laplacian_2d(f, grid), 8 * pts),
         "fd_d1_central_2d_x": (lambda: d1_central_2d(f, grid, 1), 8 * pts),
         "fd_d1_upwind_2d_y": (lambda: d1_upwind_2d(f, 0.7, grid, 0), 8 * pts),
         "bc_fill_periodic": (lambda: fill_ghosts(f, grid, spec, 0.0, None), 2 * ring),
         "tree_norm2_interior": (lambda: tree_norm2({"u": f}, grid), 4 * pts),
+        "reduce2d_sumsq_entry_point": (reduce_kernel, 4 * pts),
+        # reference: a device copy of the same padded array (reads + writes it)
+        "device_copy_same_shape": (lambda: f.clone(), 8 * B * (n + 2) ** 2),
     }
     out = {}
     for name, (fn, nbytes) in legs.items():

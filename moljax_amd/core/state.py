@@ -94,7 +94,7 @@ def _reduce(op: int, fields_a, fields_b=None, atol: float = 0.0, rtol: float = 0
             if Bb != B or tb.shape != ta.shape:
                 raise ValueError("fields of a and b must have the same shape")
         if out is None:
-            out = torch.zeros(B, dtype=torch.float64, device=ta.device)
+            out = torch.empty(B, dtype=torch.float64, device=ta.device)  # written by the first field
             work = torch.empty(lib.mjx_reduce_workspace_elems(B), dtype=torch.float64, device=ta.device)
         elif out.numel() != B:
             raise ValueError("all fields must have the same batch shape")

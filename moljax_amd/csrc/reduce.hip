@@ -3,15 +3,17 @@
 // moljax/core/utils.py:186-197).
 //
 // Two launches per call: k_reduce_partial gives every instance a fixed set
-// of RED_BLOCKS workgroups; each walks the instance's points in a fixed
-// order, reduces its 256 lanes by wave64 shuffles and LDS in a fixed tree,
-// and writes one partial.  k_reduce_final folds an instance's partials with
+// of RED_BLOCKS workgroups; each walks its rows of the instance in a fixed
+// order (16-byte loads where the rows allow), reduces its 256 lanes by
+// wave64 shuffles and LDS in a fixed tree, and writes one partial.  k_reduce_final folds an instance's partials with
 // one wave, again in a fixed order, and combines the result with out[b]
 // when accumulating over fields.  The same inputs therefore give bitwise
 // the same result on every run.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
+#include <cstdint>
 
 #include "mjx_common.h"
 #include "moljax_hip.h"
@@ -50,25 +52,66 @@ __device__ __forceinline__ double term(T a, T b, double atol, double rtol) {
   } else return isfinite(x) ? 0.0 : 1.0;  // MJX_RED_NONFINITE
 }
 
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+};
+
+// One wave per row: rows blk*4 + wave, then every 4 * nblk rows (nblk
+// workgroups per instance, at most RED_BLOCKS).  A
+// row is read as an unaligned head of scalars, a body of 16-byte loads (one
+// per lane per step) and a tail of scalars; when b's row is not aligned like
+// a's, the row is read one element per lane.  The combination order depends
+// only on the shapes and on the rows' 16-byte alignment.
 template <typename T, int OP>
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const T* __restrict__ a, long long ap, long long ai,
                                                                 const T* __restrict__ b, long long bp, long long bi,
-                                                                int ny, int nx, double atol, double rtol,
+                                                                int ny, int nx, double atol, double rtol, int nblk,
                                                                 double* __restrict__ partial) {
-  const int inst = blockIdx.x / RED_BLOCKS, blk = blockIdx.x - inst * RED_BLOCKS;
+  using V = typename Vec16<T>::type;
+  constexpr int VE = 16 / sizeof(T);
+  const int inst = blockIdx.x / nblk, blk = blockIdx.x - inst * nblk;
   __shared__ double lds[RED_THREADS / 64];
   const T* A = a + inst * ai;
   const T* Bp = b ? b + inst * bi : nullptr;
-  const long long n = (long long)ny * nx;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double acc = 0.0;  // identity of + and of max over |x|
-  for (long long i = (long long)blk * RED_THREADS + threadIdx.x; i < n; i += (long long)RED_BLOCKS * RED_THREADS) {
-    const int y = (int)(i / nx), x = (int)(i - (long long)y * nx);
-    const T av = A[(long long)y * ap + x];
-    const T bv = Bp ? Bp[(long long)y * bp + x] : (T)0;
-    acc = combine<OP>(acc, term<T, OP>(av, bv, atol, rtol));
+  for (int y = blk * (RED_THREADS / 64) + wave; y < ny; y += nblk * (RED_THREADS / 64)) {
+    const T* ra = A + (long long)y * ap;
+    const T* rb = Bp ? Bp + (long long)y * bp : nullptr;
+    const uintptr_t ua = (uintptr_t)ra;
+    if (rb && ((ua ^ (uintptr_t)rb) & 15)) {
+      for (int x = lane; x < nx; x += 64) acc = combine<OP>(acc, term<T, OP>(ra[x], rb[x], atol, rtol));
+      continue;
+    }
+    int h = (int)(((16 - (ua & 15)) & 15) / sizeof(T));
+    if (h > nx) h = nx;
+    const int nb = (nx - h) / VE;
+    const int t0 = h + nb * VE;
+    if (lane < h) acc = combine<OP>(acc, term<T, OP>(ra[lane], rb ? rb[lane] : (T)0, atol, rtol));
+    for (int v = lane; v < nb; v += 64) {
+      const V av = *reinterpret_cast<const V*>(ra + h + v * VE);
+      const T* ae = reinterpret_cast<const T*>(&av);
+      if (rb) {
+        const V bv = *reinterpret_cast<const V*>(rb + h + v * VE);
+        const T* be = reinterpret_cast<const T*>(&bv);
+#pragma unroll
+        for (int e = 0; e < VE; ++e) acc = combine<OP>(acc, term<T, OP>(ae[e], be[e], atol, rtol));
+      } else {
+#pragma unroll
+        for (int e = 0; e < VE; ++e) acc = combine<OP>(acc, term<T, OP>(ae[e], (T)0, atol, rtol));
+      }
+    }
+    if (t0 + lane < nx) acc = combine<OP>(acc, term<T, OP>(ra[t0 + lane], rb ? rb[t0 + lane] : (T)0, atol, rtol));
   }
   acc = wave_reduce<OP>(acc);
-  if ((threadIdx.x & 63) == 0) lds[threadIdx.x / 64] = acc;
+  if (lane == 0) lds[wave] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = lds[0];
@@ -79,9 +122,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const T* __restr
 
 template <int OP>
 __global__ __launch_bounds__(64) void k_reduce_final(const double* __restrict__ partial, double* __restrict__ out,
-                                                     int accumulate) {
+                                                     int accumulate, int nblk) {
   const int inst = blockIdx.x;
-  double s = partial[(long long)inst * RED_BLOCKS + threadIdx.x];  // RED_BLOCKS == 64 lanes
+  // one partial per lane (RED_BLOCKS == 64); lanes past nblk hold the
+  // identity (0 for every reduction here, including the max of |x|)
+  double s = (int)threadIdx.x < nblk ? partial[(long long)inst * RED_BLOCKS + threadIdx.x] : 0.0;
   s = wave_reduce<OP>(s);
   if (threadIdx.x == 0) out[inst] = accumulate ? combine<OP>(out[inst], s) : s;
 }
@@ -91,11 +136,14 @@ static_assert(RED_BLOCKS == 64, "k_reduce_final folds one partial per lane");
 template <typename T, int OP>
 int launch(int B, int ny, int nx, const T* a, long long ap, long long ai, const T* b, long long bp, long long bi,
            double atol, double rtol, double* out, double* work, int accumulate, hipStream_t st) {
-  hipLaunchKernelGGL((k_reduce_partial<T, OP>), dim3(B * RED_BLOCKS), dim3(RED_THREADS), 0, st, a, ap, ai, b, bp,
-                     bi, ny, nx, atol, rtol, work);
+  // workgroups per instance: one per 32 rows (4 waves x 8 rows), at most
+  // RED_BLOCKS
+  const int nblk = std::min(RED_BLOCKS, std::max(1, (ny + 31) / 32));
+  hipLaunchKernelGGL((k_reduce_partial<T, OP>), dim3(B * nblk), dim3(RED_THREADS), 0, st, a, ap, ai, b, bp, bi, ny,
+                     nx, atol, rtol, nblk, work);
   int rc = check_launch("reduce_partial");
   if (rc) return rc;
-  hipLaunchKernelGGL((k_reduce_final<OP>), dim3(B), dim3(64), 0, st, work, out, accumulate);
+  hipLaunchKernelGGL((k_reduce_final<OP>), dim3(B), dim3(64), 0, st, work, out, accumulate, nblk);
   return check_launch("reduce_final");
 }
 

@@ -7,13 +7,13 @@
 // as the reference does (moljax/core/operators.py:159-322 fill zeros_like and
 // set the interior).
 //
-// Memory-bound streaming kernels: one thread per output element along x
-// (coalesced rows), the +-1 neighbours come from the same or adjacent rows
-// already in L2/L1.
+// Memory-bound streaming kernels with coalesced row access; the FD
+// operators stage a tile and its halo in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
+#include <cstdint>
 #include <string>
 
 #include "mjx_common.h"
@@ -25,7 +25,8 @@ enum FdOp { FD_LAP = 0, FD_D2 = 1, FD_D1C = 2, FD_D1UP = 3 };
 // Batched stencil launches put the instance on grid y (grid z for the FD
 // kernel), at most this many per launch; larger batches loop on the host.
 constexpr long long MAX_GRID_Y = 65535;
-constexpr int FD_RY = 8;  // padded rows per thread in the FD kernel
+constexpr int FD_TX = 64;  // padded columns per FD tile (one per lane)
+constexpr int FD_TY = 32;  // padded rows per FD tile (8 per thread row)
 
 struct FdArgs {
   long long ip, is;  // input pitch / instance stride
@@ -40,65 +41,245 @@ struct FdArgs {
 template <typename T, int OP>
 __global__ __launch_bounds__(256) void k_fd2d(const T* __restrict__ in, T* __restrict__ out,
                                               const T* __restrict__ vel, FdArgs a, long long b0) {
-  // 64 x 4 threads; each thread marches down FD_RY padded rows of one padded
-  // column, keeping the y-neighbours in registers (the Laplacian loads 3
-  // values per output instead of 5).  Instance on grid z.
+  // One 32 x 64 tile of the padded output per workgroup (64 x 4 threads).
+  // The tile and its one-cell halo are staged in LDS with coalesced row
+  // loads (each input value is read from memory once per tile, 10 % halo);
+  // every thread then forms 8 outputs of one column from LDS and writes
+  // them as coalesced row segments.  Output ghost cells are written as 0.
+  __shared__ T tile[FD_TY + 2][FD_TX + 2];
   const int W = a.nx + 2, H = a.ny + 2;
-  const int xp = blockIdx.x * 64 + threadIdx.x;
-  const int yp0 = (blockIdx.y * 4 + threadIdx.y) * FD_RY;
-  if (xp >= W || yp0 >= H) return;
+  const int x0 = blockIdx.x * FD_TX, y0 = blockIdx.y * FD_TY;  // padded origin of the tile
   const long long b = b0 + blockIdx.z;
+  const T* pin = in + b * a.is - a.ip - 1;  // padded (0, 0)
+  const int tid = threadIdx.y * FD_TX + threadIdx.x;
+  for (int idx = tid; idx < (FD_TY + 2) * (FD_TX + 2); idx += FD_TX * 4) {
+    const int ly = idx / (FD_TX + 2), lx = idx - ly * (FD_TX + 2);
+    const int yp = y0 - 1 + ly, xp = x0 - 1 + lx;
+    T v = (T)0;
+    if (yp >= 0 && yp < H && xp >= 0 && xp < W) v = pin[(long long)yp * a.ip + xp];
+    tile[ly][lx] = v;
+  }
+  __syncthreads();
+  const int lx = threadIdx.x, xp = x0 + lx;
+  if (xp >= W) return;
   const int x = xp - 1;
   const bool xin = x >= 0 && x < a.nx;
-  const int yend = yp0 + FD_RY < H ? yp0 + FD_RY : H;
-  const long long sy = a.ip;
-  const bool ymarch = OP == FD_LAP || a.axis == 0;
-  T up = (T)0, c = (T)0;
-  bool have = false;
-  for (int yp = yp0; yp < yend; ++yp) {
+  T* pout = out + b * a.os - a.op - 1;
+#pragma unroll
+  for (int r = 0; r < FD_TY / 4; ++r) {
+    const int ly = threadIdx.y * (FD_TY / 4) + r;
+    const int yp = y0 + ly;
+    if (yp >= H) break;
     const int y = yp - 1;
-    T* o = out + b * a.os + (long long)y * a.op + x;
-    if (!xin || y < 0 || y >= a.ny) {
-      *o = (T)0;
-      continue;
-    }
-    const T* f = in + b * a.is + (long long)y * a.ip + x;
-    T dn = (T)0;
-    if (ymarch) {
-      if (!have) {
-        up = f[-sy];
-        c = f[0];
-        have = true;
-      }
-      dn = f[sy];
-    } else {
-      c = f[0];
-    }
-    if constexpr (OP == FD_LAP) {
-      // d2_central_2d(axis=0) + d2_central_2d(axis=1), operators.py:311-322
-      const T yy = (dn - (T)2 * c + up) / (T)a.dy2;
-      const T xx = (f[1] - (T)2 * c + f[-1]) / (T)a.dx2;
-      *o = yy + xx;
-    } else {
-      const T lo = a.axis == 0 ? up : f[-1];
-      const T hi = a.axis == 0 ? dn : f[1];
-      if constexpr (OP == FD_D2) {
-        *o = (hi - (T)2 * c + lo) / (T)a.h2;  // operators.py:271-308
-      } else if constexpr (OP == FD_D1C) {
-        *o = (hi - lo) / ((T)2 * (T)a.h);  // operators.py:159-197
+    T res = (T)0;
+    if (xin && y >= 0 && y < a.ny) {
+      const T c = tile[ly + 1][lx + 1];
+      const T up = tile[ly][lx + 1], dn = tile[ly + 2][lx + 1];
+      const T lf = tile[ly + 1][lx], rt = tile[ly + 1][lx + 2];
+      if constexpr (OP == FD_LAP) {
+        // d2_central_2d(axis=0) + d2_central_2d(axis=1), operators.py:311-322
+        res = (dn - (T)2 * c + up) / (T)a.dy2 + (rt - (T)2 * c + lf) / (T)a.dx2;
       } else {
-        // operators.py:200-268: backward where v >= 0, forward otherwise
-        const T v = vel ? vel[b * a.vs + (long long)y * a.vp + x] : (T)a.vscalar;
-        const T bwd = (c - lo) / (T)a.h;
-        const T fwd = (hi - c) / (T)a.h;
-        *o = v >= (T)0 ? bwd : fwd;
+        const T lo = a.axis == 0 ? up : lf;
+        const T hi = a.axis == 0 ? dn : rt;
+        if constexpr (OP == FD_D2) {
+          res = (hi - (T)2 * c + lo) / (T)a.h2;  // operators.py:271-308
+        } else if constexpr (OP == FD_D1C) {
+          res = (hi - lo) / ((T)2 * (T)a.h);  // operators.py:159-197
+        } else {
+          // operators.py:200-268: backward where v >= 0, forward otherwise
+          const T v = vel ? vel[b * a.vs + (long long)y * a.vp + x] : (T)a.vscalar;
+          res = v >= (T)0 ? (c - lo) / (T)a.h : (hi - c) / (T)a.h;
+        }
       }
     }
-    if (ymarch) {
-      up = c;
-      c = dn;
+    pout[(long long)yp * a.op + xp] = res;
+  }
+}
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+template <typename T, int OP>
+__device__ __forceinline__ T fd_point(T c, T up, T dn, T lf, T rt, const T* vel, const FdArgs& a, long long b, int y,
+                                      int x) {
+  if constexpr (OP == FD_LAP) {
+    return (dn - (T)2 * c + up) / (T)a.dy2 + (rt - (T)2 * c + lf) / (T)a.dx2;  // operators.py:311-322
+  } else {
+    const T lo = a.axis == 0 ? up : lf;
+    const T hi = a.axis == 0 ? dn : rt;
+    if constexpr (OP == FD_D2) {
+      return (hi - (T)2 * c + lo) / (T)a.h2;  // operators.py:271-308
+    } else if constexpr (OP == FD_D1C) {
+      return (hi - lo) / ((T)2 * (T)a.h);  // operators.py:159-197
+    } else {
+      // operators.py:200-268: backward where v >= 0, forward otherwise
+      const T v = vel ? vel[b * a.vs + (long long)y * a.vp + x] : (T)a.vscalar;
+      return v >= (T)0 ? (c - lo) / (T)a.h : (hi - c) / (T)a.h;
     }
   }
+}
+
+// The FD tile kernel with two-element (8-byte FP32, 16-byte FP64) accesses,
+// for arrays whose padded rows start on a two-element boundary (even pitch
+// and instance stride, aligned origin): the LDS tile holds padded columns
+// [x0 - 2, x0 + 66) as 34 pairs per row; each thread forms a pair of
+// adjacent outputs in 4 rows and stores it as one pair.
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_fd2d_pair(const T* __restrict__ in, T* __restrict__ out,
+                                                   const T* __restrict__ vel, FdArgs a, long long b0) {
+  using P = typename Vec2<T>::type;
+  constexpr int NP = FD_TX / 2 + 2;  // pairs per LDS row
+  __shared__ P tile[FD_TY + 2][NP];
+  const int W = a.nx + 2, H = a.ny + 2;
+  const int x0 = blockIdx.x * FD_TX, y0 = blockIdx.y * FD_TY;
+  const long long b = b0 + blockIdx.z;
+  const T* pin = in + b * a.is - a.ip - 1;  // padded (0, 0), two-element aligned
+  const int tid = threadIdx.y * FD_TX + threadIdx.x;
+  for (int idx = tid; idx < (FD_TY + 2) * NP; idx += FD_TX * 4) {
+    const int ly = idx / NP, q = idx - ly * NP;
+    const int yp = y0 - 1 + ly, c = x0 - 2 + 2 * q;
+    P v;
+    v.x = (T)0;
+    v.y = (T)0;
+    if (yp >= 0 && yp < H) {
+      const T* row = pin + (long long)yp * a.ip;
+      if (c >= 0 && c + 1 < W) {
+        v = *reinterpret_cast<const P*>(row + c);
+      } else {
+        if (c >= 0 && c < W) v.x = row[c];
+        if (c + 1 >= 0 && c + 1 < W) v.y = row[c + 1];
+      }
+    }
+    tile[ly][q] = v;
+  }
+  __syncthreads();
+  const int cp = tid & 31, rg = tid >> 5;
+  const int xp0 = x0 + 2 * cp;
+  if (xp0 >= W) return;
+  T* pout = out + b * a.os - a.op - 1;
+  const int xa = xp0 - 1, xb = xp0;  // interior x of the two outputs
+  const bool ina = xa >= 0 && xa < a.nx, inb = xb >= 0 && xb < a.nx;
+#pragma unroll
+  for (int r = 0; r < FD_TY / 8; ++r) {
+    const int ly = rg * (FD_TY / 8) + r;
+    const int yp = y0 + ly;
+    if (yp >= H) break;
+    const int y = yp - 1;
+    const bool yin = y >= 0 && y < a.ny;
+    const P C = tile[ly + 1][cp + 1], U = tile[ly][cp + 1], D = tile[ly + 2][cp + 1];
+    const P L = tile[ly + 1][cp], R = tile[ly + 1][cp + 2];
+    P o;
+    o.x = (yin && ina) ? fd_point<T, OP>(C.x, U.x, D.x, L.y, C.y, vel, a, b, y, xa) : (T)0;
+    o.y = (yin && inb) ? fd_point<T, OP>(C.y, U.y, D.y, C.x, R.x, vel, a, b, y, xb) : (T)0;
+    T* orow = pout + (long long)yp * a.op;
+    if (xp0 + 1 < W) {
+      *reinterpret_cast<P*>(orow + xp0) = o;
+    } else {
+      orow[xp0] = o.x;
+    }
+  }
+}
+
+// The FD operators on instances whose padded rows are contiguous (pitch =
+// nx + 2, even) with 16-byte aligned origins: the instance is one flat
+// stream of (ny + 2)(nx + 2) values, each thread forms VE consecutive
+// outputs from one 16-byte load of the centres, the 16-byte-pair loads of
+// the values one row above and below, and the two end neighbours, and
+// writes them with one 16-byte store.  The rows above and below were loaded
+// by neighbouring workgroups just before, so they come from cache; HBM sees
+// each value read once and written once.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double2;
+};
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_fd2d_flat(const T* __restrict__ in, T* __restrict__ out,
+                                                   const T* __restrict__ vel, FdArgs a, long long b0, double invW) {
+  constexpr int VE = 16 / sizeof(T);  // outputs per thread
+  using V = typename Vec4<T>::type;
+  using P = typename Vec2<T>::type;
+  const int W = a.nx + 2, H = a.ny + 2;
+  const int N = H * W;
+  const int i0 = (blockIdx.x * 256 + threadIdx.x) * VE;
+  if (i0 >= N) return;
+  const long long b = b0 + blockIdx.y;
+  const T* p = in + b * a.is - a.ip - 1;  // padded (0, 0)
+  T* q = out + b * a.os - a.op - 1;
+  T c[VE], up[VE], dn[VE];
+  {
+    const V cv = *reinterpret_cast<const V*>(p + i0);
+    const T* cs = reinterpret_cast<const T*>(&cv);
+#pragma unroll
+    for (int e = 0; e < VE; ++e) c[e] = cs[e];
+  }
+  if (i0 >= W) {
+#pragma unroll
+    for (int e = 0; e < VE; e += 2) {
+      const P v = *reinterpret_cast<const P*>(p + i0 - W + e);
+      up[e] = v.x;
+      up[e + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < VE; ++e) up[e] = i0 + e >= W ? p[i0 + e - W] : (T)0;
+  }
+  if (i0 + VE - 1 + W < N) {
+#pragma unroll
+    for (int e = 0; e < VE; e += 2) {
+      const P v = *reinterpret_cast<const P*>(p + i0 + W + e);
+      dn[e] = v.x;
+      dn[e + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < VE; ++e) dn[e] = i0 + e + W < N ? p[i0 + e + W] : (T)0;
+  }
+  const T lf0 = i0 > 0 ? p[i0 - 1] : (T)0;
+  const T rtN = i0 + VE < N ? p[i0 + VE] : (T)0;
+  // row and column of the first output (exact: the quotient is corrected)
+  int row = (int)((double)i0 * invW);
+  int col = i0 - row * W;
+  if (col < 0) {
+    --row;
+    col += W;
+  } else if (col >= W) {
+    ++row;
+    col -= W;
+  }
+  T o[VE];
+#pragma unroll
+  for (int e = 0; e < VE; ++e) {
+    const int y = row - 1, x = col - 1;
+    const T lf = e == 0 ? lf0 : c[e - 1];
+    const T rt = e == VE - 1 ? rtN : c[e + 1];
+    o[e] = (y >= 0 && y < a.ny && x >= 0 && x < a.nx) ? fd_point<T, OP>(c[e], up[e], dn[e], lf, rt, vel, a, b, y, x)
+                                                       : (T)0;
+    if (++col == W) {
+      col = 0;
+      ++row;
+    }
+  }
+  V ov;
+  T* os = reinterpret_cast<T*>(&ov);
+#pragma unroll
+  for (int e = 0; e < VE; ++e) os[e] = o[e];
+  *reinterpret_cast<V*>(q + i0) = ov;
 }
 
 // --------------------------------------------------------------- BC fill
@@ -316,12 +497,55 @@ int mjx_fd_op2d(int dtype, int op, int B, int ny, int nx, const void* in, long l
   a.dx2 = dx * dx;
   a.dy2 = dy * dy;
   a.vscalar = vel_scalar;
-  if (nx > INT_MAX - 66 || ny > 65535 * 4 * FD_RY - 2) return fail(MJX_E_LIMIT, "grid too large for one launch");
+  if (nx > INT_MAX - 66 || ny > 65535 * FD_TY - 2) return fail(MJX_E_LIMIT, "grid too large for one launch");
   hipStream_t st = (hipStream_t)stream;
+  // two-element accesses when every padded row starts on a two-element
+  // boundary of input, output and velocity
+  const size_t es = dtype == MJX_F64 ? 8 : 4;
+  auto pair_ok = [&](const void* p, long long pitch, long long istride) {
+    if (!p) return true;
+    const uintptr_t origin = (uintptr_t)p - (uintptr_t)((pitch + 1) * (long long)es);
+    return pitch % 2 == 0 && (B == 1 || istride % 2 == 0) && origin % (2 * es) == 0;
+  };
+  const bool pairs = pair_ok(in, in_pitch, in_istride) && pair_ok(out, out_pitch, out_istride);
+  // the flat form: contiguous padded rows (pitch = nx + 2, even), 16-byte
+  // aligned instance origins, a whole number of 16-byte units per instance
+  const long long Wp = nx + 2, Np = (long long)(ny + 2) * Wp;
+  const long long ve = 16 / (long long)es;
+  auto flat_ok = [&](const void* ptr, long long pitch, long long istride) {
+    const uintptr_t origin = (uintptr_t)ptr - (uintptr_t)((pitch + 1) * (long long)es);
+    return pitch == Wp && Wp % 2 == 0 && Np % ve == 0 && (B == 1 || istride % ve == 0) && origin % 16 == 0;
+  };
+  const bool flat = Np < (1LL << 30) && flat_ok(in, in_pitch, in_istride) && flat_ok(out, out_pitch, out_istride);
+  if (flat) {
+    const double invW = 1.0 / (double)Wp;
+    const unsigned gx = (unsigned)((Np / ve + 255) / 256);
+#define MJX_FD_FLAT(T, OPV)                                                                           \
+  for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y)                                                    \
+  hipLaunchKernelGGL((k_fd2d_flat<T, OPV>), dim3(gx, (unsigned)std::min<long long>(MAX_GRID_Y, B - b0)), \
+                     dim3(256), 0, st, (const T*)in, (T*)out, (const T*)vel, a, b0, invW)
+    if (dtype == MJX_F32) {
+      switch (op) {
+        case FD_LAP: MJX_FD_FLAT(float, FD_LAP); break;
+        case FD_D2: MJX_FD_FLAT(float, FD_D2); break;
+        case FD_D1C: MJX_FD_FLAT(float, FD_D1C); break;
+        default: MJX_FD_FLAT(float, FD_D1UP); break;
+      }
+    } else {
+      switch (op) {
+        case FD_LAP: MJX_FD_FLAT(double, FD_LAP); break;
+        case FD_D2: MJX_FD_FLAT(double, FD_D2); break;
+        case FD_D1C: MJX_FD_FLAT(double, FD_D1C); break;
+        default: MJX_FD_FLAT(double, FD_D1UP); break;
+      }
+    }
+#undef MJX_FD_FLAT
+    return check_launch("fd_op2d");
+  }
 #define MJX_FD_LAUNCH(T, OPV)                                                                         \
   for (long long b0 = 0; b0 < B; b0 += MAX_GRID_Y)                                                    \
-  hipLaunchKernelGGL((k_fd2d<T, OPV>),                                                                \
-                     dim3((nx + 2 + 63) / 64, (ny + 2 + 4 * FD_RY - 1) / (4 * FD_RY),                 \
+  hipLaunchKernelGGL(pairs ? (k_fd2d_pair<T, OPV>) : (k_fd2d<T, OPV>),                                \
+                     dim3((nx + 2 + FD_TX - 1) / FD_TX, (ny + 2 + FD_TY - 1) / FD_TY,                   \
                           (unsigned)std::min<long long>(MAX_GRID_Y, B - b0)),                         \
                      dim3(64, 4), 0, st, (const T*)in, (T*)out, (const T*)vel, a, b0)
   if (dtype == MJX_F32) {

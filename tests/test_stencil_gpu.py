@@ -17,7 +17,7 @@ def _field(ny, nx, seed=0):
     return np.random.default_rng(seed).standard_normal((ny + 2, nx + 2))
 
 
-@pytest.mark.parametrize("ny,nx", [(16, 16), (33, 70), (256, 256)])
+@pytest.mark.parametrize("ny,nx", [(16, 16), (33, 70), (256, 256), (24, 37), (70, 129)])
 def test_fd_operators(gpu, ny, nx):
     from moljax_amd.core import Grid2D
     from moljax_amd.core import operators as op
@@ -95,3 +95,33 @@ def test_fd_upwind_velocity_broadcast(gpu):
             assert np.allclose(out[b], ref, rtol=1e-13, atol=1e-11)
     with pytest.raises(ValueError, match="broadcast"):
         op.d1_upwind_2d(torch.from_numpy(f).cuda(), torch.from_numpy(np.zeros((2, 26, 42))).cuda(), grid, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fd_strided_views_odd_width(gpu, dtype):
+    """Padded fields as views of wider rows (even pitch, odd padded width:
+    two-element accesses with a single last column) and contiguous odd
+    widths (one-element accesses) give the oracle's values, with the output
+    ghost ring zero."""
+    from moljax_amd.core import Grid2D
+    from moljax_amd.core import operators as op
+    ny, nx = 41, 63  # padded width 65
+    grid = Grid2D.uniform(nx, ny, 0.0, 1.1, 0.0, 0.9)
+    f = np.stack([_field(ny, nx, s) for s in range(3)])
+    wide = torch.zeros((3, ny + 2, nx + 3), dtype=dtype, device="cuda")
+    wide[:, :, :nx + 2] = torch.from_numpy(f).to("cuda", dtype)
+    view = wide[:, :, :nx + 2]
+    tol = dict(rtol=1e-13, atol=1e-9) if dtype == torch.float64 else dict(rtol=2e-5, atol=2e-2)
+    for t in (view, view.contiguous()):
+        out = _np(op.laplacian_2d(t, grid))
+        for b in range(3):
+            ref = ofd.laplacian_2d(f[b].astype(np.float32).astype(np.float64) if dtype == torch.float32 else f[b],
+                                   grid.dy, grid.dx)
+            assert np.allclose(out[b], ref, **tol)
+            assert np.all(out[b, 0] == 0) and np.all(out[b, -1] == 0)
+            assert np.all(out[b, :, 0] == 0) and np.all(out[b, :, -1] == 0)
+        d1 = _np(op.d1_central_2d(t, grid, 1))
+        for b in range(3):
+            ref = ofd.d1_central_2d(f[b].astype(np.float32).astype(np.float64) if dtype == torch.float32 else f[b],
+                                    grid.dy, grid.dx, 1)
+            assert np.allclose(d1[b], ref, **tol)
