@@ -485,8 +485,8 @@ int etd_store_n(int B, int n, const EtdPlanes<T>& p, const cpx<T>* W, hipStream_
 }
 
 template <typename T>
-int etd_steps(int B, int n, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
-              hipStream_t st) {
+int etd_steps_all(int B, int n, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
+                  hipStream_t st) {
   switch (n) {
     case 32: return etd_steps_sized<T, 32>(B, p, coef, ip, n_steps, st);
     case 64: return etd_steps_sized<T, 64>(B, p, coef, ip, n_steps, st);
@@ -496,6 +496,12 @@ int etd_steps(int B, int n, const EtdPlanes<T>& p, const T* coef, const InstPara
     case 1024: return etd_steps_sized<T, 1024>(B, p, coef, ip, n_steps, st);
     default: return fail(MJX_E_SIZE, "ETDRK4 grid " + std::to_string(n) + " not supported: square n in 32..1024");
   }
+}
+
+template <typename T>
+int etd_steps(int B, int n, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
+              hipStream_t st) {
+  return etd_steps_all<T>(B, n, p, coef, ip, n_steps, st);
 }
 
 }  // namespace mjx
