@@ -221,6 +221,21 @@ int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const in
                      void* const* v, const long long* pitch, const long long* istride,
                      const void* const* tables, const double* const* inst, void* const* work,
                      int steps_per_graph, void** plan);
+/* As mjx_batch_create with one scheme per group, so that Gray-Scott groups
+ * and single-field Helmholtz groups share one plan and one graph.  Schemes
+ * 0-2 as above; 3, 4, 5: B[g] fields u[g] (v[g] unused, may be NULL) each
+ * advanced by implicit diffusion steps u <- (I - beta lap_h)^-1 u, beta =
+ * dt D from column 0 of its inst row, with 3 the periodic 5-point FD symbol
+ * (FFT; moljax/core/fft_solvers.py:261-294), 4 homogeneous Dirichlet on a
+ * vertex grid (DST-I) and 5 homogeneous Neumann on a cell grid (DCT-II)
+ * (moljax/core/fft_nonperiodic.py:308-355; the batched copy
+ * moljax/core/jit_kernels.py:510-531); tables {lamx, lamy} are the 1-D
+ * symbols, work[g] holds mjx_spectral_workspace_bytes (3) or B n^2 values
+ * (4, 5).  Sizes as for mjx_r2r_solve2d / mjx_spectral_pair. */
+int mjx_batch_create_mixed(int dtype, int n_groups, const int* schemes, const int* n, const int* B,
+                           void* const* u, void* const* v, const long long* pitch, const long long* istride,
+                           const void* const* tables, const double* const* inst, void* const* work,
+                           int steps_per_graph, void** plan);
 int mjx_batch_run(void* plan, int n_steps, int fill_ghosts, void* stream);
 int mjx_batch_destroy(void* plan);
 

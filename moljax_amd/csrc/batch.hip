@@ -10,9 +10,16 @@
 //      groups' kernels are independent branches of one graph and small
 //      groups run alongside large ones instead of behind them;
 //   3. the remainder (n_steps mod K) and the unpack run as direct launches.
-// Schemes: 0 ETDRK4 (tables: coefficients), 1 IMEX-Strang (tables: lamx,
-// lamy), 2 the Gray-Scott benchmark's splitting (tables: lamx, lamy; inst
-// beta = dt D; benchmarks/benchmark_gray_scott.py:268-289).
+// Schemes (per group): 0 ETDRK4 (tables: coefficients), 1 IMEX-Strang
+// (tables: lamx, lamy), 2 the Gray-Scott benchmark's splitting (tables:
+// lamx, lamy; inst beta = dt D; benchmarks/benchmark_gray_scott.py:268-289),
+// and single-field Helmholtz groups whose step is one implicit diffusion
+// solve u <- (I - beta lap_h)^-1 u with beta = dt D from the instance row
+// (tables: lamx, lamy of the grid's 5-point Laplacian): 3 periodic (FFT),
+// 4 homogeneous Dirichlet on a vertex grid (DST-I), 5 homogeneous Neumann
+// on a cell grid (DCT-II) -- the solves of moljax/core/fft_solvers.py:261-294,
+// moljax/core/fft_nonperiodic.py:308-355 and the batched copy of
+// moljax/core/jit_kernels.py:510-531, repeated step by step.
 // Kernel tiles (lines per workgroup) come from the autotuner table, which
 // mjx_autotune fills by timing the candidate tiles on scratch data of the
 // group's shape.  Results are bitwise those of the same instances advanced
@@ -49,7 +56,18 @@ int etd_advance(int B, int n, const double* coef, const InstParams* ip, int n_st
 
 namespace {
 
-enum { SCHEME_ETDRK4 = 0, SCHEME_STRANG = 1, SCHEME_BENCH_STRANG = 2 };
+enum {
+  SCHEME_ETDRK4 = 0,
+  SCHEME_STRANG = 1,
+  SCHEME_BENCH_STRANG = 2,
+  SCHEME_HELM_PERIODIC = 3,
+  SCHEME_HELM_DIRICHLET = 4,
+  SCHEME_HELM_NEUMANN = 5
+};
+bool helm_scheme(int s) { return s >= SCHEME_HELM_PERIODIC && s <= SCHEME_HELM_NEUMANN; }
+// Strang-type chains run their first and last half-steps in pack / finish.
+bool split_ends(int s) { return s == SCHEME_STRANG || s == SCHEME_BENCH_STRANG; }
+int r2r_kind(int s) { return s == SCHEME_HELM_DIRICHLET ? 0 : 1; }
 
 // Sizes of the Stockham ETDRK4 chain; other sides run the chirp-z one.
 bool etd_stockham(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024; }
@@ -66,7 +84,23 @@ struct Group {
 
 template <typename T>
 struct Ops {
+  // One implicit diffusion solve per instance of a Helmholtz group, in place.
+  static int helm_step(const Group& g, hipStream_t st) {
+    constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
+    const double* inst = reinterpret_cast<const double*>(g.ip);
+    if (g.scheme == SCHEME_HELM_PERIODIC)
+      return mjx_spectral_pair(dt_, g.B, g.n, g.n, g.u, nullptr, g.pitch, g.istride, g.u, nullptr, g.pitch,
+                               g.istride, 0, g.t0, g.t1, nullptr, inst, 1, g.work, st);
+    return mjx_r2r_solve2d(dt_, r2r_kind(g.scheme), g.B, g.n, g.n, g.u, g.pitch, g.istride, g.u, g.pitch, g.istride,
+                           g.t0, g.t1, inst, 0, g.work, st);
+  }
   static int pack(const Group& g, hipStream_t st) {
+    if (helm_scheme(g.scheme)) {
+      // the state stays in place; any-length sides get their chirp-z tables
+      // now, before a graph capture replays the solves
+      if (g.scheme == SCHEME_HELM_PERIODIC) return is_pow2_len(g.n) ? MJX_OK : czt_prepare_periodic(g.n);
+      return czt_prepare_r2r(r2r_kind(g.scheme), g.n);
+    }
     if (g.scheme == SCHEME_ETDRK4) {
       if (!etd_stockham(g.n))
         return czt_etdrk4_pack(g.B, g.n, g.n, (const T*)g.u, (const T*)g.v, g.pitch, g.istride, (cpx<T>*)g.work, st);
@@ -89,6 +123,11 @@ struct Ops {
   // Benchmark splitting: each middle step closes the previous step's
   // reaction and opens its own (two half-steps in one row pass).
   static int advance(const Group& g, int steps, hipStream_t st) {
+    if (helm_scheme(g.scheme)) {
+      for (int s = 0; s < steps; ++s)
+        if (int rc = helm_step(g, st)) return rc;
+      return MJX_OK;
+    }
     if (g.scheme == SCHEME_ETDRK4) {
       if (!etd_stockham(g.n))
         return czt_etdrk4_advance(g.B, g.n, g.n, (const T*)g.t0, g.ip, steps, (cpx<T>*)g.work, st);
@@ -105,6 +144,7 @@ struct Ops {
     return MJX_OK;
   }
   static int finish(const Group& g, int ghosts, hipStream_t st) {
+    if (helm_scheme(g.scheme)) return MJX_OK;
     if (g.scheme == SCHEME_ETDRK4) {
       if (!etd_stockham(g.n))
         return czt_etdrk4_unpack(g.B, g.n, g.n, (T*)g.u, (T*)g.v, g.pitch, g.istride, ghosts, (cpx<T>*)g.work, st);
@@ -124,7 +164,7 @@ struct Ops {
 }  // namespace
 
 struct BatchPlan {
-  int dtype = MJX_F32, scheme = SCHEME_ETDRK4, K = 1;
+  int dtype = MJX_F32, K = 1;
   std::vector<Group> groups;
   hipStream_t cap = nullptr;
   std::vector<hipStream_t> side;
@@ -140,13 +180,6 @@ struct BatchPlan {
     if (cap) (void)hipStreamDestroy(cap);
   }
 
-  int advance_all(int steps, hipStream_t st) {
-    for (auto& g : groups) {
-      const int rc = dtype == MJX_F32 ? Ops<float>::advance(g, steps, st) : Ops<double>::advance(g, steps, st);
-      if (rc) return rc;
-    }
-    return MJX_OK;
-  }
 
   // One graph advancing every group by K steps, one branch per group.
   int capture() {
@@ -197,17 +230,22 @@ struct BatchPlan {
     // Strang: the first step's leading half-solve is in pack(), the last
     // step's row pass and trailing half-solve in finish().  Benchmark
     // splitting: pack() runs the first step but its closing half-step,
-    // which finish() applies.
-    const int middle = scheme == SCHEME_ETDRK4 ? n_steps : n_steps - 1;
-    if (middle > 0) {
-      const int reps = middle / K;
-      if (reps > 0 && !exec) {
-        rc = capture();
-        if (rc) return rc;
-      }
-      for (int r = 0; r < reps; ++r)
-        if (hipGraphLaunch(exec, st) != hipSuccess) return fail(MJX_E_HIP, "batch: hipGraphLaunch failed");
-      rc = advance_all(middle - reps * K, st);
+    // which finish() applies.  Other schemes advance all n_steps in the
+    // middle.  The graph advances every group K steps per replay; each
+    // group takes its remaining middle steps as direct launches.
+    int min_middle = n_steps;
+    for (auto& g : groups) min_middle = std::min(min_middle, split_ends(g.scheme) ? n_steps - 1 : n_steps);
+    const int reps = min_middle > 0 ? min_middle / K : 0;
+    if (reps > 0 && !exec) {
+      rc = capture();
+      if (rc) return rc;
+    }
+    for (int r = 0; r < reps; ++r)
+      if (hipGraphLaunch(exec, st) != hipSuccess) return fail(MJX_E_HIP, "batch: hipGraphLaunch failed");
+    for (auto& g : groups) {
+      const int rest = (split_ends(g.scheme) ? n_steps - 1 : n_steps) - reps * K;
+      if (rest <= 0) continue;
+      rc = dtype == MJX_F32 ? Ops<float>::advance(g, rest, st) : Ops<double>::advance(g, rest, st);
       if (rc) return rc;
     }
     for (auto& g : groups) {
@@ -320,40 +358,57 @@ using namespace mjx;
 
 extern "C" {
 
-int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const int* B, void* const* u, void* const* v,
-                     const long long* pitch, const long long* istride, const void* const* tables,
-                     const double* const* inst, void* const* work, int steps_per_graph, void** plan) {
+int mjx_batch_create_mixed(int dtype, int n_groups, const int* schemes, const int* n, const int* B, void* const* u,
+                           void* const* v, const long long* pitch, const long long* istride,
+                           const void* const* tables, const double* const* inst, void* const* work,
+                           int steps_per_graph, void** plan) {
   if (!plan) return fail(MJX_E_ARG, "null plan pointer");
   *plan = nullptr;
   if (dtype != MJX_F32 && dtype != MJX_F64) return fail(MJX_E_ARG, "dtype must be MJX_F32 or MJX_F64");
-  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG && scheme != SCHEME_BENCH_STRANG)
-    return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4), 1 (Strang) or 2 (benchmark splitting)");
-  if (n_groups < 1 || !n || !B || !u || !v || !pitch || !istride || !tables || !inst || !work)
+  if (n_groups < 1 || !schemes || !n || !B || !u || !v || !pitch || !istride || !tables || !inst || !work)
     return fail(MJX_E_ARG, "null pointer argument or no groups");
   if (steps_per_graph < 1 || steps_per_graph > 256) return fail(MJX_E_ARG, "steps_per_graph must be in [1, 256]");
+  for (int g = 0; g < n_groups; ++g)
+    if (schemes[g] < SCHEME_ETDRK4 || schemes[g] > SCHEME_HELM_NEUMANN)
+      return fail(MJX_E_ARG, "batch group " + std::to_string(g) +
+                                 ": scheme must be 0 (ETDRK4), 1 (Strang), 2 (benchmark splitting), 3 (periodic "
+                                 "Helmholtz), 4 (Dirichlet Helmholtz) or 5 (Neumann Helmholtz)");
   int rc = ensure_tables(nullptr);
   if (rc) return rc;
   std::unique_ptr<BatchPlan> p(new BatchPlan());
   p->dtype = dtype;
-  p->scheme = scheme;
   p->K = steps_per_graph;
   for (int g = 0; g < n_groups; ++g) {
-    const int ng = n[g];
+    const int ng = n[g], scheme = schemes[g];
+    const std::string gs = "batch group " + std::to_string(g);
     // power-of-two sides run the Stockham kernels, other sides in [2, 2048]
     // the chirp-z ones (tables are created by the pack, before any capture)
     const bool ok = (scheme == SCHEME_ETDRK4 ? etd_stockham(ng) : is_pow2_len(ng)) || czt_len_ok(ng);
-    if (!ok) return fail(MJX_E_SIZE, "batch group " + std::to_string(g) + ": unsupported size " + std::to_string(ng));
-    if (B[g] < 1 || !u[g] || !v[g] || !inst[g] || !work[g] || !tables[2 * g] ||
+    if (!ok) return fail(MJX_E_SIZE, gs + ": unsupported size " + std::to_string(ng));
+    const bool two_fields = !helm_scheme(scheme);
+    if (B[g] < 1 || !u[g] || (two_fields && !v[g]) || !inst[g] || !work[g] || !tables[2 * g] ||
         (scheme != SCHEME_ETDRK4 && !tables[2 * g + 1]))
-      return fail(MJX_E_ARG, "batch group " + std::to_string(g) + ": null pointer or empty");
-    if (pitch[g] < ng) return fail(MJX_E_ARG, "batch group " + std::to_string(g) + ": pitch smaller than n");
-    Group gr{scheme, ng, B[g], u[g], v[g], pitch[g], istride[g], tables[2 * g],
+      return fail(MJX_E_ARG, gs + ": null pointer or empty");
+    if (pitch[g] < ng) return fail(MJX_E_ARG, gs + ": pitch smaller than n");
+    Group gr{scheme, ng, B[g], u[g], two_fields ? v[g] : nullptr, pitch[g], istride[g], tables[2 * g],
              scheme != SCHEME_ETDRK4 ? tables[2 * g + 1] : nullptr, reinterpret_cast<const InstParams*>(inst[g]),
              work[g]};
     p->groups.push_back(gr);
   }
   *plan = p.release();
   return MJX_OK;
+}
+
+int mjx_batch_create(int dtype, int scheme, int n_groups, const int* n, const int* B, void* const* u, void* const* v,
+                     const long long* pitch, const long long* istride, const void* const* tables,
+                     const double* const* inst, void* const* work, int steps_per_graph, void** plan) {
+  if (plan) *plan = nullptr;
+  if (scheme != SCHEME_ETDRK4 && scheme != SCHEME_STRANG && scheme != SCHEME_BENCH_STRANG)
+    return fail(MJX_E_ARG, "scheme must be 0 (ETDRK4), 1 (Strang) or 2 (benchmark splitting)");
+  if (n_groups < 1 || n_groups > 4096) return fail(MJX_E_ARG, "n_groups must be in [1, 4096]");
+  std::vector<int> schemes((size_t)n_groups, scheme);
+  return mjx_batch_create_mixed(dtype, n_groups, schemes.data(), n, B, u, v, pitch, istride, tables, inst, work,
+                                steps_per_graph, plan);
 }
 
 int mjx_batch_run(void* plan, int n_steps, int fill_ghosts, void* stream) {
@@ -363,7 +418,7 @@ int mjx_batch_run(void* plan, int n_steps, int fill_ghosts, void* stream) {
   BatchPlan* p = static_cast<BatchPlan*>(plan);
   if (fill_ghosts)
     for (auto& g : p->groups)
-      if (g.pitch < g.n + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= n + 2");
+      if (!helm_scheme(g.scheme) && g.pitch < g.n + 2) return fail(MJX_E_ARG, "ghost fill needs pitch >= n + 2");
   return p->run(n_steps, fill_ghosts, (hipStream_t)stream);
 }
 

@@ -129,6 +129,22 @@ int inv_tf(int kind) { return kind == R2R_DST ? TF_DST1 : (kind == R2R_DCT ? TF_
 
 }  // namespace
 
+int czt_prepare_r2r(int kind, int n) {
+  for (int tf : {fwd_tf(kind), inv_tf(kind)}) {
+    int P, K2, N2, part;
+    tf_params(tf, n, P, K2, N2, part);
+    CztTables h;
+    const int rc = czt_tables(n, P, K2, N2, h);
+    if (rc) return rc;
+  }
+  return MJX_OK;
+}
+
+int czt_prepare_periodic(int n) {
+  CztTables h;
+  return czt_tables(n, n, 0, 0, h);
+}
+
 int czt_r2r(const CztR2RCall& c, hipStream_t st) {
   const int tf1 = c.mode == R2R_INV ? inv_tf(c.kind) : fwd_tf(c.kind);
   int P, K2, N2, part1, part2 = CZT_RE;

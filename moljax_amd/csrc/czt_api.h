@@ -30,6 +30,12 @@ struct CztTables {
 };
 int czt_tables(int n, int P, int K2, int N2, CztTables& out);
 
+// Create (once per process and device) the tables that the periodic passes,
+// or the DST/DCT transforms of one kind, need for side n, so that later
+// launches -- for example under stream capture -- only look them up.
+int czt_prepare_periodic(int n);
+int czt_prepare_r2r(int kind, int n);
+
 #define MJX_CZT_DECL(T)                                                                                       \
   int czt_row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,     \
                        cpx<T>* W, hipStream_t st, double in_scale);                                           \

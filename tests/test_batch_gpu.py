@@ -198,3 +198,103 @@ def test_c4_four_size_plan(gpu, dtype):
         assert ogs.rel_l2(mu.double().cpu().numpy(), ru) < tol
         assert ogs.rel_l2(mv.double().cpu().numpy(), rv) < tol
     mix.close()
+
+
+def _helm_oracle(kind, u, n, D, dt, steps):
+    from oracle import nonperiodic as onp
+    from oracle import spectral as osp
+    for _ in range(steps):
+        if kind == "periodic":
+            u = osp.solve_helmholtz_2d(u, osp.laplacian_symbol_2d(n, n, 1.0 / n, 1.0 / n), dt, D)
+        elif kind == "dirichlet":
+            lam = onp.laplacian_symbol_dirichlet(n, 1.0 / (n + 1))
+            u = onp.solve_helmholtz_2d("dirichlet", u, lam, lam, dt, D)
+        else:
+            lam = onp.laplacian_symbol_neumann(n, 1.0 / n)
+            u = onp.solve_helmholtz_2d("neumann", u, lam, lam, dt, D)
+    return u
+
+
+def _mixed(dtype, spg, helm_only=None):
+    """Gray-Scott (ETDRK4, Strang) and Helmholtz (periodic, Dirichlet,
+    Neumann) instances of uneven power-of-two and other sizes."""
+    from moljax_amd.batch import MixedBatch
+    mb = MixedBatch(dtype=dtype, steps_per_graph=spg)
+    specs = []
+    gs = [("etdrk4", [64, 48]), ("strang", [32])]
+    helm = [("periodic", [64, 40]), ("dirichlet", [63, 50]), ("neumann", [64, 37])]
+    rng = np.random.default_rng(5)
+    for scheme, sizes in gs:
+        F = [0.03 + 0.004 * i for i in range(len(sizes))]
+        k = [0.06 + 0.001 * i for i in range(len(sizes))]
+        ids = mb.add_gray_scott(sizes, F, k, DU, DV, DT, scheme=scheme)
+        for i, n, f, kk in zip(ids, sizes, F, k):
+            specs.append(("gs", scheme, n, dict(F=f, k=kk), _ic(n, seed=i)))
+    for kind, sizes in helm:
+        D = [10 ** rng.uniform(-3, -1) for _ in sizes]
+        dt = [10 ** rng.uniform(-3, -1) for _ in sizes]
+        ids = mb.add_helmholtz(kind, sizes, D, dt)
+        for i, n, d, t in zip(ids, sizes, D, dt):
+            specs.append(("helm", kind, n, dict(D=d, dt=t), (rng.standard_normal((n, n)),)))
+    mb.build()
+    for i, s in enumerate(specs):
+        mb.set_state(i, *[torch.from_numpy(a) for a in s[4]])
+    return mb, specs
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_mixed_kinds_one_plan(gpu, dtype):
+    """Gray-Scott and Helmholtz groups of uneven sizes in one plan and one
+    graph: each instance bitwise equal to the same instance in a plan of its
+    own; in FP64 each Helmholtz instance within 1e-12 of the oracle's
+    step-by-step solves and each Gray-Scott instance within 1e-12 of the
+    oracle."""
+    from moljax_amd.batch import MixedBatch, spectral_symbol_1d
+    steps = 5
+    mb, specs = _mixed(dtype, spg=2)
+    mb.run(steps)
+    for i, (fam, kind, n, prm, ic) in enumerate(specs):
+        print(f"mixed plan instance {i}: {fam} {kind} n={n} {dtype}", flush=True)
+        alone = MixedBatch(dtype=dtype, steps_per_graph=2)
+        if fam == "gs":
+            alone.add_gray_scott([n], [prm["F"]], [prm["k"]], DU, DV, DT, scheme=kind)
+        else:
+            alone.add_helmholtz(kind, [n], prm["D"], prm["dt"])
+        alone.build()
+        alone.set_state(0, *[torch.from_numpy(a) for a in ic])
+        alone.run(steps)
+        for a, b in zip(mb.state(i), alone.state(0)):
+            assert torch.equal(a, b), f"instance {i} ({fam} {kind}, n={n}) differs from its lone run"
+        alone.close()
+        if dtype != torch.float64:
+            continue
+        got = [t.cpu().numpy() for t in mb.state(i)]
+        if fam == "helm":
+            ref = [_helm_oracle(kind, ic[0], n, prm["D"], prm["dt"], steps)]
+        elif kind == "etdrk4":
+            lam = spectral_symbol_1d(n)
+            ref = list(oetd.gs_etdrk4(ic[0], ic[1], lam, lam, DU, DV, prm["F"], prm["k"], DT, steps))
+        else:
+            lam = spectral_symbol_1d(n)
+            P = lambda a: ogs.apply_bc_periodic(np.pad(a, 1))
+            y = {"u": P(ic[0]), "v": P(ic[1])}
+            for _ in range(steps):
+                y = ogs.imex_strang_step(y, DT, prm["F"], prm["k"], DU, DV, lam[:, None] + lam[None, :])
+            ref = [y["u"][1:-1, 1:-1], y["v"][1:-1, 1:-1]]
+        for g, r in zip(got, ref):
+            err = ogs.rel_l2(g, r)
+            assert err < 1e-12, (fam, kind, n, err)
+    mb.close()
+
+
+def test_mixed_plan_graph_length_independent(gpu):
+    """The same mixed plan run with different graph lengths gives the same
+    bits (the Strang groups' middle steps differ from the others' by one)."""
+    outs = []
+    for spg in (1, 3, 8):
+        mb, specs = _mixed(torch.float32, spg)
+        mb.run(7)
+        outs.append([t.clone() for i in range(len(specs)) for t in mb.state(i)])
+        mb.close()
+    for o in outs[1:]:
+        assert all(torch.equal(a, c) for a, c in zip(outs[0], o))
