@@ -106,6 +106,19 @@ int mjx_gs_imex_strang(int dtype, int B, int ny, int nx, void* u, void* v, long 
                        long long istride, int fill_ghosts, const void* lamx, const void* lamy,
                        const void* lam2d, const double* inst, int n_steps, void* work, void* stream);
 
+/* Spectral multiplier tables of the jitted ETD kernels, elementwise in FP64
+ * over the (ny, nx) eigenvalues lam (device; ny = 1 for one line), z = dt
+ * lam: mode 0 phi_1, phi_2, phi_3 of lam itself (out0..out2); 1 ETD1 {e^z,
+ * dt phi1}; 2 ETD2 {e^z, dt phi1 + dt phi2, -dt phi2}; 3 ETDRK4 {e^z, dt b1,
+ * dt b2 (= dt b3), dt b4} with b1 = phi1 - 3 phi2 + 4 phi3, b2 = 2 phi2 -
+ * 4 phi3, b4 = -phi2 + 4 phi3; series below |z| = 1e-4.  even != 0 writes
+ * each table's even part (m(k) + m(-k))/2 (indices reversed modulo ny, nx),
+ * the multiplier the real part of ifft(m fft(x)) applies to real x.
+ * Replaces: moljax/core/jit_kernels.py:28-53 (phi1-3) and the multiplier
+ * arithmetic of :60-180 (etd1/etd2/etdrk4 kernels). */
+int mjx_phi_tables(int ny, int nx, const double* lam, double dt, int mode, int even, double* out0, double* out1,
+                   double* out2, double* out3, void* stream);
+
 /* mjx_gs_imex_strang without any allocation: no reference counterpart; for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  The spectral workspace

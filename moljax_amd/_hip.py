@@ -69,6 +69,7 @@ PROTOTYPES = {
     "mjx_gs_bench_workspace_bytes": (_ll, [_i, _i, _i, _i]),
     "mjx_gs_rk4_spectral": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _p, _p, _p, _i, _p, _p]),
     "mjx_gs_cn_fixed_point": (_i, [_i, _i, _i, _i, _p, _p, _ll, _ll, _p, _p, _p, _p, _d, _i, _i, _p, _p, _p]),
+    "mjx_phi_tables": (_i, [_i, _i, _p, _d, _i, _i, _p, _p, _p, _p, _p]),
     "mjx_etd_phi_coeff_elems": (_ll, [_i, _i]),
     "mjx_etd_phi_coeffs": (_i, [_i, _i, _i, _p, _p, _d, _d, _d, _p, _p]),
     "mjx_gs_etd_exp": (_i, [_i, _i, _i, _i, _i, _p, _p, _ll, _ll, _i, _p, _p, _i, _p, _p, _i, _p, _p]),

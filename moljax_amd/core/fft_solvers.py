@@ -115,8 +115,9 @@ class _SymbolTables:
     def __init__(self, lam: torch.Tensor, dtype: torch.dtype):
         sep = getattr(lam, "_mjx_separable", None)
         if sep is not None:
-            self.lamx = torch.from_numpy(np.asarray(sep[0], dtype=np.float64)).to("cuda", dtype)
-            self.lamy = torch.from_numpy(np.asarray(sep[1], dtype=np.float64)).to("cuda", dtype)
+            self.lamx, self.lamy = (s.to("cuda", dtype).contiguous() if isinstance(s, torch.Tensor) else
+                                    torch.from_numpy(np.asarray(s, dtype=np.float64)).to("cuda", dtype)
+                                    for s in sep)
             self.lam2d = None
         else:
             self.lamx = self.lamy = None
@@ -200,7 +201,33 @@ _LINE_SYMBOLS: Dict[tuple, torch.Tensor] = {}
 _POW2_SIDES = (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
 
 
-def _lines_spectral(t: torch.Tensor, sym: np.ndarray, row, nrep: int) -> torch.Tensor:
+def _line_symbol_table(sym, rows: int) -> torch.Tensor:
+    """The (rows, nx) symbol of stacked lines (y symbol 0), tagged separable:
+    host symbols are cached by value, device symbols on the tensor itself."""
+    if isinstance(sym, torch.Tensor):
+        cache = getattr(sym, "_mjx_lines", None)
+        if cache is None:
+            cache = {}
+            try:
+                sym._mjx_lines = cache
+            except AttributeError:
+                pass
+        lam2d = cache.get(rows)
+        if lam2d is None:
+            lam2d = _tag_separable(sym.reshape(1, -1).expand(rows, -1).contiguous(), sym,
+                                   torch.zeros(rows, dtype=torch.float64, device=sym.device))
+            cache[rows] = lam2d
+        return lam2d
+    key = (sym.tobytes(), rows)
+    lam2d = _LINE_SYMBOLS.get(key)
+    if lam2d is None:
+        lam2d = _tag_separable(torch.from_numpy(np.broadcast_to(sym, (rows, sym.shape[-1])).copy()).to(default_device()),
+                               sym, np.zeros(rows))
+        _LINE_SYMBOLS[key] = lam2d
+    return lam2d
+
+
+def _lines_spectral(t: torch.Tensor, sym, row, nrep: int) -> torch.Tensor:
     """Apply the 1-D spectral multiplier of (sym, row, nrep) along the last
     axis of t (a batch of periodic lines): the lines are stacked as the rows
     of a 2-D array whose y symbol is zero, so the 2-D kernels apply the 1-D
@@ -217,12 +244,7 @@ def _lines_spectral(t: torch.Tensor, sym: np.ndarray, row, nrep: int) -> torch.T
         chunk = lines[i0:i0 + step]
         L = chunk.shape[0]
         rows = max(16, 1 << (L - 1).bit_length()) if pow2 else max(2, L)
-        key = (sym.tobytes(), rows)
-        lam2d = _LINE_SYMBOLS.get(key)
-        if lam2d is None:
-            lam2d = _tag_separable(torch.from_numpy(np.broadcast_to(sym, (rows, nx)).copy()).to(default_device()),
-                                   sym, np.zeros(rows))
-            _LINE_SYMBOLS[key] = lam2d
+        lam2d = _line_symbol_table(sym, rows)
         buf = torch.zeros((rows, nx), dtype=t.dtype, device=t.device)
         buf[:L] = chunk
         res, _ = spectral_pair(buf, None, _tables(lam2d, t.dtype), [row], nrep=nrep)
@@ -239,7 +261,7 @@ def solve_helmholtz_1d(rhs_interior, laplacian_symbol, dt: float, D: float) -> t
     lam = as_device(laplacian_symbol, torch.float64).reshape(-1)
     if lam.numel() != nx:
         raise ValueError(f"symbol length {lam.numel()} does not match the line length {nx}")
-    return _lines_spectral(t, lam.cpu().numpy(), [float(dt) * float(D), 0.0, 0.0, 0.0], 1)
+    return _lines_spectral(t, lam.contiguous(), [float(dt) * float(D), 0.0, 0.0, 0.0], 1)
 
 
 def solve_helmholtz(rhs_interior, laplacian_symbol, dt: float, D: float) -> torch.Tensor:
@@ -349,6 +371,10 @@ def _filter_tables(filter_kernel, dtype: torch.dtype):
     if t.is_complex():
         raise NotImplementedError("spectral filters take real kernels")
     cache = getattr(t, "_mjx_filter", None)
+    if cache is None and getattr(t, "_mjx_even", False):
+        # tables made even on the device (mjx_phi_tables): used as they are
+        d = t.detach().to(torch.float64).contiguous()
+        cache = {"sym": d, "full": d}
     if cache is None:
         sym = _symmetrised(t.detach().double().cpu().numpy())
         cache = {"sym": sym, "full": torch.from_numpy(sym).to(default_device())}
