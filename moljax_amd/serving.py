@@ -18,14 +18,16 @@ Scheduling:
   * every request advances in chunks of ``chunk`` steps counted from its own
     first step (the last chunk is the remainder), so its arithmetic does not
     depend on what else is running;
-  * requests of one (kind, size) live in the slots of a resident pool; each
-    iteration is ONE batched launch per pool in which every resident request
+  * requests of one (kind, size) live in the slots of a resident pool; a
+    pool's launch is ONE batched launch in which every resident request
     advances min(chunk, its remaining steps): the instance rows carry each
     request's parameters and step count (per-instance spectral powers for
-    the linear kinds, a masked Strang chain for Gray-Scott), and the pools'
-    launches go to different streams so they overlap on the device;
-  * requests that arrive while others run are admitted at the next
-    iteration; finished ones retire and release their footprint.
+    the linear kinds, a masked Strang chain for Gray-Scott);
+  * each pool runs on a stream of its own and is relaunched as soon as its
+    previous launch has finished (an event query, no device-wide wait), so
+    pools overlap on the device; requests that arrive while others run are
+    admitted into free slots at once, and finished ones retire and release
+    their footprint when their last launch completes.
 Because every kernel is batch-invariant, a request's result is bitwise the
 same alone and in any mix.
 """
@@ -251,7 +253,15 @@ def _rows(req: Request) -> List[float]:
 
 class Server:
     """The continuous-batching loop.  ``hbm_budget`` bounds the bytes of
-    resident request state plus working buffers (footprint = 4x state)."""
+    resident request state plus working buffers (footprint = 4x state).
+
+    Every pool runs on a stream of its own and is launched again as soon as
+    its previous launch has finished (an event query, no device-wide wait):
+    requests are admitted into a pool's free slots at any moment (the input
+    copy is ordered after the pool's launch in flight on its stream), pools
+    of different kinds and sizes overlap on the device, and a request retires
+    -- its result copied out, its slot and footprint freed -- when the launch
+    that completes its last chunk has finished."""
 
     def __init__(self, dtype: torch.dtype = torch.float32, hbm_budget: int = 16 << 30, chunk: int = 32,
                  n_streams: int = 4):
@@ -274,8 +284,29 @@ class Server:
         key = (r.kind, r.grid)
         p = self.pools.get(key)
         if p is None:
-            p = self.pools[key] = _Pool(r.kind, r.grid, self.dtype)
+            st = self.streams[len(self.pools) % len(self.streams)]
+            st.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(st):
+                p = self.pools[key] = _Pool(r.kind, r.grid, self.dtype)
+            p.stream = st
+            p.event = None      # the launch in flight, if any
+            p.flight = []       # (slot, request, steps) advanced by it
         return p
+
+    def _finish_launch(self, p: _Pool, t_done: float):
+        """The pool's launch has finished: advance its requests and retire
+        the completed ones (their results copied on the pool's stream)."""
+        with torch.cuda.stream(p.stream):
+            for i, r, n in p.flight:
+                r.done_steps += n
+                if r.done_steps >= r.steps:
+                    r.finished = t_done
+                    r.state = [f[i].clone() for f in p.fields]
+                    p.slots[i] = None
+                    self.used -= self.footprint(r)
+                    self._n_active -= 1
+        p.event = None
+        p.flight = []
 
     def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
         """Serve ``requests`` (arrival times in seconds from the start);
@@ -284,47 +315,48 @@ class Server:
             if self.footprint(r) > self.budget:
                 raise ValueError(f"request {r.rid} needs {self.footprint(r)} bytes, above the HBM budget")
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        n_active = 0
+        self._n_active = 0
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
-        while pending or n_active:
+        while pending or self._n_active:
+            progressed = False
+            # admission: FIFO by arrival while the footprint budget allows
             now = clock()
             while pending and pending[0].arrival <= now and self.used + self.footprint(pending[0]) <= self.budget:
                 r = pending.popleft()
-                self._pool(r).admit(r, initial_state(r, self.dtype))
+                p = self._pool(r)
+                # on the pool's stream: after its launch in flight (a pool that
+                # grows copies its old slots there too)
+                with torch.cuda.stream(p.stream):
+                    p.admit(r, initial_state(r, self.dtype))
                 r.admitted = time.perf_counter() - t0
                 self.used += self.footprint(r)
                 self.peak_used = max(self.peak_used, self.used)
-                n_active += 1
-            if not n_active:
-                if pending:
+                self._n_active += 1
+                progressed = True
+            self.peak_active = max(self.peak_active, self._n_active)
+            # completions, then new launches of every idle pool with requests
+            for p in self.pools.values():
+                if p.event is not None and p.event.query():
+                    self._finish_launch(p, time.perf_counter() - t0)
+                    progressed = True
+                if p.event is None and p.active():
+                    with torch.cuda.stream(p.stream):
+                        hi = max(i for i, x in enumerate(p.slots) if x is not None) + 1
+                        p.flight = [(i, p.slots[i], min(self.chunk, p.slots[i].steps - p.slots[i].done_steps))
+                                    for i in range(hi) if p.slots[i] is not None]
+                        p.launch(self.chunk, p.stream.cuda_stream)
+                        p.event = torch.cuda.Event()
+                        p.event.record(p.stream)
+                    self.iterations += 1
+                    self.max_group = max(self.max_group, len(p.flight))
+                    progressed = True
+            if not progressed:
+                if pending and not self._n_active:
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.002)))
-                continue
-            self.peak_active = max(self.peak_active, n_active)
-            cur = torch.cuda.current_stream()
-            live = [p for p in self.pools.values() if p.active()]
-            for gi, p in enumerate(live):
-                s = self.streams[gi % len(self.streams)]
-                s.wait_stream(cur)
-                with torch.cuda.stream(s):
-                    p.launch(self.chunk, s.cuda_stream)
-                self.max_group = max(self.max_group, sum(x is not None for x in p.slots))
-            for s in self.streams:
-                cur.wait_stream(s)
-            torch.cuda.synchronize()
-            self.iterations += 1
-            t_done = time.perf_counter() - t0
-            for p in live:
-                for i, r in enumerate(p.slots):
-                    if r is None:
-                        continue
-                    r.done_steps += min(self.chunk, r.steps - r.done_steps)
-                    if r.done_steps >= r.steps:
-                        r.finished = t_done
-                        r.state = [f[i].clone() for f in p.fields]
-                        p.slots[i] = None
-                        self.used -= self.footprint(r)
-                        n_active -= 1
+                else:
+                    time.sleep(2e-5)
+        torch.cuda.synchronize()
         return requests
 
 
