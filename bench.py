@@ -5,7 +5,9 @@ Workload (one "step" = one IMEX-Strang time step of every instance in the
 batch): Grid2D 256x256 on [0,2.5]^2, periodic, Du=2e-5, Dv=1e-5, F=0.035,
 k=0.065, dt=1.0; B instances with seeded ICs (seeds 0..B-1), FP32 state,
 resident in HBM before the timed region.  Also reported in the same line:
-the FP64 single-instance GPU form, the FP32-vs-FP64 relative L2 error curve,
+the FP64 single-instance GPU form, the FP64 batched form (the same batch in
+FP64, the one form that stays within 1e-5 relative L2 error of an FP64
+reference at 10000 steps), the FP32-vs-FP64 relative L2 error curve,
 per-step latency percentiles, the roofline of the dominant kernel (HIP events
 on the launch stream) and the CPU oracle timed on this host.  Secondary
 workloads in the same line: the uneven-size batched ETDRK4 plan (1024
