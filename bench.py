@@ -342,7 +342,8 @@ def kernel_legs(B=512, n=N, reps=50):
     """Per-kernel HBM rates of the row-independent operators at the headline
     shape (B padded FP32 fields of (n+2)^2): the FD Laplacian, central and
     upwind first derivatives (each reads the field once and writes the
-    interior, 8 B per interior point), the periodic ghost fill (reads the
+    interior, 8 B per interior point), the fused stencil + reaction explicit
+    stages (Euler and RK4, ten steps per call), the periodic ghost fill (reads the
     two interior rings it copies and writes the ghost ring) and the interior
     sum of squares (reads 4 B per interior point, FP64 accumulation).  Each
     leg: torch events on the launch stream around `reps` back-to-back calls."""
@@ -364,7 +365,23 @@ def kernel_legs(B=512, n=N, reps=50):
     def reduce_kernel():  # the C entry point alone: per-instance sums of squares of the interiors
         _hip.check(_hip.lib().mjx_reduce2d(0, 0, B, n, n, _hip.ptr(fi), n + 2, (n + 2) ** 2, None, 0, 0, 0.0, 0.0,
                                            _hip.ptr(red_out), 0, _hip.ptr(red_work), _hip.stream_ptr()), "reduce2d")
+    # fused FD stencil + Gray-Scott reaction stages (mjx_gs_explicit, periodic,
+    # no ghost fill), `ksteps` steps per call.  Euler reads and writes both
+    # fields once per step (16 B per point); an RK4 step moves 136 B per point
+    # (stage 0: 24, stages 1-2: 40 each, stage 3: 32).
+    g1, g2 = (torch.rand((B, n + 2, n + 2), generator=gen, device="cuda", dtype=torch.float32) for _ in range(2))
+    from moljax_amd.core._arrays import inst_table
+    xrows = inst_table([[0.0, 0.0, P["Du"], P["Dv"], P["F"], P["k"], 0.5]] * B)
+    xwork = torch.empty(_hip.lib().mjx_gs_explicit_workspace_bytes(0, B, n, n), dtype=torch.uint8, device="cuda")
+    ksteps = 10
+
+    def explicit(method):
+        _hip.check(_hip.lib().mjx_gs_explicit(0, method, B, n, n, _hip.ptr(g1[:, 1:, 1:]), _hip.ptr(g2[:, 1:, 1:]), n + 2,
+                                              (n + 2) ** 2, 0, grid.dy, grid.dx, _hip.ptr(xrows), ksteps,
+                                              _hip.ptr(xwork), _hip.stream_ptr()), "gs_explicit")
     legs = {
+        "gs_explicit_euler_x10": (lambda: explicit(0), 16 * ksteps * pts),
+        "gs_explicit_rk4_x10": (lambda: explicit(2), 136 * ksteps * pts),
         "fd_laplacian_2d": (lambda: laplacian_2d(f, grid), 8 * pts),
         "fd_d1_central_2d_x": (lambda: d1_central_2d(f, grid, 1), 8 * pts),
         "fd_d1_upwind_2d_y": (lambda: d1_upwind_2d(f, 0.7, grid, 0), 8 * pts),
