@@ -98,7 +98,7 @@ struct Ops {
     if (helm_scheme(g.scheme)) {
       // the state stays in place; any-length sides get their chirp-z tables
       // now, before a graph capture replays the solves
-      if (g.scheme == SCHEME_HELM_PERIODIC) return is_pow2_len(g.n) ? MJX_OK : czt_prepare_periodic(g.n);
+      if (g.scheme == SCHEME_HELM_PERIODIC) return spec_generic(g.n, g.n) ? czt_prepare_periodic(g.n) : MJX_OK;
       return czt_prepare_r2r(r2r_kind(g.scheme), g.n);
     }
     if (g.scheme == SCHEME_ETDRK4) {
@@ -381,9 +381,9 @@ int mjx_batch_create_mixed(int dtype, int n_groups, const int* schemes, const in
   for (int g = 0; g < n_groups; ++g) {
     const int ng = n[g], scheme = schemes[g];
     const std::string gs = "batch group " + std::to_string(g);
-    // power-of-two sides run the Stockham kernels, other sides in [2, 2048]
-    // the chirp-z ones (tables are created by the pack, before any capture)
-    const bool ok = (scheme == SCHEME_ETDRK4 ? etd_stockham(ng) : is_pow2_len(ng)) || czt_len_ok(ng);
+    // Stockham sides run the Stockham kernels, other sides in [2, 2048] the
+    // chirp-z ones (tables are created by the pack, before any capture)
+    const bool ok = (scheme == SCHEME_ETDRK4 ? etd_stockham(ng) : supported_len(ng)) || czt_len_ok(ng);
     if (!ok) return fail(MJX_E_SIZE, gs + ": unsupported size " + std::to_string(ng));
     const bool two_fields = !helm_scheme(scheme);
     if (B[g] < 1 || !u[g] || (two_fields && !v[g]) || !inst[g] || !work[g] || !tables[2 * g] ||

@@ -11,13 +11,14 @@ namespace mjx {
 constexpr int CZT_MIN = 2, CZT_MAX = 2048;
 inline bool czt_len_ok(int n) { return n >= CZT_MIN && n <= CZT_MAX; }
 
-// The periodic spectral passes run the chirp-z kernels unless both sides are
-// powers of two in [16, 4096] (the Stockham kernels).
-inline bool spec_generic(int nx, int ny) { return !(is_pow2_len(nx) && is_pow2_len(ny)); }
+// The periodic spectral passes run the chirp-z kernels unless both sides
+// have a Stockham plan: powers of two in [16, 4096] and the mixed-radix
+// lengths of MJX_FOR_MIXED.
+inline bool spec_generic(int nx, int ny) { return !(supported_len(nx) && supported_len(ny)); }
 
 // Complex columns per row of the periodic spectral work array: nx for the
-// packed power-of-two layout, 2 (nx/2 + 1) for the half-spectra of the
-// chirp-z layout.
+// packed Stockham layout, 2 (nx/2 + 1) for the half-spectra of the chirp-z
+// layout.
 inline long long spec_work_cols(int nx, int ny) { return spec_generic(nx, ny) ? 2LL * (nx / 2 + 1) : nx; }
 
 // Chirp-z tables of one plan: n points, period P, doubled offsets K2 = 2 ko

@@ -8,19 +8,19 @@ using T = float;
 int etd_run(int B, int n, T* u, T* v, long long pitch, long long istride, int ghosts, const T* coef,
             const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st) {
   const EtdPlanes<T> p = etd_planes(work, B, n);
-  int rc = Launch<T>::row_fwd_pack(n, B, n, u, v, pitch, istride, p.Ws, st, (T)(1.0 / n));
+  int rc = Launch<T, SET_POW2>::row_fwd_pack(n, B, n, u, v, pitch, istride, p.Ws, st, (T)(1.0 / n));
   if (!rc) rc = etd_steps<T>(B, n, p, coef, ip, n_steps, st);
   if (rc) return rc;
-  return Launch<T>::row_inv_unpack(n, B, n, p.Ws, u, v, pitch, istride, ghosts, st);
+  return Launch<T, SET_POW2>::row_inv_unpack(n, B, n, p.Ws, u, v, pitch, istride, ghosts, st);
 }
 
 int etd_pack(int B, int n, const T* u, const T* v, long long pitch, long long istride, cpx<T>* work, hipStream_t st) {
-  return Launch<T>::row_fwd_pack(n, B, n, u, v, pitch, istride, etd_planes(work, B, n).Ws, st, (T)(1.0 / n));
+  return Launch<T, SET_POW2>::row_fwd_pack(n, B, n, u, v, pitch, istride, etd_planes(work, B, n).Ws, st, (T)(1.0 / n));
 }
 
 int etd_unpack(int B, int n, T* u, T* v, long long pitch, long long istride, int ghosts, cpx<T>* work,
                hipStream_t st) {
-  return Launch<T>::row_inv_unpack(n, B, n, etd_planes(work, B, n).Ws, u, v, pitch, istride, ghosts, st);
+  return Launch<T, SET_POW2>::row_inv_unpack(n, B, n, etd_planes(work, B, n).Ws, u, v, pitch, istride, ghosts, st);
 }
 
 int etd_advance(int B, int n, const T* coef, const InstParams* ip, int n_steps, cpx<T>* work, hipStream_t st) {
@@ -37,14 +37,14 @@ int etd_exp_run(int B, int n, T* u, T* v, long long pitch, long long istride, in
   const EtdPlanes<T> p = etd_planes(work, B, n);
   int rc = MJX_OK;
   if (order == 2 && prev == 1) {
-    rc = Launch<T>::row_fwd_pack(n, B, n, nu, nv, pitch, istride, p.Wg, st, (T)(1.0 / n));
+    rc = Launch<T, SET_POW2>::row_fwd_pack(n, B, n, nu, nv, pitch, istride, p.Wg, st, (T)(1.0 / n));
     if (!rc) rc = etd_store_n<T>(B, n, p, p.Wg, st);
   }
-  if (!rc) rc = Launch<T>::row_fwd_pack(n, B, n, u, v, pitch, istride, p.Ws, st, (T)(1.0 / n));
+  if (!rc) rc = Launch<T, SET_POW2>::row_fwd_pack(n, B, n, u, v, pitch, istride, p.Ws, st, (T)(1.0 / n));
   cpx<T>* cur = p.Ws;
   if (!rc) rc = etd_exp_steps<T>(B, n, p, cur, coef, ip, order, prev != 0, n_steps, st);
   if (rc) return rc;
-  return Launch<T>::row_inv_unpack(n, B, n, cur, u, v, pitch, istride, ghosts, st);
+  return Launch<T, SET_POW2>::row_inv_unpack(n, B, n, cur, u, v, pitch, istride, ghosts, st);
 }
 
 int etd_phi_coeffs(int ny, int nx, const double* lamx, const double* lamy, double Du, double Dv, double dt, T* coef,

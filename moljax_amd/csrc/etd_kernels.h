@@ -53,7 +53,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_nl(const cpx<T>* _
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
   cpx<T> v[C::P];
-  load_unpack<T, NX, TR>(Win + roff, lrow, j, v);
+  load_unpack<T, NX, TR>(Win + roff, true, lrow, j, v);
   line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {

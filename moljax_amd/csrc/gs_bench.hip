@@ -121,7 +121,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_bench(BenchArgs<T>
       a0_[m] = need ? q0[x] : (T)0;
       a1_[m] = need ? q1[x] : (T)0;
     }
-    load_unpack<T, NX, TR>(wrow, lrow, j, v);
+    load_unpack<T, NX, TR>(wrow, true, lrow, j, v);
     line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
     if constexpr (OP == BOP_RK_UNPACK || OP == BOP_RK_NEXT) {
       const T w6 = (T)(a.ip[inst].dt / 6.0);

@@ -47,7 +47,7 @@ static void cpu_checks() {
                            P, nullptr) == MJX_E_SIZE, "strang size");
   CHECK(mjx_gs_imex_strang_noalloc(MJX_F32, 129, 256, 256, P, P, 258, 258LL * 258, 1, P, P, nullptr,
                                    (const double*)P, 3, nullptr) == MJX_E_LIMIT, "noalloc limit");
-  CHECK(mjx_gs_imex_strang_noalloc(MJX_F32, 2, 250, 250, P, P, 252, 252LL * 252, 1, P, P, nullptr,
+  CHECK(mjx_gs_imex_strang_noalloc(MJX_F32, 2, 252, 252, P, P, 254, 254LL * 254, 1, P, P, nullptr,
                                    (const double*)P, 3, nullptr) == MJX_E_SIZE, "noalloc size");
   CHECK(mjx_gs_imex(MJX_F32, 7, 1, 64, 64, P, P, 66, 66 * 66, 1, P, P, nullptr, (const double*)P, 1, P, nullptr) ==
             MJX_E_ARG, "imex scheme");

@@ -13,13 +13,22 @@
 //   w^(r*(b mod Ns)) with w = exp(SIGN*2*pi*i/(Ns*R)), runs a length-R DFT and
 //   writes output r to (b/Ns)*Ns*R + (b mod Ns) + r*Ns.
 // Twiddles come from a table of exp(-2*pi*i*t/MJX_TWN) evaluated once in FP64
-// on the device (sincospi), rounded once to FP32 for the FP32 table.
+// on the device (sincospi), rounded once to FP32 for the FP32 table.  The
+// mixed-radix lengths (radix-3 and radix-5 stages beside the power-of-two
+// ones) index a table of their own, exp(-2*pi*i*t/N) for t < N, stored after
+// the power-of-two table (tw_base).
+// A radix-R stage needs R | P (every thread runs P/R whole butterflies), so
+// a length's points per thread P is a multiple of each of its radices.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <type_traits>
 
 #define MJX_TWN 4096
+
+// Lengths that are not powers of two but have a Stockham plan of their own
+// (radices 2, 3, 4, 5, 8); every other length runs the chirp-z kernels.
+#define MJX_FOR_MIXED(X) X(96) X(100) X(192) X(200) X(250) X(384) X(500) X(768) X(1000)
 
 namespace mjx {
 
@@ -125,31 +134,88 @@ __device__ __forceinline__ void dft16(cpx<T>* a) {
   t = a[11]; a[11] = a[14]; a[14] = t;
 }
 
+// Radix-3: w = exp(SIGN 2 pi i / 3) = -1/2 + SIGN i sqrt(3)/2.
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft3(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2) {
+  const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+  const cpx<T> t = a1 + a2;
+  const cpx<T> m = a0 - scale(t, (T)0.5);
+  const cpx<T> d = mul_si<SIGN>(scale(a1 - a2, h));
+  a0 = a0 + t;
+  a1 = m + d;
+  a2 = m - d;
+}
+
+// Radix-5 from the symmetric and antisymmetric input pairs:
+// X1,4 = a1 +- SIGN i b1, X2,3 = a2 +- SIGN i b2 with
+// a1 = x0 + c1 (x1+x4) + c2 (x2+x3), b1 = s1 (x1-x4) + s2 (x2-x3),
+// a2 = x0 + c2 (x1+x4) + c1 (x2+x3), b2 = s2 (x1-x4) - s1 (x2-x3),
+// c_k = cos(2 pi k / 5), s_k = sin(2 pi k / 5).
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft5(cpx<T>* a) {
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+  const cpx<T> t1 = a[1] + a[4], t2 = a[2] + a[3], t3 = a[1] - a[4], t4 = a[2] - a[3];
+  const cpx<T> a1 = a[0] + scale(t1, c1) + scale(t2, c2);
+  const cpx<T> a2 = a[0] + scale(t1, c2) + scale(t2, c1);
+  const cpx<T> b1 = mul_si<SIGN>(scale(t3, s1) + scale(t4, s2));
+  const cpx<T> b2 = mul_si<SIGN>(scale(t3, s2) - scale(t4, s1));
+  a[0] = a[0] + t1 + t2;
+  a[1] = a1 + b1;
+  a[4] = a1 - b1;
+  a[2] = a2 + b2;
+  a[3] = a2 - b2;
+}
+
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dft(cpx<T>* a) {
   if constexpr (R == 2) {
     dft2<SIGN>(a[0], a[1]);
+  } else if constexpr (R == 3) {
+    dft3<SIGN>(a[0], a[1], a[2]);
+  } else if constexpr (R == 5) {
+    dft5<SIGN>(a);
   } else if constexpr (R == 4) {
     dft4<SIGN>(a[0], a[1], a[2], a[3]);
   } else if constexpr (R == 8) {
     dft8<SIGN>(a);
   } else {
-    static_assert(R == 16, "radix must be 2, 4, 8 or 16");
+    static_assert(R == 16, "radix must be 2, 3, 4, 5, 8 or 16");
     dft16<SIGN>(a);
   }
 }
 
 // ------------------------------------------------------------------- plans
-// Points per thread and radix sequence for each supported power-of-two length.
+__host__ __device__ constexpr bool pow2_len(int n) {
+  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 ||
+         n == 4096;
+}
+__host__ __device__ constexpr bool mixed_len(int n) {
+  return n == 96 || n == 100 || n == 192 || n == 200 || n == 250 || n == 384 || n == 500 || n == 768 ||
+         n == 1000;
+}
+__host__ __device__ constexpr bool supported_len(int n) { return pow2_len(n) || mixed_len(n); }
+
+// Points per thread and radix sequence for each supported length.
+//   power of two: 16: 4x4  32: 8x4  64: 8x8  128: 16x8  256: 16x16
+//                 512: 8x8x8  1024: 16x16x4  2048: 16x16x8  4096: 16x16x16
+//   mixed (P):    96: 4x4x3x2 (12)  100: 4x5x5 (20)  192: 4x4x4x3 (12)
+//                 200: 4x5x5x2 (20)  250: 5x5x5x2 (10)  384: 4x4x4x3x2 (12)
+//                 500: 4x5x5x5 (20)  768: 4x4x4x4x3 (12)  1000: 4x5x5x5x2 (20)
 __host__ __device__ constexpr int plan_p(int n) {
-  return n == 16 ? 4 : n == 32 ? 8 : n == 64 ? 8 : n == 512 ? 8 : 16;
+  return n == 16 ? 4 : n == 32 ? 8 : n == 64 ? 8 : n == 512 ? 8
+       : (n == 96 || n == 192 || n == 384 || n == 768) ? 12
+       : (n == 100 || n == 200 || n == 500 || n == 1000) ? 20
+       : n == 250 ? 10
+       : 16;
 }
 __host__ __device__ constexpr int plan_nstages(int n) {
-  return n <= 256 ? 2 : 3;
+  return n == 100 ? 3
+       : (n == 96 || n == 192 || n == 200 || n == 250 || n == 500) ? 4
+       : (n == 384 || n == 768 || n == 1000) ? 5
+       : n <= 256 ? 2 : 3;
 }
 __host__ __device__ constexpr int plan_radix(int n, int s) {
-  // 16: 4x4  32: 8x4  64: 8x8  128: 16x8  256: 16x16
-  // 512: 8x8x8  1024: 16x16x4  2048: 16x16x8  4096: 16x16x16
   return n == 16 ? 4
        : n == 32 ? (s == 0 ? 8 : 4)
        : n == 64 ? 8
@@ -158,15 +224,46 @@ __host__ __device__ constexpr int plan_radix(int n, int s) {
        : n == 512 ? 8
        : n == 1024 ? (s < 2 ? 16 : 4)
        : n == 2048 ? (s < 2 ? 16 : 8)
-       : 16;
+       : n == 4096 ? 16
+       : n == 96 ? (s < 2 ? 4 : s == 2 ? 3 : 2)
+       : (n == 100 || n == 500) ? (s == 0 ? 4 : 5)
+       : n == 192 ? (s < 3 ? 4 : 3)
+       : n == 200 ? (s == 0 ? 4 : s < 3 ? 5 : 2)
+       : n == 250 ? (s < 3 ? 5 : 2)
+       : n == 384 ? (s < 3 ? 4 : s == 3 ? 3 : 2)
+       : n == 768 ? (s < 4 ? 4 : 3)
+       : (s == 0 ? 4 : s < 4 ? 5 : 2);  // 1000
 }
 __host__ __device__ constexpr int plan_ns_before(int n, int s) {
   return s == 0 ? 1 : plan_ns_before(n, s - 1) * plan_radix(n, s - 1);
 }
-__host__ __device__ constexpr bool supported_len(int n) {
-  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048 ||
-         n == 4096;
+
+// Twiddle table of a length: the shared power-of-two table (MJX_TWN entries)
+// or the length's own N entries, stored after it in the order of
+// MJX_FOR_MIXED.
+__host__ __device__ constexpr int tw_len(int n) { return mixed_len(n) ? n : MJX_TWN; }
+__host__ __device__ constexpr int tw_base(int n) {
+  return !mixed_len(n) ? 0
+       : MJX_TWN + (n > 96 ? 96 : 0) + (n > 100 ? 100 : 0) + (n > 192 ? 192 : 0) + (n > 200 ? 200 : 0) +
+             (n > 250 ? 250 : 0) + (n > 384 ? 384 : 0) + (n > 500 ? 500 : 0) + (n > 768 ? 768 : 0);
 }
+constexpr int MJX_TW_TOTAL = MJX_TWN + 96 + 100 + 192 + 200 + 250 + 384 + 500 + 768 + 1000;
+
+// Compile-time checks of every plan: the radices multiply to N and divide P.
+__host__ __device__ constexpr bool plan_ok(int n) {
+  int prod = 1;
+  for (int s = 0; s < plan_nstages(n); ++s) {
+    if (plan_p(n) % plan_radix(n, s) != 0) return false;
+    prod *= plan_radix(n, s);
+  }
+  return prod == n && n % plan_p(n) == 0;
+}
+#define MJX_PLAN_CHECK(N) static_assert(plan_ok(N), "bad FFT plan for " #N);
+MJX_FOR_MIXED(MJX_PLAN_CHECK)
+MJX_PLAN_CHECK(16) MJX_PLAN_CHECK(32) MJX_PLAN_CHECK(64) MJX_PLAN_CHECK(128) MJX_PLAN_CHECK(256)
+MJX_PLAN_CHECK(512) MJX_PLAN_CHECK(1024) MJX_PLAN_CHECK(2048) MJX_PLAN_CHECK(4096)
+#undef MJX_PLAN_CHECK
+static_assert(tw_base(1000) + 1000 == MJX_TW_TOTAL, "mixed twiddle tables overlap");
 
 template <int I, int E>
 struct SFor {
@@ -199,13 +296,20 @@ __host__ __device__ constexpr int tw_offset(int n, int s) {
 }
 __host__ __device__ constexpr int tw_count(int n) { return tw_offset(n, plan_nstages(n)); }
 
+// The mixed-radix plans whose twiddles would take more than 128 bytes of
+// registers per thread (32 FP32 or 16 FP64 values) read them from the table
+// (L1-resident) at each use instead.
 template <typename T, int N>
 struct TwCache {
-  cpx<T> w[tw_count(N)];
+  static constexpr bool CACHED = pow2_len(N) || tw_count(N) * (int)sizeof(T) <= 128;
+  cpx<T> w[CACHED ? tw_count(N) : 1];
+  const cpx<T>* tw;
 };
 
 template <typename T, int N>
 __device__ __forceinline__ void load_twiddles(TwCache<T, N>& c, int j, const cpx<T>* __restrict__ tw) {
+  c.tw = tw;
+  if constexpr (!TwCache<T, N>::CACHED) return;
   constexpr int P = plan_p(N);
   constexpr int TN = N / P;
   constexpr int NS = plan_nstages(N);
@@ -214,11 +318,12 @@ __device__ __forceinline__ void load_twiddles(TwCache<T, N>& c, int j, const cpx
     constexpr int R = plan_radix(N, s);
     constexpr int Ns = plan_ns_before(N, s);
     constexpr int NB = P / R;
-    constexpr int TSTEP = MJX_TWN / (Ns * R);
+    constexpr int TSTEP = tw_len(N) / (Ns * R);
+    static_assert(TSTEP * Ns * R == tw_len(N), "twiddle table step");
     constexpr int off = tw_offset(N, s);
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-      const int k = (j + TN * q) & (Ns - 1);
+      const int k = pow2_len(N) ? ((j + TN * q) & (Ns - 1)) : (int)((unsigned)(j + TN * q) % (unsigned)Ns);
 #pragma unroll
       for (int r = 1; r < R; ++r) c.w[off + q * (R - 1) + (r - 1)] = tw[r * k * TSTEP];
     }
@@ -268,7 +373,14 @@ __device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* 
         constexpr int off = tw_offset(N, s);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const cpx<T> w = twc.w[off + q * (R - 1) + (r - 1)];
+          cpx<T> w;
+          if constexpr (TwCache<T, N>::CACHED) {
+            w = twc.w[off + q * (R - 1) + (r - 1)];
+          } else {
+            constexpr int TSTEP = tw_len(N) / (Ns * R);
+            const int k = (int)((unsigned)(j + TN * q) % (unsigned)Ns);
+            w = twc.tw[r * k * TSTEP];
+          }
           t[r] = cmul(t[r], SIGN < 0 ? w : conjg(w));
         }
       }
@@ -280,8 +392,14 @@ __device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* 
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
         const int b = j + TN * q;
-        const int k = b & (Ns - 1);
-        const int idxD = (b / Ns) * Ns * R + k;
+        int k, idxD;
+        if constexpr (pow2_len(N)) {
+          k = b & (Ns - 1);
+          idxD = (b / Ns) * Ns * R + k;
+        } else {
+          k = (int)((unsigned)b % (unsigned)Ns);
+          idxD = (int)((unsigned)b / (unsigned)Ns) * Ns * R + k;
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) lds[lds_addr<ES, PAD>(idxD + r * Ns)] = v[q + r * NB];
       }

@@ -10,15 +10,24 @@
 
 namespace mjx {
 
-// exp(-2*pi*i*t/MJX_TWN), evaluated in FP64 (sincospi) and rounded once to FP32.
+// Entry i of the tables: exp(-2*pi*i*t/n) for the power-of-two table
+// (n = MJX_TWN, t = i) or the mixed-radix length whose table holds i
+// (t = i - tw_base(n)), evaluated in FP64 (sincospi) and rounded once to FP32.
 __global__ void k_init_twiddles(cpx<double>* tw64, cpx<float>* tw32) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < MJX_TWN) {
-    double s, c;
-    sincospi(2.0 * (double)t / (double)MJX_TWN, &s, &c);
-    tw64[t] = mk(c, -s);
-    tw32[t] = mk((float)c, (float)-s);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MJX_TW_TOTAL) return;
+  int n = MJX_TWN, t = i;
+#define X(N)                                  \
+  if (i >= tw_base(N) && i < tw_base(N) + N) { \
+    n = N;                                    \
+    t = i - tw_base(N);                       \
   }
+  MJX_FOR_MIXED(X)
+#undef X
+  double s, c;
+  sincospi(2.0 * (double)t / (double)n, &s, &c);
+  tw64[i] = mk(c, -s);
+  tw32[i] = mk((float)c, (float)-s);
 }
 
 static thread_local std::string g_err;
@@ -56,12 +65,13 @@ int ensure_tables(hipStream_t) {
   if (g_tw64[dev]) return MJX_OK;
   cpx<double>* t64 = nullptr;
   cpx<float>* t32 = nullptr;
-  if (hipMalloc((void**)&t64, sizeof(cpx<double>) * MJX_TWN) != hipSuccess) return fail(MJX_E_HIP, "hipMalloc twiddles");
-  if (hipMalloc((void**)&t32, sizeof(cpx<float>) * MJX_TWN) != hipSuccess) {
+  if (hipMalloc((void**)&t64, sizeof(cpx<double>) * MJX_TW_TOTAL) != hipSuccess)
+    return fail(MJX_E_HIP, "hipMalloc twiddles");
+  if (hipMalloc((void**)&t32, sizeof(cpx<float>) * MJX_TW_TOTAL) != hipSuccess) {
     (void)hipFree(t64);
     return fail(MJX_E_HIP, "hipMalloc twiddles");
   }
-  hipLaunchKernelGGL(k_init_twiddles, dim3((MJX_TWN + 255) / 256), dim3(256), 0, 0, t64, t32);
+  hipLaunchKernelGGL(k_init_twiddles, dim3((MJX_TW_TOTAL + 255) / 256), dim3(256), 0, 0, t64, t32);
   int rc = check_launch("init_twiddles");
   if (rc) return rc;
   const hipError_t e = hipDeviceSynchronize();

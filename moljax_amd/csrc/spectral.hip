@@ -69,8 +69,8 @@ static int check_grid(int dtype, int B, int ny, int nx) {
   if (B < 1) return fail(MJX_E_ARG, "batch must be >= 1");
   if (spec_generic(nx, ny) && (!czt_len_ok(nx) || !czt_len_ok(ny)))
     return fail(MJX_E_SIZE, "grid " + std::to_string(ny) + "x" + std::to_string(nx) +
-                                " not supported: nx and ny must be in [2, 2048], or both powers of two in "
-                                "[16, 4096]");
+                                " not supported: nx and ny must be in [2, 2048], or both Stockham lengths "
+                                "(powers of two in [16, 4096], 96, 100, 192, 200, 250, 384, 500, 768, 1000)");
   if ((long long)B * ny * (nx / 16) > 0x7fffffffLL) return fail(MJX_E_LIMIT, "batch too large for one launch");
   return MJX_OK;
 }
@@ -225,7 +225,9 @@ int mjx_gs_imex_strang_noalloc(int dtype, int B, int ny, int nx, void* u, void* 
   int rc = check_grid(dtype, B, ny, nx);
   if (rc) return rc;
   if (spec_generic(nx, ny))
-    return fail(MJX_E_SIZE, "mjx_gs_imex_strang_noalloc serves power-of-two sides in [16, 4096] only");
+    return fail(MJX_E_SIZE,
+                "mjx_gs_imex_strang_noalloc serves Stockham sides only (powers of two in [16, 4096] and the "
+                "mixed-radix lengths 96, 100, 192, 200, 250, 384, 500, 768, 1000)");
   if (!u || !v || !inst) return fail(MJX_E_ARG, "null pointer argument");
   if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
   if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");

@@ -155,14 +155,17 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(
   const int tile = w - inst * rows_per_inst_tiles;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
   const int y = tile * C::TR + rl;
+  // the last tile of an instance may hold rows past ny (ny not a multiple of
+  // TR): they run the transforms on zeros, for the barriers, and store nothing
+  const bool live = y < ny;
   cpx<T>* lrow = lds + rl * C::LP;
-  const T* ra = a + inst * istride + (long long)y * pitch;
-  const T* rb = b ? b + inst * istride + (long long)y * pitch : nullptr;
+  const T* ra = a + inst * istride + (long long)(live ? y : 0) * pitch;
+  const T* rb = b ? b + inst * istride + (long long)(live ? y : 0) * pitch : nullptr;
   cpx<T> v[C::P];
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
     const int x = j + C::TN * m;
-    v[m] = mk(ra[x] * in_scale, rb ? rb[x] * in_scale : (T)0);
+    v[m] = live ? mk(ra[x] * in_scale, rb ? rb[x] * in_scale : (T)0) : mk((T)0, (T)0);
   }
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
@@ -170,6 +173,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(
 #pragma unroll
   for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
   __syncthreads();
+  if (!live) return;
   cpx<T>* wrow = W + ((long long)inst * ny + y) * NX;
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
@@ -180,12 +184,12 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_fwd_pack(
 
 // Load one packed row into LDS and return the unpacked Z(e) at e = j + TN*m.
 template <typename T, int NX, int TR>
-__device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, cpx<T>* lrow, int j,
+__device__ __forceinline__ void load_unpack(const cpx<T>* __restrict__ wrow, bool live, cpx<T>* lrow, int j,
                                             cpx<T> (&v)[RowCfg<NX, TR>::P]) {
   using C = RowCfg<NX, TR>;
   cpx<T> g[C::P];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) g[m] = wrow[j + C::TN * m];
+  for (int m = 0; m < C::P; ++m) g[m] = live ? wrow[j + C::TN * m] : mk((T)0, (T)0);
 #pragma unroll
   for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, g[m]);
   __syncthreads();
@@ -212,12 +216,14 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
   const int tile = w - inst * rows_per_inst_tiles;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
   const int y = tile * C::TR + rl;
+  const bool live = y < ny;  // rows past ny: transforms only
   cpx<T>* lrow = lds + rl * C::LP;
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
   cpx<T> v[C::P];
-  load_unpack<T, NX, TR>(W + ((long long)inst * ny + y) * NX, lrow, j, v);
+  load_unpack<T, NX, TR>(W + ((long long)inst * ny + (live ? y : 0)) * NX, live, lrow, j, v);
   line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+  if (!live) return;
   if (react) {
     const T F = (T)react[inst].F, kk = (T)react[inst].k, hdt = (T)(0.5 * react[inst].dt);
 #pragma unroll
@@ -298,14 +304,15 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
   if (mstep >= 0 && mstep >= (int)ip[inst].aux) return;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
   const int y = tile * C::TR + rl;
+  const bool live = y < ny;  // rows past ny: transforms only
   cpx<T>* lrow = lds + rl * C::LP;
-  cpx<T>* wrow = W + ((long long)inst * ny + y) * NX;
+  cpx<T>* wrow = W + ((long long)inst * ny + (live ? y : 0)) * NX;
   const T F = (T)ip[inst].F, kk = (T)ip[inst].k, dt = (T)ip[inst].dt;
   const T hdt = (T)(0.5 * ip[inst].dt);
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
   cpx<T> v[C::P];
-  load_unpack<T, NX, TR>(wrow, lrow, j, v);
+  load_unpack<T, NX, TR>(wrow, live, lrow, j, v);
   line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
@@ -334,6 +341,7 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
 #pragma unroll
   for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
   __syncthreads();
+  if (!live) return;
 #pragma unroll
   for (int m = 0; m < C::P; ++m) {
     const int p = j + C::TN * m;
@@ -383,16 +391,23 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   const int p0 = tile * TC;
   const int p = p0 + c;
   const int half = nx >> 1;
-  cpx<T>* col = W + (long long)inst * NY * nx + p;
+  // the last tile may hold columns past nx (nx/2 not a multiple of TC): they
+  // run the transforms on zeros, for the barriers, and store nothing
+  const bool live = p < nx;
+  cpx<T>* col = W + (long long)inst * NY * nx + (live ? p : 0);
   cpx<T> v[C::P];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) v[m] = col[(long long)(j + C::TN * m) * nx];
-  // Every column of a tile belongs to one field (half is a multiple of TC).
+  for (int m = 0; m < C::P; ++m) v[m] = live ? col[(long long)(j + C::TN * m) * nx] : mk((T)0, (T)0);
   // The exponential multiplier e^{beta (lx + ly)} = e^{beta lx} e^{beta ly}
-  // stages e^{beta ly} in LDS (NY exponentials per workgroup, not per thread).
+  // stages e^{beta ly} in LDS (NY exponentials per workgroup, not per thread)
+  // for the field of the tile's first column; when nx/2 is a multiple of TC
+  // every column of a tile belongs to that field, otherwise the columns of
+  // the other field in the one tile that straddles nx/2 take e^{beta ly}
+  // directly.
   const bool expmode = NREP == 0 && !LAM2D && nrep == -2;
+  const bool f0 = p0 >= half;
   if constexpr (!LAM2D) {
-    const T beta_w = (T)(p0 >= half ? ip[inst].beta_b : ip[inst].beta_a);
+    const T beta_w = (T)(f0 ? ip[inst].beta_b : ip[inst].beta_a);
     for (int i = threadIdx.x; i < NY; i += C::BT) lds_lamy[i] = expmode ? exp(beta_w * lamy[i]) : lamy[i];
   }
   TwCache<T, NY> twc;
@@ -400,15 +415,18 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
   line_fft<T, NY, -1, TC, 0>(v, j, lds + c, twc);  // its barriers also publish lds_lamy
 
   const bool fb = p >= half;
-  const int kx = fb ? p - half : p;
+  const int kx = !live ? 0 : fb ? p - half : p;
   const T alpha = (T)(fb ? ip[inst].alpha_b : ip[inst].alpha_a);
   const T beta = (T)(fb ? ip[inst].beta_b : ip[inst].beta_a);
+  const bool staged = fb == f0;
+  auto ey = [&](int ky) -> T { return staged ? lds_lamy[ky] : exp(beta * lamy[ky]); };
   auto multiply_plain = [&]() {
+    if (!live) return;
     const T lx = LAM2D ? (T)0 : lamx[kx];
     if (expmode) {
       const T ex = exp(beta * lx) * s;
 #pragma unroll
-      for (int m = 0; m < C::P; ++m) v[m] = scale(v[m], ex * lds_lamy[j + C::TN * m]);
+      for (int m = 0; m < C::P; ++m) v[m] = scale(v[m], ex * ey(j + C::TN * m));
       return;
     }
 #pragma unroll
@@ -420,18 +438,18 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
       v[m] = scale(v[m], spec_mult<T, NREP>(lam, alpha, beta, nrep, s));
     }
   };
-  if (p0 == 0 || p0 == half) {
+  if (p0 == 0 || (p0 <= half && half < p0 + TC)) {
     // This tile holds packed column 0 or nx/2: two real sequences with the
     // multipliers of kx = 0 (ma) and kx = nx/2 (mb),
     // out(ky) = (ma+mb)/2 Z(ky) + (ma-mb)/2 conj(Z(-ky)).
 #pragma unroll
     for (int m = 0; m < C::P; ++m) lds[(j + C::TN * m) * TC + c] = v[m];
     __syncthreads();
-    if (c == 0) {
+    if (p == 0 || p == half) {
 #pragma unroll
       for (int m = 0; m < C::P; ++m) {
         const int ky = j + C::TN * m;
-        const int kym = (NY - ky) & (NY - 1);
+        const int kym = ky == 0 ? 0 : NY - ky;
         const cpx<T> zm = conjg(lds[kym * TC + c]);
         T la, lb, ma, mb;
         if constexpr (LAM2D) {
@@ -442,8 +460,8 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
           lb = lamx[half] + lds_lamy[ky];
         }
         if (expmode) {
-          ma = exp(beta * lamx[0]) * s * lds_lamy[ky];
-          mb = exp(beta * lamx[half]) * s * lds_lamy[ky];
+          ma = exp(beta * lamx[0]) * s * ey(ky);
+          mb = exp(beta * lamx[half]) * s * ey(ky);
         } else {
           ma = spec_mult<T, NREP>(la, alpha, beta, nrep, s);
           mb = spec_mult<T, NREP>(lb, alpha, beta, nrep, s);
@@ -459,6 +477,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
     multiply_plain();
   }
   line_fft<T, NY, +1, TC, 0>(v, j, lds + c, twc);
+  if (!live) return;
 #pragma unroll
   for (int m = 0; m < C::P; ++m) col[(long long)(j + C::TN * m) * nx] = v[m];
 }
@@ -466,15 +485,23 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_apply(
 // ------------------------------------------------------------- dispatch
 // Lines per workgroup: about 256 threads in FP32 and 128 in FP64 (LDS), at
 // least one full line, and never more lines than the grid has (rows: ny,
-// column tile: nx/2 so that the packed special columns 0 and nx/2 fall at
-// tile starts).
+// column tile: nx/2, so that with power-of-two sides the packed special
+// columns 0 and nx/2 fall at tile starts).  Mixed-radix sides may leave a
+// partial last tile; the kernels skip its missing lines.
 template <typename T, int N>
 constexpr int lines_default() {
   constexpr int bt = sizeof(T) == 8 ? 128 : 256;
   return (N / plan_p(N)) >= bt ? 1 : bt / (N / plan_p(N));
 }
 
-inline int pick_lines(int def, int limit) { return def < limit ? def : limit; }
+// The largest power of two no larger than def or limit (with_lines
+// instantiates powers of two).
+inline int pick_lines(int def, int limit) {
+  const int l = def < limit ? def : limit;
+  int p = 1;
+  while (2 * p <= l) p *= 2;
+  return p;
+}
 
 // Small batches: halve the lines per workgroup until a launch has about 512
 // workgroups (two per CU), so that one instance still spreads over the chip.
@@ -497,26 +524,26 @@ template <typename T, int N, int L>
 struct RowLaunch {
   static int fwd_pack(int B, int ny, const T* a, const T* b, long long pitch, long long istride, cpx<T>* W,
                       T in_scale, hipStream_t st) {
-    const int tiles = ny / L;
+    const int tiles = (ny + L - 1) / L;
     hipLaunchKernelGGL((k_row_fwd_pack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, a, b, pitch,
-                       istride, W, ny, tiles, in_scale, twiddles<T>());
+                       istride, W, ny, tiles, in_scale, twiddles<T>() + tw_base(N));
     return check_launch("row_fwd_pack");
   }
   static int inv_unpack(int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch, long long istride,
                         int ghosts, T keep, const InstParams* react, hipStream_t st) {
-    const int tiles = ny / L;
+    const int tiles = (ny + L - 1) / L;
     hipLaunchKernelGGL((k_row_inv_unpack<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, a, b,
-                       pitch, istride, ny, tiles, ghosts, keep, twiddles<T>(), react);
+                       pitch, istride, ny, tiles, ghosts, keep, twiddles<T>() + tw_base(N), react);
     return check_launch("row_inv_unpack");
   }
   static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, int mstep, hipStream_t st) {
-    const int tiles = ny / L;
+    const int tiles = (ny + L - 1) / L;
     if (rxn != RXN_HEUN)
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
-                         ny, tiles, twiddles<T>(), mstep, rxn - RXN_EULER);
+                         ny, tiles, twiddles<T>() + tw_base(N), mstep, rxn - RXN_EULER);
     else
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
-                         ny, tiles, twiddles<T>(), mstep, 0);
+                         ny, tiles, twiddles<T>() + tw_base(N), mstep, 0);
     return check_launch("row_gs");
   }
 };
@@ -526,9 +553,9 @@ struct ColLaunch {
   template <bool LAM2D, int NREP>
   static void go(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
                  int nrep, T s, int mstep, hipStream_t st) {
-    const int tiles = nx / L;
+    const int tiles = (nx + L - 1) / L;
     hipLaunchKernelGGL((k_col_apply<T, N, L, LAM2D, NREP>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W,
-                       nx, lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>(), mstep);
+                       nx, lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>() + tw_base(N), mstep);
   }
   static int apply(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
                    int nrep, T s, int mstep, hipStream_t st) {
@@ -548,24 +575,51 @@ struct ColLaunch {
   }
 };
 
+// Line counts instantiated for a length: every power of two up to the
+// default for the power-of-two lengths (the autotuner picks among them); for
+// the mixed-radix lengths only the default rounded down to a power of two and
+// its quarters, sixteenths, ... (enough for small batches, at a third of the
+// code).
+__host__ __device__ constexpr int pow2_floor(int d) {
+  int p = 1;
+  while (2 * p <= d) p *= 2;
+  return p;
+}
+template <typename T, int N>
+constexpr bool lines_built(int l) {
+  constexpr int D = lines_default<T, N>();
+  if (l > D) return false;
+  if (!mixed_len(N) || l == 1) return true;
+  int q = pow2_floor(D);
+  while (q > l) q /= 4;
+  return q == l;
+}
+
 // Calls F<L>::run(args...) for the runtime line count l (a power of two no
-// larger than lines_default<T, N>()).
+// larger than lines_default<T, N>()), rounded down to a built count.
 template <typename T, int N, template <int> class F, typename... A>
 int with_lines(int l, A... args) {
-  constexpr int D = lines_default<T, N>();
-  if constexpr (D >= 64) if (l == 64) return F<64>::run(args...);
-  if constexpr (D >= 32) if (l == 32) return F<32>::run(args...);
-  if constexpr (D >= 16) if (l == 16) return F<16>::run(args...);
-  if constexpr (D >= 8) if (l == 8) return F<8>::run(args...);
-  if constexpr (D >= 4) if (l == 4) return F<4>::run(args...);
-  if constexpr (D >= 2) if (l == 2) return F<2>::run(args...);
+  while (l > 1 && !lines_built<T, N>(l)) l /= 2;
+  if constexpr (lines_built<T, N>(64)) if (l == 64) return F<64>::run(args...);
+  if constexpr (lines_built<T, N>(32)) if (l == 32) return F<32>::run(args...);
+  if constexpr (lines_built<T, N>(16)) if (l == 16) return F<16>::run(args...);
+  if constexpr (lines_built<T, N>(8)) if (l == 8) return F<8>::run(args...);
+  if constexpr (lines_built<T, N>(4)) if (l == 4) return F<4>::run(args...);
+  if constexpr (lines_built<T, N>(2)) if (l == 2) return F<2>::run(args...);
   if (l == 1) return F<1>::run(args...);
   return fail(MJX_E_SIZE, "no kernel for tile of " + std::to_string(l) + " lines");
 }
 
-#define MJX_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#define MJX_FOR_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) MJX_FOR_MIXED(X)
 
-template <typename T>
+// The launchers of one precision are compiled in three translation units,
+// one per set of line lengths, so that they build in parallel.
+enum { SET_POW2 = 0, SET_MIX5 = 1, SET_MIX3 = 2 };
+__host__ __device__ constexpr int size_set(int n) {
+  return pow2_len(n) ? SET_POW2 : (n % 5 == 0) ? SET_MIX5 : SET_MIX3;
+}
+
+template <typename T, int SET>
 struct Launch {
   template <int N>
   struct FwdPack {
@@ -610,14 +664,17 @@ struct Launch {
   static int row_fwd_pack(int nx, int B, int ny, const T* a, const T* b, long long pitch, long long istride,
                           cpx<T>* W, hipStream_t st, T in_scale = (T)1) {
     switch (nx) {
-#define X(N) \
-  case N:    \
-    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, a, b, pitch, istride, W, in_scale, st);
+#define X(N)                                 \
+  case N:                                    \
+    if constexpr (size_set(N) == SET)        \
+    return with_lines<T, N, FwdPack<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, a, b, pitch, istride, W, in_scale, st); \
+    else break;
       MJX_FOR_SIZES(X)
 #undef X
       default:
         return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
     }
+    return fail(MJX_E_SIZE, "nx " + std::to_string(nx) + " belongs to another launcher set");
   }
   static int row_inv_unpack(int nx, int B, int ny, const cpx<T>* W, T* a, T* b, long long pitch,
                             long long istride, int ghosts, hipStream_t st, T keep = (T)0,
@@ -625,25 +682,31 @@ struct Launch {
     switch (nx) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
+    if constexpr (size_set(N) == SET)                                                                     \
     return with_lines<T, N, InvUnpack<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, W, a, b, pitch, \
-                                                   istride, ghosts, keep, react, st);
+                                                   istride, ghosts, keep, react, st); \
+    else break;
       MJX_FOR_SIZES(X)
 #undef X
       default:
         return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
     }
+    return fail(MJX_E_SIZE, "nx " + std::to_string(nx) + " belongs to another launcher set");
   }
   static int row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStream_t st, int rxn = RXN_HEUN,
                     int mstep = -1) {
     switch (nx) {
-#define X(N) \
-  case N:    \
-    return with_lines<T, N, Gs<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, W, ip, rxn, mstep, st);
+#define X(N)                          \
+  case N:                             \
+    if constexpr (size_set(N) == SET) \
+    return with_lines<T, N, Gs<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), B, ny, W, ip, rxn, mstep, st); \
+    else break;
       MJX_FOR_SIZES(X)
 #undef X
       default:
         return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
     }
+    return fail(MJX_E_SIZE, "nx " + std::to_string(nx) + " belongs to another launcher set");
   }
   static int col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
                        const InstParams* ip, int nrep, hipStream_t st, int mstep = -1) {
@@ -651,13 +714,16 @@ struct Launch {
     switch (ny) {
 #define X(N)                                                                                                 \
   case N:                                                                                                    \
+    if constexpr (size_set(N) == SET)                                                                        \
     return with_lines<T, N, Col<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_COL), (long long)B * nx), nx / 2), B, nx, W, lamx, lamy, lam2d, \
-                                             ip, nrep, s, mstep, st);
+                                             ip, nrep, s, mstep, st); \
+    else break;
       MJX_FOR_SIZES(X)
 #undef X
       default:
         return fail(MJX_E_SIZE, "unsupported ny " + std::to_string(ny));
     }
+    return fail(MJX_E_SIZE, "ny " + std::to_string(ny) + " belongs to another launcher set");
   }
 };
 

@@ -2,8 +2,9 @@
 entry point, workspace in a fixed device array) against mjx_gs_imex_strang
 with a caller-allocated workspace, through the C ABI: bitwise equal results
 on two calls in a row, the second with a smaller input (the scratch keeps
-the first call's contents), in FP32 and FP64; oversize and non-power-of-two
-requests are refused with MJX_E_LIMIT / MJX_E_SIZE before any launch."""
+the first call's contents), in FP32 and FP64, and a third call on a
+mixed-radix side (250); oversize requests and sides without a Stockham plan
+are refused with MJX_E_LIMIT / MJX_E_SIZE before any launch."""
 import numpy as np
 import pytest
 import torch
@@ -37,7 +38,7 @@ def test_noalloc_equals_allocating_entry_point(gpu, dtype):
     st = _hip.stream_ptr()
     # second call smaller than the first: the scratch holds stale data from
     # the first call when the second starts
-    for n, B, steps, seed0 in ((256, 16, 7, 0), (64, 3, 5, 40)):
+    for n, B, steps, seed0 in ((256, 16, 7, 0), (64, 3, 5, 40), (250, 4, 5, 80)):
         print(f"noalloc {dtype} n={n} B={B} steps={steps}", flush=True)
         Ua, Va = _state(n, B, dtype, seed0)
         Ub, Vb = Ua.clone(), Va.clone()
@@ -65,6 +66,7 @@ def test_noalloc_refuses_oversize_and_other_sides(gpu):
     rc = lib.mjx_gs_imex_strang_noalloc(0, 129, 256, 256, P, P, 258, 258 * 258, 1, P, P, None, P, 3,
                                         _hip.stream_ptr())
     assert rc == _hip.MJX_E_LIMIT and "scratch" in lib.mjx_last_error().decode()
-    rc = lib.mjx_gs_imex_strang_noalloc(0, 2, 250, 250, P, P, 252, 252 * 252, 1, P, P, None, P, 3,
+    # 252 has no Stockham plan (chirp-z sides are not served)
+    rc = lib.mjx_gs_imex_strang_noalloc(0, 2, 252, 252, P, P, 254, 254 * 254, 1, P, P, None, P, 3,
                                         _hip.stream_ptr())
     assert rc == _hip.MJX_E_SIZE
