@@ -14,9 +14,9 @@
 //   writes output r to (b/Ns)*Ns*R + (b mod Ns) + r*Ns.
 // Twiddles come from a table of exp(-2*pi*i*t/MJX_TWN) evaluated once in FP64
 // on the device (sincospi), rounded once to FP32 for the FP32 table.  The
-// mixed-radix lengths (radix-3 and radix-5 stages beside the power-of-two
-// ones) index a table of their own, exp(-2*pi*i*t/N) for t < N, stored after
-// the power-of-two table (tw_base).
+// mixed-radix lengths (stages of radix 5, 6, 10 and 12 beside the
+// power-of-two ones) index a table of their own, exp(-2*pi*i*t/N) for t < N,
+// stored after the power-of-two table (tw_base).
 // A radix-R stage needs R | P (every thread runs P/R whole butterflies), so
 // a length's points per thread P is a multiple of each of its radices.
 #pragma once
@@ -27,7 +27,7 @@
 #define MJX_TWN 4096
 
 // Lengths that are not powers of two but have a Stockham plan of their own
-// (radices 2, 3, 4, 5, 8); every other length runs the chirp-z kernels.
+// (radices 2, 4, 5, 6, 10, 12); every other length runs the chirp-z kernels.
 #define MJX_FOR_MIXED(X) X(96) X(100) X(192) X(200) X(250) X(384) X(500) X(768) X(1000)
 
 namespace mjx {
@@ -168,6 +168,39 @@ __device__ __forceinline__ void dft5(cpx<T>* a) {
 }
 
 template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft(cpx<T>* a);
+
+// Prime-factor (Good-Thomas) DFT of length N1*N2 with gcd(N1, N2) = 1: input
+// n = (N2 n1 + N1 n2) mod N, N1-point DFTs over n1, N2-point DFTs over n2, output k = (N2 e1 k1 + N1 e2 k2) mod N with e1 = N2^-1 mod N1 and
+// e2 = N1^-1 mod N2; W_N^(nk) = W_N1^(n1 k1) W_N2^(n2 k2), so no twiddles.
+__host__ __device__ constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+template <int N1, int N2, int SIGN, typename T>
+__device__ __forceinline__ void dft_pfa(cpx<T>* a) {
+  constexpr int N = N1 * N2;
+  constexpr int E1 = inv_mod(N2 % N1, N1), E2 = inv_mod(N1 % N2, N2);
+  cpx<T> y[N1][N2];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    cpx<T> t[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) t[n1] = a[(N2 * n1 + N1 * n2) % N];
+    dft<N1, SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) y[k1][n2] = t[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+    dft<N2, SIGN>(y[k1]);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) a[(N2 * E1 * k1 + N1 * E2 * k2) % N] = y[k1][k2];
+  }
+}
+
+template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dft(cpx<T>* a) {
   if constexpr (R == 2) {
     dft2<SIGN>(a[0], a[1]);
@@ -175,12 +208,18 @@ __device__ __forceinline__ void dft(cpx<T>* a) {
     dft3<SIGN>(a[0], a[1], a[2]);
   } else if constexpr (R == 5) {
     dft5<SIGN>(a);
+  } else if constexpr (R == 6) {
+    dft_pfa<2, 3, SIGN>(a);
+  } else if constexpr (R == 10) {
+    dft_pfa<2, 5, SIGN>(a);
+  } else if constexpr (R == 12) {
+    dft_pfa<4, 3, SIGN>(a);
   } else if constexpr (R == 4) {
     dft4<SIGN>(a[0], a[1], a[2], a[3]);
   } else if constexpr (R == 8) {
     dft8<SIGN>(a);
   } else {
-    static_assert(R == 16, "radix must be 2, 3, 4, 5, 8 or 16");
+    static_assert(R == 16, "radix must be 2, 3, 4, 5, 6, 8, 10, 12 or 16");
     dft16<SIGN>(a);
   }
 }
@@ -199,20 +238,20 @@ __host__ __device__ constexpr bool supported_len(int n) { return pow2_len(n) || 
 // Points per thread and radix sequence for each supported length.
 //   power of two: 16: 4x4  32: 8x4  64: 8x8  128: 16x8  256: 16x16
 //                 512: 8x8x8  1024: 16x16x4  2048: 16x16x8  4096: 16x16x16
-//   mixed (P):    96: 4x4x3x2 (12)  100: 4x5x5 (20)  192: 4x4x4x3 (12)
-//                 200: 4x5x5x2 (20)  250: 5x5x5x2 (10)  384: 4x4x4x3x2 (12)
-//                 500: 4x5x5x5 (20)  768: 4x4x4x4x3 (12)  1000: 4x5x5x5x2 (20)
+//   mixed (P):    96: 4x4x6 (12)  100: 10x10 (10)  192: 4x4x12 (12)
+//                 200: 10x10x2 (10)  250: 10x5x5 (10)  384: 4x4x12x2 (12)
+//                 500: 10x10x5 (10)  768: 4x4x4x12 (12)  1000: 10x10x10 (10)
+// (radices 6, 10 and 12 are prime-factor 2x3, 2x5 and 4x3 DFTs).
 __host__ __device__ constexpr int plan_p(int n) {
   return n == 16 ? 4 : n == 32 ? 8 : n == 64 ? 8 : n == 512 ? 8
        : (n == 96 || n == 192 || n == 384 || n == 768) ? 12
-       : (n == 100 || n == 200 || n == 500 || n == 1000) ? 20
-       : n == 250 ? 10
+       : mixed_len(n) ? 10
        : 16;
 }
 __host__ __device__ constexpr int plan_nstages(int n) {
-  return n == 100 ? 3
-       : (n == 96 || n == 192 || n == 200 || n == 250 || n == 500) ? 4
-       : (n == 384 || n == 768 || n == 1000) ? 5
+  return n == 100 ? 2
+       : (n == 96 || n == 192 || n == 200 || n == 250 || n == 500 || n == 1000) ? 3
+       : (n == 384 || n == 768) ? 4
        : n <= 256 ? 2 : 3;
 }
 __host__ __device__ constexpr int plan_radix(int n, int s) {
@@ -225,14 +264,14 @@ __host__ __device__ constexpr int plan_radix(int n, int s) {
        : n == 1024 ? (s < 2 ? 16 : 4)
        : n == 2048 ? (s < 2 ? 16 : 8)
        : n == 4096 ? 16
-       : n == 96 ? (s < 2 ? 4 : s == 2 ? 3 : 2)
-       : (n == 100 || n == 500) ? (s == 0 ? 4 : 5)
-       : n == 192 ? (s < 3 ? 4 : 3)
-       : n == 200 ? (s == 0 ? 4 : s < 3 ? 5 : 2)
-       : n == 250 ? (s < 3 ? 5 : 2)
-       : n == 384 ? (s < 3 ? 4 : s == 3 ? 3 : 2)
-       : n == 768 ? (s < 4 ? 4 : 3)
-       : (s == 0 ? 4 : s < 4 ? 5 : 2);  // 1000
+       : n == 96 ? (s < 2 ? 4 : 6)
+       : (n == 100 || n == 1000) ? 10
+       : n == 192 ? (s < 2 ? 4 : 12)
+       : n == 200 ? (s < 2 ? 10 : 2)
+       : n == 250 ? (s == 0 ? 10 : 5)
+       : n == 384 ? (s < 2 ? 4 : s == 2 ? 12 : 2)
+       : n == 500 ? (s < 2 ? 10 : 5)
+       : (s < 3 ? 4 : 12);  // 768
 }
 __host__ __device__ constexpr int plan_ns_before(int n, int s) {
   return s == 0 ? 1 : plan_ns_before(n, s - 1) * plan_radix(n, s - 1);
