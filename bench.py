@@ -14,7 +14,8 @@ workloads in the same line: the uneven-size batched ETDRK4 plan (1024
 instances of {64,128,256,512}^2), backward-Euler Newton-Krylov on a 512^2
 Dirichlet grid (iteration counts per step), the serving loop (Poisson
 arrivals at 200/s, p50/p99 request latency, throughput vs sequential FP64),
-and the same IMEX-Strang batch on a 250^2 grid (the chirp-z kernels).
+and the same IMEX-Strang batch on 250^2 (mixed-radix Stockham kernels) and
+251^2 (chirp-z kernels) grids.
 
 Prints ONE JSON line.  Only the cpu_baseline leg touches the oracle.
 """
@@ -423,8 +424,9 @@ def main():
                     help="comma-separated batch sizes to time as well ('' to skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (byte-counter calibration)")
-    ap.add_argument("--anylen-n", type=int, default=250,
-                    help="side of the any-length (chirp-z) IMEX-Strang leg at the headline batch (0: skip)")
+    ap.add_argument("--anylen-n", type=str, default="250,251",
+                    help="comma-separated sides of the non-power-of-two IMEX-Strang legs at the headline batch "
+                         "(250: mixed-radix Stockham kernels, 251: chirp-z kernels; '' or 0 to skip)")
     ap.add_argument("--c4-steps", type=int, default=100, help="steps of the uneven batched ETDRK4 workload (0: skip)")
     ap.add_argument("--c4-instances", type=int, default=1024)
     ap.add_argument("--c3-steps", type=int, default=10, help="Newton-Krylov backward-Euler steps (0: skip)")
@@ -491,15 +493,20 @@ def main():
         sweep[str(b)] = b * N * N * K / t
 
     anylen = None
-    if args.anylen_n > 0:
-        n2 = args.anylen_n
+    mixed = (96, 100, 192, 200, 250, 384, 500, 768, 1000)
+    for n2 in [int(x) for x in args.anylen_n.split(",") if x and int(x) > 0]:
         log(f"any-length IMEX-Strang {n2}^2 B={B}")
         ta, _, fina, _, first = time_form(torch.float32, B, K, W, n=n2, keep_first=True)
         ref = fp64_reference_state(n2, W + K, seed=0)
-        anylen = {"grid": [n2, n2], "batch": B, "steps": K, "value": B * n2 * n2 * K / ta,
-                  "unit": "grid-point-steps/s", "ms_per_step": 1e3 * ta / K, "finite": fina,
-                  "vs_power_of_two_headline": (B * n2 * n2 * K / ta) / gps32,
-                  "rel_l2_fp32_vs_fp64_instance0": rel_l2_pair(first, ref), "rel_l2_after_steps": W + K}
+        leg = {"grid": [n2, n2], "batch": B, "steps": K, "value": B * n2 * n2 * K / ta,
+               "kernels": "mixed-radix Stockham" if n2 in mixed else "chirp-z",
+               "unit": "grid-point-steps/s", "ms_per_step": 1e3 * ta / K, "finite": fina,
+               "vs_power_of_two_headline": (B * n2 * n2 * K / ta) / gps32,
+               "rel_l2_fp32_vs_fp64_instance0": rel_l2_pair(first, ref), "rel_l2_after_steps": W + K}
+        if anylen is None:
+            anylen = leg
+        else:
+            anylen.setdefault("other_sides", []).append(leg)
 
     curve = None
     if not args.no_curve:

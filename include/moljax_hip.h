@@ -58,8 +58,9 @@ int mjx_init(void);
 
 /* Number of bytes of complex work buffer needed by the periodic spectral
  * entry points for B instances of ny x nx in the given dtype:
- * B*ny*nx complex values when nx and ny are both powers of two in
- * [16, 4096] (packed x-spectra), else B*ny*2*(nx/2+1) (the x-half-spectra
+ * B*ny*nx complex values when nx and ny both have a Stockham plan (powers of
+ * two in [16, 4096] and the mixed-radix lengths 96, 100, 192, 200, 250, 384,
+ * 500, 768, 1000; packed x-spectra), else B*ny*2*(nx/2+1) (the x-half-spectra
  * of the two fields, chirp-z path). */
 long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx);
 
@@ -76,9 +77,10 @@ long long mjx_spectral_workspace_bytes(int dtype, int B, int ny, int nx);
  *           apply_diffusion_inverse_fft) on periodic 2-D grids, and the
  *           FFT-CN step of benchmarks/benchmark_vs_scipy.py:74-91
  *           (alpha = beta = dt*D/2, lam = -(kx^2+ky^2)).
- * Sizes: nx, ny both powers of two in [16, 4096] (Stockham line FFTs), or
- * any nx, ny in [2, 2048] (chirp-z line transforms, the same results to
- * rounding).  The same rule holds for mjx_gs_imex_strang,
+ * Sizes: nx, ny both Stockham lengths (powers of two in [16, 4096], and 96,
+ * 100, 192, 200, 250, 384, 500, 768, 1000 with radix-5/6/10/12 stages:
+ * Stockham line FFTs), or any nx, ny in [2, 2048] (chirp-z line transforms,
+ * the same results to rounding).  The same rule holds for mjx_gs_imex_strang,
  * mjx_gs_imex_strang_var and mjx_gs_imex.  The first call for a side the
  * chirp-z kernels serve evaluates and uploads that side's plan tables
  * (blocking copies, kept for the process): make it before any stream
@@ -125,8 +127,8 @@ int mjx_phi_tables(int ny, int nx, const double* lam, double dt, int mode, int e
  * is a fixed, library-owned device array of MJX_NOALLOC_SCRATCH_BYTES per GPU
  * (held until the process ends); requests whose
  * mjx_spectral_workspace_bytes(dtype, B, ny, nx) exceed it return
- * MJX_E_LIMIT.  Power-of-two sides in [16, 4096] only (MJX_E_SIZE
- * otherwise).  mjx_init must have run on the device (MJX_E_ARG otherwise).
+ * MJX_E_LIMIT.  Stockham sides only (powers of two in [16, 4096] and the
+ * mixed-radix lengths of mjx_spectral_pair; MJX_E_SIZE otherwise).  mjx_init must have run on the device (MJX_E_ARG otherwise).
  * Calls no hipMalloc/hipFree and creates no events.  Every call on a GPU
  * shares that GPU's scratch: calls that overlap on one GPU (different
  * streams or threads) race; calls on one stream are ordered.  Results are
