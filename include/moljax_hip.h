@@ -403,8 +403,10 @@ int mjx_etdrk4_coeffs_form(int dtype, int form, int ny, int nx, const double* la
  * (the workspace is then six planes of ny * (nx+2) complex values per
  * instance; the coefficient tables are the same); per-instance F, k
  * from inst rows; coef from mjx_etdrk4_coeffs.  Four row passes (inverse
- * x-FFT, N(u,v), forward x-FFT) and four column passes (y-FFT, stage
- * combination, inverse y-FFT) per step.  Replaces the coupled ETDRK4 of
+ * x-FFT, N(u,v), forward x-FFT) and four column passes (y-FFT of the
+ * stage's operands, stage combination, inverse y-FFT) per step; on the
+ * Stockham kernels every operand is a packed n x n plane (four per
+ * instance, 4 n^2 + 2 n complex values with ETD2's N^ plane).  Replaces the coupled ETDRK4 of
  * benchmarks/benchmark_gray_scott.py:369-423 and serves
  * moljax/core/fft_integrators.py:252-341 (etdrk4_step) for Gray-Scott.
  * work: mjx_etdrk4_workspace_bytes(dtype, B, ny, nx). */

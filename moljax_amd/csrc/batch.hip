@@ -316,7 +316,7 @@ int autotune_t(int scheme, int n, int B, int reps, hipStream_t st, int* chosen) 
   InstParams* ip = nullptr;
   void* work = nullptr;
   T* tab = nullptr;
-  const long long work_bytes = (long long)sizeof(T) * 2 * B * (3LL * n * n + 3LL * n * (n + 2));
+  const long long work_bytes = mjx_etdrk4_workspace_bytes(sizeof(T) == 8 ? MJX_F64 : MJX_F32, B, n, n);
   const long long tab_elems = 6LL * n * (n + 2);
   int rc = MJX_OK;
   if (hipMalloc(&u, 2 * pts * sizeof(T)) != hipSuccess || hipMalloc(&ip, B * sizeof(InstParams)) != hipSuccess ||

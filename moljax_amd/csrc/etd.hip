@@ -118,7 +118,9 @@ long long mjx_etdrk4_coeff_elems(int ny, int nx) { return 6LL * ny * (nx + 2); }
 long long mjx_etdrk4_workspace_bytes(int dtype, int B, int ny, int nx) {
   const long long es = dtype == MJX_F64 ? 16 : 8;
   if (!etd_fast(ny, nx)) return es * czt_etd_work_elems(B, ny, nx);
-  return es * (long long)B * (3LL * ny * nx + 3LL * ny * (nx + 2));
+  // four packed planes (etd_planes in etd_kernels.h), the last one with room
+  // for the separated N^ plane of ETD2
+  return es * (long long)B * (4LL * ny * nx + 2LL * ny);
 }
 
 int mjx_etdrk4_coeffs_form(int dtype, int form, int ny, int nx, const double* lamx, const double* lamy, double Du,

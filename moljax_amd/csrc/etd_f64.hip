@@ -37,8 +37,8 @@ int etd_exp_run(int B, int n, T* u, T* v, long long pitch, long long istride, in
   const EtdPlanes<T> p = etd_planes(work, B, n);
   int rc = MJX_OK;
   if (order == 2 && prev == 1) {
-    rc = Launch<T, SET_POW2>::row_fwd_pack(n, B, n, nu, nv, pitch, istride, p.Wg, st, (T)(1.0 / n));
-    if (!rc) rc = etd_store_n<T>(B, n, p, p.Wg, st);
+    rc = Launch<T, SET_POW2>::row_fwd_pack(n, B, n, nu, nv, pitch, istride, p.P1, st, (T)(1.0 / n));
+    if (!rc) rc = etd_store_n<T>(B, n, p, p.P1, st);
   }
   if (!rc) rc = Launch<T, SET_POW2>::row_fwd_pack(n, B, n, u, v, pitch, istride, p.Ws, st, (T)(1.0 / n));
   cpx<T>* cur = p.Ws;

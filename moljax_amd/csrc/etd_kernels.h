@@ -1,31 +1,32 @@
 // ETDRK4 (Cox-Matthews) for periodic Gray-Scott, batched, with FP64
 // contour-integral coefficients.
 //
-// Data per instance (complex, precision T):
-//   Ws  packed x-spectra of the state (layout of spectral_kernels.h)
-//   Wg  packed x-spectra of the current stage value (a, b or c)
-//   Wn  packed x-spectra of N(.) of the last stage
-//   Uh, Nn, Fh  full 2-D spectra in the separated layout: ny rows of nx+2
-//       columns, s = kx (0..nx/2) for u and s = nx/2+1+kx for v, so that
-//       every column is one field at one kx (the packed special columns 0
-//       and nx/2 are split into their two real-sequence spectra).
+// Square power-of-two grids (the Stockham chain): four packed x-spectral
+// planes per instance (layout of spectral_kernels.h, ny rows of nx complex
+// values): Ws the state, P1..P3 the stage operands (etd_planes).
 // Coefficients E, E2, Q, f1, f2, f3 are six real (ny, nx+2) tables in the
-// same separated layout, shared by all instances with the same (dt, Du, Dv).
+// separated layout (s = kx (0..nx/2) for u and s = nx/2+1+kx for v, so every
+// column is one field at one kx), shared by all instances with the same
+// (dt, Du, Dv).
 //
 // One step (benchmarks/benchmark_gray_scott.py:369-417, Kassam-Trefethen form):
-//   row_nl(Ws -> Wn)             N_n
-//   col stage 1: Uh = F2 Ws, Nn = F2 Wn, a^ = E2 Uh + Q Nn      -> Wg
-//   row_nl(Wg -> Wn)             N_a
-//   col stage 2: b^ = E2 Uh + Q Na^, Fh = E Uh + f1 Nn + f2 Na^  -> Wg
-//   row_nl(Wg -> Wn)             N_b
-//   col stage 3: c^ = E2 a^ + Q (2 Nb^ - Nn), a^ = E2 Uh + Q Nn,
-//                Fh += f2 Nb^                                     -> Wg
-//   row_nl(Wg -> Wn)             N_c
-//   col stage 4: u^new = Fh + f3 Nc^                              -> Ws
-// ("F2 W" is the y-FFT of packed x-spectra.)  Every packed x-spectral buffer
-// holds the x-DFT divided by nx (the state is packed with scale 1/nx and N is
-// scaled by 1/nx before its x-FFT), so all spectra share one normalisation
-// and the stage outputs, which go back through the inverse y-FFT, carry 1/ny.
+//   row_nl(Ws -> P1)        N_n
+//   k_col_rk4 stage 1       a = E2 u + Q N_n                         -> P2
+//   row_nl(P2 -> P2)        N_a
+//   k_col_rk4 stage 2       b = E2 u + Q N_a                         -> P3
+//   row_nl(P3 -> P3)        N_b
+//   k_col_rk4 stage 3       c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)  -> P2
+//                           Fh = E u + f1 N_n + f2 (N_a + N_b)       -> P1
+//   row_nl(P2 -> P2)        N_c
+//   k_col_rk4 stage 4       u_new = Fh + f3 N_c                      -> Ws
+// Every packed x-spectral buffer holds the x-DFT divided by nx (the state is
+// packed with scale 1/nx and N is scaled by 1/nx before its x-FFT), so all
+// spectra share one normalisation and the stage outputs, which go back
+// through the inverse y-FFT, carry 1/ny.
+//
+// The any-length (chirp-z) chains keep the separated y-spectra Uh, Nn, Fh
+// (ny rows of nx+2 columns) and the per-value stage arithmetic etd_stage;
+// ETD1 / ETD2 use k_col_etd with the separated N^ plane.
 #pragma once
 #include "spectral_kernels.h"
 
@@ -33,9 +34,11 @@ namespace mjx {
 
 // Row pass: packed x-spectra -> physical (u, v) -> Gray-Scott reaction
 // N(u, v) = (-u v^2 + F(1-u), u v^2 - (F+k) v) -> packed x-spectra.
+// Win == Wout is allowed (in place): a row is read whole into LDS before any
+// of it is written, and one workgroup owns the row.
 template <typename T, int NX, int TR>
-__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_nl(const cpx<T>* __restrict__ Win,
-                                                                  cpx<T>* __restrict__ Wout,
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_nl(const cpx<T>* Win,
+                                                                  cpx<T>* Wout,
                                                                   const InstParams* __restrict__ ip, int ny,
                                                                   int rows_per_inst_tiles,
                                                                   const cpx<T>* __restrict__ tw) {
@@ -220,6 +223,178 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
   for (int m = 0; m < C::P; ++m) Wout[woff + (long long)(j + C::TN * m) * nx] = v[m];
 }
 
+// ETDRK4 column stages on the packed planes of the square power-of-two
+// chain.  Every operand stays a packed x-spectral plane (ny rows of nx
+// complex values, rows 128-byte aligned); a stage y-transforms each plane it
+// needs again instead of keeping separated y-spectra, and the final
+// combination is accumulated once, in stage 3, as the y-spectrum Fh in the
+// packed layout (Fh_a + i Fh_b in the special columns, as the y-FFT of a
+// packed column would be):
+//   stage 1  Ws = u, P1 = N_n                 -> P2 = a = E2 u + Q N_n
+//   stage 2  Ws, P2 = N_a                      -> P3 = b = E2 u + Q N_a
+//   stage 3  Ws, P1, P2 = N_a, P3 = N_b        -> P2 = c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)
+//                                                 P1 = Fh = E u + f1 N_n + f2 (N_a + N_b)
+//   stage 4  P1 = Fh, P2 = N_c                 -> Ws = Fh + f3 N_c
+// (benchmarks/benchmark_gray_scott.py:369-417; the same combinations as
+// etd_stage, with the final sum gathered in stage 3).  Stage 3 writes P2 and
+// P1 over the inputs it read: the workgroup owns its columns and reads them
+// all before its first store.  Each stage is linear in its inputs, so in a
+// special column (packed kx = 0 and nx/2 of one field) an input Z with
+// per-field coefficients cA, cB contributes (cA+cB)/2 Z(ky) + (cA-cB)/2
+// conj Z(-ky).
+enum { RK_S1 = 1, RK_S2 = 2, RK_S3 = 3, RK_S4 = 4 };
+enum { CF_E = 0, CF_E2 = 1, CF_Q = 2, CF_F1 = 3, CF_F2 = 4, CF_F3 = 5 };
+
+template <typename T, int NY, int TC_, int STAGE>
+__global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, cpx<T>* P1, cpx<T>* P2,
+                                                                   cpx<T>* P3, const T* coef, int col_tiles,
+                                                                   int B, int kgrp,
+                                                                   const cpx<T>* __restrict__ tw) {
+  using C = ColCfg<NY, TC_>;
+  constexpr int TC = C::TC, P = C::P, TN = C::TN;
+  constexpr int NX = NY;                    // square grids
+  constexpr int SW = NX + 2;                // row length of the separated-layout coefficient tables
+  constexpr unsigned TSTRIDE = NY * SW;     // one coefficient table
+  constexpr unsigned RSTEP = TN * NX;       // plane offset between a thread's points m and m + 1
+  constexpr unsigned CSTEP = TN * SW;       // the same in a coefficient table
+  __shared__ cpx<T> lds[NY * TC];
+  __shared__ cpx<T> lds_f[STAGE == RK_S3 ? NY * TC : 1];
+  // Workgroups run tile-major within groups of kgrp instances (group,
+  // tile, instance), so that the coefficient slab of a tile is read from
+  // L2 by the group's other instances; the last group may be smaller.
+  const int w = xcd_remap_rev(blockIdx.x, gridDim.x);
+  const int gsz = kgrp * col_tiles;
+  const int g = w / gsz;
+  const int r = w - g * gsz;
+  const int kg = B - g * kgrp < kgrp ? B - g * kgrp : kgrp;
+  const int tile = r / kg;
+  const int inst = g * kgrp + (r - tile * kg);
+  const int c = threadIdx.x % TC, j = threadIdx.x / TC;
+  const int p0 = tile * TC;
+  const int p = p0 + c;
+  constexpr int half = NX / 2;
+  const T s = (T)(1.0 / (double)NY);
+  // per-instance plane bases (uniform) and 32-bit offsets within them
+  const long long ibase = (long long)inst * NY * NX;
+  cpx<T>* const ws = Ws + ibase;
+  cpx<T>* const p1 = P1 + ibase;
+  cpx<T>* const p2 = P2 + ibase;
+  cpx<T>* const p3 = P3 + ibase;
+  const unsigned toff = (unsigned)(j * NX + p);
+  const bool stile = p0 == 0 || p0 == half;  // tile whose first column is a packed special column
+  const bool spec = stile && c == 0;
+  // coefficient columns: the field's kx column, or the pair (kx = 0, nx/2)
+  const int sa = spec ? (p0 == 0 ? 0 : half + 1) : (p < half ? p : p + 1);
+  const int sb = spec ? (p0 == 0 ? half : NX + 1) : sa;
+  const unsigned ca = (unsigned)(j * SW + sa), cb = (unsigned)(j * SW + sb);
+  TwCache<T, NY> twc;
+  load_twiddles(twc, j, tw);
+
+  // ld: the tile's columns of plane X; xf: their y-FFT, which in a special
+  // tile is also left in LDS for the conj Z(-ky) partners (the caller
+  // releases LDS with done()).  Each stage issues the loads of its next
+  // input before transforming the current one.
+  auto ld = [&](const cpx<T>* X, cpx<T> (&z)[P]) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) z[m] = X[toff + m * RSTEP];
+  };
+  auto xf = [&](cpx<T> (&z)[P]) {
+    line_fft<T, NY, -1, TC, 0>(z, j, lds + c, twc);
+    if (stile) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) lds[(j + TN * m) * TC + c] = z[m];
+      __syncthreads();
+    }
+  };
+  auto done = [&]() {
+    if (stile) __syncthreads();
+  };
+  // a[m] (+)= k * Z at ky = j + TN m, with k from mult(table index)
+  auto add = [&](cpx<T> (&a)[P], const cpx<T> (&z)[P], bool first, auto mult) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const T cA = mult(ca + m * CSTEP);
+      cpx<T> t;
+      if (spec) {
+        const int ky = j + TN * m;
+        const T cB = mult(cb + m * CSTEP);
+        const cpx<T> zm = conjg(lds[((NY - ky) & (NY - 1)) * TC]);
+        const T cp = (T)0.5 * (cA + cB), cm = (T)0.5 * (cA - cB);
+        t = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
+      } else {
+        t = scale(z[m], cA);
+      }
+      a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
+    }
+  };
+  auto E = [&](unsigned i) { return coef[i + CF_E * TSTRIDE]; };
+  auto E2 = [&](unsigned i) { return coef[i + CF_E2 * TSTRIDE]; };
+  auto Q = [&](unsigned i) { return coef[i + CF_Q * TSTRIDE]; };
+
+  cpx<T> za[P], zb[P], acc[P];
+  cpx<T>* out;
+  if constexpr (STAGE == RK_S1 || STAGE == RK_S2) {
+    ld(ws, za);
+    ld(STAGE == RK_S1 ? p1 : p2, zb);
+    xf(za);
+    add(acc, za, true, E2);
+    done();
+    xf(zb);
+    add(acc, zb, false, Q);
+    done();
+    out = STAGE == RK_S1 ? p2 : p3;
+  } else if constexpr (STAGE == RK_S3) {
+    // Fh accumulates in this thread's own LDS slots (no sharing, no
+    // barriers), which leaves the registers to the transforms and to c.
+    cpx<T>* fh = lds_f + j * TC + c;
+    auto addf = [&](const cpx<T> (&zz)[P], bool first, auto mult) {
+      cpx<T> t[P];
+      add(t, zz, true, mult);
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const cpx<T> o = first ? mk((T)0, (T)0) : fh[m * TN * TC];
+        fh[m * TN * TC] = mk(o.re + t[m].re, o.im + t[m].im);
+      }
+    };
+    ld(p2, za);  // N_a
+    ld(p3, zb);  // N_b
+    xf(za);
+    addf(za, true, [&](unsigned i) { return coef[i + CF_F2 * TSTRIDE]; });
+    done();
+    ld(ws, za);  // u
+    xf(zb);
+    add(acc, zb, true, [&](unsigned i) { return (T)2 * Q(i); });
+    addf(zb, false, [&](unsigned i) { return coef[i + CF_F2 * TSTRIDE]; });
+    done();
+    ld(p1, zb);  // N_n
+    xf(za);
+    add(acc, za, false, [&](unsigned i) { const T e2 = E2(i); return e2 * e2; });
+    addf(za, false, E);
+    done();
+    xf(zb);
+    add(acc, zb, false, [&](unsigned i) { const T q = Q(i); return E2(i) * q - q; });
+    addf(zb, false, [&](unsigned i) { return coef[i + CF_F1 * TSTRIDE]; });
+    done();
+#pragma unroll
+    for (int m = 0; m < P; ++m) p1[toff + m * RSTEP] = fh[m * TN * TC];
+    out = p2;
+  } else {
+    ld(p2, za);  // N_c
+    ld(p1, zb);  // Fh
+    xf(za);
+    add(acc, za, true, [&](unsigned i) { return coef[i + CF_F3 * TSTRIDE]; });
+    done();
+#pragma unroll
+    for (int m = 0; m < P; ++m) acc[m] = mk(acc[m].re + zb[m].re, acc[m].im + zb[m].im);
+    out = ws;
+  }
+#pragma unroll
+  for (int m = 0; m < P; ++m) acc[m] = scale(acc[m], s);
+  line_fft<T, NY, +1, TC, 0>(acc, j, lds + c, twc);
+#pragma unroll
+  for (int m = 0; m < P; ++m) out[toff + m * RSTEP] = acc[m];
+}
+
 // ------------------------------------------------- contour coefficients
 // Kassam-Trefethen contour means over M = 32 points r_j = exp(i pi (j-1/2)/M)
 // (benchmarks/benchmark_gray_scott.py:336-356), in FP64 for one table entry:
@@ -332,6 +507,14 @@ struct EtdLaunch {
                        Nn, Fh, coef, Wout, nx, tiles, twiddles<T>());
     return check_launch("col_etd");
   }
+  template <int N, int L, int STAGE>
+  static int rk4(int B, int nx, cpx<T>* Ws, cpx<T>* P1, cpx<T>* P2, cpx<T>* P3, const T* coef, hipStream_t st) {
+    const int tiles = nx / L;
+    constexpr int kgrp = 4;  // instances per tile-major group (measured: 1, 4, 8, 16 -> 4 best)
+    hipLaunchKernelGGL((k_col_rk4<T, N, L, STAGE>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, Ws, P1, P2, P3,
+                       coef, tiles, B, kgrp, twiddles<T>());
+    return check_launch("col_rk4");
+  }
 };
 
 // Runtime tile dispatch for the ETDRK4 kernels.  Rows: the default lines
@@ -374,30 +557,62 @@ struct EtdTiles {
       if (L == CM / 8) return EtdLaunch<T>::template col<N, CM / 8, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
     return EtdLaunch<T>::template col<N, DC, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
   }
+  // Stage 3 holds c and a transform in registers and Fh in a second LDS
+  // array: its tile is capped at 256 threads (512 with 8 points per thread),
+  // so that a wave may hold 256 VGPRs, and at 64 KiB of LDS for both arrays.
+  static constexpr int S3T = ((plan_p(N) <= 8 ? 512 : 256) / TN) > 0 ? ((plan_p(N) <= 8 ? 512 : 256) / TN) : 1;
+  static constexpr int S3L = 32768 / (N * (int)sizeof(cpx<T>)) > 0 ? 32768 / (N * (int)sizeof(cpx<T>)) : 1;
+  static constexpr int S3CAP = S3T < S3L ? S3T : S3L;
+  template <int STAGE>
+  static int rk4(int L, int B, int nx, cpx<T>* Ws, cpx<T>* P1, cpx<T>* P2, cpx<T>* P3, const T* coef,
+                 hipStream_t st) {
+    if constexpr (STAGE == RK_S3) {
+      if (L > S3CAP) L = S3CAP;
+      if constexpr (CM <= S3CAP) {
+        if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+      }
+    } else {
+      if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    }
+    if constexpr (CM >= 2)
+      if (L == CM / 2) return EtdLaunch<T>::template rk4<N, CM / 2, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    if constexpr (CM >= 4)
+      if (L == CM / 4) return EtdLaunch<T>::template rk4<N, CM / 4, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    if constexpr (CM >= 8)
+      if (L == CM / 8) return EtdLaunch<T>::template rk4<N, CM / 8, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    return EtdLaunch<T>::template rk4<N, DC, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+  }
 };
 
 
-// n_steps ETDRK4 steps on the packed state Ws of B square N x N instances
-// (workspace planes as laid out by etd_planes), without the pack/unpack.
+// Workspace of the square power-of-two ETD chains: four packed planes of
+// B n^2 complex values (Ws the state; P1, P2, P3 the stage operands of
+// k_col_rk4).  ETD1 / ETD2 use Ws and P1 for the state (ping-pong), P2 for
+// N^ and, at the start of P3, the separated-layout plane Nn (B n (n+2)
+// values) holding N^ of the previous step; hence B (4 n^2 + 2 n) in all.
 template <typename T>
 struct EtdPlanes {
-  cpx<T>*Ws, *Wg, *Wn, *Uh, *Nn, *Fh;
+  cpx<T>*Ws, *P1, *P2, *P3;
+  cpx<T>* Nn;
 };
+
+inline long long etd_work_elems(int B, int n) { return (long long)B * (4LL * n * n + 2LL * n); }
 
 template <typename T>
 inline EtdPlanes<T> etd_planes(cpx<T>* work, int B, int n) {
   const long long plane = (long long)B * n * n;
-  const long long splane = (long long)B * n * (n + 2);
   EtdPlanes<T> p;
   p.Ws = work;
-  p.Wg = p.Ws + plane;
-  p.Wn = p.Wg + plane;
-  p.Uh = p.Wn + plane;
-  p.Nn = p.Uh + splane;
-  p.Fh = p.Nn + splane;
+  p.P1 = p.Ws + plane;
+  p.P2 = p.P1 + plane;
+  p.P3 = p.P2 + plane;
+  p.Nn = p.P3;
   return p;
 }
 
+// n_steps ETDRK4 steps on the packed state Ws of B square N x N instances,
+// without the pack/unpack: four row passes (N of the stage value, in place
+// from the second stage on) and the four k_col_rk4 stages.
 template <typename T, int N>
 int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
                     hipStream_t st) {
@@ -407,20 +622,20 @@ int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParam
   const int LC = E::pick_col(tuned_lines(TK_ETD_COL, dt_, N));
   int rc = MJX_OK;
   for (int s = 0; s < n_steps && rc == 0; ++s) {
-    rc = E::row_nl(LR, B, N, p.Ws, p.Wn, ip, st);
-    if (!rc) rc = E::template col<1>(LC, B, N, p.Ws, p.Wn, p.Uh, p.Nn, p.Fh, coef, p.Wg, st);
-    if (!rc) rc = E::row_nl(LR, B, N, p.Wg, p.Wn, ip, st);
-    if (!rc) rc = E::template col<2>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Wg, st);
-    if (!rc) rc = E::row_nl(LR, B, N, p.Wg, p.Wn, ip, st);
-    if (!rc) rc = E::template col<3>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Wg, st);
-    if (!rc) rc = E::row_nl(LR, B, N, p.Wg, p.Wn, ip, st);
-    if (!rc) rc = E::template col<4>(LC, B, N, p.Wn, nullptr, p.Uh, p.Nn, p.Fh, coef, p.Ws, st);
+    rc = E::row_nl(LR, B, N, p.Ws, p.P1, ip, st);
+    if (!rc) rc = E::template rk4<RK_S1>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.P2, p.P2, ip, st);
+    if (!rc) rc = E::template rk4<RK_S2>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.P3, p.P3, ip, st);
+    if (!rc) rc = E::template rk4<RK_S3>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.P2, p.P2, ip, st);
+    if (!rc) rc = E::template rk4<RK_S4>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
   }
   return rc;
 }
 
 // n_steps of exponential Euler (order 1) or AB2 (order 2) on the packed
-// state in plane cur (Ws or Wg; each step writes the other one, and cur
+// state in plane cur (Ws or P1; each step writes the other one, and cur
 // names the plane holding the result).  AB2 keeps N^ of the previous step in
 // the separated plane Nn (have_prev: Nn already holds it, else the first step
 // is ETD1).
@@ -433,15 +648,15 @@ int etd_exp_steps_sized(int B, const EtdPlanes<T>& p, cpx<T>*& cur, const T* coe
   const int LC = E::pick_col(tuned_lines(TK_ETD_COL, dt_, N));
   int rc = MJX_OK;
   for (int s = 0; s < n_steps && rc == 0; ++s) {
-    cpx<T>* next = cur == p.Ws ? p.Wg : p.Ws;
-    rc = E::row_nl(LR, B, N, cur, p.Wn, ip, st);
+    cpx<T>* next = cur == p.Ws ? p.P1 : p.Ws;
+    rc = E::row_nl(LR, B, N, cur, p.P2, ip, st);
     if (rc) break;
     if (order == 1)
-      rc = E::template col<STAGE_ETD1>(LC, B, N, cur, p.Wn, p.Uh, p.Nn, p.Fh, coef, next, st);
+      rc = E::template col<STAGE_ETD1>(LC, B, N, cur, p.P2, nullptr, p.Nn, nullptr, coef, next, st);
     else if (s == 0 && !have_prev)
-      rc = E::template col<STAGE_ETD2_FIRST>(LC, B, N, cur, p.Wn, p.Uh, p.Nn, p.Fh, coef, next, st);
+      rc = E::template col<STAGE_ETD2_FIRST>(LC, B, N, cur, p.P2, nullptr, p.Nn, nullptr, coef, next, st);
     else
-      rc = E::template col<STAGE_ETD2>(LC, B, N, cur, p.Wn, p.Uh, p.Nn, p.Fh, coef, next, st);
+      rc = E::template col<STAGE_ETD2>(LC, B, N, cur, p.P2, nullptr, p.Nn, nullptr, coef, next, st);
     cur = next;
   }
   return rc;
@@ -454,7 +669,7 @@ int etd_store_n_sized(int B, const EtdPlanes<T>& p, const cpx<T>* W, hipStream_t
   using E = EtdTiles<T, N>;
   constexpr int dt_ = sizeof(T) == 8 ? MJX_F64 : MJX_F32;
   const int LC = E::pick_col(tuned_lines(TK_ETD_COL, dt_, N));
-  return E::template col<STAGE_STORE_N>(LC, B, N, W, nullptr, p.Uh, p.Nn, p.Fh, nullptr, nullptr, st);
+  return E::template col<STAGE_STORE_N>(LC, B, N, W, nullptr, nullptr, p.Nn, nullptr, nullptr, nullptr, st);
 }
 
 template <typename T>

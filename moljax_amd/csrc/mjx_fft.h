@@ -338,17 +338,19 @@ __host__ __device__ constexpr int tw_count(int n) { return tw_offset(n, plan_nst
 // The mixed-radix plans whose twiddles would take more than 128 bytes of
 // registers per thread (32 FP32 or 16 FP64 values) read them from the table
 // (L1-resident) at each use instead.
-template <typename T, int N>
+template <typename T, int N, bool USE = true>
 struct TwCache {
-  static constexpr bool CACHED = pow2_len(N) || tw_count(N) * (int)sizeof(T) <= 128;
+  // USE = false: always read the table at each use (kernels that need the
+  // registers for other operands)
+  static constexpr bool CACHED = USE && (pow2_len(N) || tw_count(N) * (int)sizeof(T) <= 128);
   cpx<T> w[CACHED ? tw_count(N) : 1];
   const cpx<T>* tw;
 };
 
-template <typename T, int N>
-__device__ __forceinline__ void load_twiddles(TwCache<T, N>& c, int j, const cpx<T>* __restrict__ tw) {
+template <typename T, int N, bool USE>
+__device__ __forceinline__ void load_twiddles(TwCache<T, N, USE>& c, int j, const cpx<T>* __restrict__ tw) {
   c.tw = tw;
-  if constexpr (!TwCache<T, N>::CACHED) return;
+  if constexpr (!TwCache<T, N, USE>::CACHED) return;
   constexpr int P = plan_p(N);
   constexpr int TN = N / P;
   constexpr int NS = plan_nstages(N);
@@ -383,8 +385,8 @@ __host__ __device__ constexpr int lds_line_len(int n, int pad) { return pad ? n 
 // lds   : this line's LDS base; natural index idx lives at lds[lds_addr(idx)]
 // twc   : this thread's twiddles (load_twiddles)
 // The caller guarantees every thread of the block calls this (barriers).
-template <typename T, int N, int SIGN, int ES, int PAD>
-__device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* lds, const TwCache<T, N>& twc) {
+template <typename T, int N, int SIGN, int ES, int PAD, bool USE>
+__device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* lds, const TwCache<T, N, USE>& twc) {
   constexpr int P = plan_p(N);
   constexpr int TN = N / P;
   constexpr int NS = plan_nstages(N);
@@ -413,7 +415,7 @@ __device__ __forceinline__ void line_fft(cpx<T> (&v)[plan_p(N)], int j, cpx<T>* 
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           cpx<T> w;
-          if constexpr (TwCache<T, N>::CACHED) {
+          if constexpr (TwCache<T, N, USE>::CACHED) {
             w = twc.w[off + q * (R - 1) + (r - 1)];
           } else {
             constexpr int TSTEP = tw_len(N) / (Ns * R);
