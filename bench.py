@@ -7,7 +7,10 @@ k=0.065, dt=1.0; B instances with seeded ICs (seeds 0..B-1), FP32 state,
 resident in HBM before the timed region.  Also reported in the same line:
 the FP64 single-instance GPU form, the FP64 batched form (the same batch in
 FP64, the one form that stays within 1e-5 relative L2 error of an FP64
-reference at 10000 steps), the FP32-vs-FP64 relative L2 error curve,
+reference at 10000 steps), the FP32 increment form (the same operator with
+only the diffusion increments through the FP32 transforms: within 1e-5 to
+3000 steps, against 2000 for the plain FP32 chain), the FP32-vs-FP64
+relative L2 error curves of both FP32 forms,
 per-step latency percentiles, the roofline of the dominant kernel (HIP events
 on the launch stream) and the CPU oracle timed on this host.  Secondary
 workloads in the same line: the uneven-size batched ETDRK4 plan (1024
@@ -58,12 +61,15 @@ def setup(dtype, B, seeds, n=N):
     return cache.laplacian_symbol, U, V, rows, work
 
 
-def run(U, V, lam, rows, work, steps):
-    from moljax_amd.core.stepping import gs_imex_strang_inplace
-    gs_imex_strang_inplace(U, V, lam, rows, steps, work=work)
+def run(U, V, lam, rows, work, steps, scheme=0):
+    """IMEX-Strang steps in place: scheme 0 the plain chain, 4 the increment
+    form (GS_STRANG_INC; the same operator, only the diffusion increments go
+    through the transforms)."""
+    from moljax_amd.core.stepping import gs_imex_inplace
+    gs_imex_inplace(U, V, lam, rows, steps, scheme, work=work)
 
 
-def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N, keep_first=False):
+def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N, keep_first=False, scheme=0):
     """Seconds for ``steps`` steps of a B-instance batch after ``warmup``
     untimed ones.  With kernel_timing, every launch of the chain is also
     bracketed by HIP events on its stream (a separate pass: the events cost
@@ -71,13 +77,13 @@ def time_form(dtype, B, steps, warmup, kernel_timing=False, n=N, keep_first=Fals
     import torch
     from moljax_amd import _hip
     lam, U, V, rows, work = setup(dtype, B, list(range(B)), n)
-    run(U, V, lam, rows, work, warmup)
+    run(U, V, lam, rows, work, warmup, scheme)
     torch.cuda.synchronize()
     if kernel_timing:
         _hip.lib().mjx_kernel_timing_enable(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run(U, V, lam, rows, work, steps)
+    run(U, V, lam, rows, work, steps, scheme)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kt = kts = None
@@ -111,20 +117,27 @@ def fp64_reference_state(n, steps, seed=0):
 
 def error_curve(checkpoints, seed=0):
     """FP32 vs FP64 (both on the GPU path; the FP64 path is pinned to the
-    CPU oracle by the test suite) relative L2 of (u, v) at each checkpoint."""
+    CPU oracle by the test suite) relative L2 of (u, v) at each checkpoint,
+    for the plain FP32 chain and for the FP32 increment form."""
     import torch
-    out = {}
-    l32, U32, V32, r32, w32 = setup(torch.float32, 1, [seed])
-    l64, U64, V64, r64, w64 = setup(torch.float64, 1, [seed])
+    from moljax_amd.core.stepping import GS_STRANG, GS_STRANG_INC
+    out, out_inc = {}, {}
+    forms = {"f32": (torch.float32, GS_STRANG), "inc": (torch.float32, GS_STRANG_INC),
+             "f64": (torch.float64, GS_STRANG)}
+    st = {k: setup(dt, 1, [seed]) for k, (dt, _) in forms.items()}
     done = 0
     for c in checkpoints:
-        run(U32, V32, l32, r32, w32, c - done)
-        run(U64, V64, l64, r64, w64, c - done)
+        for k, (_, sc) in forms.items():
+            lam, U, V, rows, work = st[k]
+            run(U, V, lam, rows, work, c - done, sc)
         done = c
-        num = torch.sqrt(((U32.double() - U64) ** 2).sum() + ((V32.double() - V64) ** 2).sum())
+        U64, V64 = st["f64"][1], st["f64"][2]
         den = torch.sqrt((U64 ** 2).sum() + (V64 ** 2).sum())
-        out[str(c)] = float(num / den)
-    return out
+        for k, dst in (("f32", out), ("inc", out_inc)):
+            U, V = st[k][1], st[k][2]
+            num = torch.sqrt(((U.double() - U64) ** 2).sum() + ((V.double() - V64) ** 2).sum())
+            dst[str(c)] = float(num / den)
+    return out, out_inc
 
 
 def _cpu_variant(workers, target_s):
@@ -475,6 +488,13 @@ def main():
         fp64b = {"value": b64 * N * N * k64 / tb, "unit": "grid-point-steps/s", "batch": b64, "steps": k64,
                  "ms_per_step": 1e3 * tb / k64, "finite": finb}
 
+    log(f"fp32 increment form batched B={B} steps={K}")
+    tinc, _, fininc, _ = time_form(torch.float32, B, K, W, scheme=4)
+    inc = {"value": B * N * N * K / tinc, "unit": "grid-point-steps/s", "batch": B, "steps": K,
+           "ms_per_step": 1e3 * tinc / K, "finite": fininc,
+           "form": "IMEX-Strang increment form (scheme 4): state kept physical as (u - 1, v), only the diffusion "
+                   "increments (H^k - 1) y through the FP32 transforms; 48 B/GPS moved against 32"}
+
     # roofline of the dominant kernel: the step's compulsory 16 B/GPS (FP32
     # u, v read + written once) is split evenly over its two kernels.
     (ms_gs, n_gs), (ms_col, n_col) = kt[KT_ROW_GS], kt[KT_COL]
@@ -508,10 +528,11 @@ def main():
         else:
             anylen.setdefault("other_sides", []).append(leg)
 
-    curve = None
+    curve = curve_inc = None
     if not args.no_curve:
-        log("error curve")
-        curve = error_curve([100, 1000, 2000, 3000, 5000, 10000])
+        log("error curves")
+        curve, curve_inc = error_curve([100, 1000, 2000, 3000, 5000, 10000])
+        inc["rel_l2_vs_fp64"] = curve_inc
 
     c4 = c3 = c5 = None
     if args.c4_steps > 0:
@@ -546,18 +567,26 @@ def main():
     if curve:
         ok = [int(c) for c, e in curve.items() if e <= 1e-5]
         horizon = max(ok) if ok else 0
-    # which form meets the metric's 1e-5 at config 2's 10 000-step horizon
+    # which form meets the metric's 1e-5 at config 2's 10 000-step horizon:
+    # the fastest of the forms whose measured error there is within it (the
+    # FP64 batch is held to the oracle at 1e-12 step for step)
     target = None
     if curve and fp64b:
-        e10k = curve.get("10000")
-        fp32_ok = e10k is not None and e10k <= 1e-5
-        form = ("fp32_batched", gps32) if fp32_ok else ("fp64_batched", fp64b["value"])
+        e10k, e10k_inc = curve.get("10000"), curve_inc.get("10000")
+        cands = [("fp64_batched", fp64b["value"])]
+        if e10k is not None and e10k <= 1e-5:
+            cands.append(("fp32_batched", gps32))
+        if e10k_inc is not None and e10k_inc <= 1e-5:
+            cands.append(("fp32_increment_batched", inc["value"]))
+        form = max(cands, key=lambda c: c[1])
         target = {"horizon_steps": 10000, "tolerance": 1e-5, "form": form[0], "value": form[1],
                   "unit": "grid-point-steps/s", "ratio_to_fp64_single": form[1] / gps64,
-                  "fp32_batched_rel_l2_at_10000": e10k,
+                  "fp32_batched_rel_l2_at_10000": e10k, "fp32_increment_rel_l2_at_10000": e10k_inc,
                   "fp64_rel_l2_vs_oracle": "<= 1e-12 per 20 steps (test suite, step for step)",
-                  "note": "FP32 state drifts above 1e-5 after ~2000 steps (the dynamics amplify rounding); the "
-                          "FP64 batch carries the 1e-5 target to 10 000 steps"}
+                  "note": "the pattern-forming dynamics amplify every rounding after ~2000 steps: the plain FP32 "
+                          "chain leaves 1e-5 between 2000 and 3000 steps, the FP32 increment form after 3000 "
+                          "(1.2e-5..5.5e-5 at 10 000 over five seeds); the FP64 batch carries the 1e-5 target "
+                          "to 10 000 steps"}
     line = {
         "metric": METRIC,
         "value": gps32,
@@ -578,6 +607,8 @@ def main():
                                  "ms_per_step": 1e3 * t64 / args.fp64_steps},
         "speedup_fp32_batched_vs_fp64_single": gps32 / gps64,
         "fp64_batched": fp64b,
+        "fp32_increment_batched": inc,
+        "speedup_fp32_increment_vs_fp64_single": inc["value"] / gps64,
         "speedup_fp64_batched_vs_fp64_single": fp64b["value"] / gps64 if fp64b else None,
         "form_within_1e-5_at_10000_steps": target,
         "rel_l2_fp32_vs_fp64": curve,

@@ -144,8 +144,13 @@ int mjx_gs_imex_strang_noalloc(int dtype, int B, int ny, int nx, void* u, void* 
  * 2 IMEX-SSPRK2  y_new = y/2 + E(E(y))/2 with E the IMEX-Euler map
  *   (stepping.py:438-495), 3 the splitting of the Gray-Scott benchmark
  *   (benchmarks/benchmark_gray_scott.py:268-289): y += dt/2 R(y);
- *   y^ <- e^{dt D lam} y^; y += dt/2 R(y).  inst beta columns: dt/2*D for
- * Strang, dt*D for Euler, SSPRK2 and the benchmark splitting.  Replaces
+ *   y^ <- e^{dt D lam} y^; y += dt/2 R(y), 4 Strang in increment form: the
+ *   operator of scheme 0 with the state kept physical in (u, v) and each
+ *   diffusion solve applied as y + (H^k - 1)(y - (1, 0)), so that only the
+ *   increment passes through the transforms (their FP32 rounding then
+ *   scales with the increment; power-of-two sides only, MJX_E_SIZE
+ *   otherwise).  inst beta columns: dt/2*D for Strang (0, 4), dt*D for
+ *   Euler, SSPRK2 and the benchmark splitting.  Replaces
  * moljax/core/stepping.py:317-495 for the model of
  * moljax/core/model.py:529-584.  work as mjx_gs_imex_strang. */
 int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, long long pitch,

@@ -17,15 +17,27 @@
 //           half-step, exact diffusion e^{dt D lam}, reaction Euler half-step:
 //           fwd_pack -> row(half) -> col(E) -> [row(half, half) -> col(E)]*(n-1)
 //           -> inv_unpack(closing half-step)
-// The instance table's beta columns carry dt/2 D (STRANG) or dt D (EULER,
-// SSPRK2, BENCH_STRANG); its dt column is the step size.
+//   STRANG_INC  the STRANG operator in increment form, in place: the state
+//           stays physical, held as its deviation (u - 1, v) from the steady
+//           state during the chain, and every diffusion solve is
+//           y + (H^k - 1) y, with only the increment going through the
+//           transforms:
+//           row_inc(first) -> col(H - 1) -> [row_inc(update, Heun,
+//           transform) -> col(H^2 - 1)]*(n-1) -> row_inc(update, Heun,
+//           transform) -> col(H - 1) -> row_inc(update)
+//           (power-of-two sides; 48 B per point and step in FP32 against 32
+//           for STRANG, with transform rounding that scales with the
+//           increment instead of the state; run by gs_imex_inc_t in
+//           spectral.hip, whose hot launches take the per-kernel events)
+// The instance table's beta columns carry dt/2 D (STRANG, STRANG_INC) or
+// dt D (EULER, SSPRK2, BENCH_STRANG); its dt column is the step size.
 #pragma once
 #include "mjx_common.h"
 #include "spectral_api.h"
 
 namespace mjx {
 
-enum { GS_STRANG = 0, GS_EULER = 1, GS_SSPRK2 = 2, GS_BENCH_STRANG = 3 };
+enum { GS_STRANG = 0, GS_EULER = 1, GS_SSPRK2 = 2, GS_BENCH_STRANG = 3, GS_STRANG_INC = 4 };
 
 template <typename T>
 int gs_chain_t(int scheme, int B, int ny, int nx, const T* u_in, const T* v_in, long long in_pitch,

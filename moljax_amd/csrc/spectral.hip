@@ -109,6 +109,26 @@ static int gs_imex_t(int B, int ny, int nx, T* u, T* v, long long pitch, long lo
   return spec_row_inv_unpack(nx, B, ny, W, u, v, pitch, istride, ghosts, st);
 }
 
+// The same operator in increment form (GS_STRANG_INC, gs_chain.h), in place;
+// the step's row pass and column pass take the per-kernel events.
+template <typename T>
+static int gs_imex_inc_t(int B, int ny, int nx, T* u, T* v, long long pitch, long long istride, int ghosts,
+                         const T* lamx, const T* lamy, const T* lam2d, const InstParams* inst, int n_steps,
+                         cpx<T>* W, hipStream_t st) {
+  int rc = spec_row_gs_inc(nx, B, ny, W, u, v, pitch, istride, inst, INC_FIRST, 0, st);
+  if (!rc) rc = spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, NREP_INC1, st);
+  for (int s = 0; s < n_steps && !rc; ++s) {
+    rc = timed_launch(KT_ROW_GS, st,
+                      [&] { return spec_row_gs_inc(nx, B, ny, W, u, v, pitch, istride, inst, INC_MID, 0, st); });
+    const int nrep = s == n_steps - 1 ? NREP_INC1 : NREP_INC2;
+    if (!rc)
+      rc = timed_launch(KT_COL_APPLY, st,
+                        [&] { return spec_col_apply(ny, B, nx, W, lamx, lamy, lam2d, inst, nrep, st); });
+  }
+  if (rc) return rc;
+  return spec_row_gs_inc(nx, B, ny, W, u, v, pitch, istride, inst, INC_LAST, ghosts, st);
+}
+
 // Strang chain whose instances take their own number of steps (instance row
 // aux, 0..max_steps): the masked column and row passes leave an instance
 // untouched once its steps are done, so each instance's arithmetic is that of
@@ -282,10 +302,13 @@ int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, 
   if (scheme == GS_STRANG)
     return mjx_gs_imex_strang(dtype, B, ny, nx, u, v, pitch, istride, fill_ghosts, lamx, lamy, lam2d, inst,
                               n_steps, work, stream);
-  if (scheme != GS_EULER && scheme != GS_SSPRK2 && scheme != GS_BENCH_STRANG)
-    return fail(MJX_E_ARG, "scheme must be 0 (Strang), 1 (Euler), 2 (SSPRK2) or 3 (benchmark Strang)");
+  if (scheme != GS_EULER && scheme != GS_SSPRK2 && scheme != GS_BENCH_STRANG && scheme != GS_STRANG_INC)
+    return fail(MJX_E_ARG,
+                "scheme must be 0 (Strang), 1 (Euler), 2 (SSPRK2), 3 (benchmark Strang) or 4 (Strang, increment form)");
   int rc = check_grid(dtype, B, ny, nx);
   if (rc) return rc;
+  if (scheme == GS_STRANG_INC && (!is_pow2_len(nx) || !is_pow2_len(ny)))
+    return fail(MJX_E_SIZE, "increment-form IMEX-Strang: nx and ny must be powers of two in [16, 4096]");
   if (!u || !v || !inst || !work) return fail(MJX_E_ARG, "null pointer argument");
   if (!lam2d && (!lamx || !lamy)) return fail(MJX_E_ARG, "need lamx and lamy, or lam2d");
   if (n_steps < 0) return fail(MJX_E_ARG, "n_steps must be >= 0");
@@ -296,6 +319,14 @@ int mjx_gs_imex(int dtype, int scheme, int B, int ny, int nx, void* u, void* v, 
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   const InstParams* ip = reinterpret_cast<const InstParams*>(inst);
+  if (scheme == GS_STRANG_INC) {
+    if (dtype == MJX_F32)
+      return gs_imex_inc_t<float>(B, ny, nx, (float*)u, (float*)v, pitch, istride, fill_ghosts, (const float*)lamx,
+                                  (const float*)lamy, (const float*)lam2d, ip, n_steps, (cpx<float>*)work, st);
+    return gs_imex_inc_t<double>(B, ny, nx, (double*)u, (double*)v, pitch, istride, fill_ghosts,
+                                 (const double*)lamx, (const double*)lamy, (const double*)lam2d, ip, n_steps,
+                                 (cpx<double>*)work, st);
+  }
   if (dtype == MJX_F32)
     return gs_chain_t<float>(scheme, B, ny, nx, (float*)u, (float*)v, pitch, istride, (float*)u, (float*)v, pitch,
                              istride, fill_ghosts, (const float*)lamx, (const float*)lamy, (const float*)lam2d, ip,

@@ -37,6 +37,12 @@ int spec_row_gs(int nx, int B, int ny, cpx<T>* W, const InstParams* ip, hipStrea
   if (spec_generic(nx, ny)) return czt_row_gs(nx, B, ny, W, ip, st, rxn, mstep);
   MJX_BY_SET(nx, row_gs(nx, B, ny, W, ip, st, rxn, mstep))
 }
+int spec_row_gs_inc(int nx, int B, int ny, cpx<T>* W, T* a, T* b, long long pitch, long long istride,
+                    const InstParams* ip, int mode, int ghosts, hipStream_t st) {
+  if (!is_pow2_len(nx) || !is_pow2_len(ny))
+    return fail(MJX_E_SIZE, "increment-form IMEX-Strang: nx and ny must be powers of two in [16, 4096]");
+  return Launch<T, SET_POW2>::row_gs_inc(nx, B, ny, W, a, b, pitch, istride, ip, mode, ghosts, st);
+}
 int spec_col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
                    const InstParams* ip, int nrep, hipStream_t st, int mstep) {
   if (spec_generic(nx, ny)) return czt_col_apply(ny, B, nx, W, lamx, lamy, lam2d, ip, nrep, st, mstep);

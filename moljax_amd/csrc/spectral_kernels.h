@@ -92,13 +92,21 @@ __device__ __forceinline__ T recip(T d) {
 // s * ((1 + alpha*lam) / (1 - beta*lam))^nrep with |1 - beta*lam| clamped
 // below at 1e-14 (moljax/core/fft_solvers.py:280-283).  NREP > 0 fixes the
 // power at compile time; NREP == 0 takes the runtime nrep, where nrep = -1
-// selects s * alpha * lam and nrep = -2 selects s * exp(beta * lam);
+// selects s * alpha * lam, nrep = -2 selects s * exp(beta * lam) and
+// NREP_INC1 / NREP_INC2 select the increments s * (g - 1) / s * (g^2 - 1) of
+// g = (1 + alpha*lam) / (1 - beta*lam);
 // NREP == -1 is the s * alpha * lam form fixed at compile time.
 template <typename T, int NREP>
 __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
   if constexpr (NREP == -1) return alpha * lam * s;  // the matvec form, without the runtime dispatch
   T den = (T)1 - beta * lam;
   if (fabs(den) < (T)1e-14) den = (T)1e-14;
+  if constexpr (NREP == NREP_INC1 || NREP == NREP_INC2) {  // the increment forms at compile time
+    T g1;
+    if constexpr (std::is_same<T, double>::value) g1 = ((alpha + beta) * lam) / den;
+    else g1 = ((alpha + beta) * lam) * recip(den);
+    return (NREP == NREP_INC1 ? g1 : g1 * (g1 + (T)2)) * s;
+  }
   const T num = (T)1 + alpha * lam;
   T g;
   if constexpr (std::is_same<T, double>::value) g = num / den;
@@ -110,6 +118,14 @@ __device__ __forceinline__ T spec_mult(T lam, T alpha, T beta, int nrep, T s) {
   } else {
     if (nrep == -1) return alpha * lam * s;
     if (nrep == -2) return exp(beta * lam) * s;
+    if (nrep == NREP_INC1 || nrep == NREP_INC2) {
+      // increments g - 1 = (alpha + beta) lam / den and g^2 - 1 = (g - 1)(g - 1 + 2),
+      // formed without the cancellation of g - 1 near lam = 0
+      T g1;
+      if constexpr (std::is_same<T, double>::value) g1 = ((alpha + beta) * lam) / den;
+      else g1 = ((alpha + beta) * lam) * recip(den);
+      return (nrep == NREP_INC1 ? g1 : g1 * (g1 + (T)2)) * s;
+    }
     // the fixed-power forms, so a runtime power of 1 or 2 rounds exactly as
     // the NREP = 1 / 2 instantiations do
     if (nrep == 1) return g * s;
@@ -349,6 +365,142 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
   }
 }
 
+// Increment-form IMEX-Strang row pass.  The state stays physical in the
+// caller's arrays, held during the chain as (w, v) = (u - 1, v), its
+// deviation from (1, 0), the homogeneous steady state of Gray-Scott; W
+// carries the packed x-spectra of a diffusion increment:
+//   INC_FIRST  w <- u - 1 (written back); W <- x-FFT of (w, v)
+//   INC_MID    (w, v) += x-iFFT(W); Heun reaction step of R written for w
+//              (u = w + 1); W <- x-FFT of (w, v)
+//   INC_LAST   (w, v) += x-iFFT(W); u <- w + 1; optional periodic ghost ring
+// Between row passes the column pass applies s (g - 1) or s (g^2 - 1)
+// (NREP_INC1 / NREP_INC2), so (w, v) + x-iFFT(W) is H (w, v) or H^2 (w, v):
+// the operator of mjx_gs_imex_strang (H leaves constants unchanged).  Only
+// the increment passes through the transforms, so their rounding scales
+// with it instead of with the state, and the stored deviation rounds
+// relative to |u - 1| instead of |u|.
+template <typename T, int NX, int TR, int MODE>
+__global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs_inc(cpx<T>* __restrict__ W, T* __restrict__ a,
+                                                                      T* __restrict__ b, long long pitch,
+                                                                      long long istride,
+                                                                      const InstParams* __restrict__ ip, int ny,
+                                                                      int rows_per_inst_tiles, int ghosts,
+                                                                      const cpx<T>* __restrict__ tw) {
+  using C = RowCfg<NX, TR>;
+  __shared__ cpx<T> lds[C::TR * C::LP];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int inst = w / rows_per_inst_tiles;
+  const int tile = w - inst * rows_per_inst_tiles;
+  const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
+  const int y = tile * C::TR + rl;
+  const bool live = y < ny;  // rows past ny: transforms only
+  cpx<T>* lrow = lds + rl * C::LP;
+  cpx<T>* wrow = W + ((long long)inst * ny + (live ? y : 0)) * NX;
+  T* ra = a + inst * istride + (long long)(live ? y : 0) * pitch;
+  T* rb = b + inst * istride + (long long)(live ? y : 0) * pitch;
+  TwCache<T, NX> twc;
+  load_twiddles(twc, j, tw);
+  T su[C::P], sv[C::P];  // (w, v)
+#pragma unroll
+  for (int m = 0; m < C::P; ++m) {
+    const int x = j + C::TN * m;
+    su[m] = live ? (MODE == INC_FIRST ? ra[x] - (T)1 : ra[x]) : (T)0;
+    sv[m] = live ? rb[x] : (T)0;
+  }
+  if constexpr (MODE == INC_FIRST) {
+    if (live) {
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) ra[j + C::TN * m] = su[m];
+    }
+  }
+  cpx<T> v[C::P];
+  if constexpr (MODE != INC_FIRST) {
+    load_unpack<T, NX, TR>(wrow, live, lrow, j, v);
+    line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      su[m] += v[m].re;
+      sv[m] += v[m].im;
+    }
+  }
+  if constexpr (MODE == INC_MID) {
+    const T F = (T)ip[inst].F, kk = (T)ip[inst].k, dt = (T)ip[inst].dt;
+    const T hdt = (T)(0.5 * ip[inst].dt);
+    // R of moljax/core/operators.py:655-667 with u = w + 1:
+    // (-u v^2 - F w, u v^2 - (F + k) v)
+    auto react_w = [&](T w_, T v_, T& rw, T& rv) {
+      const T uvv = (w_ + (T)1) * v_ * v_;
+      rw = -uvv - F * w_;
+      rv = uvv - (F + kk) * v_;
+    };
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      T r1u, r1v, r2u, r2v;
+      react_w(su[m], sv[m], r1u, r1v);
+      const T ut = su[m] + dt * r1u, vt = sv[m] + dt * r1v;
+      react_w(ut, vt, r2u, r2v);
+      su[m] = su[m] + hdt * (r1u + r2u);
+      sv[m] = sv[m] + hdt * (r1v + r2v);
+    }
+  }
+  if constexpr (MODE == INC_LAST) {
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) su[m] += (T)1;
+  }
+  if constexpr (MODE != INC_FIRST) {
+    if (live) {
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) {
+        const int x = j + C::TN * m;
+        ra[x] = su[m];
+        rb[x] = sv[m];
+      }
+    }
+  }
+  if constexpr (MODE == INC_LAST) {
+    if (!live || !ghosts) return;
+    // periodic ghost ring as k_row_inv_unpack writes it (x ghosts with the
+    // row, first/last interior rows also into the opposite ghost row)
+    const long long offs[3] = {0, (y == 0) ? (long long)ny * pitch : 0, (y == ny - 1) ? -(long long)ny * pitch : 0};
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      if (o > 0 && offs[o] == 0) continue;
+      T* qa = ra + offs[o];
+      T* qb = rb + offs[o];
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) {
+        const int x = j + C::TN * m;
+        if (o > 0) {
+          qa[x] = su[m];
+          qb[x] = sv[m];
+        }
+        if (x == 0) {
+          qa[NX] = su[m];
+          qb[NX] = sv[m];
+        }
+        if (x == NX - 1) {
+          qa[-1] = su[m];
+          qb[-1] = sv[m];
+        }
+      }
+    }
+    return;
+  } else {
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) v[m] = mk(su[m], sv[m]);
+    line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int p = j + C::TN * m;
+      wrow[p] = pack_at<T, NX>(lrow, p);
+    }
+  }
+}
+
 // ----------------------------------------------------------- column pass
 template <int NY, int TC_>
 struct ColCfg {
@@ -546,6 +698,22 @@ struct RowLaunch {
                          ny, tiles, twiddles<T>() + tw_base(N), mstep, 0);
     return check_launch("row_gs");
   }
+  static int gs_inc(int B, int ny, cpx<T>* W, T* a, T* b, long long pitch, long long istride, const InstParams* ip,
+                    int mode, int ghosts, hipStream_t st) {
+    const int tiles = (ny + L - 1) / L;
+    const dim3 g(tiles * B), bt(RowCfg<N, L>::BT);
+    const cpx<T>* tw = twiddles<T>() + tw_base(N);
+    if (mode == INC_FIRST)
+      hipLaunchKernelGGL((k_row_gs_inc<T, N, L, INC_FIRST>), g, bt, 0, st, W, a, b, pitch, istride, ip, ny, tiles,
+                         ghosts, tw);
+    else if (mode == INC_MID)
+      hipLaunchKernelGGL((k_row_gs_inc<T, N, L, INC_MID>), g, bt, 0, st, W, a, b, pitch, istride, ip, ny, tiles,
+                         ghosts, tw);
+    else
+      hipLaunchKernelGGL((k_row_gs_inc<T, N, L, INC_LAST>), g, bt, 0, st, W, a, b, pitch, istride, ip, ny, tiles,
+                         ghosts, tw);
+    return check_launch("row_gs_inc");
+  }
 };
 
 template <typename T, int N, int L>
@@ -556,6 +724,14 @@ struct ColLaunch {
     const int tiles = (nx + L - 1) / L;
     hipLaunchKernelGGL((k_col_apply<T, N, L, LAM2D, NREP>), dim3(tiles * B), dim3(ColCfg<N, L>::BT), 0, st, W,
                        nx, lamx, lamy, lam2d, ip, nrep, s, tiles, twiddles<T>() + tw_base(N), mstep);
+  }
+  // compile-time increment forms of the increment-form Strang chain, which
+  // runs on power-of-two lengths only
+  template <int NREP>
+  static void go_pow2(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
+                      int nrep, T s, int mstep, hipStream_t st) {
+    if constexpr (pow2_len(N)) go<false, NREP>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+    else go<false, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
   }
   static int apply(int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d, const InstParams* ip,
                    int nrep, T s, int mstep, hipStream_t st) {
@@ -569,6 +745,8 @@ struct ColLaunch {
       if (fixed == 1) go<false, 1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
       else if (fixed == 2) go<false, 2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
       else if (fixed == -1) go<false, -1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else if (pow2_len(N) && fixed == NREP_INC1) go_pow2<NREP_INC1>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
+      else if (pow2_len(N) && fixed == NREP_INC2) go_pow2<NREP_INC2>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
       else go<false, 0>(B, nx, W, lamx, lamy, lam2d, ip, nrep, s, mstep, st);
     }
     return check_launch("col_apply");
@@ -651,6 +829,16 @@ struct Launch {
     };
   };
   template <int N>
+  struct GsInc {
+    template <int L>
+    struct F {
+      static int run(int B, int ny, cpx<T>* W, T* a, T* b, long long pitch, long long istride,
+                     const InstParams* ip, int mode, int ghosts, hipStream_t st) {
+        return RowLaunch<T, N, L>::gs_inc(B, ny, W, a, b, pitch, istride, ip, mode, ghosts, st);
+      }
+    };
+  };
+  template <int N>
   struct Col {
     template <int L>
     struct F {
@@ -707,6 +895,24 @@ struct Launch {
         return fail(MJX_E_SIZE, "unsupported nx " + std::to_string(nx));
     }
     return fail(MJX_E_SIZE, "nx " + std::to_string(nx) + " belongs to another launcher set");
+  }
+  // The increment-form row pass is built for the power-of-two lengths only.
+  static int row_gs_inc(int nx, int B, int ny, cpx<T>* W, T* a, T* b, long long pitch, long long istride,
+                        const InstParams* ip, int mode, int ghosts, hipStream_t st) {
+    if constexpr (SET != SET_POW2) {
+      return fail(MJX_E_SIZE, "increment-form IMEX-Strang: nx must be a power of two in [16, 4096]");
+    } else {
+      switch (nx) {
+#define X(N)                                                                                                   \
+  case N:                                                                                                      \
+    return with_lines<T, N, GsInc<N>::template F>(pick_lines(fill_lines(lines_for<T, N>(TK_ROW), (long long)B * ny), ny), \
+                                                  B, ny, W, a, b, pitch, istride, ip, mode, ghosts, st);
+        X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#undef X
+        default:
+          return fail(MJX_E_SIZE, "increment-form IMEX-Strang: nx must be a power of two in [16, 4096]");
+      }
+    }
   }
   static int col_apply(int ny, int B, int nx, cpx<T>* W, const T* lamx, const T* lamy, const T* lam2d,
                        const InstParams* ip, int nrep, hipStream_t st, int mstep = -1) {
