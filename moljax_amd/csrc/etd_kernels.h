@@ -15,9 +15,9 @@
 //   row_nl(P2 -> P2)        N_a
 //   k_col_rk4 stage 2       b = E2 u + Q N_a                         -> P3
 //   row_nl(P3 -> P3)        N_b
-//   k_col_rk4 stage 3       c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)  -> P2
-//                           Fh = E u + f1 N_n + f2 (N_a + N_b)       -> P1
-//   row_nl(P2 -> P2)        N_c
+//   k_col_rk4 stage 3F      Fh = E u + f1 N_n + f2 (N_a + N_b)       -> P2
+//   k_col_rk4 stage 3C      c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)  -> P1
+//   row_nl(P1 -> P1)        N_c
 //   k_col_rk4 stage 4       u_new = Fh + f3 N_c                      -> Ws
 // Every packed x-spectral buffer holds the x-DFT divided by nx (the state is
 // packed with scale 1/nx and N is scaled by 1/nx before its x-FFT), so all
@@ -227,22 +227,24 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
 // chain.  Every operand stays a packed x-spectral plane (ny rows of nx
 // complex values, rows 128-byte aligned); a stage y-transforms each plane it
 // needs again instead of keeping separated y-spectra, and the final
-// combination is accumulated once, in stage 3, as the y-spectrum Fh in the
-// packed layout (Fh_a + i Fh_b in the special columns, as the y-FFT of a
-// packed column would be):
-//   stage 1  Ws = u, P1 = N_n                 -> P2 = a = E2 u + Q N_n
-//   stage 2  Ws, P2 = N_a                      -> P3 = b = E2 u + Q N_a
-//   stage 3  Ws, P1, P2 = N_a, P3 = N_b        -> P2 = c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)
-//                                                 P1 = Fh = E u + f1 N_n + f2 (N_a + N_b)
-//   stage 4  P1 = Fh, P2 = N_c                 -> Ws = Fh + f3 N_c
-// (benchmarks/benchmark_gray_scott.py:369-417; the same combinations as
-// etd_stage, with the final sum gathered in stage 3).  Stage 3 writes P2 and
-// P1 over the inputs it read: the workgroup owns its columns and reads them
-// all before its first store.  Each stage is linear in its inputs, so in a
-// special column (packed kx = 0 and nx/2 of one field) an input Z with
-// per-field coefficients cA, cB contributes (cA+cB)/2 Z(ky) + (cA-cB)/2
-// conj Z(-ky).
-enum { RK_S1 = 1, RK_S2 = 2, RK_S3 = 3, RK_S4 = 4 };
+// combination Fh is formed once, as a y-spectrum in the packed layout
+// (Fh_a + i Fh_b in the special columns, as the y-FFT of a packed column
+// would be):
+//   stage 1   Ws = u, P1 = N_n                -> P2 = a = E2 u + Q N_n
+//   stage 2   Ws, P2 = N_a                     -> P3 = b = E2 u + Q N_a
+//   stage 3F  Ws, P1, P2 = N_a, P3 = N_b       -> P2 = Fh = E u + f1 N_n + f2 (N_a + N_b)
+//             (N_a + N_b summed before one transform: the y-FFT is linear)
+//   stage 3C  Ws, P1, P3                       -> P1 = c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)
+//   stage 4   P2 = Fh, P1 = N_c                -> Ws = Fh + f3 N_c
+// (benchmarks/benchmark_gray_scott.py:369-417; the combinations of
+// etd_stage, with the final sum gathered once).  Stages 3F and 3C write over
+// an input they read: the workgroup owns its columns and reads them all
+// before its first store.  (One fused stage 3 held c, Fh and a transform at
+// once and ran 3-5x slower than the two.)  Each stage is linear in its
+// inputs, so in a special column (packed kx = 0 and nx/2 of one field) an
+// input Z with per-field coefficients cA, cB contributes
+// (cA+cB)/2 Z(ky) + (cA-cB)/2 conj Z(-ky).
+enum { RK_S1 = 1, RK_S2 = 2, RK_S3F = 3, RK_S3C = 5, RK_S4 = 4 };
 enum { CF_E = 0, CF_E2 = 1, CF_Q = 2, CF_F1 = 3, CF_F2 = 4, CF_F3 = 5 };
 
 template <typename T, int NY, int TC_, int STAGE>
@@ -258,7 +260,6 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   constexpr unsigned RSTEP = TN * NX;       // plane offset between a thread's points m and m + 1
   constexpr unsigned CSTEP = TN * SW;       // the same in a coefficient table
   __shared__ cpx<T> lds[NY * TC];
-  __shared__ cpx<T> lds_f[STAGE == RK_S3 ? NY * TC : 1];
   // Workgroups run tile-major within groups of kgrp instances (group,
   // tile, instance), so that the coefficient slab of a tile is read from
   // L2 by the group's other instances; the last group may be smaller.
@@ -343,44 +344,44 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
     add(acc, zb, false, Q);
     done();
     out = STAGE == RK_S1 ? p2 : p3;
-  } else if constexpr (STAGE == RK_S3) {
-    // Fh accumulates in this thread's own LDS slots (no sharing, no
-    // barriers), which leaves the registers to the transforms and to c.
-    cpx<T>* fh = lds_f + j * TC + c;
-    auto addf = [&](const cpx<T> (&zz)[P], bool first, auto mult) {
-      cpx<T> t[P];
-      add(t, zz, true, mult);
-#pragma unroll
-      for (int m = 0; m < P; ++m) {
-        const cpx<T> o = first ? mk((T)0, (T)0) : fh[m * TN * TC];
-        fh[m * TN * TC] = mk(o.re + t[m].re, o.im + t[m].im);
-      }
-    };
-    ld(p2, za);  // N_a
-    ld(p3, zb);  // N_b
-    xf(za);
-    addf(za, true, [&](unsigned i) { return coef[i + CF_F2 * TSTRIDE]; });
-    done();
+  } else if constexpr (STAGE == RK_S3F) {
     ld(ws, za);  // u
-    xf(zb);
-    add(acc, zb, true, [&](unsigned i) { return (T)2 * Q(i); });
-    addf(zb, false, [&](unsigned i) { return coef[i + CF_F2 * TSTRIDE]; });
-    done();
     ld(p1, zb);  // N_n
     xf(za);
-    add(acc, za, false, [&](unsigned i) { const T e2 = E2(i); return e2 * e2; });
-    addf(za, false, E);
+    add(acc, za, true, E);
     done();
+    ld(p2, za);  // N_a + N_b
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const cpx<T> nb = p3[toff + m * RSTEP];
+      za[m] = mk(za[m].re + nb.re, za[m].im + nb.im);
+    }
     xf(zb);
-    add(acc, zb, false, [&](unsigned i) { const T q = Q(i); return E2(i) * q - q; });
-    addf(zb, false, [&](unsigned i) { return coef[i + CF_F1 * TSTRIDE]; });
+    add(acc, zb, false, [&](unsigned i) { return coef[i + CF_F1 * TSTRIDE]; });
+    done();
+    xf(za);
+    add(acc, za, false, [&](unsigned i) { return coef[i + CF_F2 * TSTRIDE]; });
     done();
 #pragma unroll
-    for (int m = 0; m < P; ++m) p1[toff + m * RSTEP] = fh[m * TN * TC];
-    out = p2;
+    for (int m = 0; m < P; ++m) p2[toff + m * RSTEP] = acc[m];  // y-spectral, unscaled
+    return;
+  } else if constexpr (STAGE == RK_S3C) {
+    ld(ws, za);  // u
+    ld(p1, zb);  // N_n
+    xf(za);
+    add(acc, za, true, [&](unsigned i) { const T e2 = E2(i); return e2 * e2; });
+    done();
+    ld(p3, za);  // N_b
+    xf(zb);
+    add(acc, zb, false, [&](unsigned i) { const T q = Q(i); return E2(i) * q - q; });
+    done();
+    xf(za);
+    add(acc, za, false, [&](unsigned i) { return (T)2 * Q(i); });
+    done();
+    out = p1;
   } else {
-    ld(p2, za);  // N_c
-    ld(p1, zb);  // Fh
+    ld(p1, za);  // N_c
+    ld(p2, zb);  // Fh
     xf(za);
     add(acc, za, true, [&](unsigned i) { return coef[i + CF_F3 * TSTRIDE]; });
     done();
@@ -557,23 +558,10 @@ struct EtdTiles {
       if (L == CM / 8) return EtdLaunch<T>::template col<N, CM / 8, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
     return EtdLaunch<T>::template col<N, DC, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
   }
-  // Stage 3 holds c and a transform in registers and Fh in a second LDS
-  // array: its tile is capped at 256 threads (512 with 8 points per thread),
-  // so that a wave may hold 256 VGPRs, and at 64 KiB of LDS for both arrays.
-  static constexpr int S3T = ((plan_p(N) <= 8 ? 512 : 256) / TN) > 0 ? ((plan_p(N) <= 8 ? 512 : 256) / TN) : 1;
-  static constexpr int S3L = 32768 / (N * (int)sizeof(cpx<T>)) > 0 ? 32768 / (N * (int)sizeof(cpx<T>)) : 1;
-  static constexpr int S3CAP = S3T < S3L ? S3T : S3L;
   template <int STAGE>
   static int rk4(int L, int B, int nx, cpx<T>* Ws, cpx<T>* P1, cpx<T>* P2, cpx<T>* P3, const T* coef,
                  hipStream_t st) {
-    if constexpr (STAGE == RK_S3) {
-      if (L > S3CAP) L = S3CAP;
-      if constexpr (CM <= S3CAP) {
-        if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
-      }
-    } else {
-      if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
-    }
+    if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
     if constexpr (CM >= 2)
       if (L == CM / 2) return EtdLaunch<T>::template rk4<N, CM / 2, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
     if constexpr (CM >= 4)
@@ -612,7 +600,7 @@ inline EtdPlanes<T> etd_planes(cpx<T>* work, int B, int n) {
 
 // n_steps ETDRK4 steps on the packed state Ws of B square N x N instances,
 // without the pack/unpack: four row passes (N of the stage value, in place
-// from the second stage on) and the four k_col_rk4 stages.
+// from the second stage on) and the five k_col_rk4 stages.
 template <typename T, int N>
 int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParams* ip, int n_steps,
                     hipStream_t st) {
@@ -627,8 +615,9 @@ int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParam
     if (!rc) rc = E::row_nl(LR, B, N, p.P2, p.P2, ip, st);
     if (!rc) rc = E::template rk4<RK_S2>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
     if (!rc) rc = E::row_nl(LR, B, N, p.P3, p.P3, ip, st);
-    if (!rc) rc = E::template rk4<RK_S3>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
-    if (!rc) rc = E::row_nl(LR, B, N, p.P2, p.P2, ip, st);
+    if (!rc) rc = E::template rk4<RK_S3F>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    if (!rc) rc = E::template rk4<RK_S3C>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    if (!rc) rc = E::row_nl(LR, B, N, p.P1, p.P1, ip, st);
     if (!rc) rc = E::template rk4<RK_S4>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
   }
   return rc;
