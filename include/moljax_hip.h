@@ -304,6 +304,14 @@ int mjx_reduce2d(int dtype, int op, int B, int ny, int nx, const void* a, long l
                  const void* b, long long b_pitch, long long b_istride, double atol, double rtol, double* out,
                  int accumulate, double* work, void* stream);
 
+/* Batch total of the per-instance results of mjx_reduce2d (B doubles on
+ * the device): the sum (the maximum for MJX_RED_MAXABS) in a fixed order,
+ * then its square root when take_sqrt != 0, into total[0].  With the
+ * per-field calls of mjx_reduce2d it gives tree_norm2 / tree_vdot /
+ * tree_norm_inf of a whole batch in one more launch.
+ * Replaces: the final sum and sqrt of moljax/core/state.py:115-189. */
+int mjx_reduce_finish(int op, int B, const double* per_inst, int take_sqrt, double* total, void* stream);
+
 /* IMEX-Strang in place where instance b takes n_b = inst[8b + 7] steps
  * (0 <= n_b <= max_steps) in one batched chain: the column and row passes of
  * step s leave an instance with s >= n_b untouched, so every instance's

@@ -43,6 +43,7 @@ PROTOTYPES = {
                                   _i, _ll, ctypes.POINTER(ctypes.c_longlong), _p, _p]),
     "mjx_reduce_workspace_elems": (_ll, [_i]),
     "mjx_reduce2d": (_i, [_i, _i, _i, _i, _i, _p, _ll, _ll, _p, _ll, _ll, _d, _d, _p, _i, _p, _p]),
+    "mjx_reduce_finish": (_i, [_i, _i, _p, _i, _p, _p]),
     "mjx_batch_create": (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
     "mjx_batch_create_mixed": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
     "mjx_batch_run": (_i, [_p, _i, _i, _p]),

@@ -5,7 +5,7 @@ run the native a x + b y + c kernel and allocate new arrays (the reference's
 functional semantics).  Reductions used for accept/reject decisions
 accumulate in float64.
 """
-from typing import Any, Callable, Dict, Tuple
+from typing import Optional, Any, Callable, Dict, Tuple
 
 import torch
 
@@ -79,11 +79,30 @@ def _rview(f: torch.Tensor):
     return view2d(f)
 
 
-def _reduce(op: int, fields_a, fields_b=None, atol: float = 0.0, rtol: float = 0.0) -> torch.Tensor:
+_RED_WORK = {}
+
+
+def _red_work(lib, B: int, device, st: int) -> torch.Tensor:
+    """The reduction partials buffer for batch B on a device and stream st,
+    kept between calls (work on one stream is ordered; every call rewrites
+    it)."""
+    key = (B, device.index, st)
+    w = _RED_WORK.get(key)
+    if w is None:
+        w = torch.empty(lib.mjx_reduce_workspace_elems(B), dtype=torch.float64, device=device)
+        _RED_WORK[key] = w
+    return w
+
+
+def _reduce(op: int, fields_a, fields_b=None, atol: float = 0.0, rtol: float = 0.0, st: Optional[int] = None
+            ) -> torch.Tensor:
     """Per-instance FP64 reduction over several fields (mjx_reduce2d, fixed
-    order, bitwise reproducible); returns a (B,) float64 device tensor."""
+    order, bitwise reproducible) on stream st (the current stream when None);
+    returns a (B,) float64 device tensor."""
     from .. import _hip
     lib = _hip.lib()
+    if st is None:
+        st = _hip.stream_ptr()
     out = work = None
     for i, fa in enumerate(fields_a):
         ta, ap, ai, B = _rview(fa)
@@ -95,12 +114,12 @@ def _reduce(op: int, fields_a, fields_b=None, atol: float = 0.0, rtol: float = 0
                 raise ValueError("fields of a and b must have the same shape")
         if out is None:
             out = torch.empty(B, dtype=torch.float64, device=ta.device)  # written by the first field
-            work = torch.empty(lib.mjx_reduce_workspace_elems(B), dtype=torch.float64, device=ta.device)
+            work = _red_work(lib, B, ta.device, st)
         elif out.numel() != B:
             raise ValueError("all fields must have the same batch shape")
         rc = lib.mjx_reduce2d(_hip.dtype_code(ta.dtype), op, B, ta.shape[-2], ta.shape[-1], _hip.ptr(ta), ap, ai,
                               _hip.ptr(tb), bp, bi, float(atol), float(rtol), _hip.ptr(out), int(i > 0),
-                              _hip.ptr(work), _hip.stream_ptr())
+                              _hip.ptr(work), st)
         _hip.check(rc, "mjx_reduce2d")
     return out
 
@@ -114,17 +133,30 @@ def _device_fields(fields):
     return out
 
 
+def _total(op: int, fields_a, fields_b=None, take_sqrt: bool = False) -> torch.Tensor:
+    """Batch total (max for RED_MAXABS) of a reduction over several fields,
+    in a fixed order on the device (mjx_reduce2d per field, then
+    mjx_reduce_finish), optionally its square root: a 0-d float64 tensor."""
+    from .. import _hip
+    st = _hip.stream_ptr()
+    per_inst = _reduce(op, fields_a, fields_b, st=st)
+    total = torch.empty((), dtype=torch.float64, device=per_inst.device)
+    rc = _hip.lib().mjx_reduce_finish(int(op), per_inst.numel(), per_inst.data_ptr(), int(bool(take_sqrt)),
+                                      total.data_ptr(), st)
+    _hip.check(rc, "mjx_reduce_finish")
+    return total
+
+
 def tree_norm2(state: StateDict, grid, interior_only: bool = True) -> torch.Tensor:
     """state.py:115-137: sqrt of the sum of squares over all fields, float64
     accumulation."""
-    f = _device_fields([_interior(v, grid, interior_only) for v in state.values()])
-    return torch.sqrt(torch.sum(_reduce(RED_SUMSQ, f)))
+    return _total(RED_SUMSQ, _device_fields([_interior(v, grid, interior_only) for v in state.values()]),
+                  take_sqrt=True)
 
 
 def tree_norm_inf(state: StateDict, grid, interior_only: bool = True) -> torch.Tensor:
     """state.py:140-160: max |x| over all fields (NaN propagates)."""
-    f = _device_fields([_interior(v, grid, interior_only) for v in state.values()])
-    return torch.max(_reduce(RED_MAXABS, f))
+    return _total(RED_MAXABS, _device_fields([_interior(v, grid, interior_only) for v in state.values()]))
 
 
 def tree_vdot(a: StateDict, b: StateDict, grid, interior_only: bool = True) -> torch.Tensor:
@@ -132,7 +164,7 @@ def tree_vdot(a: StateDict, b: StateDict, grid, interior_only: bool = True) -> t
     keys = list(a.keys())
     fa = _device_fields([_interior(a[k], grid, interior_only) for k in keys])
     fb = _device_fields([_interior(b[k], grid, interior_only) for k in keys])
-    return torch.sum(_reduce(RED_DOT, fa, fb))
+    return _total(RED_DOT, fa, fb)
 
 
 def state_interior_size(state: StateDict, grid) -> int:

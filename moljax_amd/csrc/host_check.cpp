@@ -32,7 +32,10 @@ static void cpu_checks() {
   CHECK(mjx_version() > 0, "version");
   CHECK(mjx_spectral_workspace_bytes(MJX_F32, 2, 64, 32) == 2LL * 64 * 32 * 8, "spectral workspace pow2");
   CHECK(mjx_spectral_workspace_bytes(MJX_F32, 2, 50, 30) == 2LL * 50 * 32 * 8, "spectral workspace any-length");
-  CHECK(mjx_etdrk4_workspace_bytes(MJX_F32, 2, 64, 64) == 8LL * 2 * (3 * 64 * 64 + 3 * 64 * 66), "etd workspace");
+  CHECK(mjx_etdrk4_workspace_bytes(MJX_F32, 2, 64, 64) == 8LL * 2 * (4 * 64 * 64 + 2 * 64), "etd workspace");
+  CHECK(mjx_gs_imex(MJX_F32, 4, 1, 96, 96, nullptr, nullptr, 98, 98 * 98, 1, nullptr, nullptr, nullptr, nullptr, 1,
+                    nullptr, nullptr) == MJX_E_SIZE, "increment form takes power-of-two sides");
+  CHECK(mjx_reduce_finish(MJX_RED_SUMSQ, 0, nullptr, 1, nullptr, nullptr) == MJX_E_ARG, "reduce finish empty");
   CHECK(mjx_nk_workspace_bytes(64, 64, 30) > 31LL * 64 * 64 * 8, "nk workspace");
   CHECK(mjx_reduce_workspace_elems(3) == 3 * 64, "reduce workspace");
   CHECK(mjx_etdrk4_coeff_elems(64, 64) == 6LL * 64 * 66, "coeff elems");
@@ -271,6 +274,9 @@ static void misc(int n) {
   Dev<double> red(B), rw(mjx_reduce_workspace_elems(B));
   CHECK(mjx_reduce2d(MJX_F64, 0, B, n, n, in.p + W + 1, W, (long long)W * W, nullptr, 0, 0, 0, 0, red.p, 0, rw.p,
                      nullptr) == MJX_OK && finite(red.get()), "reduction");
+  Dev<double> tot(1);
+  CHECK(mjx_reduce_finish(MJX_RED_SUMSQ, B, red.p, 1, tot.p, nullptr) == MJX_OK && finite(tot.get()),
+        "reduction batch total");
   auto sym = fd_symbol(n, 1.0);
   Dev<double> lam(sym.size(), &sym), o0(n), o1(n), o2(n), o3(n);
   CHECK(mjx_phi_tables(1, n, lam.p, 0.1, 3, 1, o0.p, o1.p, o2.p, o3.p, nullptr) == MJX_OK && finite(o3.get()),

@@ -82,7 +82,58 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const T* __restr
   const T* Bp = b ? b + inst * bi : nullptr;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double acc = 0.0;  // identity of + and of max over |x|
-  for (int y = blk * (RED_THREADS / 64) + wave; y < ny; y += nblk * (RED_THREADS / 64)) {
+  const int ystep = nblk * (RED_THREADS / 64);
+  int y = blk * (RED_THREADS / 64) + wave;
+  // Rows whose body fits one 16-byte load per lane (nx <= 64 * VE, the
+  // interior of a padded field) with b absent or aligned like a on every
+  // row: RROWS rows at a time, all their loads issued before any is used.
+  // Each row keeps its own unaligned head and tail (an odd pitch alternates
+  // the rows' alignment).
+  constexpr int RROWS = 4;
+  const bool batched = nx <= 64 * VE &&
+                       (!Bp || ((((uintptr_t)A ^ (uintptr_t)Bp) & 15) == 0 &&
+                                ((uintptr_t)((ap - bp) * (long long)sizeof(T)) & 15) == 0));
+  if (batched) {
+    for (; y + (RROWS - 1) * ystep < ny; y += RROWS * ystep) {
+      V av[RROWS], bv[RROWS];
+      T ha[RROWS], hb[RROWS], ta[RROWS], tb[RROWS];
+      int hh[RROWS], nbb[RROWS], tt[RROWS];
+#pragma unroll
+      for (int r = 0; r < RROWS; ++r) {
+        const T* ra = A + (long long)(y + r * ystep) * ap;
+        const T* rb = Bp ? Bp + (long long)(y + r * ystep) * bp : nullptr;
+        int h = (int)(((16 - ((uintptr_t)ra & 15)) & 15) / sizeof(T));
+        if (h > nx) h = nx;
+        const int nb = (nx - h) / VE;
+        hh[r] = h;
+        nbb[r] = nb;
+        tt[r] = h + nb * VE;
+        if (lane < nb) av[r] = *reinterpret_cast<const V*>(ra + h + lane * VE);
+        if (rb && lane < nb) bv[r] = *reinterpret_cast<const V*>(rb + h + lane * VE);
+        if (lane < h) {
+          ha[r] = ra[lane];
+          hb[r] = rb ? rb[lane] : (T)0;
+        }
+        if (tt[r] + lane < nx) {
+          ta[r] = ra[tt[r] + lane];
+          tb[r] = rb ? rb[tt[r] + lane] : (T)0;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RROWS; ++r) {
+        if (lane < hh[r]) acc = combine<OP>(acc, term<T, OP>(ha[r], hb[r], atol, rtol));
+        if (lane < nbb[r]) {
+          const T* ae = reinterpret_cast<const T*>(&av[r]);
+          const T* be = reinterpret_cast<const T*>(&bv[r]);
+#pragma unroll
+          for (int e = 0; e < VE; ++e) acc = combine<OP>(acc, term<T, OP>(ae[e], Bp ? be[e] : (T)0, atol, rtol));
+        }
+        if (tt[r] + lane < nx) acc = combine<OP>(acc, term<T, OP>(ta[r], tb[r], atol, rtol));
+      }
+    }
+  }
+  // remaining rows (and every row of other layouts) one at a time
+  for (; y < ny; y += ystep) {
     const T* ra = A + (long long)y * ap;
     const T* rb = Bp ? Bp + (long long)y * bp : nullptr;
     const uintptr_t ua = (uintptr_t)ra;
@@ -132,6 +183,17 @@ __global__ __launch_bounds__(64) void k_reduce_final(const double* __restrict__ 
 }
 
 static_assert(RED_BLOCKS == 64, "k_reduce_final folds one partial per lane");
+
+// Batch total of per-instance results in a fixed order (lane l folds
+// instances l, l + 64, ..., then a wave tree), optionally its square root.
+template <int OP>
+__global__ __launch_bounds__(64) void k_reduce_total(const double* __restrict__ per_inst, int B, int take_sqrt,
+                                                     double* __restrict__ total) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < B; b += 64) s = combine<OP>(s, per_inst[b]);
+  s = wave_reduce<OP>(s);
+  if (threadIdx.x == 0) total[0] = take_sqrt ? sqrt(s) : s;
+}
 
 template <typename T, int OP>
 int launch(int B, int ny, int nx, const T* a, long long ap, long long ai, const T* b, long long bp, long long bi,
@@ -187,6 +249,17 @@ int mjx_reduce2d(int dtype, int op, int B, int ny, int nx, const void* a, long l
                            atol, rtol, out, work, accumulate, st);
   return dispatch<double>(op, B, ny, nx, (const double*)a, a_pitch, a_istride, (const double*)b, b_pitch,
                           b_istride, atol, rtol, out, work, accumulate, st);
+}
+
+int mjx_reduce_finish(int op, int B, const double* per_inst, int take_sqrt, double* total, void* stream) {
+  if (B < 1 || !per_inst || !total) return fail(MJX_E_ARG, "empty batch or null pointer argument");
+  if (op < MJX_RED_SUMSQ || op > MJX_RED_SUM) return fail(MJX_E_ARG, "unknown reduction op");
+  hipStream_t st = (hipStream_t)stream;
+  if (op == MJX_RED_MAXABS)
+    hipLaunchKernelGGL((k_reduce_total<MJX_RED_MAXABS>), dim3(1), dim3(64), 0, st, per_inst, B, take_sqrt, total);
+  else
+    hipLaunchKernelGGL((k_reduce_total<MJX_RED_SUM>), dim3(1), dim3(64), 0, st, per_inst, B, take_sqrt, total);
+  return check_launch("reduce_total");
 }
 
 }  // extern "C"

@@ -36,11 +36,17 @@ def test_etd_exp_checks(lib):
     assert lib.mjx_etd_phi_coeff_elems(64, 64) == 3 * 64 * 66
     # any-length ETD workspace: six planes of ny * (nx + 2)
     assert lib.mjx_etdrk4_workspace_bytes(0, 2, 48, 50) == 8 * 6 * 2 * 48 * 52
-    assert lib.mjx_etdrk4_workspace_bytes(0, 2, 64, 64) == 8 * 2 * (3 * 64 * 64 + 3 * 64 * 66)
+    # power-of-two square ETD workspace: four packed planes of n^2, the last
+    # with room for ETD2's separated N^ plane of n (n + 2)
+    assert lib.mjx_etdrk4_workspace_bytes(0, 2, 64, 64) == 8 * 2 * (4 * 64 * 64 + 2 * 64)
+    assert lib.mjx_etdrk4_workspace_bytes(1, 3, 512, 512) == 16 * 3 * (4 * 512 * 512 + 2 * 512)
 
 
 def test_imex_and_batch_scheme_checks(lib):
     assert lib.mjx_gs_imex(1, 7, 1, 64, 64, P, P, 66, 66 * 66, 1, P, P, None, P, 1, P, None) == E_ARG
+    # the increment form (scheme 4) takes power-of-two sides only
+    assert lib.mjx_gs_imex(0, 4, 1, 96, 96, P, P, 98, 98 * 98, 1, P, P, None, P, 1, P, None) == E_SIZE
+    assert "power" in _msg(lib)
     plan = ctypes.c_void_p()
     n = (ctypes.c_int * 1)(64)
     assert lib.mjx_batch_create(1, 5, 1, n, n, None, None, None, None, None, None, None, 4,
@@ -78,6 +84,9 @@ def test_benchmark_method_checks(lib):
     assert cn(lap=None) == E_ARG
     assert lib.mjx_gs_bench_workspace_bytes(1, 2, 64, 32) == 6 * 8 * 2 * 64 * 32 + 16 * 2 * 64 + 4 * 2
     assert lib.mjx_reduce2d(1, 9, 1, 4, 4, P, 4, 16, None, 4, 16, 0.0, 0.0, P, 0, P, None) == E_ARG
+    assert lib.mjx_reduce_finish(0, 0, P, 1, P, None) == E_ARG
+    assert lib.mjx_reduce_finish(9, 4, P, 1, P, None) == E_ARG
+    assert lib.mjx_reduce_finish(0, 4, None, 1, P, None) == E_ARG
 
 
 def test_spectral_any_length_checks(lib):
