@@ -558,10 +558,16 @@ struct EtdTiles {
       if (L == CM / 8) return EtdLaunch<T>::template col<N, CM / 8, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
     return EtdLaunch<T>::template col<N, DC, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
   }
+  // Stages 3F and 3C hold three inputs' worth of values: at 1024 threads
+  // (128 VGPRs a wave) they spill, so they run half the widest tile there.
   template <int STAGE>
   static int rk4(int L, int B, int nx, cpx<T>* Ws, cpx<T>* P1, cpx<T>* P2, cpx<T>* P3, const T* coef,
                  hipStream_t st) {
-    if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    if constexpr ((STAGE == RK_S3F || STAGE == RK_S3C) && CM >= 2 && ColCfg<N, CM>::BT > 512) {
+      if (L == CM) return EtdLaunch<T>::template rk4<N, CM / 2, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    } else {
+      if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
+    }
     if constexpr (CM >= 2)
       if (L == CM / 2) return EtdLaunch<T>::template rk4<N, CM / 2, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
     if constexpr (CM >= 4)
