@@ -132,12 +132,20 @@ __device__ __forceinline__ double czt_norm(int kind, int k, int n) {
 
 // APPLY: forward transform, spectral multiplier, inverse transform (second
 // plan); otherwise one transform times sc1.
-template <typename T, int L, int ORIENT, bool APPLY>
-__global__ __launch_bounds__((CztCfg<T, L>::BT)) void k_czt_r2r(CztR2RArgs a, CztPlan<T> p1, CztPlan<T> p2,
-                                                              const cpx<T>* __restrict__ tw) {
+// NLW lines per workgroup (0: CztCfg's default).
+template <typename T, int L, int NLW>
+struct CztR2RCfg {
+  static constexpr int NL = NLW > 0 ? NLW : CztCfg<T, L>::NL;
+  static constexpr int BT = NL * CztCfg<T, L>::TN;
+  static constexpr int LDS = NL * CztCfg<T, L>::LP;
+};
+
+template <typename T, int L, int ORIENT, bool APPLY, int NLW = 0>
+__global__ __launch_bounds__((CztR2RCfg<T, L, NLW>::BT)) void k_czt_r2r(CztR2RArgs a, CztPlan<T> p1, CztPlan<T> p2,
+                                                                      const cpx<T>* __restrict__ tw) {
   using C = CztCfg<T, L>;
-  constexpr int P = C::P, TN = C::TN, NL = C::NL;
-  __shared__ cpx<T> lds[C::LDS];
+  constexpr int P = C::P, TN = C::TN, NL = CztR2RCfg<T, L, NLW>::NL;
+  __shared__ cpx<T> lds[CztR2RCfg<T, L, NLW>::LDS];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int inst = w / a.tiles;
   int tl, j;

@@ -121,23 +121,35 @@ struct CztLaunch {
       return check_launch("czt_col_apply");
     }
   };
+  // Lines per workgroup: CztCfg's default, or a quarter of it when the
+  // default would leave most CUs idle (one 512^2 instance: 128 workgroups);
+  // every line is its own transform, so the results are the same bitwise.
   template <int L>
   struct R2R {
-    static int run(CztR2RArgs a, int orient, bool apply, CztPlan<T> p1, CztPlan<T> p2, int B, hipStream_t st) {
-      using C = CztCfg<T, L>;
+    template <int NLW>
+    static int go(CztR2RArgs a, int orient, bool apply, CztPlan<T> p1, CztPlan<T> p2, int B, hipStream_t st) {
+      using C = CztR2RCfg<T, L, NLW>;
       a.tiles = (a.nlines + C::NL - 1) / C::NL;
       int rc = czt_grid_check((long long)a.tiles * B);
       if (rc) return rc;
       const dim3 g(a.tiles * B), b(C::BT);
       const cpx<T>* tw = twiddles<T>();
       if (orient == CZT_ROW) {
-        if (apply) hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_ROW, true>), g, b, 0, st, a, p1, p2, tw);
-        else hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_ROW, false>), g, b, 0, st, a, p1, p2, tw);
+        if (apply) hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_ROW, true, NLW>), g, b, 0, st, a, p1, p2, tw);
+        else hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_ROW, false, NLW>), g, b, 0, st, a, p1, p2, tw);
       } else {
-        if (apply) hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_COL, true>), g, b, 0, st, a, p1, p2, tw);
-        else hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_COL, false>), g, b, 0, st, a, p1, p2, tw);
+        if (apply) hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_COL, true, NLW>), g, b, 0, st, a, p1, p2, tw);
+        else hipLaunchKernelGGL((k_czt_r2r<T, L, CZT_COL, false, NLW>), g, b, 0, st, a, p1, p2, tw);
       }
       return check_launch("czt_r2r");
+    }
+    static int run(CztR2RArgs a, int orient, bool apply, CztPlan<T> p1, CztPlan<T> p2, int B, hipStream_t st) {
+      constexpr int NL0 = CztCfg<T, L>::NL;
+      if constexpr (NL0 >= 4) {
+        if ((long long)((a.nlines + NL0 - 1) / NL0) * B < 256)
+          return go<NL0 / 4>(a, orient, apply, p1, p2, B, st);
+      }
+      return go<0>(a, orient, apply, p1, p2, B, st);
     }
   };
 

@@ -318,13 +318,25 @@ constexpr int r2r_lines() {
   return (L / plan_p(L)) >= bt ? 1 : bt / (L / plan_p(L));
 }
 
+// Lines per workgroup: the default (about 256 threads in FP32, 128 in
+// FP64), or a quarter of it when the default would leave most CUs idle (a
+// single 512^2 instance gives 64 workgroups): each pair of real lines shares
+// one complex transform either way, so the results are the same bitwise.
 template <typename T, int KIND, int L, int ORIENT, int MODE>
 int r2r_launch(R2RArgs a, int B, hipStream_t st) {
   constexpr int TL = r2r_lines<T, L>();
-  a.tiles = (a.nlines + 2 * TL - 1) / (2 * TL);
+  constexpr int TLS = TL >= 4 ? TL / 4 : 1;
   const cpx<T>* tw;
   if constexpr (std::is_same<T, float>::value) tw = twiddles_f32();
   else tw = twiddles_f64();
+  const long long wide = (long long)((a.nlines + 2 * TL - 1) / (2 * TL)) * B;
+  if (TLS < TL && wide < 256) {
+    a.tiles = (a.nlines + 2 * TLS - 1) / (2 * TLS);
+    hipLaunchKernelGGL((k_r2r<T, KIND, L, ORIENT, MODE, TLS>), dim3(a.tiles * B), dim3(TLS * R2RCfg<KIND, L>::TN),
+                       0, st, a, tw);
+    return check_launch("r2r");
+  }
+  a.tiles = (a.nlines + 2 * TL - 1) / (2 * TL);
   hipLaunchKernelGGL((k_r2r<T, KIND, L, ORIENT, MODE, TL>), dim3(a.tiles * B), dim3(TL * R2RCfg<KIND, L>::TN), 0,
                      st, a, tw);
   return check_launch("r2r");
