@@ -108,6 +108,28 @@ def test_helmholtz_2d_batched_and_poisson(gpu):
     assert ogs.rel_l2(_np(p), ref) < 1e-12
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("kind,n", [("dirichlet", 511), ("dirichlet", 512), ("neumann", 512)])
+def test_line_tiles_do_not_change_results(gpu, kind, n, dtype):
+    # A lone instance runs its line passes on narrow workgroups (a quarter of
+    # the default tile, so that the launch spreads over the CUs); a batch of
+    # eight runs the default tile.  The per-line arithmetic is the same, so
+    # each instance must match its lone solve bit for bit.  Dirichlet 511
+    # and Neumann 512 use the Stockham kernels, Dirichlet 512 the chirp-z ones.
+    from moljax_amd.core import fft_nonperiodic as fn
+    h = 1.0 / (n + 1) if kind == "dirichlet" else 1.0 / n
+    cache = fn.create_nonperiodic_fft_cache_2d(n, n, h, h, kind)
+    solve = fn.solve_helmholtz_dirichlet if kind == "dirichlet" else fn.solve_helmholtz_neumann
+    rhs = np.random.default_rng(n).standard_normal((8, n, n))
+    out = solve(torch.from_numpy(rhs).to("cuda", dtype), cache, 1.0, 0.01)
+    for b in (0, 5):
+        single = solve(torch.from_numpy(rhs[b]).to("cuda", dtype), cache, 1.0, 0.01)
+        assert torch.equal(out[b], single), b
+    if dtype == torch.float64:
+        ref = onp.solve_helmholtz_2d(kind, rhs[0], _np(cache.lam_y), _np(cache.lam_x), 1.0, 0.01)
+        assert ogs.rel_l2(_np(out[0]), ref) < 1e-12
+
+
 def test_unsupported_nonperiodic_size(gpu):
     from moljax_amd.core import fft_nonperiodic as fn
     with pytest.raises(ValueError):
