@@ -87,6 +87,7 @@ PROTOTYPES = {
 }
 
 _lib = None
+_gpu_checked = False
 
 
 class MjxError(RuntimeError):
@@ -99,9 +100,11 @@ def load_library(require_gpu: bool = True):
     With ``require_gpu`` (the default for every compute path) a missing GPU is
     an error; tests that only inspect exported symbols pass False.
     """
-    global _lib
-    if require_gpu and not torch.cuda.is_available():
-        raise MjxError("moljax_amd needs a ROCm GPU; torch.cuda.is_available() is False")
+    global _lib, _gpu_checked
+    if require_gpu and not _gpu_checked:
+        if not torch.cuda.is_available():
+            raise MjxError("moljax_amd needs a ROCm GPU; torch.cuda.is_available() is False")
+        _gpu_checked = True  # a process keeps its GPUs: checked once
     if _lib is None:
         if not LIB_PATH.exists():
             raise MjxError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")

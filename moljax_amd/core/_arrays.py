@@ -1,6 +1,7 @@
 """Array plumbing shared by the host modules: device placement, strided 2-D
 views for the C ABI and per-instance parameter tables.  No arithmetic on
 field data happens here."""
+import math
 from typing import Optional, Sequence, Tuple
 
 import numpy as np
@@ -47,7 +48,7 @@ def view2d(t: torch.Tensor) -> Tuple[torch.Tensor, int, int, int]:
     if t.dim() == 2:
         return t.unsqueeze(0), t.stride(0), ny * t.stride(0), 1
     lead = t.shape[:-2]
-    B = int(np.prod(lead))
+    B = math.prod(lead)
     try:
         tb = t.view(B, ny, nx)
     except RuntimeError:
