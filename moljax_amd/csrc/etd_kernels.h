@@ -15,10 +15,13 @@
 //   row_nl(P2 -> P2)        N_a
 //   k_col_rk4 stage 2       b = E2 u + Q N_a                         -> P3
 //   row_nl(P3 -> P3)        N_b
-//   k_col_rk4 stage 3F      Fh = E u + f1 N_n + f2 (N_a + N_b)       -> P2
-//   k_col_rk4 stage 3C      c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)  -> P1
+//   k_col_rk4 stage 3M      Fh = E u + f1 N_n + f2 (N_a + N_b)       -> P2
+//                           c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)  -> P1
 //   row_nl(P1 -> P1)        N_c
 //   k_col_rk4 stage 4       u_new = Fh + f3 N_c                      -> Ws
+// Stage 3M reads four planes and writes two.  In FP64 its two accumulators
+// overflow the register file, so FP64 runs it as two kernels: 3F (Fh, with
+// N_a + N_b through one transform) and 3C (c).
 // Every packed x-spectral buffer holds the x-DFT divided by nx (the state is
 // packed with scale 1/nx and N is scaled by 1/nx before its x-FFT), so all
 // spectra share one normalisation and the stage outputs, which go back
@@ -232,19 +235,34 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_etd(
 // would be):
 //   stage 1   Ws = u, P1 = N_n                -> P2 = a = E2 u + Q N_n
 //   stage 2   Ws, P2 = N_a                     -> P3 = b = E2 u + Q N_a
-//   stage 3F  Ws, P1, P2 = N_a, P3 = N_b       -> P2 = Fh = E u + f1 N_n + f2 (N_a + N_b)
-//             (N_a + N_b summed before one transform: the y-FFT is linear)
-//   stage 3C  Ws, P1, P3                       -> P1 = c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)
+//   stage 3M  Ws, P1, P2 = N_a, P3 = N_b       -> P2 = Fh = E u + f1 N_n + f2 (N_a + N_b)
+//                                                 P1 = c = E2 (E2 u + Q N_n) + Q (2 N_b - N_n)
+//             (one y-FFT of each of the four inputs feeds both sums; FP32)
+//   stage 3F  Ws, P1, P2, P3                   -> P2 = Fh, with N_a + N_b summed
+//             before one transform (the y-FFT is linear)             (FP64)
+//   stage 3C  Ws, P1, P3                       -> P1 = c                (FP64)
 //   stage 4   P2 = Fh, P1 = N_c                -> Ws = Fh + f3 N_c
 // (benchmarks/benchmark_gray_scott.py:369-417; the combinations of
-// etd_stage, with the final sum gathered once).  Stages 3F and 3C write over
-// an input they read: the workgroup owns its columns and reads them all
-// before its first store.  (One fused stage 3 held c, Fh and a transform at
-// once and ran 3-5x slower than the two.)  Each stage is linear in its
-// inputs, so in a special column (packed kx = 0 and nx/2 of one field) an
-// input Z with per-field coefficients cA, cB contributes
-// (cA+cB)/2 Z(ky) + (cA-cB)/2 conj Z(-ky).
-enum { RK_S1 = 1, RK_S2 = 2, RK_S3F = 3, RK_S3C = 5, RK_S4 = 4 };
+// etd_stage, with the final sum gathered once).  Stage 3 writes over inputs
+// it reads: the workgroup owns its columns and reads them all before its
+// first store.  Each stage is linear in its inputs, so in a special column
+// (packed kx = 0 and nx/2 of one field) an input Z with per-field
+// coefficients cA, cB contributes (cA+cB)/2 Z(ky) + (cA-cB)/2 conj Z(-ky).
+
+// Loads and stores at a 32-bit byte offset from a uniform base: the compiler
+// then addresses with the base in SGPRs and one 32-bit VGPR offset, shared by
+// every table or plane read at the same offset, instead of a 64-bit VGPR
+// address per access.
+template <typename V>
+__device__ __forceinline__ V ld_at(const V* base, unsigned byte_off) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename V>
+__device__ __forceinline__ void st_at(V* base, unsigned byte_off, const V& v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+enum { RK_S1 = 1, RK_S2 = 2, RK_S3F = 3, RK_S3C = 5, RK_S4 = 4, RK_S3M = 6 };
 enum { CF_E = 0, CF_E2 = 1, CF_Q = 2, CF_F1 = 3, CF_F2 = 4, CF_F3 = 5 };
 
 template <typename T, int NY, int TC_, int STAGE>
@@ -257,8 +275,10 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   constexpr int NX = NY;                    // square grids
   constexpr int SW = NX + 2;                // row length of the separated-layout coefficient tables
   constexpr unsigned TSTRIDE = NY * SW;     // one coefficient table
-  constexpr unsigned RSTEP = TN * NX;       // plane offset between a thread's points m and m + 1
-  constexpr unsigned CSTEP = TN * SW;       // the same in a coefficient table
+  // byte offsets (32-bit: one plane or table is < 4 GiB) between a thread's
+  // points m and m + 1 in a plane and in a coefficient table
+  constexpr unsigned RSTEP = TN * NX * (unsigned)sizeof(cpx<T>);
+  constexpr unsigned CSTEP = TN * SW * (unsigned)sizeof(T);
   __shared__ cpx<T> lds[NY * TC];
   // Workgroups run tile-major within groups of kgrp instances (group,
   // tile, instance), so that the coefficient slab of a tile is read from
@@ -281,13 +301,13 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   cpx<T>* const p1 = P1 + ibase;
   cpx<T>* const p2 = P2 + ibase;
   cpx<T>* const p3 = P3 + ibase;
-  const unsigned toff = (unsigned)(j * NX + p);
+  const unsigned toff = (unsigned)(j * NX + p) * (unsigned)sizeof(cpx<T>);
   const bool stile = p0 == 0 || p0 == half;  // tile whose first column is a packed special column
   const bool spec = stile && c == 0;
   // coefficient columns: the field's kx column, or the pair (kx = 0, nx/2)
   const int sa = spec ? (p0 == 0 ? 0 : half + 1) : (p < half ? p : p + 1);
   const int sb = spec ? (p0 == 0 ? half : NX + 1) : sa;
-  const unsigned ca = (unsigned)(j * SW + sa), cb = (unsigned)(j * SW + sb);
+  const unsigned ca = (unsigned)(j * SW + sa) * (unsigned)sizeof(T), cb = (unsigned)(j * SW + sb) * (unsigned)sizeof(T);
   TwCache<T, NY> twc;
   load_twiddles(twc, j, tw);
 
@@ -297,7 +317,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   // input before transforming the current one.
   auto ld = [&](const cpx<T>* X, cpx<T> (&z)[P]) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) z[m] = X[toff + m * RSTEP];
+    for (int m = 0; m < P; ++m) z[m] = ld_at(X, toff + m * RSTEP);
   };
   auto xf = [&](cpx<T> (&z)[P]) {
     line_fft<T, NY, -1, TC, 0>(z, j, lds + c, twc);
@@ -310,27 +330,77 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   auto done = [&]() {
     if (stile) __syncthreads();
   };
-  // a[m] (+)= k * Z at ky = j + TN m, with k from mult(table index)
+  // a[m] (+)= k * Z at ky = j + TN m, with k from mult (a table read by
+  // byte offset).  The multipliers of all P points are loaded first, outside
+  // the branch on the special column, so that their loads are in flight
+  // together (loads inside the per-point branch each waited for the last).
   auto add = [&](cpx<T> (&a)[P], const cpx<T> (&z)[P], bool first, auto mult) {
+    T kA[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-      const T cA = mult(ca + m * CSTEP);
-      cpx<T> t;
-      if (spec) {
+    for (int m = 0; m < P; ++m) kA[m] = mult(ca + m * CSTEP);
+    if (spec) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
         const int ky = j + TN * m;
         const T cB = mult(cb + m * CSTEP);
         const cpx<T> zm = conjg(lds[((NY - ky) & (NY - 1)) * TC]);
-        const T cp = (T)0.5 * (cA + cB), cm = (T)0.5 * (cA - cB);
-        t = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
-      } else {
-        t = scale(z[m], cA);
+        const T cp = (T)0.5 * (kA[m] + cB), cm = (T)0.5 * (kA[m] - cB);
+        const cpx<T> t = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
+        a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
       }
-      a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const cpx<T> t = scale(z[m], kA[m]);
+        a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
+      }
     }
   };
-  auto E = [&](unsigned i) { return coef[i + CF_E * TSTRIDE]; };
-  auto E2 = [&](unsigned i) { return coef[i + CF_E2 * TSTRIDE]; };
-  auto Q = [&](unsigned i) { return coef[i + CF_Q * TSTRIDE]; };
+  // the same for two accumulators a (multiplier ma) and b (mb), reading a
+  // special column's conj partner once
+  auto add2 = [&](cpx<T> (&a)[P], cpx<T> (&b)[P], const cpx<T> (&z)[P], bool first, auto ma, auto mb) {
+    T kA[P], kB[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      kA[m] = ma(ca + m * CSTEP);
+      kB[m] = mb(ca + m * CSTEP);
+    }
+    if (spec) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const int ky = j + TN * m;
+        const T aB = ma(cb + m * CSTEP), bB = mb(cb + m * CSTEP);
+        const cpx<T> zm = conjg(lds[((NY - ky) & (NY - 1)) * TC]);
+        T cp = (T)0.5 * (kA[m] + aB), cm = (T)0.5 * (kA[m] - aB);
+        const cpx<T> t = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
+        cp = (T)0.5 * (kB[m] + bB);
+        cm = (T)0.5 * (kB[m] - bB);
+        const cpx<T> u = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
+        a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
+        b[m] = first ? u : mk(b[m].re + u.re, b[m].im + u.im);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const cpx<T> t = scale(z[m], kA[m]), u = scale(z[m], kB[m]);
+        a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
+        b[m] = first ? u : mk(b[m].re + u.re, b[m].im + u.im);
+      }
+    }
+  };
+  // multiplier tables by byte offset
+  const T* const tE = coef + CF_E * TSTRIDE;
+  const T* const tE2 = coef + CF_E2 * TSTRIDE;
+  const T* const tQ = coef + CF_Q * TSTRIDE;
+  const T* const tF1 = coef + CF_F1 * TSTRIDE;
+  const T* const tF2 = coef + CF_F2 * TSTRIDE;
+  const T* const tF3 = coef + CF_F3 * TSTRIDE;
+  auto E = [&](unsigned o) { return ld_at(tE, o); };
+  auto E2 = [&](unsigned o) { return ld_at(tE2, o); };
+  auto Q = [&](unsigned o) { return ld_at(tQ, o); };
+  auto F1 = [&](unsigned o) { return ld_at(tF1, o); };
+  auto F2 = [&](unsigned o) { return ld_at(tF2, o); };
+  auto F3 = [&](unsigned o) { return ld_at(tF3, o); };
 
   cpx<T> za[P], zb[P], acc[P];
   cpx<T>* out;
@@ -353,18 +423,43 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
     ld(p2, za);  // N_a + N_b
 #pragma unroll
     for (int m = 0; m < P; ++m) {
-      const cpx<T> nb = p3[toff + m * RSTEP];
+      const cpx<T> nb = ld_at(p3, toff + m * RSTEP);
       za[m] = mk(za[m].re + nb.re, za[m].im + nb.im);
     }
     xf(zb);
-    add(acc, zb, false, [&](unsigned i) { return coef[i + CF_F1 * TSTRIDE]; });
+    add(acc, zb, false, F1);
     done();
     xf(za);
-    add(acc, za, false, [&](unsigned i) { return coef[i + CF_F2 * TSTRIDE]; });
+    add(acc, za, false, F2);
     done();
 #pragma unroll
-    for (int m = 0; m < P; ++m) p2[toff + m * RSTEP] = acc[m];  // y-spectral, unscaled
+    for (int m = 0; m < P; ++m) st_at(p2, toff + m * RSTEP, acc[m]);  // y-spectral, unscaled
     return;
+  } else if constexpr (STAGE == RK_S3M) {
+    // Fh = E u + f1 N_n + f2 (N_a + N_b) and c = E2^2 u + (E2 Q - Q) N_n
+    // + 2 Q N_b from one y-FFT of each of u, N_n, N_b, N_a
+    cpx<T> cc[P];
+    ld(ws, za);  // u
+    ld(p1, zb);  // N_n
+    xf(za);
+    add2(acc, cc, za, true, E, [&](unsigned i) { const T e2 = E2(i); return e2 * e2; });
+    done();
+    ld(p3, za);  // N_b
+    xf(zb);
+    add2(acc, cc, zb, false, F1, [&](unsigned i) { const T q = Q(i); return E2(i) * q - q; });
+    done();
+    ld(p2, zb);  // N_a
+    xf(za);
+    add2(acc, cc, za, false, F2, [&](unsigned i) { return (T)2 * Q(i); });
+    done();
+    xf(zb);
+    add(acc, zb, false, F2);
+    done();
+#pragma unroll
+    for (int m = 0; m < P; ++m) st_at(p2, toff + m * RSTEP, acc[m]);  // Fh: y-spectral, unscaled
+#pragma unroll
+    for (int m = 0; m < P; ++m) acc[m] = cc[m];
+    out = p1;
   } else if constexpr (STAGE == RK_S3C) {
     ld(ws, za);  // u
     ld(p1, zb);  // N_n
@@ -383,7 +478,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
     ld(p1, za);  // N_c
     ld(p2, zb);  // Fh
     xf(za);
-    add(acc, za, true, [&](unsigned i) { return coef[i + CF_F3 * TSTRIDE]; });
+    add(acc, za, true, F3);
     done();
 #pragma unroll
     for (int m = 0; m < P; ++m) acc[m] = mk(acc[m].re + zb[m].re, acc[m].im + zb[m].im);
@@ -393,7 +488,7 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   for (int m = 0; m < P; ++m) acc[m] = scale(acc[m], s);
   line_fft<T, NY, +1, TC, 0>(acc, j, lds + c, twc);
 #pragma unroll
-  for (int m = 0; m < P; ++m) out[toff + m * RSTEP] = acc[m];
+  for (int m = 0; m < P; ++m) st_at(out, toff + m * RSTEP, acc[m]);
 }
 
 // ------------------------------------------------- contour coefficients
@@ -558,12 +653,13 @@ struct EtdTiles {
       if (L == CM / 8) return EtdLaunch<T>::template col<N, CM / 8, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
     return EtdLaunch<T>::template col<N, DC, STAGE>(B, nx, Win, Win2, Uh, Nn, Fh, coef, Wout, st);
   }
-  // Stages 3F and 3C hold three inputs' worth of values: at 1024 threads
-  // (128 VGPRs a wave) they spill, so they run half the widest tile there.
+  // Stages 3M, 3F and 3C hold three inputs' worth of values: at 1024
+  // threads (128 VGPRs a wave) they spill, so they run half the widest tile
+  // there.
   template <int STAGE>
   static int rk4(int L, int B, int nx, cpx<T>* Ws, cpx<T>* P1, cpx<T>* P2, cpx<T>* P3, const T* coef,
                  hipStream_t st) {
-    if constexpr ((STAGE == RK_S3F || STAGE == RK_S3C) && CM >= 2 && ColCfg<N, CM>::BT > 512) {
+    if constexpr ((STAGE == RK_S3F || STAGE == RK_S3C || STAGE == RK_S3M) && CM >= 2 && ColCfg<N, CM>::BT > 512) {
       if (L == CM) return EtdLaunch<T>::template rk4<N, CM / 2, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
     } else {
       if (L == CM) return EtdLaunch<T>::template rk4<N, CM, STAGE>(B, nx, Ws, P1, P2, P3, coef, st);
@@ -621,8 +717,12 @@ int etd_steps_sized(int B, const EtdPlanes<T>& p, const T* coef, const InstParam
     if (!rc) rc = E::row_nl(LR, B, N, p.P2, p.P2, ip, st);
     if (!rc) rc = E::template rk4<RK_S2>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
     if (!rc) rc = E::row_nl(LR, B, N, p.P3, p.P3, ip, st);
-    if (!rc) rc = E::template rk4<RK_S3F>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
-    if (!rc) rc = E::template rk4<RK_S3C>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    if constexpr (sizeof(T) == 4) {
+      if (!rc) rc = E::template rk4<RK_S3M>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    } else {
+      if (!rc) rc = E::template rk4<RK_S3F>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+      if (!rc) rc = E::template rk4<RK_S3C>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
+    }
     if (!rc) rc = E::row_nl(LR, B, N, p.P1, p.P1, ip, st);
     if (!rc) rc = E::template rk4<RK_S4>(LC, B, N, p.Ws, p.P1, p.P2, p.P3, coef, st);
   }
