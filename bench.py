@@ -197,19 +197,24 @@ def step_latency(kt_samples):
             "step_max_ms": float(np.max(steps)), "samples": int(len(steps))}
 
 
+def _profile_value(name, *keys):
+    """The number on the line ``<keys...> <value>`` of a committed rocprofv3
+    summary under profiles/, or None when there is no such line."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            f = line.split()
+            if len(f) == len(keys) + 1 and tuple(f[:-1]) == keys:
+                return float(f[-1])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(kernel, B):
     """HBM bytes per launch of ``kernel`` at batch B from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x2 + WRITE_SIZE x1, copy-kernel
     calibrated), or None when the summary has no matching line."""
-    path = os.path.join(ROOT, "profiles", "gray_scott_256_imex_traffic.md")
-    try:
-        for line in open(path):
-            f = line.split()
-            if len(f) >= 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == f"B={B}":
-                return float(f[3])
-    except OSError:
-        pass
-    return None
+    return _profile_value("gray_scott_256_imex_traffic.md", "traffic_bytes_per_launch", kernel, f"B={B}")
 
 
 def c4_batched_etdrk4(n_inst, steps, seed=0):
@@ -250,7 +255,18 @@ def c4_batched_etdrk4(n_inst, steps, seed=0):
     ref.close()
     from moljax_amd import _hip
     tiles = {str(n): [_hip.lib().mjx_tune_get(2, 0, n), _hip.lib().mjx_tune_get(3, 0, n)] for n in b.order}
-    out = {"value": b.n_points * steps / el, "unit": "grid-point-steps/s", "instances": n_inst,
+    gps = b.n_points * steps / el
+    # Roofline: the state's 16 B per grid-point step, and the chain's own
+    # compulsory traffic (FP32: four row passes of 2 planes, stages of 3, 3,
+    # 6 and 3 planes, 8 B per point a plane), beside the bytes per point the
+    # committed counter summary measured for the 512^2 group.
+    chain = 8.0 * (4 * 2 + 3 + 3 + 6 + 3)
+    meas = _profile_value("batched_etdrk4_uneven_counters.md", "traffic_bytes_per_point", "etdrk4_512")
+    roof = {"bound": "hbm", "peak": HBM_PEAK, "unit": "B/s", "algorithmic_bytes_per_gps": 16.0,
+            "frac": 16.0 * gps / HBM_PEAK, "chain_bytes_per_point": chain, "chain_frac": chain * gps / HBM_PEAK,
+            "measured_bytes_per_point_512": meas,
+            "traffic_ratio_512": None if meas is None else meas / chain}
+    out = {"value": gps, "unit": "grid-point-steps/s", "instances": n_inst, "roofline": roof,
            "points": b.n_points, "state_bytes": b.state_bytes(), "steps": steps, "ms_per_step": 1e3 * el / steps,
            "sizes": {str(n): len(b.members[n]) for n in b.order}, "tiles_row_col": tiles,
            "setup_s": setup_s, "finite": finite,
