@@ -257,6 +257,9 @@ int mjx_batch_create_mixed(int dtype, int n_groups, const int* schemes, const in
                            const void* const* tables, const double* const* inst, void* const* work,
                            int steps_per_graph, void** plan);
 int mjx_batch_run(void* plan, int n_steps, int fill_ghosts, void* stream);
+/* mjx_batch_destroy waits for the plan's device (pending replays of its
+ * graph) before it releases the graph, streams and events; the caller's
+ * buffers stay the caller's. */
 int mjx_batch_destroy(void* plan);
 
 /* Autotuner (no reference counterpart): times the candidate tiles (lines per

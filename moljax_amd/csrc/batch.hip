@@ -165,6 +165,8 @@ struct Ops {
 
 struct BatchPlan {
   int dtype = MJX_F32, K = 1;
+  int device = 0;         // the device the plan was created on
+  bool launched = false;  // work of this plan was enqueued
   std::vector<Group> groups;
   hipStream_t cap = nullptr;
   std::vector<hipStream_t> side;
@@ -172,7 +174,17 @@ struct BatchPlan {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
 
+  // A plan may be destroyed right after an asynchronous run: wait for its
+  // device first, so that no replay of the graph (nor a launch reading its
+  // nodes) is still pending when the graph, its streams and events go.
   ~BatchPlan() {
+    if (launched) {
+      int cur = device;
+      (void)hipGetDevice(&cur);
+      if (cur != device) (void)hipSetDevice(device);
+      (void)hipDeviceSynchronize();
+      if (cur != device) (void)hipSetDevice(cur);
+    }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -223,6 +235,7 @@ struct BatchPlan {
 
   int run(int n_steps, int ghosts, hipStream_t st) {
     int rc = MJX_OK;
+    launched = true;
     for (auto& g : groups) {
       rc = dtype == MJX_F32 ? Ops<float>::pack(g, st) : Ops<double>::pack(g, st);
       if (rc) return rc;
@@ -377,6 +390,7 @@ int mjx_batch_create_mixed(int dtype, int n_groups, const int* schemes, const in
   if (rc) return rc;
   std::unique_ptr<BatchPlan> p(new BatchPlan());
   p->dtype = dtype;
+  if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
   p->K = steps_per_graph;
   for (int g = 0; g < n_groups; ++g) {
     const int ng = n[g], scheme = schemes[g];
