@@ -38,44 +38,61 @@ namespace mjx {
 // Row pass: packed x-spectra -> physical (u, v) -> Gray-Scott reaction
 // N(u, v) = (-u v^2 + F(1-u), u v^2 - (F+k) v) -> packed x-spectra.
 // Win == Wout is allowed (in place): a row is read whole into LDS before any
-// of it is written, and one workgroup owns the row.
-template <typename T, int NX, int TR>
+// of it is written, and one workgroup owns the row.  A workgroup takes NT
+// consecutive row tiles and issues the loads of all of them first, so the
+// later tiles' loads are in flight while the first is transformed (at
+// P = 8 values a thread, one tile keeps too few bytes in flight).
+template <typename T, int NX, int TR, int NT>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_nl(const cpx<T>* Win,
                                                                   cpx<T>* Wout,
                                                                   const InstParams* __restrict__ ip, int ny,
-                                                                  int rows_per_inst_tiles,
+                                                                  int groups_per_inst,
                                                                   const cpx<T>* __restrict__ tw) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
-  const int inst = w / rows_per_inst_tiles;
-  const int tile = w - inst * rows_per_inst_tiles;
+  const int inst = w / groups_per_inst;
+  const int tile0 = (w - inst * groups_per_inst) * NT;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
-  const int y = tile * C::TR + rl;
   cpx<T>* lrow = lds + rl * C::LP;
-  const long long roff = ((long long)inst * ny + y) * NX;
+  const long long rbase = ((long long)inst * ny + tile0 * C::TR + rl) * NX;  // this thread's row in tile0
+  constexpr long long TSTEP = (long long)C::TR * NX;                          // the same row one tile on
   const T F = (T)ip[inst].F, kk = (T)ip[inst].k;
   const T inv_nx = (T)(1.0 / NX);
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
-  cpx<T> v[C::P];
-  load_unpack<T, NX, TR>(Win + roff, true, lrow, j, v);
-  line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+  cpx<T> g[NT][C::P];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) {
-    T ru, rv;
-    gs_react(v[m].re, v[m].im, F, kk, ru, rv);
-    v[m] = mk(ru * inv_nx, rv * inv_nx);
-  }
-  line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
+  for (int t = 0; t < NT; ++t)
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
-  __syncthreads();
-  cpx<T>* wrow = Wout + roff;
+    for (int m = 0; m < C::P; ++m) g[t][m] = Win[rbase + t * TSTEP + j + C::TN * m];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) {
-    const int p = j + C::TN * m;
-    wrow[p] = pack_at<T, NX>(lrow, p);
+  for (int t = 0; t < NT; ++t) {
+    if (t > 0) __syncthreads();  // the previous tile's pack reads of LDS are done
+    cpx<T> v[C::P];
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, g[t][m]);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) v[m] = unpack_at<T, NX>(lrow, j + C::TN * m);
+    __syncthreads();
+    line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      T ru, rv;
+      gs_react(v[m].re, v[m].im, F, kk, ru, rv);
+      v[m] = mk(ru * inv_nx, rv * inv_nx);
+    }
+    line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
+    __syncthreads();
+    cpx<T>* wrow = Wout + rbase + t * TSTEP;
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const int p = j + C::TN * m;
+      wrow[p] = pack_at<T, NX>(lrow, p);
+    }
   }
 }
 
@@ -591,8 +608,18 @@ struct EtdLaunch {
   template <int N, int L>
   static int row_nl(int B, int ny, const cpx<T>* Win, cpx<T>* Wout, const InstParams* ip, hipStream_t st) {
     const int tiles = ny / L;
-    hipLaunchKernelGGL((k_row_nl<T, N, L>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip, ny, tiles,
-                       twiddles<T>());
+    // Two row tiles per workgroup where one tile leaves too few bytes in
+    // flight (FP32, n >= 256; measured at 512^2: 335 -> 251 us a pass) and
+    // the grid keeps >= 2048 workgroups (small batches lose more from half
+    // the workgroups than they gain).
+    constexpr bool two = sizeof(T) == 4 && N >= 256;
+    if (two && tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
+      hipLaunchKernelGGL((k_row_nl<T, N, L, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip,
+                         ny, tiles / 2, twiddles<T>());
+    } else {
+      hipLaunchKernelGGL((k_row_nl<T, N, L, 1>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip, ny,
+                         tiles, twiddles<T>());
+    }
     return check_launch("row_nl");
   }
   template <int N, int L, int STAGE>

@@ -71,6 +71,32 @@ def test_batched_per_instance_params_and_invariance(gpu):
         assert ogs.rel_l2(_np(U[i]), ru) < 1e-12 and ogs.rel_l2(_np(V[i]), rv) < 1e-12
 
 
+@pytest.mark.parametrize("n,B", [(256, 64), (512, 32)])
+def test_row_tiles_per_workgroup_do_not_change_results(gpu, n, B):
+    """FP32 batches this large run the row pass with two row tiles per
+    workgroup (the loads of the second tile issued before the first is
+    transformed); a lone instance runs one tile per workgroup.  Each row's
+    arithmetic is the same, so the results are bitwise equal."""
+    from moljax_amd.core.fft_integrators import etdrk4_coefficients, gs_etdrk4_inplace, spectral_symbol_1d
+    steps = 3
+    lam = spectral_symbol_1d(n)
+    ics = [_ic(n, s % 4) for s in range(B)]
+    U = torch.from_numpy(np.stack([a for a, _ in ics])).to("cuda", torch.float32)
+    V = torch.from_numpy(np.stack([b for _, b in ics])).to("cuda", torch.float32)
+    coef = etdrk4_coefficients(lam, lam, 2e-5, 1e-5, 1.0, torch.float32)
+    rows = [[0, 0, 0, 0, 0.03 + 0.0003 * i, 0.06, 1.0] for i in range(B)]
+    pick = [0, B // 2 + 1, B - 1]
+    alone = []
+    for i in pick:
+        u1, v1 = U[i].clone(), V[i].clone()
+        gs_etdrk4_inplace(u1, v1, coef, rows[i:i + 1], steps)
+        alone.append((u1, v1))
+    gs_etdrk4_inplace(U, V, coef, rows, steps)
+    for i, (u1, v1) in zip(pick, alone):
+        print(f"n={n} B={B} instance {i}: max |diff| u={float((U[i] - u1).abs().max()):.3e}", flush=True)
+        assert torch.equal(U[i], u1) and torch.equal(V[i], v1), i
+
+
 def test_etdrk4_step_api_fd_symbol(gpu):
     """etdrk4_step / etd_integrate with DiffusionOperator (library FD symbol)."""
     from moljax_amd.core import Grid2D
