@@ -305,63 +305,88 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_inv_unpack(
 //   y + dt R(y); with rx = 1 y + (dt/2) R(y), with rx = 2 two such half
 //   steps in a row (the reaction half-steps that close one step and open
 //   the next of benchmark_gray_scott.py:268-289).
-template <typename T, int NX, int TR, int RXN>
+//
+// A workgroup takes NT consecutive row tiles of one instance and issues the
+// loads of all of them first, so that the later tiles' loads are in flight
+// while the first is transformed (a 512-point line gives a thread only 8
+// values, too few bytes in flight with one tile).  The arithmetic of every
+// row is the same for any NT.
+template <typename T, int NX, int TR, int RXN, int NT>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restrict__ W,
                                                                   const InstParams* __restrict__ ip, int ny,
-                                                                  int rows_per_inst_tiles,
+                                                                  int groups_per_inst,
                                                                   const cpx<T>* __restrict__ tw, int mstep,
                                                                   int rx) {
   using C = RowCfg<NX, TR>;
   __shared__ cpx<T> lds[C::TR * C::LP];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
-  const int inst = w / rows_per_inst_tiles;
-  const int tile = w - inst * rows_per_inst_tiles;
+  const int inst = w / groups_per_inst;
+  const int tile0 = (w - inst * groups_per_inst) * NT;
   // masked chain step: an instance past its own step count is left as is
   if (mstep >= 0 && mstep >= (int)ip[inst].aux) return;
   const int j = threadIdx.x % C::TN, rl = threadIdx.x / C::TN;
-  const int y = tile * C::TR + rl;
-  const bool live = y < ny;  // rows past ny: transforms only
   cpx<T>* lrow = lds + rl * C::LP;
-  cpx<T>* wrow = W + ((long long)inst * ny + (live ? y : 0)) * NX;
   const T F = (T)ip[inst].F, kk = (T)ip[inst].k, dt = (T)ip[inst].dt;
   const T hdt = (T)(0.5 * ip[inst].dt);
   TwCache<T, NX> twc;
   load_twiddles(twc, j, tw);
-  cpx<T> v[C::P];
-  load_unpack<T, NX, TR>(wrow, live, lrow, j, v);
-  line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+  // rows past ny (the last tile when TR does not divide ny): transforms only
+  bool live[NT];
+  cpx<T>* wrow[NT];
+  cpx<T> g[NT][C::P];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) {
-    const T u0 = v[m].re, v0 = v[m].im;
-    T r1u, r1v, r2u, r2v;
-    gs_react(u0, v0, F, kk, r1u, r1v);
-    if constexpr (RXN == RXN_EULER) {
-      if (rx == 0) {
-        v[m] = mk(u0 + dt * r1u, v0 + dt * r1v);
-      } else {
-        T ut = u0 + hdt * r1u, vt = v0 + hdt * r1v;
-        if (rx == 2) {
-          gs_react(ut, vt, F, kk, r2u, r2v);
-          ut = ut + hdt * r2u;
-          vt = vt + hdt * r2v;
-        }
-        v[m] = mk(ut, vt);
-      }
-    } else {
-      const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
-      gs_react(ut, vt, F, kk, r2u, r2v);
-      v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
-    }
+  for (int t = 0; t < NT; ++t) {
+    const int y = (tile0 + t) * C::TR + rl;
+    live[t] = y < ny;
+    wrow[t] = W + ((long long)inst * ny + (live[t] ? y : 0)) * NX;
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) g[t][m] = live[t] ? wrow[t][j + C::TN * m] : mk((T)0, (T)0);
   }
-  line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
-  __syncthreads();
-  if (!live) return;
+  for (int t = 0; t < NT; ++t) {
+    if (t > 0) __syncthreads();  // the previous tile's pack reads of LDS are done
+    cpx<T> v[C::P];
 #pragma unroll
-  for (int m = 0; m < C::P; ++m) {
-    const int p = j + C::TN * m;
-    wrow[p] = pack_at<T, NX>(lrow, p);
+    for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, g[t][m]);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) v[m] = unpack_at<T, NX>(lrow, j + C::TN * m);
+    __syncthreads();
+    line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) {
+      const T u0 = v[m].re, v0 = v[m].im;
+      T r1u, r1v, r2u, r2v;
+      gs_react(u0, v0, F, kk, r1u, r1v);
+      if constexpr (RXN == RXN_EULER) {
+        if (rx == 0) {
+          v[m] = mk(u0 + dt * r1u, v0 + dt * r1v);
+        } else {
+          T ut = u0 + hdt * r1u, vt = v0 + hdt * r1v;
+          if (rx == 2) {
+            gs_react(ut, vt, F, kk, r2u, r2v);
+            ut = ut + hdt * r2u;
+            vt = vt + hdt * r2v;
+          }
+          v[m] = mk(ut, vt);
+        }
+      } else {
+        const T ut = u0 + dt * r1u, vt = v0 + dt * r1v;
+        gs_react(ut, vt, F, kk, r2u, r2v);
+        v[m] = mk(u0 + hdt * (r1u + r2u), v0 + hdt * (r1v + r2v));
+      }
+    }
+    line_fft<T, NX, -1, 1, 1>(v, j, lrow, twc);
+#pragma unroll
+    for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, v[m]);
+    __syncthreads();
+    if (live[t]) {
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) {
+        const int p = j + C::TN * m;
+        wrow[t][p] = pack_at<T, NX>(lrow, p);
+      }
+    }
   }
 }
 
@@ -690,11 +715,23 @@ struct RowLaunch {
   }
   static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, int mstep, hipStream_t st) {
     const int tiles = (ny + L - 1) / L;
+    // Two row tiles per workgroup for FP32 lines of at most 8 values a
+    // thread (n >= 512) when the grid keeps >= 2048 workgroups.
+    constexpr bool two = sizeof(T) == 4 && N >= 512 && plan_p(N) <= 8;
+    if (two && tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
+      if (rxn != RXN_HEUN)
+        hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, W,
+                           ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, rxn - RXN_EULER);
+      else
+        hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, W,
+                           ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, 0);
+      return check_launch("row_gs");
+    }
     if (rxn != RXN_HEUN)
-      hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
+      hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER, 1>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
                          ny, tiles, twiddles<T>() + tw_base(N), mstep, rxn - RXN_EULER);
     else
-      hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
+      hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN, 1>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
                          ny, tiles, twiddles<T>() + tw_base(N), mstep, 0);
     return check_launch("row_gs");
   }

@@ -96,6 +96,27 @@ def test_batch_equals_single_bitwise(gpu):
     assert torch.equal(U[2], u1) and torch.equal(V[2], v1)
 
 
+def test_two_row_tiles_per_workgroup_bitwise(gpu):
+    """A 512^2 FP32 batch of 32 runs the row pass with two row tiles per
+    workgroup (the second tile's loads in flight during the first); a lone
+    instance runs one.  Each row's arithmetic is the same: bitwise equal."""
+    from moljax_amd.core.stepping import _gs_rows, gs_imex_strang_inplace
+    n, B, steps = 512, 32, 4
+    grid, (model, cache, diff) = _model(n, torch.float32)
+    ys = [ogs.initial_condition(n, n, seed % 3) for seed in range(B)]
+    U = torch.from_numpy(np.stack([y["u"] for y in ys])).to("cuda", torch.float32)
+    V = torch.from_numpy(np.stack([y["v"] for y in ys])).to("cuda", torch.float32)
+    pick = [0, 17, B - 1]
+    alone = []
+    for i in pick:
+        u1, v1 = U[i].clone(), V[i].clone()
+        gs_imex_strang_inplace(u1, v1, cache.laplacian_symbol, _gs_rows(model, 1.0, diff, 1), steps)
+        alone.append((u1, v1))
+    gs_imex_strang_inplace(U, V, cache.laplacian_symbol, _gs_rows(model, 1.0, diff, B), steps)
+    for i, (u1, v1) in zip(pick, alone):
+        assert torch.equal(U[i], u1) and torch.equal(V[i], v1), i
+
+
 def test_per_instance_parameters(gpu):
     """Instances with different (F, k, D, dt) in one launch each match the
     oracle run with their own parameters."""
