@@ -612,14 +612,15 @@ struct EtdLaunch {
     // flight (FP32, n >= 256; measured at 512^2: 335 -> 251 us a pass) and
     // the grid keeps >= 2048 workgroups (small batches lose more from half
     // the workgroups than they gain).
-    constexpr bool two = sizeof(T) == 4 && N >= 256;
-    if (two && tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
-      hipLaunchKernelGGL((k_row_nl<T, N, L, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip,
-                         ny, tiles / 2, twiddles<T>());
-    } else {
-      hipLaunchKernelGGL((k_row_nl<T, N, L, 1>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip, ny,
-                         tiles, twiddles<T>());
+    if constexpr (sizeof(T) == 4 && N >= 256) {
+      if (tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
+        hipLaunchKernelGGL((k_row_nl<T, N, L, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip,
+                           ny, tiles / 2, twiddles<T>());
+        return check_launch("row_nl");
+      }
     }
+    hipLaunchKernelGGL((k_row_nl<T, N, L, 1>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip, ny,
+                       tiles, twiddles<T>());
     return check_launch("row_nl");
   }
   template <int N, int L, int STAGE>

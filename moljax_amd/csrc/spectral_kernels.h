@@ -333,25 +333,31 @@ __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_gs(cpx<T>* __restr
   // rows past ny (the last tile when TR does not divide ny): transforms only
   bool live[NT];
   cpx<T>* wrow[NT];
-  cpx<T> g[NT][C::P];
+  cpx<T> g[NT > 1 ? NT : 1][C::P];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int y = (tile0 + t) * C::TR + rl;
     live[t] = y < ny;
     wrow[t] = W + ((long long)inst * ny + (live[t] ? y : 0)) * NX;
+    if constexpr (NT > 1) {
 #pragma unroll
-    for (int m = 0; m < C::P; ++m) g[t][m] = live[t] ? wrow[t][j + C::TN * m] : mk((T)0, (T)0);
+      for (int m = 0; m < C::P; ++m) g[t][m] = live[t] ? wrow[t][j + C::TN * m] : mk((T)0, (T)0);
+    }
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    if (t > 0) __syncthreads();  // the previous tile's pack reads of LDS are done
     cpx<T> v[C::P];
+    if constexpr (NT == 1) {
+      load_unpack<T, NX, TR>(wrow[0], live[0], lrow, j, v);
+    } else {
+      if (t > 0) __syncthreads();  // the previous tile's pack reads of LDS are done
 #pragma unroll
-    for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, g[t][m]);
-    __syncthreads();
+      for (int m = 0; m < C::P; ++m) lwr(lrow, j + C::TN * m, g[t][m]);
+      __syncthreads();
 #pragma unroll
-    for (int m = 0; m < C::P; ++m) v[m] = unpack_at<T, NX>(lrow, j + C::TN * m);
-    __syncthreads();
+      for (int m = 0; m < C::P; ++m) v[m] = unpack_at<T, NX>(lrow, j + C::TN * m);
+      __syncthreads();
+    }
     line_fft<T, NX, +1, 1, 1>(v, j, lrow, twc);
 #pragma unroll
     for (int m = 0; m < C::P; ++m) {
@@ -717,15 +723,16 @@ struct RowLaunch {
     const int tiles = (ny + L - 1) / L;
     // Two row tiles per workgroup for FP32 lines of at most 8 values a
     // thread (n >= 512) when the grid keeps >= 2048 workgroups.
-    constexpr bool two = sizeof(T) == 4 && N >= 512 && plan_p(N) <= 8;
-    if (two && tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
-      if (rxn != RXN_HEUN)
-        hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, W,
-                           ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, rxn - RXN_EULER);
-      else
-        hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, W,
-                           ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, 0);
-      return check_launch("row_gs");
+    if constexpr (sizeof(T) == 4 && N >= 512 && plan_p(N) <= 8) {
+      if (tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
+        if (rxn != RXN_HEUN)
+          hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st,
+                             W, ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, rxn - RXN_EULER);
+        else
+          hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_HEUN, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st,
+                             W, ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, 0);
+        return check_launch("row_gs");
+      }
     }
     if (rxn != RXN_HEUN)
       hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER, 1>), dim3(tiles * B), dim3(RowCfg<N, L>::BT), 0, st, W, ip,
