@@ -187,13 +187,15 @@ __global__ __launch_bounds__(256) void k_to_f64(const float* __restrict__ x, dou
 // ------------------------------------------------ device-resident GMRES(m)
 // The Krylov loop keeps every scalar on the device: the Hessenberg column,
 // the Givens rotations, the residual estimate and the convergence test are
-// computed by one-workgroup kernels that finish the fixed-order reductions
-// of the orthogonalisation kernels, and later launches of a finished cycle
-// exit at their first instruction.  The host enqueues Arnoldi steps in
-// chunks and reads the control block back once per chunk (a few times per
-// solve) instead of after every step.  Reductions end at kernel boundaries
-// (partials by RB workgroups, then one workgroup), so no cross-workgroup
-// fence is needed and every sum has the same order on every run.
+// computed on the device, and later launches of a finished cycle exit at
+// their first instruction.  The host enqueues Arnoldi steps in chunks and
+// reads the control block back once per chunk (a few times per solve)
+// instead of after every step.  Reductions end at kernel boundaries: RB
+// workgroups write partials, and the next kernel finishes them in a fixed
+// order, either in every one of its workgroups (the dot products that the
+// basis update consumes, final_sums) or in one workgroup (the norm and the
+// Givens step).  No cross-workgroup fence is needed and every sum has the
+// same order on every run.
 constexpr int RB = 256;  // workgroups of the partial reductions (fixed partition -> deterministic sums)
 constexpr int RT = 256;  // threads per workgroup
 constexpr int VC = 8;    // basis vectors per register chunk
@@ -246,15 +248,32 @@ __device__ void slice_dots(const double* __restrict__ V, const double* w, long l
   }
 }
 
-// One workgroup: out[idx] = fixed-order sum over workgroups of part for idx < k.
-__global__ __launch_bounds__(256) void k_gm_final_h(const double* part, int k, const GmresCtl* ctl,
-                                                    double* __restrict__ out) {
-  if (ctl->done) return;
-  __shared__ double red[4];
-  for (int idx = 0; idx < k; ++idx) {
-    const int c = idx / VC, i = idx % VC;
-    const double s = wg_sum(part[((long long)c * RB + threadIdx.x) * VC + i], red);
-    if (threadIdx.x == 0) out[idx] = s;
+// The fixed-order sums over workgroups of the partial columns idx < k
+// (part[(c*RB + b)*VC + i], idx = c*VC + i), computed by every workgroup of
+// the kernel that consumes them: thread t takes partial t of every index,
+// then each index is summed over the wave (shuffle butterfly) and over the
+// four waves in order, the tree of wg_sum, so every workgroup holds the same
+// bits.  The sums land in hs (LDS); workgroup 0 also writes them to out.
+__device__ void final_sums(const double* part, int k, double* hs, double* red, double* out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c = 0; c * VC < k; ++c) {
+    double v[VC];
+#pragma unroll
+    for (int i = 0; i < VC; ++i) v[i] = c * VC + i < k ? part[((long long)c * RB + threadIdx.x) * VC + i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < VC; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < VC; ++i) red[i * 4 + wv] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < VC && c * VC + (int)threadIdx.x < k) {
+      const int i = threadIdx.x;
+      const double sum = ((red[i * 4] + red[i * 4 + 1]) + red[i * 4 + 2]) + red[i * 4 + 3];
+      hs[c * VC + i] = sum;
+      if (blockIdx.x == 0) out[c * VC + i] = sum;
+    }
+    __syncthreads();
   }
 }
 
@@ -302,15 +321,16 @@ __global__ __launch_bounds__(256) void k_gm_dot1(const double* __restrict__ V, c
   slice_dots(V, w64, n, k, part, red);
 }
 
-// Step j, part 3: w -= V h1, then partials of h2 = V^T w.
+// Step j, part 3: h1 from the partials of part 2 (also written to h1),
+// w -= V h1, then partials of h2 = V^T w.
 __global__ __launch_bounds__(256) void k_gm_upd_dot2(const double* __restrict__ V, double* w, long long n, int k,
-                                                     const double* __restrict__ h1, double* part,
-                                                     const GmresCtl* ctl) {
+                                                     const double* __restrict__ part_in, double* __restrict__ h1,
+                                                     double* __restrict__ part, const GmresCtl* ctl) {
   if (ctl->done) return;
   __shared__ double red[4];
+  __shared__ double redk[4 * VC];
   __shared__ double hs[256];
-  for (int i = threadIdx.x; i < k; i += RT) hs[i] = h1[i];
-  __syncthreads();
+  final_sums(part_in, k, hs, redk, h1);
   for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
     double s = 0.0;
     for (int i = 0; i < k; ++i) s += hs[i] * V[(long long)i * n + g];
@@ -319,15 +339,16 @@ __global__ __launch_bounds__(256) void k_gm_upd_dot2(const double* __restrict__ 
   slice_dots(V, w, n, k, part, red);
 }
 
-// Step j, part 4: w -= V h2 and partials of ||w||^2.
+// Step j, part 4: h2 from the partials of part 3 (also written to h2),
+// w -= V h2 and partials of ||w||^2.
 __global__ __launch_bounds__(256) void k_gm_upd_norm(const double* __restrict__ V, double* w, long long n, int k,
-                                                     const double* __restrict__ h2, double* part,
-                                                     const GmresCtl* ctl) {
+                                                     const double* __restrict__ part_in, double* __restrict__ h2,
+                                                     double* __restrict__ part, const GmresCtl* ctl) {
   if (ctl->done) return;
   __shared__ double red[4];
+  __shared__ double redk[4 * VC];
   __shared__ double hs[256];
-  for (int i = threadIdx.x; i < k; i += RT) hs[i] = h2[i];
-  __syncthreads();
+  final_sums(part_in, k, hs, redk, h2);
   double acc = 0.0;
   for (long long g = (long long)blockIdx.x * RT + threadIdx.x; g < n; g += (long long)RB * RT) {
     double s = 0.0;
@@ -611,6 +632,7 @@ class NKSolver {
     h1_ = p; p += m + 1;
     h2_ = p; p += m + 1;
     part_ = p; p += part_elems(m);
+    part2_ = p; p += part_elems(m);
     p = (double*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
     V_ = p; p += (long long)(m + 1) * n_;
     w_ = p; p += n_;
@@ -645,7 +667,7 @@ class NKSolver {
     const long long n = (long long)ny * nx;
     const int m = restart;
     const long long s2 = precond_work_reals(ny, nx);
-    const long long small = (long long)(m + 1) * m + 2LL * m + 3LL * (m + 1) + part_elems(m);
+    const long long small = (long long)(m + 1) * m + 2LL * m + 3LL * (m + 1) + 2 * part_elems(m);
     long long d = (long long)(m + 1) * n + 6 * n + s2 + 8;
     return 256 + small * 8 + 256 + d * 8 + (4 * n + s2) * 4 + 256;
   }
@@ -779,13 +801,12 @@ class NKSolver {
       hipLaunchKernelGGL(k_gm_dot1<float>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const float*)w32_, w_,
                          n_, k, part_, (const GmresCtl*)ctl_);
     }
+    // partials: dot1 -> part_, upd_dot2 part_ -> part2_, upd_norm part2_ -> part_
     const GmresCtl* cc = ctl_;
-    hipLaunchKernelGGL(k_gm_final_h, dim3(1), dim3(RT), 0, st_, (const double*)part_, k, cc, h1_);
-    hipLaunchKernelGGL(k_gm_upd_dot2, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)h1_,
-                       part_, cc);
-    hipLaunchKernelGGL(k_gm_final_h, dim3(1), dim3(RT), 0, st_, (const double*)part_, k, cc, h2_);
-    hipLaunchKernelGGL(k_gm_upd_norm, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)h2_,
-                       part_, cc);
+    hipLaunchKernelGGL(k_gm_upd_dot2, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)part_,
+                       h1_, part2_, cc);
+    hipLaunchKernelGGL(k_gm_upd_norm, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)part2_,
+                       h2_, part_, cc);
     hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(RT), 0, st_, (const double*)part_, j, c_.restart,
                        (const double*)h1_, (const double*)h2_, ctl_, H_, cs_, sn_, g_);
     return check_launch("arnoldi");
@@ -851,7 +872,7 @@ class NKSolver {
   hipStream_t st_;
   long long n_;
   GmresCtl* ctl_;
-  double *H_, *cs_, *sn_, *g_, *h1_, *h2_, *part_;
+  double *H_, *cs_, *sn_, *g_, *h1_, *h2_, *part_, *part2_;
   double *V_, *w_, *b_, *dx_, *xt_, *t64_, *t64b_, *r2r64_, *inst_;
   float *v32_, *t32_, *w32_, *t32b_, *r2r32_;
   const void *lamx64_, *lamy64_, *lamx32_, *lamy32_;
