@@ -40,8 +40,9 @@ namespace mjx {
 // Win == Wout is allowed (in place): a row is read whole into LDS before any
 // of it is written, and one workgroup owns the row.  A workgroup takes NT
 // consecutive row tiles and issues the loads of all of them first, so the
-// later tiles' loads are in flight while the first is transformed (at
-// P = 8 values a thread, one tile keeps too few bytes in flight).
+// later tiles' loads are in flight while the first is transformed (one tile
+// keeps too few bytes in flight: measured 335 -> 251 us a pass at 512^2 and
+// 69 -> 56 us at 256^2 with NT = 2).
 template <typename T, int NX, int TR, int NT>
 __global__ __launch_bounds__((RowCfg<NX, TR>::BT)) void k_row_nl(const cpx<T>* Win,
                                                                   cpx<T>* Wout,
