@@ -348,51 +348,55 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
   auto done = [&]() {
     if (stile) __syncthreads();
   };
-  // a[m] (+)= k * Z at ky = j + TN m, with k from mult (a table read by
-  // byte offset).  The multipliers of all P points are loaded first, outside
-  // the branch on the special column, so that their loads are in flight
-  // together (loads inside the per-point branch each waited for the last).
-  auto add = [&](cpx<T> (&a)[P], const cpx<T> (&z)[P], bool first, auto mult) {
-    T kA[P];
+  // Multiplier values at this thread's points: A at the field's column sa,
+  // B (special column only) at the partner column sb.  Loads of all P points
+  // are issued back to back, outside the per-point branch on the special
+  // column, so that they are in flight together (loads inside the branch
+  // each waited for the last).
+  struct Mul {
+    T A[P], B[P];
+  };
+  auto tab = [&](Mul& k, auto mult) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) kA[m] = mult(ca + m * CSTEP);
+    for (int m = 0; m < P; ++m) k.A[m] = mult(ca + m * CSTEP);
+    if (spec) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) k.B[m] = mult(cb + m * CSTEP);
+    }
+  };
+  // a[m] (+)= k * Z at ky = j + TN m; a special column combines Z with the
+  // conj partner Z(-ky) from LDS
+  auto addm = [&](cpx<T> (&a)[P], const cpx<T> (&z)[P], bool first, const Mul& k) {
     if (spec) {
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const int ky = j + TN * m;
-        const T cB = mult(cb + m * CSTEP);
         const cpx<T> zm = conjg(lds[((NY - ky) & (NY - 1)) * TC]);
-        const T cp = (T)0.5 * (kA[m] + cB), cm = (T)0.5 * (kA[m] - cB);
+        const T cp = (T)0.5 * (k.A[m] + k.B[m]), cm = (T)0.5 * (k.A[m] - k.B[m]);
         const cpx<T> t = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
         a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
       }
     } else {
 #pragma unroll
       for (int m = 0; m < P; ++m) {
-        const cpx<T> t = scale(z[m], kA[m]);
+        const cpx<T> t = scale(z[m], k.A[m]);
         a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
       }
     }
   };
-  // the same for two accumulators a (multiplier ma) and b (mb), reading a
+  // the same for two accumulators a (multipliers ka) and b (kb), reading a
   // special column's conj partner once
-  auto add2 = [&](cpx<T> (&a)[P], cpx<T> (&b)[P], const cpx<T> (&z)[P], bool first, auto ma, auto mb) {
-    T kA[P], kB[P];
-#pragma unroll
-    for (int m = 0; m < P; ++m) {
-      kA[m] = ma(ca + m * CSTEP);
-      kB[m] = mb(ca + m * CSTEP);
-    }
+  auto add2m = [&](cpx<T> (&a)[P], cpx<T> (&b)[P], const cpx<T> (&z)[P], bool first, const Mul& ka,
+                   const Mul& kb) {
     if (spec) {
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const int ky = j + TN * m;
-        const T aB = ma(cb + m * CSTEP), bB = mb(cb + m * CSTEP);
         const cpx<T> zm = conjg(lds[((NY - ky) & (NY - 1)) * TC]);
-        T cp = (T)0.5 * (kA[m] + aB), cm = (T)0.5 * (kA[m] - aB);
+        T cp = (T)0.5 * (ka.A[m] + ka.B[m]), cm = (T)0.5 * (ka.A[m] - ka.B[m]);
         const cpx<T> t = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
-        cp = (T)0.5 * (kB[m] + bB);
-        cm = (T)0.5 * (kB[m] - bB);
+        cp = (T)0.5 * (kb.A[m] + kb.B[m]);
+        cm = (T)0.5 * (kb.A[m] - kb.B[m]);
         const cpx<T> u = mk(cp * z[m].re + cm * zm.re, cp * z[m].im + cm * zm.im);
         a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
         b[m] = first ? u : mk(b[m].re + u.re, b[m].im + u.im);
@@ -400,11 +404,17 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
     } else {
 #pragma unroll
       for (int m = 0; m < P; ++m) {
-        const cpx<T> t = scale(z[m], kA[m]), u = scale(z[m], kB[m]);
+        const cpx<T> t = scale(z[m], ka.A[m]), u = scale(z[m], kb.A[m]);
         a[m] = first ? t : mk(a[m].re + t.re, a[m].im + t.im);
         b[m] = first ? u : mk(b[m].re + u.re, b[m].im + u.im);
       }
     }
+  };
+  // a[m] (+)= k * Z with k from mult (a table read by byte offset)
+  auto add = [&](cpx<T> (&a)[P], const cpx<T> (&z)[P], bool first, auto mult) {
+    Mul k;
+    tab(k, mult);
+    addm(a, z, first, k);
   };
   // multiplier tables by byte offset
   const T* const tE = coef + CF_E * TSTRIDE;
@@ -455,23 +465,46 @@ __global__ __launch_bounds__((ColCfg<NY, TC_>::BT)) void k_col_rk4(cpx<T>* Ws, c
     return;
   } else if constexpr (STAGE == RK_S3M) {
     // Fh = E u + f1 N_n + f2 (N_a + N_b) and c = E2^2 u + (E2 Q - Q) N_n
-    // + 2 Q N_b from one y-FFT of each of u, N_n, N_b, N_a
+    // + 2 Q N_b from one y-FFT of each of u, N_n, N_b, N_a.  Each of the
+    // five tables is read once: E2 stays in registers from the u term to
+    // the N_n term, Q from N_n to N_b, f2 from N_b to N_a.
     cpx<T> cc[P];
+    Mul k1, k2, kq, kd;
     ld(ws, za);  // u
     ld(p1, zb);  // N_n
     xf(za);
-    add2(acc, cc, za, true, E, [&](unsigned i) { const T e2 = E2(i); return e2 * e2; });
+    tab(k1, E);
+    tab(k2, E2);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      kd.A[m] = k2.A[m] * k2.A[m];
+      if (spec) kd.B[m] = k2.B[m] * k2.B[m];
+    }
+    add2m(acc, cc, za, true, k1, kd);
     done();
     ld(p3, za);  // N_b
     xf(zb);
-    add2(acc, cc, zb, false, F1, [&](unsigned i) { const T q = Q(i); return E2(i) * q - q; });
+    tab(k1, F1);
+    tab(kq, Q);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      kd.A[m] = k2.A[m] * kq.A[m] - kq.A[m];
+      if (spec) kd.B[m] = k2.B[m] * kq.B[m] - kq.B[m];
+    }
+    add2m(acc, cc, zb, false, k1, kd);
     done();
     ld(p2, zb);  // N_a
     xf(za);
-    add2(acc, cc, za, false, F2, [&](unsigned i) { return (T)2 * Q(i); });
+    tab(k1, F2);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      kd.A[m] = (T)2 * kq.A[m];
+      if (spec) kd.B[m] = (T)2 * kq.B[m];
+    }
+    add2m(acc, cc, za, false, k1, kd);
     done();
     xf(zb);
-    add(acc, zb, false, F2);
+    addm(acc, zb, false, k1);
     done();
 #pragma unroll
     for (int m = 0; m < P; ++m) st_at(p2, toff + m * RSTEP, acc[m]);  // Fh: y-spectral, unscaled
