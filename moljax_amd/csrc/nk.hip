@@ -277,16 +277,107 @@ __device__ void final_sums(const double* part, int k, double* hs, double* red, d
   }
 }
 
-// Step j, part 1: v_j = w * inv_hn (written to V_j), t = J v_j in T.
-template <typename T>
+// Arrays of the Hessenberg / Givens recursion (m = restart length).
+struct GivensArrays {
+  const double* part;  // norm partials of the last CGS2 pass (RB values)
+  const double* h1;    // CGS2 coefficients of the two passes (k values each)
+  const double* h2;
+  double* H;           // (m + 1) x m Hessenberg matrix, rotated
+  double* cs;          // rotations
+  double* sn;
+  double* gv;          // rotated right-hand side (back substitution)
+  double* ga;          // |g_j| before rotation j (ga[0] = beta)
+};
+
+// Givens step of Arnoldi step j: hn = ||w|| from the norm partials (the
+// fixed-order tree of wg_sum), Hessenberg column j (h1 + h2, hn), the
+// previous rotations, rotation j and the residual estimate |g_{j+1}| =
+// |sn_j| |g_j|; the cycle ends on convergence (<= target) or breakdown
+// (hn == 0).  Every workgroup that calls it computes the same bits from
+// values written by earlier launches (h1, h2, cs/sn[0, j), ga[j]); only the
+// writer stores column j, rotation j, g, ga[j + 1] and the control fields.
+// Returns 1 / hn in *inv_hn (all threads) and whether the cycle is done.
+__device__ bool givens_step(const GivensArrays& ga, int j, int m, GmresCtl* ctl, bool writer, double* red,
+                            double* col, double* inv_hn) {
+  __shared__ double bc[2];
+  __shared__ double rc[2][256];  // rotations 0..j-1 (restart <= 200)
+  const int k = j + 1;
+  // the column and the rotations are loaded by the workgroup's threads in
+  // parallel (the norm's barriers publish them), then thread 0 recurses
+  const int t = threadIdx.x;
+  if (t < k) col[t] = ga.h1[t] + ga.h2[t];
+  if (t < j) {
+    rc[0][t] = ga.cs[t];
+    rc[1][t] = ga.sn[t];
+  }
+  const double hn2 = wg_sum(ga.part[t], red);
+  if (t == 0) {
+    const double hn = sqrt(hn2);
+    col[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+      const double a0 = col[i], a1 = col[i + 1];
+      col[i] = rc[0][i] * a0 + rc[1][i] * a1;
+      col[i + 1] = -rc[1][i] * a0 + rc[0][i] * a1;
+    }
+    const double hjj = col[j], hj1 = col[j + 1];
+    const double den = hypot(hjj, hj1);
+    const double c = hjj / den, s = hj1 / den;
+    const double res = fabs(s) * ga.ga[j];
+    const bool done = res <= ctl->target || hn == 0.0;
+    if (writer) {
+      for (int i = 0; i < j; ++i) ga.H[(long long)i * m + j] = col[i];
+      ga.H[(long long)j * m + j] = den;
+      ga.H[(long long)(j + 1) * m + j] = 0.0;
+      ga.cs[j] = c;
+      ga.sn[j] = s;
+      const double gj = ga.gv[j];
+      ga.gv[j + 1] = -s * gj;
+      ga.gv[j] = c * gj;
+      ga.ga[j + 1] = res;
+      ctl->res = res;
+      ctl->k = k;
+      ctl->iters += 1;
+      if (done) {
+        ctl->done = 1;
+        ctl->conv = res <= ctl->target ? 1 : 0;
+      } else {
+        ctl->inv_hn = 1.0 / hn;
+      }
+    }
+    bc[0] = done ? 0.0 : 1.0 / hn;
+    bc[1] = done ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  *inv_hn = bc[0];
+  return bc[1] != 0.0;
+}
+
+// Step j, part 1: v_j = w * inv_hn (written to V_j), t = J v_j in T.  With
+// FOLD the kernel first makes the Givens step of Arnoldi step j - 1 (in
+// every workgroup, workgroup 0 storing it), which saves its one-workgroup
+// launch; without it 1 / hn comes from the control block.
+template <typename T, bool FOLD>
 __global__ __launch_bounds__(256) void k_gm_jvp(const double* __restrict__ u, const double* __restrict__ w,
-                                                const GmresCtl* ctl, double* __restrict__ Vj, T* __restrict__ t,
-                                                RDArgs a) {
-  if (ctl->done) return;
+                                                GmresCtl* ctl, double* __restrict__ Vj, T* __restrict__ t,
+                                                RDArgs a, GivensArrays gva, int jprev, int m) {
+  double s;
+  if constexpr (FOLD) {
+    // the control block's done flag as of this launch, read once per
+    // workgroup (workgroup 0 may set it during the launch)
+    __shared__ int done0;
+    __shared__ double red[4];
+    __shared__ double col[256];  // restart <= 200
+    if (threadIdx.x == 0) done0 = ctl->done;
+    __syncthreads();
+    if (done0) return;
+    if (givens_step(gva, jprev, m, ctl, blockIdx.x == 0, red, col, &s)) return;
+  } else {
+    if (ctl->done) return;
+    s = ctl->inv_hn;
+  }
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)a.ny * a.nx;
   if (g >= n) return;
-  const double s = ctl->inv_hn;
   const int y = (int)(g / a.nx), x = (int)(g % a.nx);
   const double vc = w[g] * s;
   Vj[g] = vc;
@@ -361,44 +452,14 @@ __global__ __launch_bounds__(256) void k_gm_upd_norm(const double* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = ps;
 }
 
-// Step j, part 5 (one workgroup): hn = ||w||, Hessenberg column j (h1 + h2,
-// hn), the previous rotations, rotation j, g and the residual estimate; ends
-// the cycle on convergence (|g_{j+1}| <= target) or breakdown (hn == 0).
-__global__ __launch_bounds__(256) void k_gm_givens(const double* part, int j, int m, const double* __restrict__ h1,
-                                                   const double* __restrict__ h2, GmresCtl* ctl, double* H,
-                                                   double* cs, double* sn, double* gv) {
+// Step j, part 5 (one workgroup): the Givens step of Arnoldi step j, where
+// the next step's JVP does not make it (the last step of a chunk).
+__global__ __launch_bounds__(256) void k_gm_givens(GivensArrays gva, int j, int m, GmresCtl* ctl) {
   if (ctl->done) return;
   __shared__ double red[4];
-  const double hn2 = wg_sum(part[threadIdx.x], red);
-  if (threadIdx.x != 0) return;
-  const int k = j + 1;
-  const double hn = sqrt(hn2);
-  for (int i = 0; i < k; ++i) H[(long long)i * m + j] = h1[i] + h2[i];
-  H[(long long)(j + 1) * m + j] = hn;
-  for (int i = 0; i < j; ++i) {
-    const double a0 = H[(long long)i * m + j], a1 = H[(long long)(i + 1) * m + j];
-    H[(long long)i * m + j] = cs[i] * a0 + sn[i] * a1;
-    H[(long long)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
-  }
-  const double hjj = H[(long long)j * m + j], hj1 = H[(long long)(j + 1) * m + j];
-  const double den = hypot(hjj, hj1);
-  cs[j] = hjj / den;
-  sn[j] = hj1 / den;
-  H[(long long)j * m + j] = den;
-  H[(long long)(j + 1) * m + j] = 0.0;
-  const double gj = gv[j];
-  gv[j + 1] = -sn[j] * gj;
-  gv[j] = cs[j] * gj;
-  const double res = fabs(gv[j + 1]);
-  ctl->res = res;
-  ctl->k = k;
-  ctl->iters += 1;
-  if (res <= ctl->target || hn == 0.0) {
-    ctl->done = 1;
-    ctl->conv = res <= ctl->target ? 1 : 0;
-  } else {
-    ctl->inv_hn = 1.0 / hn;
-  }
+  __shared__ double col[256];  // restart <= 200
+  double inv_hn;
+  givens_step(gva, j, m, ctl, true, red, col, &inv_hn);
 }
 
 // Partials of ||w||^2 (no control check: cycle starts always run).
@@ -412,7 +473,8 @@ __global__ __launch_bounds__(256) void k_norm2_partial(const double* w, long lon
 
 // Start of a restart cycle (one workgroup): beta = ||w|| (w = M (b - A x), or
 // M b in cycle 0, where the target rtol * ||M b|| is set).
-__global__ __launch_bounds__(256) void k_gm_cycle_start(const double* part, int first, GmresCtl* ctl, double* gv) {
+__global__ __launch_bounds__(256) void k_gm_cycle_start(const double* part, int first, GmresCtl* ctl, double* gv,
+                                                        double* ga) {
   __shared__ double red[4];
   const double b2 = wg_sum(part[threadIdx.x], red);
   if (threadIdx.x != 0) return;
@@ -431,6 +493,7 @@ __global__ __launch_bounds__(256) void k_gm_cycle_start(const double* part, int 
   ctl->done = 0;
   ctl->conv = 0;
   gv[0] = beta;
+  ga[0] = beta;
   ctl->inv_hn = 1.0 / beta;
 }
 
@@ -611,9 +674,11 @@ static GmresCtl* ctl_host() {
   return g_ctl_host[dev];
 }
 
-// Arnoldi steps enqueued before the first read-back of a solve: the previous
-// solve's count (a restart cycle at most).
-static thread_local int g_gmres_hint = 4;
+// Arnoldi steps enqueued before the first read-back of a solve: the count
+// of the previous solve at the same Newton iteration (successive implicit
+// steps repeat their per-iteration counts closely), a restart cycle at most.
+constexpr int NK_HINTS = 16;
+static thread_local int g_gmres_hint[NK_HINTS] = {4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
 
 class NKSolver {
  public:
@@ -631,6 +696,7 @@ class NKSolver {
     g_ = p; p += m + 1;
     h1_ = p; p += m + 1;
     h2_ = p; p += m + 1;
+    ga_ = p; p += m + 1;
     part_ = p; p += part_elems(m);
     part2_ = p; p += part_elems(m);
     p = (double*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
@@ -667,7 +733,7 @@ class NKSolver {
     const long long n = (long long)ny * nx;
     const int m = restart;
     const long long s2 = precond_work_reals(ny, nx);
-    const long long small = (long long)(m + 1) * m + 2LL * m + 3LL * (m + 1) + 2 * part_elems(m);
+    const long long small = (long long)(m + 1) * m + 2LL * m + 4LL * (m + 1) + 2 * part_elems(m);
     long long d = (long long)(m + 1) * n + 6 * n + s2 + 8;
     return 256 + small * 8 + 256 + d * 8 + (4 * n + s2) * 4 + 256;
   }
@@ -685,7 +751,7 @@ class NKSolver {
     while (it < c_.max_newton && !converged) {
       if (it > 0) residual();
       int lin = 0;
-      if ((rc = gmres(b_, dx_, hc, &lin))) return rc;
+      if ((rc = gmres(b_, dx_, hc, &lin, std::min(it, NK_HINTS - 1)))) return rc;
       lin_total += lin;
       converged = hc->newton_conv != 0;  // this iteration's test (made before its update)
       for (int t = 0; t < c_.max_backtrack; ++t) {
@@ -711,7 +777,7 @@ class NKSolver {
   // Left-preconditioned restarted GMRES(m) for J dx = b, x0 = 0.  One
   // iteration = one Arnoldi step; the Givens residual is compared with rtol *
   // ||M b|| after every step; at most max_krylov restart cycles.
-  int gmres(const double* bvec, double* x, GmresCtl* hc, int* iters_out) {
+  int gmres(const double* bvec, double* x, GmresCtl* hc, int* iters_out, int hint_slot) {
     const int m = c_.restart;
     int rc;
     hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, bvec, 0.0, x, n_);
@@ -726,17 +792,20 @@ class NKSolver {
       }
       hipLaunchKernelGGL(k_norm2_partial, dim3(RB), dim3(RT), 0, st_, (const double*)w_, n_, part_);
       hipLaunchKernelGGL(k_gm_cycle_start, dim3(1), dim3(RT), 0, st_, (const double*)part_, cyc == 0 ? 1 : 0, ctl_,
-                         g_);
+                         g_, ga_);
       if (cyc > 0) {  // a restart may find the target met before any step
         if ((rc = read_ctl(hc))) return rc;
         if (hc->done) break;
       }
       int j = 0;
-      int chunk = std::max(1, std::min(m, g_gmres_hint));
+      int chunk = std::max(1, std::min(m, g_gmres_hint[hint_slot]));
       while (j < m) {
         const int c = std::min(chunk, m - j);
+        // each step's JVP makes the previous step's Givens step, except the
+        // chunk's first; the chunk's last step launches its own before the
+        // control block is read
         for (int jj = j; jj < j + c; ++jj)
-          if ((rc = arnoldi_step(jj))) return rc;
+          if ((rc = arnoldi_step(jj, jj > j, jj == j + c - 1))) return rc;
         j += c;
         if ((rc = read_ctl(hc))) return rc;
         if (hc->done) break;
@@ -748,7 +817,7 @@ class NKSolver {
       if (hc->conv) break;
     }
     iters = hc->iters;
-    if (iters > 0) g_gmres_hint = iters;
+    if (iters > 0) g_gmres_hint[hint_slot] = iters;
     *iters_out = iters;
     return check_launch("gmres");
   }
@@ -783,19 +852,35 @@ class NKSolver {
 
   // One Arnoldi step j of the current cycle (every kernel exits at once when
   // the cycle has already finished).
-  int arnoldi_step(int j) {
+  GivensArrays givens_arrays() const {
+    return GivensArrays{part_, h1_, h2_, H_, cs_, sn_, g_, ga_};
+  }
+
+  template <typename T>
+  void launch_jvp(int j, bool fold, T* t) {
     double* Vj = V_ + (long long)j * n_;
+    const GivensArrays gva = givens_arrays();
+    if (fold)
+      hipLaunchKernelGGL((k_gm_jvp<T, true>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
+                         (const double*)w_, ctl_, Vj, t, a_, gva, j - 1, c_.restart);
+    else
+      hipLaunchKernelGGL((k_gm_jvp<T, false>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
+                         (const double*)w_, ctl_, Vj, t, a_, gva, j - 1, c_.restart);
+  }
+
+  // Arnoldi step j.  fold: its JVP makes the Givens step of step j - 1;
+  // givens_now: step j's own Givens step is launched here (otherwise the
+  // next step's JVP makes it).
+  int arnoldi_step(int j, bool fold, bool givens_now) {
     const int k = j + 1;
     if (c_.mv_dtype == MJX_F64) {
-      hipLaunchKernelGGL(k_gm_jvp<double>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, (const double*)w_,
-                         (const GmresCtl*)ctl_, Vj, t64_, a_);
+      launch_jvp<double>(j, fold, t64_);
       int rc = precond_t<double>(t64_, w_, t64b_, r2r64_, lamx64_, lamy64_);
       if (rc) return rc;
       hipLaunchKernelGGL(k_gm_dot1<double>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const double*)w_, w_,
                          n_, k, part_, (const GmresCtl*)ctl_);
     } else {
-      hipLaunchKernelGGL(k_gm_jvp<float>, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_, (const double*)w_,
-                         (const GmresCtl*)ctl_, Vj, t32_, a_);
+      launch_jvp<float>(j, fold, t32_);
       int rc = precond32(t32_, w32_);
       if (rc) return rc;
       hipLaunchKernelGGL(k_gm_dot1<float>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const float*)w32_, w_,
@@ -807,8 +892,7 @@ class NKSolver {
                        h1_, part2_, cc);
     hipLaunchKernelGGL(k_gm_upd_norm, dim3(RB), dim3(RT), 0, st_, (const double*)V_, w_, n_, k, (const double*)part2_,
                        h2_, part_, cc);
-    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(RT), 0, st_, (const double*)part_, j, c_.restart,
-                       (const double*)h1_, (const double*)h2_, ctl_, H_, cs_, sn_, g_);
+    if (givens_now) hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(RT), 0, st_, givens_arrays(), j, c_.restart, ctl_);
     return check_launch("arnoldi");
   }
 
@@ -872,7 +956,7 @@ class NKSolver {
   hipStream_t st_;
   long long n_;
   GmresCtl* ctl_;
-  double *H_, *cs_, *sn_, *g_, *h1_, *h2_, *part_, *part2_;
+  double *H_, *cs_, *sn_, *g_, *h1_, *h2_, *ga_, *part_, *part2_;
   double *V_, *w_, *b_, *dx_, *xt_, *t64_, *t64b_, *r2r64_, *inst_;
   float *v32_, *t32_, *w32_, *t32b_, *r2r32_;
   const void *lamx64_, *lamy64_, *lamx32_, *lamy32_;
