@@ -556,17 +556,65 @@ __global__ __launch_bounds__(256) void k_nk_residual_final(const double* part, G
   ctl->moved = 0;
 }
 
-// One backtracking trial (newton_krylov.py:272-316): x_t = u + alpha dx and
-// partials of ||R(x_t)||^2, its neighbours formed the same way.  Trials after
-// an acceptance leave the state unchanged in the reference, so they are
-// skipped.
+// Acceptance of a backtracking trial from its residual partials: accept
+// when ||R(x_t)|| < (1 - min_dec alpha) r_bt, r_bt being the iterate's
+// residual norm rn until a trial is accepted (no trial runs after that);
+// otherwise the next trial halves alpha.  Computed the same way in every
+// workgroup that calls it; only the writer stores the result.
+__device__ bool trial_accepts(const double* part, double alpha, GmresCtl* ctl, bool writer, double* red) {
+  __shared__ int acc;
+  const double r2 = wg_sum(part[threadIdx.x], red);
+  if (threadIdx.x == 0) {
+    const double rnew = sqrt(r2);
+    const bool a = rnew < (1.0 - ctl->min_dec * alpha) * ctl->rn;
+    if (writer) {
+      if (a) {
+        ctl->accepted = 1;
+        ctl->r_bt = rnew;
+      } else {
+        ctl->alpha = alpha * ctl->bt_factor;
+      }
+    }
+    acc = a ? 1 : 0;
+  }
+  __syncthreads();
+  return acc != 0;
+}
+
+// alpha of trial t: damping halved t times, in the order of the sequential
+// updates
+__device__ __forceinline__ double trial_alpha(const GmresCtl* ctl, int t) {
+  double a = ctl->damping;
+  for (int i = 0; i < t; ++i) a = a * ctl->bt_factor;
+  return a;
+}
+
+// The accepted flag as of this launch, read once per workgroup (workgroup 0
+// of a launch that decides the previous trial may set it meanwhile).
+__device__ __forceinline__ bool accepted_at_launch(const GmresCtl* ctl) {
+  __shared__ int a0;
+  if (threadIdx.x == 0) a0 = ctl->accepted;
+  __syncthreads();
+  return a0 != 0;
+}
+
+// Backtracking trial t (newton_krylov.py:272-316): x_t = u + alpha_t dx and
+// partials of ||R(x_t)||^2 (into part_out), its neighbours formed the same
+// way.  With FOLD the kernel first decides trial t - 1 from its partials
+// (part_prev) in every workgroup, workgroup 0 storing the decision, which
+// saves that decision's one-workgroup launch.  Trials after an acceptance
+// leave the state unchanged in the reference, so they are skipped.
+template <bool FOLD>
 __global__ __launch_bounds__(256) void k_nk_trial(const double* __restrict__ u, const double* __restrict__ dx,
                                                   double* __restrict__ xt, const double* __restrict__ u_old,
-                                                  const double* __restrict__ fold, RDArgs a, double* part,
-                                                  const GmresCtl* ctl) {
-  if (ctl->accepted) return;
+                                                  const double* __restrict__ fold, RDArgs a, const double* part_prev,
+                                                  double* part_out, int t, GmresCtl* ctl) {
+  if (accepted_at_launch(ctl)) return;
   __shared__ double red[4];
-  const double alpha = ctl->alpha;
+  if constexpr (FOLD) {
+    if (trial_accepts(part_prev, trial_alpha(ctl, t - 1), ctl, blockIdx.x == 0, red)) return;
+  }
+  const double alpha = trial_alpha(ctl, t);
   const long long n = (long long)a.ny * a.nx;
   auto xa = [&](int yy, int xx) -> double {
     if (a.bc == 1) {
@@ -591,31 +639,21 @@ __global__ __launch_bounds__(256) void k_nk_trial(const double* __restrict__ u, 
     acc += r * r;
   }
   const double ps = wg_sum(acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = ps;
-}
-
-// One workgroup: accept when ||R(x_t)|| < (1 - min_dec alpha) r_bt, halve
-// alpha otherwise.
-__global__ __launch_bounds__(256) void k_nk_trial_final(const double* part, GmresCtl* ctl) {
-  if (ctl->accepted) return;
-  __shared__ double red[4];
-  const double r2 = wg_sum(part[threadIdx.x], red);
-  if (threadIdx.x != 0) return;
-  const double rnew = sqrt(r2);
-  const double alpha = ctl->alpha;
-  if (rnew < (1.0 - ctl->min_dec * alpha) * ctl->r_bt) {
-    ctl->accepted = 1;
-    ctl->r_bt = rnew;
-  } else {
-    ctl->alpha = alpha * ctl->bt_factor;
-  }
+  if (threadIdx.x == 0) part_out[blockIdx.x] = ps;
 }
 
 // moved = 1 if any |x_t - u| > 1e-8 + 1e-5 |u| (jnp.allclose defaults), for
-// an accepted trial.
+// an accepted trial.  With FOLD the kernel first decides the last trial (t)
+// from its partials, as k_nk_trial does for the one before it.
+template <bool FOLD>
 __global__ __launch_bounds__(256) void k_nk_allclose(const double* __restrict__ xt, const double* __restrict__ u,
-                                                     long long n, GmresCtl* ctl) {
-  if (!ctl->accepted) return;
+                                                     long long n, const double* part_last, int t, GmresCtl* ctl) {
+  bool acc = accepted_at_launch(ctl);
+  if constexpr (FOLD) {
+    __shared__ double red[4];
+    if (!acc) acc = trial_accepts(part_last, trial_alpha(ctl, t), ctl, blockIdx.x == 0, red);
+  }
+  if (!acc) return;
   __shared__ int any;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
@@ -754,13 +792,24 @@ class NKSolver {
       if ((rc = gmres(b_, dx_, hc, &lin, std::min(it, NK_HINTS - 1)))) return rc;
       lin_total += lin;
       converged = hc->newton_conv != 0;  // this iteration's test (made before its update)
-      for (int t = 0; t < c_.max_backtrack; ++t) {
-        hipLaunchKernelGGL(k_nk_trial, dim3(RB), dim3(RT), 0, st_, (const double*)u_, (const double*)dx_, xt_,
-                           u_old_, fold_, a_, part_, (const GmresCtl*)ctl_);
-        hipLaunchKernelGGL(k_nk_trial_final, dim3(1), dim3(RT), 0, st_, (const double*)part_, ctl_);
+      // trial t writes its partials to buffer t % 2; the next trial (or the
+      // allclose pass after the last) decides it
+      double* tp[2] = {part_, part2_};
+      const int nbt = c_.max_backtrack;
+      for (int t = 0; t < nbt; ++t) {
+        if (t == 0)
+          hipLaunchKernelGGL((k_nk_trial<false>), dim3(RB), dim3(RT), 0, st_, (const double*)u_, (const double*)dx_,
+                             xt_, u_old_, fold_, a_, (const double*)nullptr, tp[0], 0, ctl_);
+        else
+          hipLaunchKernelGGL((k_nk_trial<true>), dim3(RB), dim3(RT), 0, st_, (const double*)u_, (const double*)dx_,
+                             xt_, u_old_, fold_, a_, (const double*)tp[(t - 1) % 2], tp[t % 2], t, ctl_);
       }
-      hipLaunchKernelGGL(k_nk_allclose, dim3(nblk(n_)), dim3(256), 0, st_, (const double*)xt_, (const double*)u_,
-                         n_, ctl_);
+      if (nbt > 0)
+        hipLaunchKernelGGL((k_nk_allclose<true>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)xt_,
+                           (const double*)u_, n_, (const double*)tp[(nbt - 1) % 2], nbt - 1, ctl_);
+      else
+        hipLaunchKernelGGL((k_nk_allclose<false>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)xt_,
+                           (const double*)u_, n_, (const double*)nullptr, 0, ctl_);
       hipLaunchKernelGGL(k_nk_update, dim3(nblk(n_)), dim3(256), 0, st_, u_, (const double*)xt_,
                          (const double*)dx_, n_, (const GmresCtl*)ctl_);
       ++it;
