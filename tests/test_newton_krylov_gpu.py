@@ -209,3 +209,30 @@ def test_c3_512_be_steps_match_oracle(gpu):
                 assert err < 1e-10
             else:
                 assert err < 1e-6
+
+
+@pytest.mark.parametrize("max_backtrack", [0, 1, 3])
+def test_rejected_backtracking_trials_match_oracle(gpu, max_backtrack):
+    """Every backtracking trial rejected (min_residual_decrease = 1.2 asks
+    ||R(x_t)|| < (1 - 1.2 alpha) r_bt, which the trials at alpha = 1, 0.5,
+    0.25 miss by 20 % or more), so each Newton iteration takes the damped
+    fallback x + damping dx (newton_krylov.py:272-316), decided on the device
+    by the launch that follows each trial; and no trials at all."""
+    from moljax_amd.core.newton_krylov import NKParams
+    from moljax_amd.core.preconditioners import DSTDiffusionPreconditioner
+    from moljax_amd.core.reaction_diffusion import be_step, create_reaction_diffusion_model, sine_mode_ic
+    n, D, dt, bc = 63, 1e-3, 1e-2, "dirichlet"
+    model = create_reaction_diffusion_model(n, D=D, reaction=AC, bc=bc)
+    u0 = sine_mode_ic(n, seed=5)
+    P = DSTDiffusionPreconditioner(field_diffusivity_keys={"u": "D"})
+    nkp = NKParams(krylov_tol=1e-8, gmres_restart=30, min_residual_decrease=1.2, max_backtrack=max_backtrack)
+    y, st = be_step(model, {"u": torch.from_numpy(u0)}, 0.0, dt, P, nkp)
+    guess = u0 + dt * onk.rhs(u0, D, AC, model.dy, model.dx, bc)
+    ref = onk.newton_krylov(guess, u0, dt, 1.0, D, AC, model.dy, model.dx, bc, precond="dst", krylov_tol=1e-8,
+                            min_decrease=1.2, max_backtrack=max_backtrack)
+    print(f"max_backtrack {max_backtrack}: newton {st.newton_iters}/{ref.newton_iters} "
+          f"gmres {st.lin_iters}/{ref.gmres_iters}", flush=True)
+    assert st.converged == ref.converged and st.newton_iters == ref.newton_iters
+    assert st.lin_iters == ref.gmres_iters
+    assert ogs.rel_l2(_np(y["u"]), ref.u) < 1e-10
+    assert abs(st.final_res_norm - ref.res_norm) <= 1e-6 * ref.res_norm + 1e-14
