@@ -23,8 +23,19 @@ Pinning (what fixes each function independently of this restatement):
                                central finite differences, GMRES against a
                                dense solve, the DST preconditioner's exact
                                inverse, BDF2 coefficients and order.
+  newton_krylov.gmres          its iteration count (tests/test_oracle_pins.py):
+                               1 for M A = I, k for a diagonal operator with k
+                               distinct eigenvalues that b touches, m per
+                               restart cycle when the tolerance is never met,
+                               0 for b = 0; planted off-by-one mistakes fail.
+  newton_krylov.newton_krylov  Newton / GMRES counts of the linear problem
+                               (one GMRES step per sine mode of the residual)
+                               and of pure diffusion (one per Newton step).
   fd.*                         sine modes (exact discrete eigenvalues) and the
                                periodic wrap of the ghost fill.
+  fd.d1_upwind_2d              exact slope of a linear field for either sign
+                               of the velocity, and the one-sided difference
+                               it picks on a quadratic.
   nonperiodic.*                scipy.fft.dst / dct of the same definitions,
                                brute-force DST-II, FD residual on filled ghosts.
   etdrk4.*                     contour limits (Q -> dt/2, f1..f3 -> dt/6), the
@@ -33,6 +44,10 @@ Pinning (what fixes each function independently of this restatement):
   adaptive.*                   controller closed forms, fixed-point dt
                                sequence, first-attempt error in closed form,
                                status codes.
+  adaptive.implicit_robustness_dt and adaptive_integrate_implicit: the dt
+                               rules case by case, and the saturated-growth
+                               dt sequence (and rejection) of y' = lam y with
+                               D = 0 in closed form.
   jit_kernels.*                phi_k(0) = 1/k! and the phi recurrences, ETD1 on
                                a Fourier mode, ETD2 / ETDRK4 with equal N
                                reduce to ETD1, the Helmholtz inverse on a mode.
