@@ -646,6 +646,13 @@ struct EtdLaunch {
     // flight (FP32, n >= 256; measured at 512^2: 335 -> 251 us a pass) and
     // the grid keeps >= 2048 workgroups (small batches lose more from half
     // the workgroups than they gain).
+    if constexpr (sizeof(T) == 4 && N >= 512) {
+      if (tiles % 4 == 0 && (long long)(tiles / 4) * B >= 2048) {
+        hipLaunchKernelGGL((k_row_nl<T, N, L, 4>), dim3(tiles / 4 * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip,
+                           ny, tiles / 4, twiddles<T>());
+        return check_launch("row_nl");
+      }
+    }
     if constexpr (sizeof(T) == 4 && N >= 256) {
       if (tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
         hipLaunchKernelGGL((k_row_nl<T, N, L, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st, Win, Wout, ip,
