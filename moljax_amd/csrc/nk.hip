@@ -161,12 +161,6 @@ __global__ __launch_bounds__(256) void k_multiaxpy(const double* __restrict__ V,
   w[g] += sgn * s;
 }
 
-// (x and y may alias)
-__global__ __launch_bounds__(256) void k_scale_copy(const double* x, double a, double* y, long long n) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) y[g] = a * x[g];
-}
-
 // (z may alias x or y)
 __global__ __launch_bounds__(256) void k_axpby(double a, const double* x, double b, const double* y, double* z,
                                                long long n) {
@@ -352,16 +346,63 @@ __device__ bool givens_step(const GivensArrays& ga, int j, int m, GmresCtl* ctl,
   return bc[1] != 0.0;
 }
 
+// Start of a restart cycle from the norm partials of w (w = M (b - A x), or
+// M b in cycle 0, where the target rtol * ||M b|| is set): beta = ||w|| and
+// the cycle's initial state.  Computed the same way in every workgroup that
+// calls it; only the writer stores it.  Returns 1 / beta in *inv_beta (all
+// threads) and whether the cycle is done before any step.
+__device__ bool cycle_start_step(const double* part, bool first, GmresCtl* ctl, double* gv, double* ga, bool writer,
+                                 double* red, double* inv_beta) {
+  __shared__ double bc[2];
+  const double b2 = wg_sum(part[threadIdx.x], red);
+  if (threadIdx.x == 0) {
+    const double beta = sqrt(b2);
+    const double target = first ? ctl->rtol * beta : ctl->target;
+    const bool done = beta == 0.0 || beta <= target;
+    if (writer) {
+      ctl->k = 0;
+      if (first) {
+        ctl->iters = 0;
+        ctl->target = target;
+      }
+      ctl->beta = beta;
+      if (done) {
+        ctl->done = 1;
+        ctl->conv = 1;
+      } else {
+        ctl->done = 0;
+        ctl->conv = 0;
+        gv[0] = beta;
+        ga[0] = beta;
+        ctl->inv_hn = 1.0 / beta;
+      }
+    }
+    bc[0] = done ? 0.0 : 1.0 / beta;
+    bc[1] = done ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  *inv_beta = bc[0];
+  return bc[1] != 0.0;
+}
+
 // Step j, part 1: v_j = w * inv_hn (written to V_j), t = J v_j in T.  With
-// FOLD the kernel first makes the Givens step of Arnoldi step j - 1 (in
-// every workgroup, workgroup 0 storing it), which saves its one-workgroup
-// launch; without it 1 / hn comes from the control block.
-template <typename T, bool FOLD>
+// JVP_GIVENS the kernel first makes the Givens step of Arnoldi step j - 1,
+// with JVP_START (step 0 of the first cycle) the cycle start, from the norm
+// partials in gva.part (in every workgroup, workgroup 0 storing the
+// result), which saves a one-workgroup launch; with JVP_PLAIN 1 / hn comes
+// from the control block.
+enum { JVP_PLAIN = 0, JVP_GIVENS = 1, JVP_START = 2 };
+
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_gm_jvp(const double* __restrict__ u, const double* __restrict__ w,
                                                 GmresCtl* ctl, double* __restrict__ Vj, T* __restrict__ t,
                                                 RDArgs a, GivensArrays gva, int jprev, int m) {
   double s;
-  if constexpr (FOLD) {
+  if constexpr (MODE == JVP_START) {
+    // a cycle start resets the done flag of the previous solve: no check
+    __shared__ double red[4];
+    if (cycle_start_step(gva.part, true, ctl, gva.gv, gva.ga, blockIdx.x == 0, red, &s)) return;
+  } else if constexpr (MODE == JVP_GIVENS) {
     // the control block's done flag as of this launch, read once per
     // workgroup (workgroup 0 may set it during the launch)
     __shared__ int done0;
@@ -471,30 +512,12 @@ __global__ __launch_bounds__(256) void k_norm2_partial(const double* w, long lon
   if (threadIdx.x == 0) part[blockIdx.x] = ps;
 }
 
-// Start of a restart cycle (one workgroup): beta = ||w|| (w = M (b - A x), or
-// M b in cycle 0, where the target rtol * ||M b|| is set).
+// Cycle start in one workgroup (restart cycles, whose start the host reads).
 __global__ __launch_bounds__(256) void k_gm_cycle_start(const double* part, int first, GmresCtl* ctl, double* gv,
                                                         double* ga) {
   __shared__ double red[4];
-  const double b2 = wg_sum(part[threadIdx.x], red);
-  if (threadIdx.x != 0) return;
-  const double beta = sqrt(b2);
-  ctl->k = 0;
-  if (first) {
-    ctl->iters = 0;
-    ctl->target = ctl->rtol * beta;
-  }
-  ctl->beta = beta;
-  if (beta == 0.0 || beta <= ctl->target) {
-    ctl->done = 1;
-    ctl->conv = 1;
-    return;
-  }
-  ctl->done = 0;
-  ctl->conv = 0;
-  gv[0] = beta;
-  ga[0] = beta;
-  ctl->inv_hn = 1.0 / beta;
+  double inv_beta;
+  cycle_start_step(part, first != 0, ctl, gv, ga, true, red, &inv_beta);
 }
 
 // End of a cycle: y = H[:k,:k]^-1 g[:k] (back substitution, repeated in every
@@ -541,12 +564,12 @@ __global__ __launch_bounds__(256) void k_nk_residual(const double* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = ps;
 }
 
-// One workgroup: rn = ||R(u)||, the convergence flag, and the backtracking
-// state of the iteration reset.
-__global__ __launch_bounds__(256) void k_nk_residual_final(const double* part, GmresCtl* ctl) {
-  __shared__ double red[4];
+// rn = ||R(u)|| from the residual partials, the convergence flag, and the
+// backtracking state of the iteration reset (stored by the writer; every
+// thread of the workgroup calls it).
+__device__ void residual_finish(const double* part, GmresCtl* ctl, bool writer, double* red) {
   const double r2 = wg_sum(part[threadIdx.x], red);
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || !writer) return;
   const double rn = sqrt(r2);
   ctl->rn = rn;
   ctl->newton_conv = rn < ctl->newton_tol ? 1 : 0;
@@ -554,6 +577,26 @@ __global__ __launch_bounds__(256) void k_nk_residual_final(const double* part, G
   ctl->alpha = ctl->damping;
   ctl->accepted = 0;
   ctl->moved = 0;
+}
+
+// One workgroup: the residual finish (before the host reads it).
+__global__ __launch_bounds__(256) void k_nk_residual_final(const double* part, GmresCtl* ctl) {
+  __shared__ double red[4];
+  residual_finish(part, ctl, true, red);
+}
+
+// GMRES start x = 0 b (as a scale of b, so non-finite entries propagate);
+// with RES workgroup 0 also finishes the Newton residual from its partials,
+// which saves that one-workgroup launch.
+template <bool RES>
+__global__ __launch_bounds__(256) void k_gm_zero_x(const double* b, double* x, long long n, const double* part,
+                                                   GmresCtl* ctl) {
+  if constexpr (RES) {
+    __shared__ double red[4];
+    if (blockIdx.x == 0) residual_finish(part, ctl, true, red);
+  }
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) x[g] = 0.0 * b[g];
 }
 
 // Acceptance of a backtracking trial from its residual partials: accept
@@ -782,14 +825,14 @@ class NKSolver {
     GmresCtl* hc = ctl_host();
     if (!hc) return fail(MJX_E_HIP, "nk: pinned control block unavailable");
     // residual at the initial guess; its flags serve the first iteration
-    residual();
+    residual(true);
     if ((rc = read_ctl(hc))) return rc;
     bool converged = hc->newton_conv != 0;
     int it = 0, lin_total = 0;
     while (it < c_.max_newton && !converged) {
-      if (it > 0) residual();
+      if (it > 0) residual(false);
       int lin = 0;
-      if ((rc = gmres(b_, dx_, hc, &lin, std::min(it, NK_HINTS - 1)))) return rc;
+      if ((rc = gmres(b_, dx_, hc, &lin, std::min(it, NK_HINTS - 1), it > 0))) return rc;
       lin_total += lin;
       converged = hc->newton_conv != 0;  // this iteration's test (made before its update)
       // trial t writes its partials to buffer t % 2; the next trial (or the
@@ -826,10 +869,17 @@ class NKSolver {
   // Left-preconditioned restarted GMRES(m) for J dx = b, x0 = 0.  One
   // iteration = one Arnoldi step; the Givens residual is compared with rtol *
   // ||M b|| after every step; at most max_krylov restart cycles.
-  int gmres(const double* bvec, double* x, GmresCtl* hc, int* iters_out, int hint_slot) {
+  // res_pending: the Newton residual's partials in part_ still need their
+  // finish (made by the first launch here)
+  int gmres(const double* bvec, double* x, GmresCtl* hc, int* iters_out, int hint_slot, bool res_pending) {
     const int m = c_.restart;
     int rc;
-    hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n_)), dim3(256), 0, st_, bvec, 0.0, x, n_);
+    if (res_pending)
+      hipLaunchKernelGGL((k_gm_zero_x<true>), dim3(nblk(n_)), dim3(256), 0, st_, bvec, x, n_, (const double*)part_,
+                         ctl_);
+    else
+      hipLaunchKernelGGL((k_gm_zero_x<false>), dim3(nblk(n_)), dim3(256), 0, st_, bvec, x, n_,
+                         (const double*)nullptr, ctl_);
     if ((rc = precond(bvec, w_))) return rc;  // w = M b
     int iters = 0;
     for (int cyc = 0; cyc < c_.max_krylov; ++cyc) {
@@ -840,9 +890,8 @@ class NKSolver {
         if ((rc = precond(xt_, w_))) return rc;
       }
       hipLaunchKernelGGL(k_norm2_partial, dim3(RB), dim3(RT), 0, st_, (const double*)w_, n_, part_);
-      hipLaunchKernelGGL(k_gm_cycle_start, dim3(1), dim3(RT), 0, st_, (const double*)part_, cyc == 0 ? 1 : 0, ctl_,
-                         g_, ga_);
       if (cyc > 0) {  // a restart may find the target met before any step
+        hipLaunchKernelGGL(k_gm_cycle_start, dim3(1), dim3(RT), 0, st_, (const double*)part_, 0, ctl_, g_, ga_);
         if ((rc = read_ctl(hc))) return rc;
         if (hc->done) break;
       }
@@ -853,8 +902,11 @@ class NKSolver {
         // each step's JVP makes the previous step's Givens step, except the
         // chunk's first; the chunk's last step launches its own before the
         // control block is read
-        for (int jj = j; jj < j + c; ++jj)
-          if ((rc = arnoldi_step(jj, jj > j, jj == j + c - 1))) return rc;
+        // (the first cycle's first step also makes the cycle start)
+        for (int jj = j; jj < j + c; ++jj) {
+          const int mode = jj > j ? JVP_GIVENS : (cyc == 0 && jj == 0 ? JVP_START : JVP_PLAIN);
+          if ((rc = arnoldi_step(jj, mode, jj == j + c - 1))) return rc;
+        }
         j += c;
         if ((rc = read_ctl(hc))) return rc;
         if (hc->done) break;
@@ -887,9 +939,10 @@ class NKSolver {
     return hipStreamSynchronize(st_) == hipSuccess ? MJX_OK : fail(MJX_E_HIP, "nk: sync failed");
   }
 
-  void residual() {
+  // finish = false leaves the residual's finish to GMRES's first launch
+  void residual(bool finish) {
     hipLaunchKernelGGL(k_nk_residual, dim3(RB), dim3(RT), 0, st_, (const double*)u_, u_old_, fold_, b_, a_, part_);
-    hipLaunchKernelGGL(k_nk_residual_final, dim3(1), dim3(RT), 0, st_, (const double*)part_, ctl_);
+    if (finish) hipLaunchKernelGGL(k_nk_residual_final, dim3(1), dim3(RT), 0, st_, (const double*)part_, ctl_);
   }
 
   int read_ctl(GmresCtl* hc) {
@@ -906,30 +959,34 @@ class NKSolver {
   }
 
   template <typename T>
-  void launch_jvp(int j, bool fold, T* t) {
+  void launch_jvp(int j, int mode, T* t) {
     double* Vj = V_ + (long long)j * n_;
     const GivensArrays gva = givens_arrays();
-    if (fold)
-      hipLaunchKernelGGL((k_gm_jvp<T, true>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
+    if (mode == JVP_GIVENS)
+      hipLaunchKernelGGL((k_gm_jvp<T, JVP_GIVENS>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
+                         (const double*)w_, ctl_, Vj, t, a_, gva, j - 1, c_.restart);
+    else if (mode == JVP_START)
+      hipLaunchKernelGGL((k_gm_jvp<T, JVP_START>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
                          (const double*)w_, ctl_, Vj, t, a_, gva, j - 1, c_.restart);
     else
-      hipLaunchKernelGGL((k_gm_jvp<T, false>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
+      hipLaunchKernelGGL((k_gm_jvp<T, JVP_PLAIN>), dim3(nblk(n_)), dim3(256), 0, st_, (const double*)u_,
                          (const double*)w_, ctl_, Vj, t, a_, gva, j - 1, c_.restart);
   }
 
-  // Arnoldi step j.  fold: its JVP makes the Givens step of step j - 1;
-  // givens_now: step j's own Givens step is launched here (otherwise the
-  // next step's JVP makes it).
-  int arnoldi_step(int j, bool fold, bool givens_now) {
+  // Arnoldi step j.  mode: JVP_GIVENS, its JVP makes the Givens step of
+  // step j - 1; JVP_START, it makes the first cycle's start; givens_now:
+  // step j's own Givens step is launched here (otherwise the next step's
+  // JVP makes it).
+  int arnoldi_step(int j, int mode, bool givens_now) {
     const int k = j + 1;
     if (c_.mv_dtype == MJX_F64) {
-      launch_jvp<double>(j, fold, t64_);
+      launch_jvp<double>(j, mode, t64_);
       int rc = precond_t<double>(t64_, w_, t64b_, r2r64_, lamx64_, lamy64_);
       if (rc) return rc;
       hipLaunchKernelGGL(k_gm_dot1<double>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const double*)w_, w_,
                          n_, k, part_, (const GmresCtl*)ctl_);
     } else {
-      launch_jvp<float>(j, fold, t32_);
+      launch_jvp<float>(j, mode, t32_);
       int rc = precond32(t32_, w32_);
       if (rc) return rc;
       hipLaunchKernelGGL(k_gm_dot1<float>, dim3(RB), dim3(RT), 0, st_, (const double*)V_, (const float*)w32_, w_,
