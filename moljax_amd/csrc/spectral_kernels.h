@@ -721,10 +721,16 @@ struct RowLaunch {
   }
   static int gs(int B, int ny, cpx<T>* W, const InstParams* ip, int rxn, int mstep, hipStream_t st) {
     const int tiles = (ny + L - 1) / L;
-    // Two row tiles per workgroup for FP32 lines of at most 8 values a
-    // thread (n >= 512) when the grid keeps >= 2048 workgroups.
-    if constexpr (sizeof(T) == 4 && N >= 512 && plan_p(N) <= 8) {
-      if (tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
+    // Two row tiles per workgroup (the second tile's loads in flight during
+    // the first) when the grid keeps >= 2048 workgroups: FP32 lines of at
+    // most 8 values a thread (n >= 512), and 256-point lines once the packed
+    // plane outgrows the 256 MiB Infinity Cache, where the row pass reads
+    // from HBM (256^2: B = 2048 gains 4.5 %; B = 512, whose plane the column
+    // pass leaves in that cache, loses 1.5 %).
+    constexpr bool two_ok = sizeof(T) == 4 && (N == 256 || (N >= 512 && plan_p(N) <= 8));
+    if constexpr (two_ok) {
+      const bool two_here = N != 256 || (long long)B * ny * N * (long long)sizeof(cpx<T>) > (256LL << 20);
+      if (two_here && tiles % 2 == 0 && (long long)(tiles / 2) * B >= 2048) {
         if (rxn != RXN_HEUN)
           hipLaunchKernelGGL((k_row_gs<T, N, L, RXN_EULER, 2>), dim3(tiles / 2 * B), dim3(RowCfg<N, L>::BT), 0, st,
                              W, ip, ny, tiles / 2, twiddles<T>() + tw_base(N), mstep, rxn - RXN_EULER);

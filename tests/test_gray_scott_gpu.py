@@ -96,12 +96,15 @@ def test_batch_equals_single_bitwise(gpu):
     assert torch.equal(U[2], u1) and torch.equal(V[2], v1)
 
 
-def test_two_row_tiles_per_workgroup_bitwise(gpu):
-    """A 512^2 FP32 batch of 32 runs the row pass with two row tiles per
-    workgroup (the second tile's loads in flight during the first); a lone
-    instance runs one.  Each row's arithmetic is the same: bitwise equal."""
+@pytest.mark.parametrize("n,B", [(512, 32), (256, 1024)])
+def test_two_row_tiles_per_workgroup_bitwise(gpu, n, B):
+    """A 512^2 FP32 batch of 32, and a 256^2 batch of 1024 (its 512 MiB
+    packed plane outgrows the Infinity Cache), run the row pass with two row
+    tiles per workgroup (the second tile's loads in flight during the
+    first); a lone instance runs one.  Each row's arithmetic is the same:
+    bitwise equal."""
     from moljax_amd.core.stepping import _gs_rows, gs_imex_strang_inplace
-    n, B, steps = 512, 32, 4
+    steps = 4
     grid, (model, cache, diff) = _model(n, torch.float32)
     ys = [ogs.initial_condition(n, n, seed % 3) for seed in range(B)]
     U = torch.from_numpy(np.stack([y["u"] for y in ys])).to("cuda", torch.float32)
