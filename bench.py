@@ -298,7 +298,10 @@ def c3_newton_krylov(n, steps, op_dtype):
         res.append(st.final_res_norm)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    arnoldi = sum(gmres)
     return {"ms_per_step": 1e3 * el / steps, "newton_iters": newton, "gmres_iters": gmres,
+            "arnoldi_steps_per_be_step": arnoldi / steps,
+            "us_per_arnoldi_step": 1e6 * el / arnoldi if arnoldi else None,
             "max_final_residual": max(res), "operator_dtype": str(op_dtype).replace("torch.", ""),
             "_u": y["u"].double()}
 
